@@ -1,0 +1,172 @@
+/*
+ * edx — C ABI of the MI355X symmetry-resolved exact-diagonalization path.
+ *
+ * Plain C types only.  Every entry point returns an int status (EDX_OK or a
+ * negative EDX_ERR_* code) and records a message retrievable with
+ * edx_last_error(ctx).  Device memory belongs to the objects created here;
+ * host buffers passed in belong to the caller.  Calls on one ctx are
+ * serialised on that ctx's HIP stream; different ctxs may be used from
+ * different threads.  "ref" citations are lines of the reference file
+ * src/realspace_exactdiagonalization_py/symmetry_resolved_ed.py (SRE) unless
+ * another file is named.
+ *
+ * Conventions shared with the reference:
+ *   - mask bit i-1 is site i (1-based sites in all term/permutation inputs);
+ *   - masks of fixed popcount are indexed by their colex rank;
+ *   - a bilinear term (i, j, t) moves a particle from site i to site j;
+ *   - group element g maps the representative to the table entry's mask.
+ */
+#ifndef EDX_H
+#define EDX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EDX_OK 0
+#define EDX_ERR_INVALID (-1)   /* invalid argument          -> ValueError      */
+#define EDX_ERR_ASSERT (-2)    /* violated precondition     -> AssertionError  */
+#define EDX_ERR_HIP (-3)       /* HIP runtime failure       -> RuntimeError    */
+#define EDX_ERR_OOM (-4)       /* device allocation failure -> MemoryError     */
+#define EDX_ERR_NOCONV (-5)    /* eigensolver not converged -> RuntimeError    */
+#define EDX_ERR_INTERNAL (-6)  /* anything else             -> RuntimeError    */
+
+#define EDX_PREC_FP64 0        /* complex128 vectors and matrix elements        */
+#define EDX_PREC_MIXED 1       /* complex64 sweeps, FP64 recurrence/refinement  */
+
+typedef struct edx_ctx edx_ctx;
+typedef struct edx_catalog edx_catalog;
+typedef struct edx_sector edx_sector;
+typedef struct edx_hamiltonian edx_hamiltonian;
+
+/* ---- context ------------------------------------------------------------
+ * Device context owning one HIP stream.  No reference counterpart (the
+ * reference is single-process numpy); every object below is created on a ctx. */
+int edx_ctx_create(int32_t device, edx_ctx** out);
+int edx_ctx_destroy(edx_ctx* ctx);
+const char* edx_last_error(const edx_ctx* ctx);
+/* Number of HIP devices visible (0 when none); never fails. */
+int32_t edx_device_count(void);
+int edx_synchronize(edx_ctx* ctx);
+
+/* ---- orbit catalog --------------------------------------------------------
+ * Replaces build_symmetry_orbit_catalog / _build_orbit_catalog_kernel
+ * (ref SRE:1298-1404, :433-575).  perm: [n_g*n_site] 1-based targets,
+ * perm_phases: [n_g*n_site] complex128 as (re, im) pairs, perm_angles:
+ * [n_g*n_site] arg(perm_phases) as numpy.angle returns them.  The operations
+ * must form a group (each has its inverse in the list); n_site <= 63 and
+ * C(n_site, n_filled) < 2^31. */
+int edx_catalog_build(edx_ctx* ctx, int32_t n_site, int32_t n_filled, int32_t n_g,
+                      const int32_t* perm, const double* perm_phases, const double* perm_angles,
+                      int32_t fermionic, edx_catalog** out);
+/* Sizes: C(n_site, n_filled), number of orbits, total stabilizer entries. */
+int edx_catalog_info(const edx_catalog* cat, int64_t* n_total, int64_t* n_orbits, int64_t* n_stab);
+/* Copies the catalog to host buffers (ref Symmetry_Orbit_Catalog fields,
+ * SRE:1179-1190).  Any pointer may be NULL to skip that array.
+ * rep_rank/rep_sym: int32[n_total]; rep_amp: complex64[n_total] as float
+ * pairs; reps: uint64[n_orbits] ascending; stab_order: int32[n_orbits];
+ * stab_offsets: int64[n_orbits+1]; stab_g: int32[n_stab] ascending per
+ * orbit; stab_phase: complex128[n_stab] as double pairs. */
+int edx_catalog_fetch(edx_ctx* ctx, const edx_catalog* cat, int32_t* rep_rank, int32_t* rep_sym,
+                      float* rep_amp, uint64_t* reps, int32_t* stab_order, int64_t* stab_offsets,
+                      int32_t* stab_g, double* stab_phase);
+/* Replaces update_orbit_stabilizer_phases (ref SRE:1220-1269): new per-site
+ * phases for the same permutations; stabilizer phases and table amplitudes
+ * recomputed as running products.  Different permutations -> EDX_ERR_INVALID. */
+int edx_catalog_update_phases(edx_ctx* ctx, edx_catalog* cat, const int32_t* perm,
+                              const double* perm_phases, const double* perm_angles);
+/* Batched O(1) canonical lookup (ref get_canonical / CanonicalMap,
+ * SRE:1595-1632): for n masks of popcount n_filled, the representative mask,
+ * the table's g (rep -> mask) and alpha (complex128 widened from the stored
+ * complex64).  Masks of another popcount -> EDX_ERR_INVALID. */
+int edx_catalog_lookup(edx_ctx* ctx, const edx_catalog* cat, int64_t n, const uint64_t* masks, uint64_t* rep,
+                       int32_t* g, double* alpha);
+int edx_catalog_destroy(edx_catalog* cat);
+
+/* ---- symmetry sector (irrep filter) ----------------------------------------
+ * Replaces build_symmetry_sector_basis / _is_orbit_compatible
+ * (ref SRE:1484-1546): keep orbit o iff |chi(h) - alpha_h| <= atol for all h in
+ * its stabilizer, in catalog order.  chi: [n_g] complex128 (re, im). */
+int edx_sector_build(edx_ctx* ctx, const edx_catalog* cat, const double* chi, double atol,
+                     edx_sector** out, int64_t* dim);
+/* reps: uint64[dim]; stab_order: int32[dim]; orbit_to_basis: int32[n_orbits]
+ * (-1 outside the sector).  NULL skips. */
+int edx_sector_fetch(edx_ctx* ctx, const edx_sector* sec, uint64_t* reps, int32_t* stab_order,
+                     int32_t* orbit_to_basis);
+int edx_sector_destroy(edx_sector* sec);
+
+/* ---- sector Hamiltonian (projection table) ---------------------------------
+ * Replaces build_matrixfree_projection_table / _vectorized_block_triplets and
+ * the CSR of MatrixFreeHamiltonian (ref SRE:595-693, :1719-1769, :1913-1963).
+ * Terms: 1-based sites; amplitudes complex128 as (re, im).  Off-diagonal
+ * entries t*s_JW*alpha*conj(chi(g))*sqrt(stab_row/stab_col) are stored per row
+ * sorted by (column, term); the density diagonal separately.  The sector and
+ * catalog must outlive the Hamiltonian. */
+int edx_hamiltonian_build(edx_ctx* ctx, const edx_catalog* cat, const edx_sector* sec,
+                          const double* chi, int32_t n_bilinear, const int32_t* i_from,
+                          const int32_t* i_to, const double* t, int32_t n_density,
+                          const int32_t* v_i, const int32_t* v_j, const double* v,
+                          edx_hamiltonian** out, int64_t* n_entries);
+/* Projection table in the reference's triplet form: row/col int64[n_entries],
+ * vals complex128[n_entries] (re, im), h_diag complex128[dim].  Triplets come
+ * grouped by row, sorted by (column, term) within a row.  NULL skips. */
+int edx_hamiltonian_fetch_table(edx_ctx* ctx, const edx_hamiltonian* ham, int64_t* row, int64_t* col,
+                                int32_t* term, double* vals, double* h_diag);
+/* Sector block given as a host CSR (off-diagonal part, any order within a
+ * row) plus a diagonal: the matrix-mode block of build_ed_Hamiltonian_
+ * symmetry_block (ref SRE:2102-2174) placed on the device for edx_eigs and
+ * edx_matvec.  row_ptr int64[dim+1], col int32[nnz], vals/h_diag complex128. */
+int edx_hamiltonian_from_csr(edx_ctx* ctx, int64_t dim, int64_t nnz, const int64_t* row_ptr, const int32_t* col,
+                             const double* vals, const double* h_diag, edx_hamiltonian** out);
+/* y = H x for n_vec host vectors of length dim (complex128, column-major
+ * [n_vec][dim]); replaces MatrixFreeHamiltonian.__call__ and
+ * apply_hamiltonian (ref SRE:1965-1976, :1979-2069). */
+int edx_matvec(edx_ctx* ctx, const edx_hamiltonian* ham, int32_t n_vec, const double* x, double* y);
+/* Same on device-resident complex128 buffers (row-major [dim][n_vec]
+ * interleaved input, column-major [n_vec][dim] output): the HBM-resident
+ * kernel the benchmarks time.  Pointers are device addresses. */
+int edx_matvec_device(edx_ctx* ctx, const edx_hamiltonian* ham, int32_t precision, int32_t n_vec,
+                      const void* x_dev, void* y_dev);
+int edx_hamiltonian_destroy(edx_hamiltonian* ham);
+
+/* ---- eigensolver ----------------------------------------------------------
+ * Replaces diagonalize_block_matrixfree (ref SRE:2248-2299): the nev lowest
+ * eigenvalues (ascending, real) of the sector block, by thick-restarted block
+ * Lanczos with full reorthogonalisation.  block = 0 picks min(nev, 8).
+ * precision: EDX_PREC_FP64 or EDX_PREC_MIXED (complex64 sweeps then FP64
+ * refinement).  vecs (optional, may be NULL): complex128 [nev][dim].
+ * info (optional): [0] matvecs (vector count), [1] restarts,
+ * [2] max residual / max(1,|E|), [3] block size. */
+int edx_eigs(edx_ctx* ctx, const edx_hamiltonian* ham, int32_t nev, int32_t block, double tol,
+             uint64_t seed, int32_t precision, double* vals, double* vecs, double* info);
+
+/* ---- SpMV launch shape -------------------------------------------------------
+ * Lanes of a wavefront that share one matrix row (1, 2, 4, 8 or 16).  No
+ * reference counterpart (launch-shape tuning of the device product). */
+int edx_set_spmv_group(edx_ctx* ctx, int32_t lanes);                 /* default for new builds; 0 = auto */
+int edx_hamiltonian_set_spmv_group(edx_ctx* ctx, edx_hamiltonian* ham, int32_t lanes);
+int edx_hamiltonian_info(const edx_hamiltonian* ham, int64_t* dim, int64_t* n_entries, int32_t* spmv_group);
+
+/* ---- host-only helper (no device needed) ------------------------------------
+ * Eigen-decomposition of a Hermitian n x n matrix (row-major (re, im) pairs)
+ * from its upper (upper != 0, as LAPACK eigh(UPLO="U") in ref SRE:2281) or
+ * lower triangle; w ascending; z (optional) eigenvectors as columns. */
+int edx_host_herm_eig(int32_t n, const double* a, int32_t upper, double* w, double* z);
+
+/* ---- device helpers for benchmarks and tests ------------------------------ */
+int edx_device_alloc(edx_ctx* ctx, uint64_t bytes, void** out);
+int edx_device_free(edx_ctx* ctx, void* p);
+int edx_memcpy_h2d(edx_ctx* ctx, void* dst, const void* src, uint64_t bytes);
+int edx_memcpy_d2h(edx_ctx* ctx, void* dst, const void* src, uint64_t bytes);
+/* Average duration (ms) of the last n_rep launches of the matvec kernel timed
+ * with HIP events on the ctx stream, inputs resident in device memory. */
+int edx_time_matvec(edx_ctx* ctx, const edx_hamiltonian* ham, int32_t precision, int32_t n_vec,
+                    int32_t n_rep, const void* x_dev, void* y_dev, double* avg_ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* EDX_H */

@@ -1,0 +1,172 @@
+"""ctypes binding of ``libedx.so`` (declared in ``include/edx.h``).
+
+The library is the only compute path of this package: importing this module
+raises ``ImportError`` if the library was not built, and every call that
+needs a device raises ``RuntimeError`` when none is visible.  Status codes map
+to the Python exceptions the reference raises for the same conditions
+(``ValueError`` / ``AssertionError``), plus ``MemoryError`` and
+``RuntimeError`` for device failures.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+import threading
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(os.environ.get("EDX_LIBRARY", Path(__file__).resolve().parent / "libedx.so"))
+
+if not LIB_PATH.exists():
+    raise ImportError(
+        f"edx HIP library not found at {LIB_PATH}; build it with "
+        "`python -m realspace_exatdiagonalization_py_amd.csrc.build` (hipcc, gfx950)"
+    )
+
+lib = C.CDLL(str(LIB_PATH))
+
+OK = 0
+ERR_INVALID, ERR_ASSERT, ERR_HIP, ERR_OOM, ERR_NOCONV, ERR_INTERNAL = -1, -2, -3, -4, -5, -6
+PREC_FP64, PREC_MIXED = 0, 1
+
+P = C.c_void_p
+i32, i64, u64, f64 = C.c_int32, C.c_int64, C.c_uint64, C.c_double
+PP = C.POINTER(C.c_void_p)
+pi32 = C.POINTER(C.c_int32)
+pi64 = C.POINTER(C.c_int64)
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "edx_ctx_create": (C.c_int, [i32, PP]),
+    "edx_ctx_destroy": (C.c_int, [P]),
+    "edx_last_error": (C.c_char_p, [P]),
+    "edx_device_count": (i32, []),
+    "edx_synchronize": (C.c_int, [P]),
+    "edx_catalog_build": (C.c_int, [P, i32, i32, i32, P, P, P, i32, PP]),
+    "edx_catalog_info": (C.c_int, [P, pi64, pi64, pi64]),
+    "edx_catalog_fetch": (C.c_int, [P, P, P, P, P, P, P, P, P, P]),
+    "edx_catalog_update_phases": (C.c_int, [P, P, P, P, P]),
+    "edx_catalog_lookup": (C.c_int, [P, P, i64, P, P, P, P]),
+    "edx_catalog_destroy": (C.c_int, [P]),
+    "edx_sector_build": (C.c_int, [P, P, P, f64, PP, pi64]),
+    "edx_sector_fetch": (C.c_int, [P, P, P, P, P]),
+    "edx_sector_destroy": (C.c_int, [P]),
+    "edx_hamiltonian_build": (C.c_int, [P, P, P, P, i32, P, P, P, i32, P, P, P, PP, pi64]),
+    "edx_hamiltonian_fetch_table": (C.c_int, [P, P, P, P, P, P, P]),
+    "edx_hamiltonian_from_csr": (C.c_int, [P, i64, i64, P, P, P, P, PP]),
+    "edx_matvec": (C.c_int, [P, P, i32, P, P]),
+    "edx_matvec_device": (C.c_int, [P, P, i32, i32, P, P]),
+    "edx_hamiltonian_destroy": (C.c_int, [P]),
+    "edx_eigs": (C.c_int, [P, P, i32, i32, f64, u64, i32, P, P, P]),
+    "edx_set_spmv_group": (C.c_int, [P, i32]),
+    "edx_hamiltonian_set_spmv_group": (C.c_int, [P, P, i32]),
+    "edx_hamiltonian_info": (C.c_int, [P, pi64, pi64, pi32]),
+    "edx_host_herm_eig": (C.c_int, [i32, P, i32, P, P]),
+    "edx_device_alloc": (C.c_int, [P, u64, PP]),
+    "edx_device_free": (C.c_int, [P, P]),
+    "edx_memcpy_h2d": (C.c_int, [P, P, P, u64]),
+    "edx_memcpy_d2h": (C.c_int, [P, P, P, u64]),
+    "edx_time_matvec": (C.c_int, [P, P, i32, i32, i32, P, P, C.POINTER(C.c_double)]),
+}
+
+for _name, (_res, _args) in SIGNATURES.items():
+    _fn = getattr(lib, _name)
+    _fn.restype = _res
+    _fn.argtypes = _args
+
+
+def ptr(a: np.ndarray | None):
+    """Data pointer of a contiguous numpy array (None -> NULL)."""
+    if a is None:
+        return None
+    assert a.flags["C_CONTIGUOUS"], "array must be C-contiguous"
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def raise_for(rc: int, ctx=None) -> None:
+    if rc == OK:
+        return
+    msg = lib.edx_last_error(ctx)
+    text = msg.decode() if msg else f"edx error {rc}"
+    if rc == ERR_INVALID:
+        raise ValueError(text)
+    if rc == ERR_ASSERT:
+        raise AssertionError(text)
+    if rc == ERR_OOM:
+        raise MemoryError(text)
+    raise RuntimeError(text)
+
+
+class Context:
+    """One device context (HIP stream).  ``Context.default()`` is shared per process."""
+
+    _default: "Context | None" = None
+    _lock = threading.Lock()
+
+    def __init__(self, device: int = 0):
+        n = lib.edx_device_count()
+        if n <= 0:
+            raise RuntimeError("edx: no HIP device is visible; the ED path runs only on the GPU")
+        h = C.c_void_p()
+        raise_for(lib.edx_ctx_create(device, C.byref(h)))
+        self.handle = h
+        self.device = device
+
+    @classmethod
+    def default(cls) -> "Context":
+        with cls._lock:
+            if cls._default is None:
+                cls._default = Context(int(os.environ.get("EDX_DEVICE", "0")))
+            return cls._default
+
+    def check(self, rc: int) -> None:
+        raise_for(rc, self.handle)
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h:
+            lib.edx_ctx_destroy(h)
+            self.handle = None
+
+
+class DeviceArray:
+    """Raw device allocation (used by benchmarks/tests for HBM-resident vectors)."""
+
+    def __init__(self, ctx: Context, nbytes: int):
+        self.ctx = ctx
+        self.nbytes = int(nbytes)
+        p = C.c_void_p()
+        ctx.check(lib.edx_device_alloc(ctx.handle, self.nbytes, C.byref(p)))
+        self.ptr = p
+
+    def upload(self, a: np.ndarray) -> None:
+        a = np.ascontiguousarray(a)
+        assert a.nbytes <= self.nbytes
+        self.ctx.check(lib.edx_memcpy_h2d(self.ctx.handle, self.ptr, ptr(a), a.nbytes))
+
+    def download(self, out: np.ndarray) -> np.ndarray:
+        assert out.flags["C_CONTIGUOUS"] and out.nbytes <= self.nbytes
+        self.ctx.check(lib.edx_memcpy_d2h(self.ctx.handle, ptr(out), self.ptr, out.nbytes))
+        return out
+
+    def __del__(self):
+        p = getattr(self, "ptr", None)
+        if p:
+            lib.edx_device_free(self.ctx.handle, p)
+            self.ptr = None
+
+
+def host_herm_eig(a: np.ndarray, upper: bool = True):
+    """Hermitian eigen-decomposition by the library's host solver (no device)."""
+    a = np.ascontiguousarray(a, dtype=np.complex128)
+    n = a.shape[0]
+    w = np.zeros(n)
+    z = np.zeros((n, n), dtype=np.complex128)
+    raise_for(lib.edx_host_herm_eig(n, ptr(a), 1 if upper else 0, ptr(w), ptr(z)))
+    return w, z
+
+
+def exported_symbols() -> list[str]:
+    return sorted(SIGNATURES)

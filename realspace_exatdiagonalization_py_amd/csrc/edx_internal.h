@@ -1,0 +1,224 @@
+// Internal declarations shared by the edx HIP translation units.
+//
+// Masks are uint64 with site i (1-based) at bit i-1.  Ranks are colex ranks
+// (sum_j C(b_j, j+1) over set bits b_0 < b_1 < ...), equal to the position of
+// the mask in ascending numeric order among masks of the same popcount.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+#include <vector>
+
+#include "../../include/edx.h"
+
+#define EDX_MAX_SITES 63
+#define EDX_MAX_GROUP 64
+#define EDX_MAX_TERMS 65535
+
+namespace edx {
+
+// ---------------------------------------------------------------------------
+// Errors: every API call returns an int status and records text in the ctx.
+// ---------------------------------------------------------------------------
+struct Error {
+  int code;
+  std::string msg;
+};
+
+#define EDX_HIP(call)                                                          \
+  do {                                                                         \
+    hipError_t _e = (call);                                                    \
+    if (_e != hipSuccess)                                                      \
+      throw ::edx::Error{_e == hipErrorOutOfMemory ? EDX_ERR_OOM : EDX_ERR_HIP, \
+                         std::string(#call) + ": " + hipGetErrorString(_e)};   \
+  } while (0)
+
+#define EDX_REQUIRE(cond, code, text)                                          \
+  do {                                                                         \
+    if (!(cond)) throw ::edx::Error{(code), (text)};                           \
+  } while (0)
+
+// Device allocation owned by an object; freed in the destructor.
+template <class T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  explicit DevBuf(size_t count) { alloc(count); }
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+    return *this;
+  }
+  ~DevBuf() { release(); }
+  void alloc(size_t count) {
+    release();
+    n = count;
+    if (count) EDX_HIP(hipMalloc(&p, count * sizeof(T)));
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  size_t bytes() const { return n * sizeof(T); }
+};
+
+struct Ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string last_error;
+  int autotune_group = 0;  // lanes per row in the SpMV (0 = pick by heuristics)
+};
+
+// Scoped device selection for API calls.
+struct DeviceGuard {
+  int prev = 0;
+  explicit DeviceGuard(int dev) {
+    EDX_HIP(hipGetDevice(&prev));
+    if (prev != dev) EDX_HIP(hipSetDevice(dev));
+  }
+  ~DeviceGuard() { (void)hipSetDevice(prev); }
+};
+
+// ---------------------------------------------------------------------------
+// Group description shared by the catalog / phase / table kernels.
+// ---------------------------------------------------------------------------
+struct GroupView {
+  int n_site;
+  int n_filled;
+  int n_g;
+  int fermionic;
+  int product_phase;        // 1: alpha as running complex products (C == 1 case)
+  const uint8_t* perm;      // [n_g][n_site] 0-based target bit
+  const double* angle;      // [n_g][n_site] arg(eta_g(i))
+  const double2* phase;     // [n_g][n_site] eta_g(i)
+  const int32_t* inv;       // [n_g] index of the inverse element
+  const uint32_t* binom;    // [(n_site+1)][(n_filled+1)] C(b, j) (entries < 2^31 used)
+};
+
+// Scan helpers (scan.hip).  Exclusive prefix sums, deterministic.
+// counts of type uint32 -> offsets uint32/int64; returns the total.
+uint64_t exclusive_scan_u32(Ctx& ctx, const uint32_t* in, uint32_t* out, size_t n);
+uint64_t exclusive_scan_u32_to_i64(Ctx& ctx, const uint32_t* in, int64_t* out, size_t n);
+// popcount of every word, scanned: out[w] = sum_{v<w} popcount(words[v]).
+uint64_t exclusive_scan_popcount64(Ctx& ctx, const uint64_t* words, uint32_t* out, size_t n);
+
+inline unsigned grid_for(size_t n, unsigned block) {
+  size_t g = (n + block - 1) / block;
+  return (unsigned)(g ? g : 1);
+}
+
+}  // namespace edx
+
+// ---------------------------------------------------------------------------
+// Device helpers (inlined into every kernel that uses them).
+// ---------------------------------------------------------------------------
+namespace edxd {
+
+__device__ __forceinline__ int ctz64(uint64_t x) { return __builtin_ctzll(x); }
+__device__ __forceinline__ int popc64(uint64_t x) { return __builtin_popcountll(x); }
+
+// Colex rank with the binomial table (row b has stride nf1 = n_filled + 1).
+__device__ __forceinline__ uint64_t rank_colex(uint64_t m, const uint32_t* binom, int nf1) {
+  uint64_t r = 0;
+  int j = 1;
+  while (m) {
+    int b = ctz64(m);
+    r += binom[b * nf1 + j];
+    ++j;
+    m &= m - 1;
+  }
+  return r;
+}
+
+// Mask of a colex rank: for j = k..1 take the largest b with C(b, j) <= r.
+__device__ __forceinline__ uint64_t unrank_colex(uint64_t r, int n_site, int k, const uint32_t* binom,
+                                                 int nf1) {
+  uint64_t m = 0;
+  int b = n_site - 1;
+  for (int j = k; j >= 1; --j) {
+    while (b > 0 && (uint64_t)binom[b * nf1 + j] > r) --b;
+    m |= 1ull << b;
+    r -= binom[b * nf1 + j];
+    --b;
+  }
+  return m;
+}
+
+// Image of m under group element g (permutation part only).
+__device__ __forceinline__ uint64_t image(uint64_t m, const uint8_t* perm_g) {
+  uint64_t img = 0;
+  while (m) {
+    img |= 1ull << perm_g[ctz64(m)];
+    m &= m - 1;
+  }
+  return img;
+}
+
+// Parity (0/1) of the permutation restricted to the occupied sites of m.
+__device__ __forceinline__ int perm_parity(uint64_t m, const uint8_t* perm_g) {
+  uint64_t placed = 0;
+  int par = 0;
+  while (m) {
+    int p = perm_g[ctz64(m)];
+    par ^= popc64(placed >> p) & 1;  // targets above p already placed (p itself is free)
+    placed |= 1ull << p;
+    m &= m - 1;
+  }
+  return par;
+}
+
+// Phase of applying g to m in exponent form: exp(i * sum angle) with the angle
+// sum formed per byte (bits ascending from 0.0), bytes added in ascending order
+// from 0.0, then multiplied by the fermion sign as the complex (-1 + 0i).
+__device__ __forceinline__ double2 alpha_exp(uint64_t m, int n_site, const double* angle_g,
+                                             const uint8_t* perm_g, int fermionic) {
+  const int n_bytes = (n_site + 7) >> 3;
+  double tot = 0.0;
+  for (int b = 0; b < n_bytes; ++b) {
+    uint32_t byte = (uint32_t)(m >> (8 * b)) & 0xffu;
+    double s = 0.0;
+    while (byte) {
+      int i = __builtin_ctz(byte);
+      s = s + angle_g[8 * b + i];
+      byte &= byte - 1;
+    }
+    tot = tot + s;
+  }
+  double sn, cs;
+  sincos(tot, &sn, &cs);
+  double2 a = make_double2(cs, sn);
+  if (fermionic && perm_parity(m, perm_g)) {
+    // (re + i im) * (-1 + 0i), formed term by term as in a complex product
+    double re = a.x * -1.0 - a.y * 0.0;
+    double im = a.x * 0.0 + a.y * -1.0;
+    a = make_double2(re, im);
+  }
+  return a;
+}
+
+// Phase of applying g to m as a running complex product over occupied sites
+// (ascending), starting at 1 + 0i, then times the integer parity (+1/-1).
+__device__ __forceinline__ double2 alpha_prod(uint64_t m, const double2* phase_g,
+                                              const uint8_t* perm_g, int fermionic) {
+  double re = 1.0, im = 0.0;
+  uint64_t t = m;
+  while (t) {
+    double2 e = phase_g[ctz64(t)];
+    double nr = re * e.x - im * e.y;
+    double ni = re * e.y + im * e.x;
+    re = nr;
+    im = ni;
+    t &= t - 1;
+  }
+  double p = (fermionic && perm_parity(m, perm_g)) ? -1.0 : 1.0;
+  double nr = re * p - im * 0.0;
+  double ni = re * 0.0 + im * p;
+  return make_double2(nr, ni);
+}
+
+}  // namespace edxd

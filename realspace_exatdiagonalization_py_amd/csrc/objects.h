@@ -1,0 +1,93 @@
+// Definitions of the opaque C-ABI objects.
+#pragma once
+
+#include <exception>
+#include <new>
+
+#include "edx_internal.h"
+
+struct edx_ctx : public edx::Ctx {};
+
+struct edx_catalog {
+  int device = 0;
+  int n_site = 0, n_filled = 0, n_g = 0, fermionic = 0, product_phase = 0;
+  int64_t n_total = 0, n_orbits = 0, n_stab = 0;
+  std::vector<int32_t> h_perm;        // host copy (1-based) to validate phase updates
+  edx::DevBuf<uint8_t> perm;          // [n_g][n_site] 0-based targets
+  edx::DevBuf<double> angle;          // [n_g][n_site]
+  edx::DevBuf<double2> phase;         // [n_g][n_site]
+  edx::DevBuf<int32_t> inv;           // [n_g]
+  edx::DevBuf<uint32_t> binom;        // [(n_site+1)][(n_filled+1)]
+  edx::DevBuf<int32_t> rep_rank;      // [C] orbit index of the mask of that rank
+  edx::DevBuf<int32_t> rep_sym;       // [C] largest g with g.rep == mask
+  edx::DevBuf<float2> rep_amp;        // [C] complex64 alpha_g(rep)
+  edx::DevBuf<uint64_t> reps;         // [n_orbits] ascending
+  edx::DevBuf<int32_t> stab_order;    // [n_orbits]
+  edx::DevBuf<uint32_t> stab_off;     // [n_orbits + 1]
+  edx::DevBuf<uint8_t> stab_g;        // [n_stab]
+  edx::DevBuf<double2> stab_phase;    // [n_stab]
+
+  edx::GroupView view() const {
+    return edx::GroupView{n_site, n_filled, n_g, fermionic, product_phase, perm.p, angle.p,
+                          phase.p, inv.p, binom.p};
+  }
+};
+
+struct edx_sector {
+  int device = 0;
+  const edx_catalog* cat = nullptr;
+  int64_t dim = 0;
+  edx::DevBuf<uint64_t> reps;         // [dim]
+  edx::DevBuf<int32_t> stab_order;    // [dim]
+  edx::DevBuf<int32_t> orbit_to_basis;  // [n_orbits]
+};
+
+struct edx_hamiltonian {
+  int device = 0;
+  const edx_sector* sec = nullptr;
+  int64_t dim = 0;
+  int64_t n_entries = 0;
+  int32_t n_bilinear = 0;
+  edx::DevBuf<int64_t> row_ptr;       // [dim + 1]
+  edx::DevBuf<int32_t> col;           // [n_entries]
+  edx::DevBuf<uint16_t> term;         // [n_entries] original term index
+  edx::DevBuf<double2> val;           // [n_entries] complex128
+  edx::DevBuf<float2> val32;          // [n_entries] complex64 copy
+  edx::DevBuf<double2> hdiag;         // [dim]
+  edx::DevBuf<float2> hdiag32;        // [dim]
+  int spmv_group = 0;                 // lanes per row chosen at build
+};
+
+namespace edx {
+
+inline void set_error(edx_ctx* ctx, const std::string& msg);
+
+template <class F>
+int guarded(edx_ctx* ctx, F&& f) {
+  try {
+    f();
+    return EDX_OK;
+  } catch (const Error& e) {
+    set_error(ctx, e.msg);
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    set_error(ctx, "host allocation failed");
+    return EDX_ERR_OOM;
+  } catch (const std::exception& e) {
+    set_error(ctx, e.what());
+    return EDX_ERR_INTERNAL;
+  } catch (...) {
+    set_error(ctx, "unknown error");
+    return EDX_ERR_INTERNAL;
+  }
+}
+
+// Thread-local fallback when no ctx is available.
+std::string& tls_error();
+
+inline void set_error(edx_ctx* ctx, const std::string& msg) {
+  if (ctx) ctx->last_error = msg;
+  tls_error() = msg;
+}
+
+}  // namespace edx

@@ -1,0 +1,687 @@
+// Sector Hamiltonian: projection table as a row-grouped sparse matrix, and
+// the complex SpMV/SpMM that applies it.
+//
+// Reference: _vectorized_block_triplets (SRE:595-693) generates, per column
+// representative m and bilinear term (i -> j) with i occupied and j empty,
+// the entry at row = sector index of the orbit of m ^ (b_i | b_j):
+//     t * s_JW(m) * alpha * conj(chi(g)) * sqrt(stab_row / stab_col)
+// with (orbit, g, alpha) read from the representative tables at
+// rank(m ^ hop).  MatrixFreeHamiltonian turns the triplets into a CSR
+// (SRE:1958-1963) and every H x is one CSR product.
+//
+// Device build:
+//   P1 count   lane per column: candidate hops, row lookup, atomic row counts
+//   P2 scan    row counts -> row_ptr (int64)
+//   P3 fill    same walk; key (col << 16 | term) at row_ptr[row] + atomic slot
+//   P4 sort    per row: keys ascending, so the row order is (col, term) and the
+//              result never depends on atomic ordering
+//   P5 values  per entry: element recomputed from (col, term) in the reference's
+//              multiplication order; complex128 and complex64 copies
+//   P6 diag    per row: density terms in term order
+// Hops are enumerated per occupied site over terms grouped by source site,
+// so every lane walks the same number of candidates.
+#include <algorithm>
+#include <cmath>
+
+#include "objects.h"
+
+using edx::DevBuf;
+
+namespace {
+
+constexpr int kBlock = 256;
+
+struct TermView {
+  int n_site;
+  int n_filled;
+  int n_terms;
+  int fermionic;
+  const int32_t* toff;      // [n_site+1] terms grouped by source bit
+  const uint16_t* tk;       // [n_terms] original index, grouped order
+  const uint8_t* tto;       // [n_terms] target bit, grouped order
+  const uint8_t* from_bit;  // [n_terms] by original index
+  const uint8_t* to_bit;    // [n_terms] by original index
+  const double2* amp;       // [n_terms] by original index
+  const uint64_t* between;  // [n_terms] JW string mask by original index
+  const uint32_t* binom;    // [(n_site+1)][(n_filled+1)]
+};
+
+struct SectorView {
+  int64_t dim;
+  const uint64_t* reps;
+  const int32_t* stab;
+  const int32_t* o2b;
+  const int32_t* rep_rank;
+  const int32_t* rep_sym;
+  const float2* rep_amp;
+};
+
+struct SharedTerms {
+  int32_t* toff;
+  uint16_t* tk;
+  uint8_t* tto;
+  uint32_t* binom;
+};
+
+__device__ SharedTerms load_terms(const TermView& tv, unsigned char* smem) {
+  const int nb = (tv.n_site + 1) * (tv.n_filled + 1);
+  uint32_t* binom = reinterpret_cast<uint32_t*>(smem);
+  int32_t* toff = reinterpret_cast<int32_t*>(binom + nb);
+  uint16_t* tk = reinterpret_cast<uint16_t*>(toff + tv.n_site + 1);
+  uint8_t* tto = reinterpret_cast<uint8_t*>(tk + tv.n_terms);
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) binom[i] = tv.binom[i];
+  for (int i = threadIdx.x; i <= tv.n_site; i += blockDim.x) toff[i] = tv.toff[i];
+  for (int i = threadIdx.x; i < tv.n_terms; i += blockDim.x) {
+    tk[i] = tv.tk[i];
+    tto[i] = tv.tto[i];
+  }
+  __syncthreads();
+  return SharedTerms{toff, tk, tto, binom};
+}
+
+size_t terms_smem_bytes(int n_site, int n_filled, int n_terms) {
+  return (size_t)(n_site + 1) * (n_filled + 1) * 4 + (size_t)(n_site + 1) * 4 + (size_t)n_terms * 3 + 16;
+}
+
+// Visits every valid hop of column c: f(row, term).
+template <class F>
+__device__ __forceinline__ void for_each_hop(const TermView& tv, const SectorView& sv, const SharedTerms& st,
+                                             uint64_t m, F&& f) {
+  const int nf1 = tv.n_filled + 1;
+  uint64_t occ = m;
+  while (occ) {
+    const int s = edxd::ctz64(occ);
+    occ &= occ - 1;
+    const int q1 = st.toff[s + 1];
+    for (int q = st.toff[s]; q < q1; ++q) {
+      const int to = st.tto[q];
+      if ((m >> to) & 1ull) continue;
+      const uint64_t nm = m ^ (1ull << s) ^ (1ull << to);
+      const uint64_t idx = edxd::rank_colex(nm, st.binom, nf1);
+      const int32_t row = sv.o2b[sv.rep_rank[idx]];
+      if (row >= 0) f(row, (int)st.tk[q]);
+    }
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) ptab_count(TermView tv, SectorView sv, uint32_t* cnt) {
+  extern __shared__ unsigned char smem[];
+  SharedTerms st = load_terms(tv, smem);
+  const int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (c >= sv.dim) return;
+  for_each_hop(tv, sv, st, sv.reps[c], [&](int32_t row, int) { atomicAdd(&cnt[row], 1u); });
+}
+
+__global__ void __launch_bounds__(kBlock)
+ptab_fill(TermView tv, SectorView sv, const int64_t* row_ptr, uint32_t* cursor, uint64_t* keys) {
+  extern __shared__ unsigned char smem[];
+  SharedTerms st = load_terms(tv, smem);
+  const int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (c >= sv.dim) return;
+  for_each_hop(tv, sv, st, sv.reps[c], [&](int32_t row, int k) {
+    const uint32_t slot = atomicAdd(&cursor[row], 1u);
+    keys[row_ptr[row] + slot] = ((uint64_t)c << 16) | (uint64_t)k;
+  });
+}
+
+constexpr int kSortThreads = 64;
+constexpr int kSortCap = 128;
+
+__device__ __forceinline__ void insertion_sort_global(uint64_t* a, int64_t n) {
+  for (int64_t i = 1; i < n; ++i) {
+    const uint64_t v = a[i];
+    int64_t j = i - 1;
+    while (j >= 0 && a[j] > v) {
+      a[j + 1] = a[j];
+      --j;
+    }
+    a[j + 1] = v;
+  }
+}
+
+__global__ void __launch_bounds__(kSortThreads) ptab_sort(int64_t dim, const int64_t* row_ptr, uint64_t* keys) {
+  __shared__ uint64_t buf[kSortCap * kSortThreads];
+  const int64_t r = (int64_t)blockIdx.x * kSortThreads + threadIdx.x;
+  if (r >= dim) return;
+  const int64_t b = row_ptr[r];
+  const int64_t n = row_ptr[r + 1] - b;
+  if (n <= 1) return;
+  if (n > kSortCap) {
+    insertion_sort_global(keys + b, n);
+    return;
+  }
+  uint64_t* s = buf + threadIdx.x;  // element i at s[i * kSortThreads]
+  for (int i = 0; i < n; ++i) s[i * kSortThreads] = keys[b + i];
+  for (int i = 1; i < n; ++i) {
+    const uint64_t v = s[i * kSortThreads];
+    int j = i - 1;
+    while (j >= 0 && s[j * kSortThreads] > v) {
+      s[(j + 1) * kSortThreads] = s[j * kSortThreads];
+      --j;
+    }
+    s[(j + 1) * kSortThreads] = v;
+  }
+  for (int i = 0; i < n; ++i) keys[b + i] = s[i * kSortThreads];
+}
+
+// Complex product formed as (a.re b.re - a.im b.im) + (a.re b.im + a.im b.re) i
+// with separate roundings (the file is compiled without FP contraction).
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+__global__ void __launch_bounds__(kBlock)
+ptab_values(TermView tv, SectorView sv, const double2* chi, const int64_t* row_ptr, const uint64_t* keys,
+            int32_t* col, uint16_t* term, double2* val, float2* val32) {
+  __shared__ double2 chi_s[EDX_MAX_GROUP];
+  __shared__ uint32_t binom_s[(EDX_MAX_SITES + 1) * (EDX_MAX_SITES + 1)];
+  const int nb = (tv.n_site + 1) * (tv.n_filled + 1);
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) binom_s[i] = tv.binom[i];
+  for (int i = threadIdx.x; i < EDX_MAX_GROUP; i += blockDim.x) chi_s[i] = chi[i];
+  __syncthreads();
+  const int64_t r = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (r >= sv.dim) return;
+  const int nf1 = tv.n_filled + 1;
+  const double stab_row = (double)sv.stab[r];
+  for (int64_t e = row_ptr[r]; e < row_ptr[r + 1]; ++e) {
+    const uint64_t key = keys[e];
+    const int64_t c = (int64_t)(key >> 16);
+    const int k = (int)(key & 0xffffu);
+    const uint64_t m = sv.reps[c];
+    const uint64_t nm = m ^ (1ull << tv.from_bit[k]) ^ (1ull << tv.to_bit[k]);
+    const uint64_t idx = edxd::rank_colex(nm, binom_s, nf1);
+    const int g = sv.rep_sym[idx];
+    const float2 a32 = sv.rep_amp[idx];
+    const double2 a = make_double2((double)a32.x, (double)a32.y);
+    const double s = (tv.fermionic && (edxd::popc64(m & tv.between[k]) & 1)) ? -1.0 : 1.0;
+    const double2 t = tv.amp[k];
+    // t * hop_phase, with hop_phase the complex (s + 0i)
+    double2 h = make_double2(t.x * s - t.y * 0.0, t.x * 0.0 + t.y * s);
+    h = cmul(h, a);
+    const double2 cc = make_double2(chi_s[g].x, -chi_s[g].y);
+    h = cmul(h, cc);
+    const double q = sqrt(stab_row / (double)sv.stab[c]);
+    h = make_double2(h.x * q - h.y * 0.0, h.x * 0.0 + h.y * q);
+    col[e] = (int32_t)c;
+    term[e] = (uint16_t)k;
+    val[e] = h;
+    val32[e] = make_float2((float)h.x, (float)h.y);
+  }
+}
+
+__global__ void __launch_bounds__(kBlock)
+ptab_diag(int64_t dim, const uint64_t* reps, int n_density, const uint8_t* vi, const uint8_t* vj,
+          const double2* v, double2* hdiag, float2* hdiag32) {
+  const int64_t r = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (r >= dim) return;
+  const uint64_t m = reps[r];
+  double re = 0.0, im = 0.0;
+  for (int k = 0; k < n_density; ++k) {
+    if (((m >> vi[k]) & 1ull) && ((m >> vj[k]) & 1ull)) {
+      re = re + v[k].x;
+      im = im + v[k].y;
+    }
+  }
+  hdiag[r] = make_double2(re, im);
+  hdiag32[r] = make_float2((float)re, (float)im);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// SpMM: Y[j][r] = hdiag[r] X[r][j] + sum_e val[e] X[col[e]][j],  j < nb.
+// X interleaved (row-major [dim][nb]) so one gather serves all nb vectors;
+// Y column-major with leading dimension ldy.  G lanes share one row.
+// ---------------------------------------------------------------------------
+namespace edx {
+
+template <typename V> struct Real;
+template <> struct Real<double2> { using T = double; };
+template <> struct Real<float2> { using T = float; };
+
+template <typename V, int NBMAX, int G>
+__global__ void __launch_bounds__(256)
+spmm_csr(int64_t dim, int nb, const int64_t* __restrict__ row_ptr, const int32_t* __restrict__ col,
+         const V* __restrict__ val, const V* __restrict__ hdiag, const V* __restrict__ X, V* __restrict__ Y,
+         int64_t ldy) {
+  using T = typename Real<V>::T;
+  const int64_t row = (int64_t)blockIdx.x * (256 / G) + threadIdx.x / G;
+  const int sub = threadIdx.x % G;
+  T ar[NBMAX], ai[NBMAX];
+#pragma unroll
+  for (int j = 0; j < NBMAX; ++j) ar[j] = ai[j] = (T)0;
+  if (row < dim) {
+    const int64_t e1 = row_ptr[row + 1];
+    for (int64_t e = row_ptr[row] + sub; e < e1; e += G) {
+      const V v = val[e];
+      const V* x = X + (int64_t)col[e] * nb;
+#pragma unroll
+      for (int j = 0; j < NBMAX; ++j) {
+        if (j < nb) {
+          const V xv = x[j];
+          ar[j] = fma(v.x, xv.x, ar[j]);
+          ar[j] = fma(-v.y, xv.y, ar[j]);
+          ai[j] = fma(v.x, xv.y, ai[j]);
+          ai[j] = fma(v.y, xv.x, ai[j]);
+        }
+      }
+    }
+  }
+  if (G > 1) {
+#pragma unroll
+    for (int off = G / 2; off >= 1; off >>= 1) {
+#pragma unroll
+      for (int j = 0; j < NBMAX; ++j) {
+        ar[j] += __shfl_xor(ar[j], off, G);
+        ai[j] += __shfl_xor(ai[j], off, G);
+      }
+    }
+  }
+  if (row < dim && sub == 0) {
+    const V d = hdiag[row];
+    const V* xr = X + row * nb;
+#pragma unroll
+    for (int j = 0; j < NBMAX; ++j) {
+      if (j < nb) {
+        const V xv = xr[j];
+        T yr = fma(d.x, xv.x, ar[j]);
+        yr = fma(-d.y, xv.y, yr);
+        T yi = fma(d.x, xv.y, ai[j]);
+        yi = fma(d.y, xv.x, yi);
+        V out;
+        out.x = yr;
+        out.y = yi;
+        Y[(int64_t)j * ldy + row] = out;
+      }
+    }
+  }
+}
+
+template <typename V, int NBMAX>
+void launch_spmm_nb(const edx_hamiltonian& h, hipStream_t st, int G, int nb, const V* val, const V* hd, const V* X,
+                    V* Y, int64_t ldy) {
+  const int64_t dim = h.dim;
+#define EDX_SPMM_CASE(GG)                                                                                  \
+  case GG:                                                                                                 \
+    hipLaunchKernelGGL((spmm_csr<V, NBMAX, GG>), dim3(grid_for(dim * GG, 256)), dim3(256), 0, st, dim, nb, \
+                       (const int64_t*)h.row_ptr.p, (const int32_t*)h.col.p, val, hd, X, Y, ldy);          \
+    break;
+  switch (G) {
+    EDX_SPMM_CASE(1)
+    EDX_SPMM_CASE(2)
+    EDX_SPMM_CASE(4)
+    EDX_SPMM_CASE(8)
+    default:
+      EDX_SPMM_CASE(16)
+  }
+#undef EDX_SPMM_CASE
+  EDX_HIP(hipGetLastError());
+}
+
+// Applies H to nb <= 16 interleaved vectors.  precision selects the
+// complex128 or complex64 element copy.
+void spmm(const edx_hamiltonian& h, hipStream_t st, int precision, int nb, const void* X, void* Y, int64_t ldy) {
+  if (h.dim == 0) return;
+  EDX_REQUIRE(nb >= 1 && nb <= 16, EDX_ERR_INVALID, "at most 16 vectors per SpMM");
+  const int G = h.spmv_group > 0 ? h.spmv_group : 4;
+  if (precision == EDX_PREC_FP64) {
+    auto v = (const double2*)h.val.p;
+    auto d = (const double2*)h.hdiag.p;
+    auto x = (const double2*)X;
+    auto y = (double2*)Y;
+    if (nb == 1) launch_spmm_nb<double2, 1>(h, st, G, nb, v, d, x, y, ldy);
+    else if (nb <= 4) launch_spmm_nb<double2, 4>(h, st, G, nb, v, d, x, y, ldy);
+    else if (nb <= 8) launch_spmm_nb<double2, 8>(h, st, G, nb, v, d, x, y, ldy);
+    else launch_spmm_nb<double2, 16>(h, st, G, nb, v, d, x, y, ldy);
+  } else {
+    auto v = (const float2*)h.val32.p;
+    auto d = (const float2*)h.hdiag32.p;
+    auto x = (const float2*)X;
+    auto y = (float2*)Y;
+    if (nb == 1) launch_spmm_nb<float2, 1>(h, st, G, nb, v, d, x, y, ldy);
+    else if (nb <= 4) launch_spmm_nb<float2, 4>(h, st, G, nb, v, d, x, y, ldy);
+    else if (nb <= 8) launch_spmm_nb<float2, 8>(h, st, G, nb, v, d, x, y, ldy);
+    else launch_spmm_nb<float2, 16>(h, st, G, nb, v, d, x, y, ldy);
+  }
+}
+
+// Column-major [nb][ld] -> interleaved [dim][nb].
+template <typename V>
+__global__ void interleave_kernel(int64_t dim, int nb, const V* src, int64_t ld, V* dst) {
+  const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (r >= dim) return;
+  for (int j = 0; j < nb; ++j) dst[r * nb + j] = src[(int64_t)j * ld + r];
+}
+
+void interleave(hipStream_t st, int precision, int64_t dim, int nb, const void* src, int64_t ld, void* dst) {
+  if (dim == 0) return;
+  if (precision == EDX_PREC_FP64)
+    hipLaunchKernelGGL(interleave_kernel<double2>, dim3(grid_for(dim, 256)), dim3(256), 0, st, dim, nb,
+                       (const double2*)src, ld, (double2*)dst);
+  else
+    hipLaunchKernelGGL(interleave_kernel<float2>, dim3(grid_for(dim, 256)), dim3(256), 0, st, dim, nb,
+                       (const float2*)src, ld, (float2*)dst);
+  EDX_HIP(hipGetLastError());
+}
+
+// Dense row-major complex128 copy of the block (entries summed per row in
+// stored order, diagonal added last).
+__global__ void dense_kernel(int64_t dim, const int64_t* row_ptr, const int32_t* col, const double2* val,
+                             const double2* hdiag, double2* out) {
+  const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (r >= dim) return;
+  double2* o = out + r * dim;
+  for (int64_t e = row_ptr[r]; e < row_ptr[r + 1]; ++e) {
+    double2 a = o[col[e]];
+    a.x = a.x + val[e].x;
+    a.y = a.y + val[e].y;
+    o[col[e]] = a;
+  }
+  o[r].x = o[r].x + hdiag[r].x;
+  o[r].y = o[r].y + hdiag[r].y;
+}
+
+void dense_matrix(const edx_hamiltonian& h, hipStream_t st, double2* out) {
+  EDX_HIP(hipMemsetAsync(out, 0, (size_t)h.dim * h.dim * sizeof(double2), st));
+  hipLaunchKernelGGL(dense_kernel, dim3(grid_for(h.dim, 256)), dim3(256), 0, st, h.dim,
+                     (const int64_t*)h.row_ptr.p, (const int32_t*)h.col.p, (const double2*)h.val.p,
+                     (const double2*)h.hdiag.p, out);
+  EDX_HIP(hipGetLastError());
+}
+
+}  // namespace edx
+
+namespace {
+
+int pick_group(double mean_row) {
+  if (mean_row <= 6) return 1;
+  if (mean_row <= 16) return 2;
+  if (mean_row <= 40) return 4;
+  if (mean_row <= 100) return 8;
+  return 16;
+}
+
+}  // namespace
+
+extern "C" {
+
+int edx_hamiltonian_build(edx_ctx* ctx, const edx_catalog* cat, const edx_sector* sec, const double* chi,
+                          int32_t n_bilinear, const int32_t* i_from, const int32_t* i_to, const double* t,
+                          int32_t n_density, const int32_t* v_i, const int32_t* v_j, const double* v,
+                          edx_hamiltonian** out, int64_t* n_entries) {
+  return edx::guarded(ctx, [&] {
+    EDX_REQUIRE(ctx && cat && sec && chi && out, EDX_ERR_INVALID, "null argument");
+    EDX_REQUIRE(sec->cat == cat, EDX_ERR_INVALID, "sector was built from a different catalog");
+    EDX_REQUIRE(n_bilinear >= 0 && n_bilinear <= EDX_MAX_TERMS, EDX_ERR_INVALID, "too many bilinear terms");
+    EDX_REQUIRE(n_density >= 0, EDX_ERR_INVALID, "negative density term count");
+    EDX_REQUIRE(n_bilinear == 0 || (i_from && i_to && t), EDX_ERR_INVALID, "null bilinear term arrays");
+    EDX_REQUIRE(n_density == 0 || (v_i && v_j && v), EDX_ERR_INVALID, "null density term arrays");
+    *out = nullptr;
+    const int N = cat->n_site;
+    // ---- host-side term preparation
+    std::vector<uint8_t> fb(n_bilinear), tb(n_bilinear);
+    std::vector<uint64_t> betw(n_bilinear);
+    std::vector<std::vector<int>> by_src(N);
+    for (int k = 0; k < n_bilinear; ++k) {
+      const int a = i_from[k], b = i_to[k];
+      EDX_REQUIRE(a >= 1 && a <= N && b >= 1 && b <= N, EDX_ERR_INVALID, "bilinear term site out of range");
+      EDX_REQUIRE(a != b, EDX_ERR_INVALID,
+                  "bilinear term with i == j: the hop gate would remove a particle; pass on-site energies as "
+                  "density terms (i, i, v)");
+      fb[k] = (uint8_t)(a - 1);
+      tb[k] = (uint8_t)(b - 1);
+      const int lo = std::min(a, b), hi = std::max(a, b);
+      betw[k] = (hi - lo > 1) ? (((1ull << (hi - lo - 1)) - 1ull) << lo) : 0ull;
+      by_src[a - 1].push_back(k);
+    }
+    std::vector<int32_t> toff(N + 1, 0);
+    std::vector<uint16_t> tk;
+    std::vector<uint8_t> tto;
+    for (int s = 0; s < N; ++s) {
+      toff[s] = (int32_t)tk.size();
+      for (int k : by_src[s]) {
+        tk.push_back((uint16_t)k);
+        tto.push_back(tb[k]);
+      }
+    }
+    toff[N] = (int32_t)tk.size();
+    std::vector<uint8_t> dvi(n_density), dvj(n_density);
+    for (int k = 0; k < n_density; ++k) {
+      EDX_REQUIRE(v_i[k] >= 1 && v_i[k] <= N && v_j[k] >= 1 && v_j[k] <= N, EDX_ERR_INVALID,
+                  "density term site out of range");
+      dvi[k] = (uint8_t)(v_i[k] - 1);
+      dvj[k] = (uint8_t)(v_j[k] - 1);
+    }
+    edx::DeviceGuard dg(ctx->device);
+    hipStream_t st = ctx->stream;
+    const int T = n_bilinear;
+    DevBuf<int32_t> d_toff(N + 1);
+    DevBuf<uint16_t> d_tk(std::max(T, 1));
+    DevBuf<uint8_t> d_tto(std::max(T, 1)), d_fb(std::max(T, 1)), d_tb(std::max(T, 1));
+    DevBuf<double2> d_amp(std::max(T, 1));
+    DevBuf<uint64_t> d_betw(std::max(T, 1));
+    DevBuf<double2> d_chi(EDX_MAX_GROUP);
+    DevBuf<uint8_t> d_vi(std::max(n_density, 1)), d_vj(std::max(n_density, 1));
+    DevBuf<double2> d_v(std::max(n_density, 1));
+    std::vector<double> chi_pad(2 * EDX_MAX_GROUP, 0.0);
+    std::copy(chi, chi + 2 * cat->n_g, chi_pad.begin());
+    auto h2d = [&](void* dst, const void* src, size_t bytes) {
+      if (bytes) EDX_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st));
+    };
+    h2d(d_toff.p, toff.data(), toff.size() * 4);
+    h2d(d_tk.p, tk.data(), tk.size() * 2);
+    h2d(d_tto.p, tto.data(), tto.size());
+    h2d(d_fb.p, fb.data(), fb.size());
+    h2d(d_tb.p, tb.data(), tb.size());
+    h2d(d_amp.p, t, (size_t)T * 16);
+    h2d(d_betw.p, betw.data(), betw.size() * 8);
+    h2d(d_chi.p, chi_pad.data(), chi_pad.size() * 8);
+    h2d(d_vi.p, dvi.data(), dvi.size());
+    h2d(d_vj.p, dvj.data(), dvj.size());
+    h2d(d_v.p, v, (size_t)n_density * 16);
+
+    TermView tv{N, cat->n_filled, T, cat->fermionic, d_toff.p, d_tk.p, d_tto.p, d_fb.p, d_tb.p, d_amp.p,
+                d_betw.p, cat->binom.p};
+    const int64_t D = sec->dim;
+    SectorView sv{D, sec->reps.p, sec->stab_order.p, sec->orbit_to_basis.p, cat->rep_rank.p, cat->rep_sym.p,
+                  cat->rep_amp.p};
+    auto ham = new edx_hamiltonian();
+    try {
+      ham->device = ctx->device;
+      ham->sec = sec;
+      ham->dim = D;
+      ham->n_bilinear = T;
+      ham->row_ptr.alloc(D + 1);
+      ham->hdiag.alloc(D);
+      ham->hdiag32.alloc(D);
+      const size_t smem = terms_smem_bytes(N, cat->n_filled, T);
+      int64_t E = 0;
+      if (D > 0) {
+        DevBuf<uint32_t> cnt(D);
+        EDX_HIP(hipMemsetAsync(cnt.p, 0, D * 4, st));
+        hipLaunchKernelGGL(ptab_count, dim3(edx::grid_for(D, kBlock)), dim3(kBlock), smem, st, tv, sv, cnt.p);
+        EDX_HIP(hipGetLastError());
+        E = (int64_t)edx::exclusive_scan_u32_to_i64(*ctx, cnt.p, ham->row_ptr.p, (size_t)D);
+        EDX_HIP(hipMemcpyAsync(ham->row_ptr.p + D, &E, 8, hipMemcpyHostToDevice, st));
+        DevBuf<uint64_t> keys(std::max<int64_t>(E, 1));
+        EDX_HIP(hipMemsetAsync(cnt.p, 0, D * 4, st));
+        hipLaunchKernelGGL(ptab_fill, dim3(edx::grid_for(D, kBlock)), dim3(kBlock), smem, st, tv, sv,
+                           (const int64_t*)ham->row_ptr.p, cnt.p, keys.p);
+        EDX_HIP(hipGetLastError());
+        hipLaunchKernelGGL(ptab_sort, dim3(edx::grid_for(D, kSortThreads)), dim3(kSortThreads), 0, st, D,
+                           (const int64_t*)ham->row_ptr.p, keys.p);
+        EDX_HIP(hipGetLastError());
+        ham->col.alloc(std::max<int64_t>(E, 1));
+        ham->term.alloc(std::max<int64_t>(E, 1));
+        ham->val.alloc(std::max<int64_t>(E, 1));
+        ham->val32.alloc(std::max<int64_t>(E, 1));
+        hipLaunchKernelGGL(ptab_values, dim3(edx::grid_for(D, kBlock)), dim3(kBlock), 0, st, tv, sv,
+                           (const double2*)d_chi.p, (const int64_t*)ham->row_ptr.p, (const uint64_t*)keys.p,
+                           ham->col.p, ham->term.p, ham->val.p, ham->val32.p);
+        EDX_HIP(hipGetLastError());
+        hipLaunchKernelGGL(ptab_diag, dim3(edx::grid_for(D, kBlock)), dim3(kBlock), 0, st, D,
+                           (const uint64_t*)sec->reps.p, n_density, (const uint8_t*)d_vi.p,
+                           (const uint8_t*)d_vj.p, (const double2*)d_v.p, ham->hdiag.p, ham->hdiag32.p);
+        EDX_HIP(hipGetLastError());
+        EDX_HIP(hipStreamSynchronize(st));
+      } else {
+        const int64_t zero = 0;
+        EDX_HIP(hipMemcpyAsync(ham->row_ptr.p, &zero, 8, hipMemcpyHostToDevice, st));
+        EDX_HIP(hipStreamSynchronize(st));
+      }
+      ham->n_entries = E;
+      ham->spmv_group = ctx->autotune_group > 0 ? ctx->autotune_group : pick_group(D ? (double)E / D : 0.0);
+    } catch (...) {
+      delete ham;
+      throw;
+    }
+    *out = ham;
+    if (n_entries) *n_entries = ham->n_entries;
+  });
+}
+
+int edx_hamiltonian_fetch_table(edx_ctx* ctx, const edx_hamiltonian* ham, int64_t* row, int64_t* col,
+                                int32_t* term, double* vals, double* h_diag) {
+  return edx::guarded(ctx, [&] {
+    EDX_REQUIRE(ctx && ham, EDX_ERR_INVALID, "null argument");
+    edx::DeviceGuard dg(ctx->device);
+    const int64_t E = ham->n_entries, D = ham->dim;
+    std::vector<int64_t> rp(D + 1);
+    std::vector<int32_t> c32(E);
+    EDX_HIP(hipMemcpyAsync(rp.data(), ham->row_ptr.p, (D + 1) * 8, hipMemcpyDeviceToHost, ctx->stream));
+    if (E) EDX_HIP(hipMemcpyAsync(c32.data(), ham->col.p, E * 4, hipMemcpyDeviceToHost, ctx->stream));
+    std::vector<uint16_t> t16(term ? E : 0);
+    if (term && E) {
+      EDX_REQUIRE(ham->term.n >= (size_t)E, EDX_ERR_INVALID, "this Hamiltonian carries no term indices");
+      EDX_HIP(hipMemcpyAsync(t16.data(), ham->term.p, E * 2, hipMemcpyDeviceToHost, ctx->stream));
+    }
+    if (vals && E) EDX_HIP(hipMemcpyAsync(vals, ham->val.p, E * 16, hipMemcpyDeviceToHost, ctx->stream));
+    if (h_diag && D) EDX_HIP(hipMemcpyAsync(h_diag, ham->hdiag.p, D * 16, hipMemcpyDeviceToHost, ctx->stream));
+    EDX_HIP(hipStreamSynchronize(ctx->stream));
+    if (row)
+      for (int64_t r = 0; r < D; ++r)
+        for (int64_t e = rp[r]; e < rp[r + 1]; ++e) row[e] = r;
+    if (col)
+      for (int64_t e = 0; e < E; ++e) col[e] = c32[e];
+    if (term)
+      for (int64_t e = 0; e < E; ++e) term[e] = t16[e];
+  });
+}
+
+namespace {
+__global__ void to_c64(int64_t n, const double2* a, float2* b) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) b[i] = make_float2((float)a[i].x, (float)a[i].y);
+}
+}  // namespace
+
+int edx_hamiltonian_from_csr(edx_ctx* ctx, int64_t dim, int64_t nnz, const int64_t* row_ptr, const int32_t* col,
+                             const double* vals, const double* h_diag, edx_hamiltonian** out) {
+  return edx::guarded(ctx, [&] {
+    EDX_REQUIRE(ctx && row_ptr && h_diag && out && (nnz == 0 || (col && vals)), EDX_ERR_INVALID, "null argument");
+    EDX_REQUIRE(dim >= 0 && nnz >= 0 && row_ptr[0] == 0 && row_ptr[dim] == nnz, EDX_ERR_INVALID,
+                "row_ptr must start at 0 and end at nnz");
+    for (int64_t r = 0; r < dim; ++r)
+      EDX_REQUIRE(row_ptr[r] <= row_ptr[r + 1], EDX_ERR_INVALID, "row_ptr must be non-decreasing");
+    for (int64_t e = 0; e < nnz; ++e)
+      EDX_REQUIRE(col[e] >= 0 && col[e] < dim, EDX_ERR_INVALID, "column index out of range");
+    *out = nullptr;
+    edx::DeviceGuard dg(ctx->device);
+    hipStream_t st = ctx->stream;
+    auto ham = new edx_hamiltonian();
+    try {
+      ham->device = ctx->device;
+      ham->dim = dim;
+      ham->n_entries = nnz;
+      ham->row_ptr.alloc(dim + 1);
+      ham->hdiag.alloc(dim);
+      ham->hdiag32.alloc(dim);
+      ham->col.alloc(std::max<int64_t>(nnz, 1));
+      ham->val.alloc(std::max<int64_t>(nnz, 1));
+      ham->val32.alloc(std::max<int64_t>(nnz, 1));
+      EDX_HIP(hipMemcpyAsync(ham->row_ptr.p, row_ptr, (dim + 1) * 8, hipMemcpyHostToDevice, st));
+      if (dim) EDX_HIP(hipMemcpyAsync(ham->hdiag.p, h_diag, dim * 16, hipMemcpyHostToDevice, st));
+      if (nnz) {
+        EDX_HIP(hipMemcpyAsync(ham->col.p, col, nnz * 4, hipMemcpyHostToDevice, st));
+        EDX_HIP(hipMemcpyAsync(ham->val.p, vals, nnz * 16, hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(to_c64, dim3(edx::grid_for(nnz, 256)), dim3(256), 0, st, nnz, (const double2*)ham->val.p,
+                           ham->val32.p);
+        EDX_HIP(hipGetLastError());
+      }
+      if (dim) {
+        hipLaunchKernelGGL(to_c64, dim3(edx::grid_for(dim, 256)), dim3(256), 0, st, dim, (const double2*)ham->hdiag.p,
+                           ham->hdiag32.p);
+        EDX_HIP(hipGetLastError());
+      }
+      EDX_HIP(hipStreamSynchronize(st));
+      ham->spmv_group = ctx->autotune_group > 0 ? ctx->autotune_group : pick_group(dim ? (double)nnz / dim : 0.0);
+    } catch (...) {
+      delete ham;
+      throw;
+    }
+    *out = ham;
+  });
+}
+
+int edx_matvec(edx_ctx* ctx, const edx_hamiltonian* ham, int32_t n_vec, const double* x, double* y) {
+  return edx::guarded(ctx, [&] {
+    EDX_REQUIRE(ctx && ham && x && y, EDX_ERR_INVALID, "null argument");
+    EDX_REQUIRE(n_vec >= 0, EDX_ERR_INVALID, "negative vector count");
+    const int64_t D = ham->dim;
+    if (D == 0 || n_vec == 0) return;
+    edx::DeviceGuard dg(ctx->device);
+    hipStream_t st = ctx->stream;
+    for (int32_t v0 = 0; v0 < n_vec; v0 += 16) {
+      const int nb = std::min<int32_t>(16, n_vec - v0);
+      DevBuf<double2> xc((size_t)nb * D), xi((size_t)nb * D), yc((size_t)nb * D);
+      EDX_HIP(hipMemcpyAsync(xc.p, x + (size_t)v0 * D * 2, (size_t)nb * D * 16, hipMemcpyHostToDevice, st));
+      edx::interleave(st, EDX_PREC_FP64, D, nb, xc.p, D, xi.p);
+      edx::spmm(*ham, st, EDX_PREC_FP64, nb, xi.p, yc.p, D);
+      EDX_HIP(hipMemcpyAsync(y + (size_t)v0 * D * 2, yc.p, (size_t)nb * D * 16, hipMemcpyDeviceToHost, st));
+      EDX_HIP(hipStreamSynchronize(st));
+    }
+  });
+}
+
+int edx_matvec_device(edx_ctx* ctx, const edx_hamiltonian* ham, int32_t precision, int32_t n_vec,
+                      const void* x_dev, void* y_dev) {
+  return edx::guarded(ctx, [&] {
+    EDX_REQUIRE(ctx && ham && x_dev && y_dev, EDX_ERR_INVALID, "null argument");
+    EDX_REQUIRE(precision == EDX_PREC_FP64 || precision == EDX_PREC_MIXED, EDX_ERR_INVALID, "bad precision");
+    edx::DeviceGuard dg(ctx->device);
+    edx::spmm(*ham, ctx->stream, precision, n_vec, x_dev, y_dev, ham->dim);
+    EDX_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+int edx_time_matvec(edx_ctx* ctx, const edx_hamiltonian* ham, int32_t precision, int32_t n_vec, int32_t n_rep,
+                    const void* x_dev, void* y_dev, double* avg_ms) {
+  return edx::guarded(ctx, [&] {
+    EDX_REQUIRE(ctx && ham && x_dev && y_dev && avg_ms && n_rep > 0, EDX_ERR_INVALID, "bad argument");
+    edx::DeviceGuard dg(ctx->device);
+    hipEvent_t a, b;
+    EDX_HIP(hipEventCreate(&a));
+    EDX_HIP(hipEventCreate(&b));
+    EDX_HIP(hipEventRecord(a, ctx->stream));
+    for (int i = 0; i < n_rep; ++i) edx::spmm(*ham, ctx->stream, precision, n_vec, x_dev, y_dev, ham->dim);
+    EDX_HIP(hipEventRecord(b, ctx->stream));
+    EDX_HIP(hipEventSynchronize(b));
+    float ms = 0.f;
+    EDX_HIP(hipEventElapsedTime(&ms, a, b));
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    *avg_ms = (double)ms / n_rep;
+  });
+}
+
+int edx_hamiltonian_destroy(edx_hamiltonian* ham) {
+  if (!ham) return EDX_OK;
+  int prev = 0;
+  (void)hipGetDevice(&prev);
+  (void)hipSetDevice(ham->device);
+  delete ham;
+  (void)hipSetDevice(prev);
+  return EDX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,20 @@
+"""Shared pytest configuration: the ``gpu`` marker, import path, library build."""
+
+import sys
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parents[1]
+if str(ROOT) not in sys.path:
+    sys.path.insert(0, str(ROOT))
+if str(ROOT / "tests") not in sys.path:
+    sys.path.insert(0, str(ROOT / "tests"))
+
+_LIB = ROOT / "realspace_exatdiagonalization_py_amd" / "libedx.so"
+if not _LIB.exists():  # build once if nothing is built yet
+    from realspace_exatdiagonalization_py_amd.csrc.build import build
+
+    build()
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs a HIP device (MI355X)")

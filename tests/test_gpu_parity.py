@@ -1,0 +1,260 @@
+"""Device path vs the CPU oracle on the same seeded inputs.
+
+Integer outputs (ranks, orbit indices, g, representatives, stabilizers,
+sector dimensions, projection-table structure) must be bit-exact.  Floating
+point: complex64 amplitudes exact at zero flux and within one float32 ulp at
+nonzero flux (device sincos vs libm); complex128 matrix elements within 4e-16
+relative; H x within 1e-13 relative; eigenvalues within 1e-10 absolute of the
+oracle's ARPACK/LAPACK values (the tolerance the north star fixes).
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from cases import is_fermionic, make_case, random_vector
+from oracle import ed_oracle as O
+import realspace_exatdiagonalization_py_amd as ed
+from realspace_exatdiagonalization_py_amd.groups import build_irrep_list, group_arrays
+
+pytestmark = pytest.mark.gpu
+
+CATALOG_CASES = ["heis16", "hal23", "hub23", "hal23_twist", "cb23_twist", "heis10_identity", "hal23_ne0",
+                 "cb23_full_twist", "hal33_twist", "hub34"]
+
+
+def _both(name):
+    model, ne, G = make_case(name)
+    perm, ph = group_arrays(G)
+    oc = O.build_catalog(model.lattice.n_site, ne, perm, ph, is_fermionic(model))
+    gc = ed.build_symmetry_orbit_catalog(second_quantized_model=model, n_filled=ne, symmetry_group=G,
+                                         particle_statistics=model.particle_statistics)
+    return model, ne, G, oc, gc
+
+
+def _amp_close(a_dev, a_ref, exact):
+    if exact:
+        return np.array_equal(a_dev, a_ref)
+    ulp = np.spacing(np.abs(a_ref.real).astype(np.float32)) + np.spacing(np.abs(a_ref.imag).astype(np.float32))
+    return bool(np.all(np.abs(a_dev - a_ref) <= 2 * ulp + 1e-45))
+
+
+@pytest.mark.parametrize("name", CATALOG_CASES)
+def test_catalog_bit_exact(name):
+    print(f"[catalog] {name}", flush=True)
+    model, ne, G, oc, gc = _both(name)
+    assert gc.n_total == len(oc.rep_rank)
+    assert gc.n_orbits == len(oc.reps)
+    np.testing.assert_array_equal(gc.representative_masks, oc.reps)
+    np.testing.assert_array_equal(gc.rep_rank_table, oc.rep_rank)
+    np.testing.assert_array_equal(gc.rep_sym_table, oc.rep_sym)
+    assert gc.stabilizer_order_list == [int(x) for x in oc.stab_order]
+    assert gc.stabilizer_g_indices_list == [[int(g) for g in s] for s in oc.stab_g]
+    zero_flux = np.allclose(np.asarray(G.operations[1 % len(G.operations)].perm_phases), 1.0)
+    assert _amp_close(gc.rep_amp_table, oc.rep_amp, exact=zero_flux)
+    ph_dev = np.concatenate([np.asarray(p, complex) for p in gc.stabilizer_phases_list])
+    ph_ref = np.concatenate([np.asarray(p, complex) for p in oc.stab_phase])
+    assert np.abs(ph_dev - ph_ref).max(initial=0.0) <= 4e-16
+
+
+@pytest.mark.parametrize("name", ["heis16", "hub23", "hal23_twist", "cb23_twist", "hub34"])
+def test_sector_filter_bit_exact(name):
+    print(f"[sector] {name}", flush=True)
+    model, ne, G, oc, gc = _both(name)
+    for irrep in build_irrep_list(G, model.lattice):
+        reps, stab, o2b = O.sector_basis(oc, irrep.values)
+        b = ed.build_symmetry_sector_basis(gc, irrep)
+        assert b.dim == len(reps), irrep.label
+        np.testing.assert_array_equal(b.representative_masks, reps)
+        np.testing.assert_array_equal(np.asarray(b.stabilizer_order_list), stab)
+        np.testing.assert_array_equal(b.orbit_to_basis, o2b)
+
+
+def _oracle_block(model, oc, irrep, fermionic):
+    reps, stab, o2b = O.sector_basis(oc, irrep.values)
+    r, c, v, hd, k = O.projection_table(oc, reps, stab, o2b, model.bilinear_terms, model.density_density_terms,
+                                        irrep.values, fermionic, with_terms=True)
+    return reps, r, c, v, hd, k
+
+
+@pytest.mark.parametrize("name", ["heis16", "hub23", "hal23", "hal23_twist", "cb23_twist", "hub34"])
+def test_projection_table(name):
+    print(f"[table] {name}", flush=True)
+    model, ne, G, oc, gc = _both(name)
+    ferm = is_fermionic(model)
+    irreps = build_irrep_list(G, model.lattice)
+    for irrep in irreps[:3]:
+        reps, r, c, v, hd, k = _oracle_block(model, oc, irrep, ferm)
+        basis = ed.build_symmetry_sector_basis(gc, irrep)
+        cmap = ed.CanonicalMap(G, model.particle_statistics, gc)
+        H = ed.MatrixFreeHamiltonian(basis, model.bilinear_terms, model.density_density_terms, cmap)
+        row, col, term, vals, hdg = H._dev.fetch_table()
+        assert len(row) == len(r)
+        o_ref = np.lexsort((k, c, r))
+        o_dev = np.lexsort((term, col, row))
+        np.testing.assert_array_equal(row[o_dev], r[o_ref])
+        np.testing.assert_array_equal(col[o_dev], c[o_ref])
+        np.testing.assert_array_equal(term[o_dev], k[o_ref])
+        scale = np.maximum(np.abs(v[o_ref]), 1e-300)
+        assert np.max(np.abs(vals[o_dev] - v[o_ref]) / scale, initial=0.0) <= 4e-16
+        np.testing.assert_array_equal(hdg, hd)
+        # the reference-order triplets of build_matrixfree_projection_table
+        rr, cc, vv, hh = ed.build_matrixfree_projection_table(basis, model.bilinear_terms,
+                                                              model.density_density_terms, cmap)
+        np.testing.assert_array_equal(rr, r)
+        np.testing.assert_array_equal(cc, c)
+        assert np.max(np.abs(vv - v), initial=0.0) <= 4e-16 * max(1.0, np.abs(v).max(initial=0.0))
+        # H x against the oracle CSR
+        Href = O.csr_hamiltonian(r, c, v, hd, len(reps))
+        for seed in (1, 2):
+            x = random_vector(len(reps), seed)
+            y_ref = Href @ x
+            y = H(x)
+            assert np.linalg.norm(y - y_ref) <= 1e-13 * max(1.0, np.linalg.norm(y_ref))
+        X = np.stack([random_vector(len(reps), s) for s in range(5)], axis=1)
+        Y = H._dev.matvec(X)
+        assert np.linalg.norm(Y - Href @ X) <= 1e-13 * max(1.0, np.linalg.norm(Href @ X))
+
+
+def test_canonical_lookup():
+    model, ne, G, oc, gc = _both("hub23")
+    cmap = ed.CanonicalMap(G, model.particle_statistics, gc)
+    masks = O.unrank_chunk(np.arange(len(oc.rep_rank)), model.lattice.n_site, ne, oc.bt)
+    rep, g, a = cmap.lookup(masks)
+    np.testing.assert_array_equal(rep, oc.reps[oc.rep_rank])
+    np.testing.assert_array_equal(g, oc.rep_sym)
+    np.testing.assert_array_equal(a, oc.rep_amp.astype(np.complex128))
+    m = int(masks[17])
+    assert ed.get_canonical(cmap, m) == (int(oc.reps[oc.rep_rank[17]]), int(oc.rep_sym[17]),
+                                         complex(oc.rep_amp[17]))
+    # outside the filling -> the O(|G|) scan, as in the reference
+    assert ed.get_canonical(cmap, 0b1) == ed.get_canonical_representative(0b1, G, model.particle_statistics)
+
+
+@pytest.mark.parametrize("name", ["hal23", "cb23_twist", "hub23"])
+def test_update_orbit_stabilizer_phases(name):
+    """Catalog built at zero flux, refreshed to the twisted group, vs the oracle's refresh."""
+    print(f"[update] {name}", flush=True)
+    model, ne, G = make_case(name)
+    from realspace_exatdiagonalization_py_amd.groups import build_translation_group
+
+    G0 = build_translation_group(model.lattice, [0.0] * model.lattice.dim)
+    Gt = build_translation_group(model.lattice, [0.23, 0.61])
+    ferm = is_fermionic(model)
+    perm, ph0 = group_arrays(G0)
+    _, pht = group_arrays(Gt)
+    oc = O.build_catalog(model.lattice.n_site, ne, perm, ph0, ferm)
+    O.update_orbit_stabilizer_phases(oc, pht, ferm)
+    gc = ed.build_symmetry_orbit_catalog(second_quantized_model=model, n_filled=ne, symmetry_group=G0,
+                                         particle_statistics=model.particle_statistics)
+    assert ed.update_orbit_stabilizer_phases(gc, G0, model.particle_statistics) is gc  # same object: no-op
+    ed.update_orbit_stabilizer_phases(gc, Gt, model.particle_statistics)
+    np.testing.assert_array_equal(gc.rep_sym_table, oc.rep_sym)
+    np.testing.assert_array_equal(gc.rep_amp_table, oc.rep_amp)  # product form: same roundings
+    ph_dev = np.concatenate([np.asarray(p, complex) for p in gc.stabilizer_phases_list])
+    ph_ref = np.concatenate([np.asarray(p, complex) for p in oc.stab_phase])
+    np.testing.assert_array_equal(ph_dev, ph_ref)
+    # a different permutation part is rejected
+    from realspace_exatdiagonalization_py_amd.groups import Finite_Symmetry_Group
+    bad = Finite_Symmetry_Group("translations", [Gt.operations[0]] + list(reversed(Gt.operations[1:])))
+    with pytest.raises(ValueError):
+        ed.update_orbit_stabilizer_phases(gc, bad, model.particle_statistics)
+
+
+def _solve_both(name, irrep_index, nev, precision=ed.PREC_FP64):
+    model, ne, G, oc, gc = _both(name)
+    ferm = is_fermionic(model)
+    irrep = build_irrep_list(G, model.lattice)[irrep_index]
+    reps, r, c, v, hd, k = _oracle_block(model, oc, irrep, ferm)
+    Href = O.csr_hamiltonian(r, c, v, hd, len(reps))
+    w_ref, _ = O.diagonalize(Href, nev)
+    basis = ed.build_symmetry_sector_basis(gc, irrep)
+    cmap = ed.CanonicalMap(G, model.particle_statistics, gc)
+    H_op, n = ed.hamiltonian_linear_operator(basis, model.bilinear_terms, model.density_density_terms, cmap)
+    w, vecs = ed.diagonalize_block_matrixfree(H_op, n, nev=nev, precision=precision)
+    return w, vecs, w_ref, Href
+
+
+@pytest.mark.parametrize("name,irrep,nev", [("heis16", 0, 4), ("hal23", 0, 3), ("hal23_twist", 1, 3),
+                                            ("hub24", 1, 6), ("cb23_twist", 0, 4)])
+def test_eigs_vs_oracle(name, irrep, nev):
+    print(f"[eigs] {name} irrep={irrep} nev={nev}", flush=True)
+    w, vecs, w_ref, Href = _solve_both(name, irrep, nev)
+    assert np.max(np.abs(w - w_ref)) <= 1e-10, (w, w_ref)
+    # eigenvector residuals
+    R = Href @ vecs - vecs * w
+    assert np.max(np.linalg.norm(R, axis=0)) <= 1e-6
+
+
+def test_eigs_mixed_precision_cfg1():
+    w64, _, w_ref, _ = _solve_both("heis16", 0, 4)
+    w32, _, _, _ = _solve_both("heis16", 0, 4, precision=ed.PREC_MIXED)
+    assert np.max(np.abs(w32 - w_ref)) <= 1e-10
+    assert np.max(np.abs(w64 - w_ref)) <= 1e-10
+
+
+def test_degenerate_multiplets_hubbard():
+    """Spin multiplets appear with full multiplicity (S_z is not fixed)."""
+    model, ne, G, oc, gc = _both("hub24")
+    irrep = build_irrep_list(G, model.lattice)[0]
+    reps, r, c, v, hd, k = _oracle_block(model, oc, irrep, True)
+    dense = O.csr_hamiltonian(r, c, v, hd, len(reps)).toarray()
+    exact = np.linalg.eigvalsh((dense + dense.conj().T) / 2)[:8]
+    basis = ed.build_symmetry_sector_basis(gc, irrep)
+    cmap = ed.CanonicalMap(G, model.particle_statistics, gc)
+    H_op, n = ed.hamiltonian_linear_operator(basis, model.bilinear_terms, model.density_density_terms, cmap)
+    w, _ = ed.diagonalize_block_matrixfree(H_op, n, nev=8)
+    assert np.max(np.abs(w - exact)) <= 1e-10, (w, exact)
+
+
+def test_golden_energy_alignment():
+    """Julia golden energies of the reference's test_energy_alignment.py (ATOL 1e-9), on the GPU."""
+    import json
+    from pathlib import Path
+
+    gold = json.loads((Path(__file__).parent / "golden" / "reference_values.json").read_text())["energy_alignment"]
+    for name, key in (("hal34", "JULIA_REF_BOSON_34"), ("cb34", "JULIA_REF_FERMION_34")):
+        model, ne, G = make_case(name)
+        data = ed.build_ed_data(model, filling_fraction=__import__("fractions").Fraction(ne, model.lattice.n_site),
+                                symmetry_group=G)
+        labels = [tuple(int(x) for x in k.split(",")) for k in gold[key]]
+        ed.ed_scan(data, nev=5, mode="matrixfree", scanned_sectors=labels)
+        for lab in labels:
+            idx = [i for i, ir in enumerate(data.irrep_list) if ir.label == lab][0]
+            vals = data.ed_scan_res[idx][0]
+            np.testing.assert_allclose(vals, gold[key][f"{lab[0]},{lab[1]}"], atol=gold["ATOL"], rtol=0)
+
+
+def test_dense_path_reference_values():
+    import json
+    from pathlib import Path
+
+    gold = json.loads((Path(__file__).parent / "golden" / "reference_values.json").read_text())
+    model, ne, G = make_case("hal23")
+    data = ed.build_ed_data(model, filling_fraction=__import__("fractions").Fraction(ne, 12), symmetry_group=G)
+    vals, _ = ed.ed_scan_at_irrep_matrixfree((0, 0), data, nev=3)
+    assert abs(vals[0] - gold["test_core_haldane_2x3_E0"]) <= 1e-9
+    vals_m, _ = ed.ed_scan_at_irrep_matrix((0, 0), data, nev=3)
+    np.testing.assert_allclose(vals_m, vals, atol=1e-8)
+    assert data.sector_dims[0] == gold["design_notebook"]["haldane_2x3_k0_block"]["dim"]
+
+
+def test_empty_sector_and_errors():
+    model, ne, G = make_case("hub23")
+    data = ed.build_ed_data(model, filling_fraction=__import__("fractions").Fraction(ne, model.lattice.n_site),
+                            symmetry_group=G)
+    # a character table that no orbit matches gives an empty sector
+    irrep = ed.OneDim_Irrep("none", np.full(len(G.operations), 0.3 + 0.1j))
+    basis = ed.build_symmetry_sector_basis(data.orbit_catalog, irrep)
+    assert basis.dim == 0
+    with pytest.raises(ValueError):
+        ed.ed_scan(data, mode="bogus")
+    with pytest.raises(ValueError):
+        ed.ed_scan_at_irrep_matrixfree((9, 9), data)
+    cmap = ed.CanonicalMap(G, model.particle_statistics, data.orbit_catalog)
+    b0 = ed.build_symmetry_sector_basis(data.orbit_catalog, data.irrep_list[0])
+    with pytest.raises(ValueError):  # i == j bilinear term
+        ed.MatrixFreeHamiltonian(b0, [(1, 1, 1.0)], [], cmap)
+    with pytest.raises(ValueError):  # site out of range
+        ed.MatrixFreeHamiltonian(b0, [(1, 99, 1.0)], [], cmap)

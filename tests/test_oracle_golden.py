@@ -1,0 +1,388 @@
+"""CPU: the oracle (and the host-side builders) against the reference's own
+known answers, and against properties the mathematics fixes.
+
+Sources of the fixed values: ``tests/golden/reference_values.json`` (written
+from the reference's test files, benchmark CSV and design notebook by
+``tests/golden/make_reference_values.py``).
+"""
+
+from __future__ import annotations
+
+import json
+import math
+from fractions import Fraction
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from cases import is_fermionic, make_case
+from oracle import ed_oracle as O
+from realspace_exatdiagonalization_py_amd import models as M
+from realspace_exatdiagonalization_py_amd import groups as Gm
+from realspace_exatdiagonalization_py_amd import lattice as La
+from realspace_exatdiagonalization_py_amd.bitwise_operations import (
+    bitmask_of_site, decode_bit_mask_to_configuration, decode_bit_mask_to_configuration_inplace,
+    empty_site_for_mask, encode_configuration_to_bit_mask, filled_site_iter_for_mask, is_site_empty,
+    is_site_occupied, n_occupied_for_mask, occupy_site_for_mask,
+)
+from realspace_exatdiagonalization_py_amd.second_quantized_model import Particle_Statistics
+
+GOLD = json.loads((Path(__file__).parent / "golden" / "reference_values.json").read_text())
+NB = GOLD["design_notebook"]
+
+
+# --- bit primitives and combinadics (test_core.py:25-48; design.ipynb cells 5, 12, 14) ------------
+
+def test_bit_primitives():
+    assert bitmask_of_site(1) == 1 and bitmask_of_site(3) == 4
+    assert is_site_occupied(0b101, 1) and not is_site_occupied(0b101, 2) and is_site_empty(0b101, 2)
+    assert occupy_site_for_mask(0b101, 2) == 0b111 and empty_site_for_mask(0b111, 2) == 0b101
+    assert n_occupied_for_mask(0b10110) == 3
+    m = encode_configuration_to_bit_mask([2, 3, 5])
+    assert m == 0b10110 and decode_bit_mask_to_configuration(m, 5) == [2, 3, 5]
+    occ = [0, 0, 0]
+    assert decode_bit_mask_to_configuration_inplace(occ, m, 5) == [2, 3, 5]
+    assert list(filled_site_iter_for_mask(m, 5)) == [2, 3, 5]
+
+
+def test_gosper_and_colex_rank_kats():
+    x, seen = 0b111, []
+    while x < (1 << 6):
+        seen.append(x)
+        x = O.gosper_next(x)
+    assert seen == NB["gosper_3_of_6"]
+    bt = O.binom_table(4, 2)
+    for m, r in NB["colex_rank_4_2"]:
+        assert O.rank_colex(m, bt) == r
+    # ranks of the Gosper order are exactly 0..C-1 (bijection) and unrank inverts rank
+    bt63 = O.binom_table(6, 3)
+    assert [O.rank_colex(m, bt63) for m in seen] == list(range(20))
+    assert O.unrank_chunk(np.arange(20), 6, 3, bt63).tolist() == seen
+
+
+def test_lin_table_kat_and_equivalence():
+    left, right, nr = O.lin_table(12, 3)
+    kat = NB["lin_table_12_3"]
+    assert (left.size, right.size, nr) == (kat["left"], kat["right"], kat["n_right"])
+    bt = O.binom_table(12, 3)
+    masks = O.unrank_chunk(np.arange(math.comb(12, 3)), 12, 3, bt)
+    lin = O.rank_vec(masks, 12, bt, (left, right, nr))
+    walk = O.rank_vec(masks, 12, bt)
+    assert np.array_equal(lin, walk) and np.array_equal(walk, np.arange(220)) and kat["mismatches"] == 0
+    # the package's vectorised LinTable builder equals the oracle's
+    from realspace_exatdiagonalization_py_amd.symmetry_resolved_ed import _lin_table
+
+    for n, k in ((12, 3), (16, 8), (9, 4)):
+        a, b = O.lin_table(n, k), _lin_table(n, k)
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]) and a[2] == b[2]
+
+
+# --- group action and groups (test_core.py:51-100; design.ipynb cells 7, 18, 27) -----------------
+
+def test_group_action_kats():
+    op = Gm.Symmetry_Operation("t", [2, 3, 1])
+    m, ph = Gm.apply_operation_to_mask(0b101, op, Particle_Statistics.BOSONIC)
+    assert m == 0b011 and abs(ph - 1) == 0
+    _, ph = Gm.apply_operation_to_mask(0b001, Gm.Symmetry_Operation("id", [1, 2, 3]), Particle_Statistics.FERMIONIC)
+    assert ph == 1
+    swap = Gm.Symmetry_Operation("swap", [2, 1, 3])
+    assert Gm.apply_operation_to_mask(0b011, swap, Particle_Statistics.FERMIONIC)[1] == -1
+    assert Gm.apply_operation_to_mask(0b111, swap, Particle_Statistics.BOSONIC) == (0b111, 1)
+    assert Gm.apply_operation_to_mask(0b111, swap, Particle_Statistics.FERMIONIC) == (0b111, -1)
+    # oracle restatement agrees
+    assert O.apply_op(0b011, [2, 1, 3], [1, 1, 1], True)[1] == -1
+    G = Gm.Finite_Symmetry_Group("cyclic", [Gm.Symmetry_Operation("id", [1, 2, 3, 4]),
+                                            Gm.Symmetry_Operation("T", [4, 1, 2, 3])])
+    rep, g, _ = Gm.get_canonical_representative(0b1000, G, Particle_Statistics.BOSONIC)
+    assert (rep, g) == (0b0100, 1)
+    perm, ph = Gm.group_arrays(G)
+    assert O.canonical_rep_scalar(0b1000, perm, ph, False)[:2] == (0b0100, 1)
+    with pytest.raises(ValueError):
+        Gm.Finite_Symmetry_Group("empty", [])
+    with pytest.raises(ValueError):
+        Gm.Finite_Symmetry_Group("bad", [Gm.Symmetry_Operation("T", [2, 1])])
+
+
+def test_translation_group_and_characters():
+    m = M.build_zero_flux_bosonic_fci_second_quantized_model(sample_size=[2, 3])
+    G = Gm.build_translation_group(m.lattice)
+    assert len(G.operations) == 6 and G.operations[0].label == (0, 0)
+    irreps = Gm.build_irrep_list(G, m.lattice)
+    assert len(irreps) == 6 and irreps[0].label == (0, 0)
+    w = np.exp(2j * np.pi / 3)
+    np.testing.assert_allclose(irreps[1].values, [1, w, w * w, 1, w, w * w], atol=1e-15)  # (0,1), cell 7
+    np.testing.assert_allclose(irreps[3].values, [1, 1, 1, -1, -1, -1], atol=1e-15)      # (1,0)
+    lat = La.initialize_real_space_lattice(sample_size=[2, 2], brav_vec_list=[[1.0, 0], [0.5, math.sqrt(3) / 2]],
+                                           sub_crys_list=[[0, 0], [1 / 3, 1 / 3]])
+    Gt = Gm.build_translation_group(lat, [0.25, 0.0])
+    ref = np.array([complex(*p) for p in NB["haldane_2x2_flux_phases_op2_theta_0.25"]])
+    assert Gt.operations[2].label == (1, 0)
+    np.testing.assert_allclose(Gt.operations[2].perm_phases, ref, atol=6e-4)  # notebook prints 3 decimals
+
+
+def test_model_term_counts_and_hermiticity():
+    m = M.build_zero_flux_bosonic_fci_second_quantized_model(sample_size=[2, 3])
+    assert m.lattice.n_site == 12 and len(m.bilinear_terms) == 144
+    summed: dict = {}
+    for i, j, t in m.bilinear_terms:
+        summed[(i, j)] = summed.get((i, j), 0j) + t
+    for (i, j), t in summed.items():
+        assert abs(t - np.conj(summed.get((j, i), 0j))) < 1e-12
+    cb = M.build_zero_flux_fermionic_fci_second_quantized_model(sample_size=[3, 4])
+    assert cb.lattice.n_site == 24 and cb.particle_statistics is Particle_Statistics.FERMIONIC
+    assert len(cb.density_density_terms) == 96
+    h = M.build_heisenberg_second_quantized_model(8)
+    assert h.lattice.n_site == 8 and h.particle_statistics is Particle_Statistics.BOSONIC
+    hb = M.build_spinful_hubbard_second_quantized_model([2, 2])
+    assert hb.lattice.n_site == 8 and len(hb.density_density_terms) == 4
+    assert len(M.build_heisenberg_second_quantized_model(16).bilinear_terms) == 32
+    assert len(M.build_spinful_hubbard_second_quantized_model([3, 4], 1.0, 4.0).bilinear_terms) == 96
+
+
+# --- catalog and block of the design notebook (cells 16, 22, 24) ----------------------------------
+
+def test_haldane_2x3_catalog_and_block():
+    model, ne, G = make_case("hal23")
+    perm, ph = Gm.group_arrays(G)
+    cat = O.build_catalog(12, 3, perm, ph, False)
+    kat = NB["haldane_2x3_catalog"]
+    assert len(cat.reps) == kat["n_orbits"] and len(cat.rep_rank) == kat["n_total"]
+    assert int((cat.rep_rank == -1).sum()) == kat["unset"] and int(cat.reps[3]) == kat["rep3"]
+    idx = O.rank_colex(int(cat.reps[3]), cat.bt)
+    assert (int(cat.reps[cat.rep_rank[idx]]), int(cat.rep_sym[idx])) == tuple(kat["canonical"])
+    assert cat.rep_amp[idx] == 1
+    irrep = Gm.build_irrep_list(G, model.lattice)[0]
+    reps, stab, o2b = O.sector_basis(cat, irrep.values)
+    r, c, v, hd = O.projection_table(cat, reps, stab, o2b, model.bilinear_terms, model.density_density_terms,
+                                     irrep.values, False)
+    blk = NB["haldane_2x3_k0_block"]
+    assert len(reps) == blk["dim"] and len(r) == NB["haldane_2x3_k0_table_entries"]
+    idxd = np.arange(len(reps))
+    import scipy.sparse as sp
+
+    Hm = sp.coo_matrix((np.concatenate([hd, v]), (np.concatenate([idxd, r]), np.concatenate([idxd, c]))),
+                       shape=(len(reps),) * 2).tocsr()
+    Hm.eliminate_zeros()
+    assert Hm.nnz == blk["nnz"]
+    w, _ = O.diagonalize(O.csr_hamiltonian(r, c, v, hd, len(reps)), 3)
+    np.testing.assert_allclose(w, blk["lowest3_8dp"], atol=5e-9)
+    assert abs(w[0] - GOLD["test_core_haldane_2x3_E0"]) < 5e-10
+
+
+def test_table_invariant_alpha_is_the_action():
+    """Every table entry satisfies apply(rep, g) == (mask, alpha): the largest g."""
+    for name in ("hub23", "hal23_twist", "cb23_twist"):
+        model, ne, G = make_case(name)
+        perm, ph = Gm.group_arrays(G)
+        ferm = is_fermionic(model)
+        cat = O.build_catalog(model.lattice.n_site, ne, perm, ph, ferm)
+        masks = O.unrank_chunk(np.arange(len(cat.rep_rank)), model.lattice.n_site, ne, cat.bt)
+        for r in range(0, len(masks), 7):
+            rep = int(cat.reps[cat.rep_rank[r]])
+            g = int(cat.rep_sym[r])
+            s, a = O.apply_op(rep, perm[g], ph[g], ferm)
+            assert s == int(masks[r])
+            assert abs(complex(cat.rep_amp[r]) - a) <= 2e-7
+            assert all(O.apply_op(rep, perm[h], ph[h], ferm)[0] != s for h in range(g + 1, len(perm)))
+
+
+# --- sizes: benchmark CSV and the character formula -----------------------------------------------
+
+def _csv_case(row):
+    label = row["label"]
+    if row["model"] == "Heisenberg":
+        N = int(label.split("=")[1])
+        m = M.build_heisenberg_second_quantized_model(N)
+        return m, N // 2
+    L = [int(x) for x in label.split("×")]
+    if row["model"] == "Haldane_Boson":
+        m = M.build_zero_flux_bosonic_fci_second_quantized_model(sample_size=L)
+        return m, L[0] * L[1] // 2
+    m = M.build_spinful_hubbard_second_quantized_model(L)
+    return m, L[0] * L[1]
+
+
+def test_csv_sizes_closed_form():
+    """Orbit counts (Burnside) and k=0 sector dims (character formula) of every CSV row."""
+    for row in GOLD["benchmark_csv_latest"]:
+        m, ne = _csv_case(row)
+        G = Gm.build_translation_group(m.lattice)
+        perm, _ = Gm.group_arrays(G)
+        irreps = Gm.build_irrep_list(G, m.lattice)
+        assert math.comb(m.lattice.n_site, ne) == int(row["full_dim"])
+        n_orb = O.sector_dims_closed_form(m.lattice.n_site, ne, perm, [np.ones(len(perm))], False)[0]
+        assert n_orb == int(row["n_orbits"]), row
+        d0 = O.sector_dims_closed_form(m.lattice.n_site, ne, perm, [irreps[0].values], is_fermionic(m))[0]
+        assert d0 == int(row["sector_dim"]), row
+
+
+@pytest.mark.parametrize("row_i", [0, 2, 12, 14, 16, 24, 26, 28, 30])
+def test_csv_energies_oracle(row_i):
+    """Sector-0 ground energies of the reference's benchmark CSV (n_site <= 20)."""
+    row = GOLD["benchmark_csv_latest"][row_i]
+    m, ne = _csv_case(row)
+    G = Gm.build_translation_group(m.lattice)
+    perm, ph = Gm.group_arrays(G)
+    cat = O.build_catalog(m.lattice.n_site, ne, perm, ph, is_fermionic(m))
+    assert len(cat.reps) == int(row["n_orbits"])
+    irrep = Gm.build_irrep_list(G, m.lattice)[0]
+    reps, stab, o2b = O.sector_basis(cat, irrep.values)
+    assert len(reps) == int(row["sector_dim"])
+    r, c, v, hd = O.projection_table(cat, reps, stab, o2b, m.bilinear_terms, m.density_density_terms,
+                                     irrep.values, is_fermionic(m))
+    w, _ = O.diagonalize(O.csr_hamiltonian(r, c, v, hd, len(reps)), 1)
+    assert abs(w[0] - float(row["energy"])) < 1e-9
+
+
+def test_energy_alignment_goldens_oracle():
+    """The 30 Julia energies of test_energy_alignment.py (ATOL 1e-9) through the oracle."""
+    gold = GOLD["energy_alignment"]
+    for name, key in (("hal34", "JULIA_REF_BOSON_34"), ("cb34", "JULIA_REF_FERMION_34")):
+        model, ne, G = make_case(name)
+        perm, ph = Gm.group_arrays(G)
+        ferm = is_fermionic(model)
+        cat = O.build_catalog(model.lattice.n_site, ne, perm, ph, ferm)
+        irreps = {ir.label: ir for ir in Gm.build_irrep_list(G, model.lattice)}
+        for lab, ref in gold[key].items():
+            irrep = irreps[tuple(int(x) for x in lab.split(","))]
+            reps, stab, o2b = O.sector_basis(cat, irrep.values)
+            r, c, v, hd = O.projection_table(cat, reps, stab, o2b, model.bilinear_terms,
+                                             model.density_density_terms, irrep.values, ferm)
+            w, _ = O.diagonalize(O.csr_hamiltonian(r, c, v, hd, len(reps)), 5)
+            np.testing.assert_allclose(w, ref, atol=gold["ATOL"], rtol=0)
+
+
+# --- physics invariants: the union of sector spectra is the full spectrum ------------------------
+
+def _union_vs_full(model, ne, G, projection=O.projection_table):
+    perm, ph = Gm.group_arrays(G)
+    ferm = is_fermionic(model)
+    N = model.lattice.n_site
+    full = np.sort(np.linalg.eigvalsh(O.fock_hamiltonian(N, ne, model.bilinear_terms,
+                                                         model.density_density_terms, ferm).toarray()))
+    cat = O.build_catalog(N, ne, perm, ph, ferm)
+    parts, nonherm = [], 0.0
+    for ir in Gm.build_irrep_list(G, model.lattice):
+        reps, stab, o2b = O.sector_basis(cat, ir.values)
+        if len(reps) == 0:
+            continue
+        r, c, v, hd = projection(cat, reps, stab, o2b, model.bilinear_terms, model.density_density_terms,
+                                 ir.values, ferm)
+        H = O.csr_hamiltonian(r, c, v, hd, len(reps)).toarray()
+        nonherm = max(nonherm, np.abs(H - H.conj().T).max())
+        parts.append(np.linalg.eigvals(H).real)
+    union = np.sort(np.concatenate(parts))
+    return (np.abs(union - full).max() if union.shape == full.shape else np.inf), nonherm
+
+
+@pytest.mark.parametrize("name,tol", [("hal23", 1e-12), ("hub23", 1e-12), ("hal23_twist", 1e-7),
+                                      ("cb23_twist", 1e-7), ("heis12", 1e-12)])
+def test_union_of_sectors_is_full_spectrum(name, tol):
+    model, ne, G = make_case(name)
+    err, nonherm = _union_vs_full(model, ne, G)
+    assert err <= tol
+    assert nonherm <= 1e-7  # Hermitian up to the complex64 rounding of the stored amplitudes
+
+
+def _projector_check(name, projection=O.projection_table, n_irreps=3):
+    """Max |spectrum(sector block) - spectrum(conj(O) on {psi : U_g psi = chi(g) psi})| over
+    the first irreps, with O the plain Fock-space operator and U_g built from
+    the group action alone (an independent construction of the k sector)."""
+    model, ne, G = make_case(name)
+    perm, ph = Gm.group_arrays(G)
+    ferm = is_fermionic(model)
+    N = model.lattice.n_site
+    Hf = O.fock_hamiltonian(N, ne, model.bilinear_terms, model.density_density_terms, ferm).toarray()
+    bt = O.binom_table(N, ne)
+    C = math.comb(N, ne)
+    masks = O.unrank_chunk(np.arange(C), N, ne, bt)
+    U = []
+    for g in range(len(perm)):
+        Ug = np.zeros((C, C), complex)
+        for c, m in enumerate(masks):
+            s_, a = O.apply_op(int(m), perm[g], ph[g], ferm)
+            Ug[O.rank_colex(s_, bt), c] = a
+        U.append(Ug)
+    cat = O.build_catalog(N, ne, perm, ph, ferm)
+    worst = 0.0
+    for ir in Gm.build_irrep_list(G, model.lattice)[:n_irreps]:
+        reps, stab, o2b = O.sector_basis(cat, ir.values)
+        r, c, v, hd = projection(cat, reps, stab, o2b, model.bilinear_terms, model.density_density_terms,
+                                 ir.values, ferm)
+        e_blk = np.sort(np.linalg.eigvals(O.csr_hamiltonian(r, c, v, hd, len(reps)).toarray()).real)
+        P = sum(np.conj(ir.values[g]) * U[g] for g in range(len(U))) / len(U)
+        w, V = np.linalg.eigh((P + P.conj().T) / 2)
+        Q = V[:, w > 0.5]
+        assert Q.shape[1] == len(reps)
+        e_proj = np.sort(np.linalg.eigvalsh(Q.conj().T @ Hf.conj() @ Q))
+        worst = max(worst, float(np.abs(e_blk - e_proj).max()))
+    return worst
+
+
+@pytest.mark.parametrize("name", ["hal23_twist", "cb23_twist", "hub23"])
+def test_sector_blocks_match_textbook_projector(name):
+    """Pins the reference's character convention (alpha * conj(chi(g)) with g: rep -> mask):
+    its k block is conj(O) on the chi_k eigenspace of U, within the complex64 amplitude rounding."""
+    assert _projector_check(name) <= 2e-7
+
+
+def test_planted_mistakes_are_caught():
+    """Mistakes planted in the oracle's element formula break the pins."""
+    def dropped_conj(cat, reps, stab, o2b, bil, den, chi, ferm):
+        return O.projection_table(cat, reps, stab, o2b, bil, den, np.conj(chi), ferm)
+
+    def no_jw(cat, reps, stab, o2b, bil, den, chi, ferm):
+        return O.projection_table(cat, reps, stab, o2b, bil, den, chi, False)
+
+    def inverted_sqrt(cat, reps, stab, o2b, bil, den, chi, ferm):
+        r, c, v, hd = O.projection_table(cat, reps, stab, o2b, bil, den, chi, ferm)
+        s = np.asarray(stab, float)
+        return r, c, v * (s[c] / s[r]), hd
+
+    def no_alpha(cat, reps, stab, o2b, bil, den, chi, ferm):
+        saved = cat.rep_amp
+        cat.rep_amp = np.ones_like(saved)
+        try:
+            return O.projection_table(cat, reps, stab, o2b, bil, den, chi, ferm)
+        finally:
+            cat.rep_amp = saved
+
+    assert _projector_check("hal23_twist", dropped_conj) > 1e-3
+    assert _projector_check("hal23_twist", no_alpha) > 1e-3
+    model, ne, G = make_case("hub23")
+    assert _union_vs_full(model, ne, G, no_jw)[0] > 1e-3
+    # sqrt(stab_row / stab_col) is a diagonal similarity: it leaves the spectrum alone
+    # but is what makes the block Hermitian, so the inverted factor breaks Hermiticity
+    model, ne, G = make_case("heis12")
+    err, nonherm = _union_vs_full(model, ne, G, inverted_sqrt)
+    assert err < 1e-9 and nonherm > 1e-3
+
+
+def test_oracle_phase_refresh():
+    """Zero flux: refresh equals a fresh catalog exactly; nonzero flux: within 1 ulp (c64)."""
+    model, ne, G = make_case("cb23_twist")
+    perm, pht = Gm.group_arrays(G)
+    _, ph0 = Gm.group_arrays(Gm.build_translation_group(model.lattice, [0.0, 0.0]))
+    fresh0 = O.build_catalog(12, ne, perm, ph0, True)
+    upd = O.build_catalog(12, ne, perm, pht, True)
+    O.update_orbit_stabilizer_phases(upd, ph0, True)
+    assert np.array_equal(upd.rep_amp, fresh0.rep_amp) and np.array_equal(upd.rep_sym, fresh0.rep_sym)
+    fresh_t = O.build_catalog(12, ne, perm, pht, True)
+    upd2 = O.build_catalog(12, ne, perm, ph0, True)
+    O.update_orbit_stabilizer_phases(upd2, pht, True)
+    d = np.abs(upd2.rep_amp.astype(complex) - fresh_t.rep_amp.astype(complex))
+    assert d.max() <= 2.5e-7
+
+
+def test_closed_form_dims_all_sectors():
+    for name in ("heis16", "hub34", "hal34"):
+        model, ne, G = make_case(name)
+        perm, ph = Gm.group_arrays(G)
+        irreps = Gm.build_irrep_list(G, model.lattice)
+        dims = O.sector_dims_closed_form(model.lattice.n_site, ne, perm, [ir.values for ir in irreps],
+                                         is_fermionic(model))
+        if name == "hub34":
+            assert min(dims) == 225264 and max(dims) == 225440 and dims[0] == 225432
+        if name == "heis16":
+            assert dims[0] == 810 and sum(dims) == math.comb(16, 8)
