@@ -142,6 +142,28 @@ int edx_hamiltonian_destroy(edx_hamiltonian* ham);
 int edx_eigs(edx_ctx* ctx, const edx_hamiltonian* ham, int32_t nev, int32_t block, double tol,
              uint64_t seed, int32_t precision, double* vals, double* vecs, double* info);
 
+/* Options of edx_eigs_ex (edx_eigs fills them from its arguments).
+ *   tol      relative residual target ||Hx - Ex|| / max(1,|E|) (ARPACK's tol)
+ *   eig_tol  > 0: a pair also counts as converged once the residual-squared-
+ *            over-gap bound of its eigenvalue cluster is below eig_tol (the
+ *            eigenvalue is then accurate to eig_tol; the vector to ~sqrt)
+ *   mixed_tol residual at which EDX_PREC_MIXED hands over to FP64
+ *   basis    Ritz basis size (0: automatic), block as in edx_eigs */
+typedef struct edx_eigs_options {
+  int32_t nev;
+  int32_t block;
+  int32_t basis;
+  int32_t precision;
+  int32_t max_restarts;
+  double tol;
+  double eig_tol;
+  double mixed_tol;
+  uint64_t seed;
+} edx_eigs_options;
+void edx_eigs_default_options(edx_eigs_options* opts);
+int edx_eigs_ex(edx_ctx* ctx, const edx_hamiltonian* ham, const edx_eigs_options* opts, double* vals,
+                double* vecs, double* info);
+
 /* ---- SpMV launch shape -------------------------------------------------------
  * Lanes of a wavefront that share one matrix row (1, 2, 4, 8 or 16).  No
  * reference counterpart (launch-shape tuning of the device product). */
@@ -154,6 +176,14 @@ int edx_hamiltonian_info(const edx_hamiltonian* ham, int64_t* dim, int64_t* n_en
  * from its upper (upper != 0, as LAPACK eigh(UPLO="U") in ref SRE:2281) or
  * lower triangle; w ascending; z (optional) eigenvectors as columns. */
 int edx_host_herm_eig(int32_t n, const double* a, int32_t upper, double* w, double* z);
+
+/* ---- live kernel timing -------------------------------------------------------
+ * No reference counterpart.  With timing enabled every SpMM (class 0), Gram
+ * (1) and block-update/combine (2) launch on the ctx stream is bracketed by
+ * HIP events; ms/launches/bytes (arrays of 4, NULL skips) receive the totals
+ * (bytes = algorithmic bytes of each launch).  enable: 1 on, 0 off, -1 keep;
+ * reset != 0 clears the totals first. */
+int edx_kernel_stats(edx_ctx* ctx, int32_t enable, int32_t reset, double* ms, int64_t* launches, double* bytes);
 
 /* ---- device helpers for benchmarks and tests ------------------------------ */
 int edx_device_alloc(edx_ctx* ctx, uint64_t bytes, void** out);

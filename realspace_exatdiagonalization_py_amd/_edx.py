@@ -69,12 +69,31 @@ SIGNATURES = {
     "edx_memcpy_h2d": (C.c_int, [P, P, P, u64]),
     "edx_memcpy_d2h": (C.c_int, [P, P, P, u64]),
     "edx_time_matvec": (C.c_int, [P, P, i32, i32, i32, P, P, C.POINTER(C.c_double)]),
+    "edx_kernel_stats": (C.c_int, [P, i32, i32, P, P, P]),
+    "edx_eigs_ex": (C.c_int, [P, P, P, P, P, P]),
+    "edx_eigs_default_options": (None, [P]),
 }
+
+
+class EigsOptions(C.Structure):
+    """Mirror of ``edx_eigs_options`` (include/edx.h)."""
+
+    _fields_ = [("nev", C.c_int32), ("block", C.c_int32), ("basis", C.c_int32), ("precision", C.c_int32),
+                ("max_restarts", C.c_int32), ("tol", C.c_double), ("eig_tol", C.c_double),
+                ("mixed_tol", C.c_double), ("seed", C.c_uint64)]
 
 for _name, (_res, _args) in SIGNATURES.items():
     _fn = getattr(lib, _name)
     _fn.restype = _res
     _fn.argtypes = _args
+
+
+def eigs_options(**kw) -> EigsOptions:
+    o = EigsOptions()
+    lib.edx_eigs_default_options(C.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
 
 
 def ptr(a: np.ndarray | None):
@@ -129,6 +148,50 @@ class Context:
         if h:
             lib.edx_ctx_destroy(h)
             self.handle = None
+
+
+_local = threading.local()
+
+
+class use_context:
+    """Routes the device calls made by this thread to ``ctx`` (its own HIP
+    stream), e.g. to solve several sectors concurrently from worker threads::
+
+        with use_context(Context(0)):
+            ed.ed_scan_at_irrep_matrixfree(label, data)
+    """
+
+    def __init__(self, ctx: "Context"):
+        self.ctx = ctx
+
+    def __enter__(self):
+        self.prev = getattr(_local, "ctx", None)
+        _local.ctx = self.ctx
+        return self.ctx
+
+    def __exit__(self, *exc):
+        _local.ctx = self.prev
+        return False
+
+
+def current_context(default: "Context | None" = None) -> "Context":
+    """The thread's context set by ``use_context``, else ``default``, else the process default."""
+    ctx = getattr(_local, "ctx", None)
+    if ctx is not None:
+        return ctx
+    return default if default is not None else Context.default()
+
+
+KERNEL_CLASSES = ("spmm", "gram", "gemm", "other")
+
+
+def kernel_stats(ctx: "Context", enable: int = -1, reset: bool = False) -> dict:
+    """Totals of the live kernel timer: {class: (ms, launches, bytes)}."""
+    ms = np.zeros(4)
+    n = np.zeros(4, dtype=np.int64)
+    b = np.zeros(4)
+    ctx.check(lib.edx_kernel_stats(ctx.handle, enable, 1 if reset else 0, ptr(ms), ptr(n), ptr(b)))
+    return {k: (float(ms[i]), int(n[i]), float(b[i])) for i, k in enumerate(KERNEL_CLASSES)}
 
 
 class DeviceArray:

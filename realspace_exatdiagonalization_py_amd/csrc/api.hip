@@ -94,6 +94,22 @@ int edx_hamiltonian_info(const edx_hamiltonian* ham, int64_t* dim, int64_t* n_en
   return EDX_OK;
 }
 
+int edx_kernel_stats(edx_ctx* ctx, int32_t enable, int32_t reset, double* ms, int64_t* launches, double* bytes) {
+  return edx::guarded(ctx, [&] {
+    EDX_REQUIRE(ctx, EDX_ERR_INVALID, "null argument");
+    edx::DeviceGuard dg(ctx->device);
+    EDX_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->timer.flush();
+    if (reset) ctx->timer.reset();
+    if (enable >= 0) ctx->timer.on = enable != 0;
+    for (int i = 0; i < edx::K_NCLASS; ++i) {
+      if (ms) ms[i] = ctx->timer.ms[i];
+      if (launches) launches[i] = ctx->timer.launches[i];
+      if (bytes) bytes[i] = ctx->timer.bytes[i];
+    }
+  });
+}
+
 int edx_device_alloc(edx_ctx* ctx, uint64_t bytes, void** out) {
   return edx::guarded(ctx, [&] {
     EDX_REQUIRE(ctx && out, EDX_ERR_INVALID, "null argument");

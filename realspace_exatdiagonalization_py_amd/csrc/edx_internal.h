@@ -67,11 +67,81 @@ struct DevBuf {
   size_t bytes() const { return n * sizeof(T); }
 };
 
+// Live per-class kernel timing with HIP events on the launch stream
+// (enabled on request; used by benchmarks to time kernels inside a solve).
+enum KernelClass { K_SPMM = 0, K_GRAM = 1, K_GEMM = 2, K_OTHER = 3, K_NCLASS = 4 };
+
+struct KernelTimer {
+  bool on = false;
+  struct Pending {
+    hipEvent_t a, b;
+    int cls;
+    double bytes;
+  };
+  std::vector<Pending> pending;
+  std::vector<hipEvent_t> pool;
+  double ms[K_NCLASS] = {0, 0, 0, 0};
+  int64_t launches[K_NCLASS] = {0, 0, 0, 0};
+  double bytes[K_NCLASS] = {0, 0, 0, 0};
+
+  hipEvent_t take() {
+    if (pool.empty()) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) != hipSuccess) return nullptr;
+      return e;
+    }
+    hipEvent_t e = pool.back();
+    pool.pop_back();
+    return e;
+  }
+  // Returns the start event (null when timing is off).
+  hipEvent_t start(hipStream_t st) {
+    if (!on) return nullptr;
+    hipEvent_t e = take();
+    if (e) (void)hipEventRecord(e, st);
+    return e;
+  }
+  void stop(hipStream_t st, hipEvent_t a, int cls, double nbytes) {
+    if (!on || !a) return;
+    hipEvent_t b = take();
+    if (!b) return;
+    (void)hipEventRecord(b, st);
+    pending.push_back(Pending{a, b, cls, nbytes});
+    if (pending.size() > 4096) flush();
+  }
+  void flush() {
+    for (auto& p : pending) {
+      (void)hipEventSynchronize(p.b);
+      float t = 0.f;
+      if (hipEventElapsedTime(&t, p.a, p.b) == hipSuccess) {
+        ms[p.cls] += t;
+        launches[p.cls] += 1;
+        bytes[p.cls] += p.bytes;
+      }
+      pool.push_back(p.a);
+      pool.push_back(p.b);
+    }
+    pending.clear();
+  }
+  void reset() {
+    flush();
+    for (int i = 0; i < K_NCLASS; ++i) ms[i] = 0, launches[i] = 0, bytes[i] = 0;
+  }
+  ~KernelTimer() {
+    for (auto& p : pending) {
+      (void)hipEventDestroy(p.a);
+      (void)hipEventDestroy(p.b);
+    }
+    for (auto e : pool) (void)hipEventDestroy(e);
+  }
+};
+
 struct Ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   std::string last_error;
   int autotune_group = 0;  // lanes per row in the SpMV (0 = pick by heuristics)
+  KernelTimer timer;
 };
 
 // Scoped device selection for API calls.

@@ -34,14 +34,13 @@ using edx::dense::cplx;
 using edx::dense::Mat;
 
 namespace edx {
-void spmm(const edx_hamiltonian& h, hipStream_t st, int precision, int nb, const void* X, void* Y, int64_t ldy);
+void spmm(const edx_hamiltonian& h, Ctx& ctx, int precision, int nb, const void* X, void* Y, int64_t ldy);
 void interleave(hipStream_t st, int precision, int64_t dim, int nb, const void* src, int64_t ld, void* dst);
 void dense_matrix(const edx_hamiltonian& h, hipStream_t st, double2* out);
 }  // namespace edx
 
 namespace {
 
-constexpr int kRS = 32;          // rows staged per Gram sub-chunk
 constexpr int kMaxVec = 64;      // basis vectors the Gram/update kernels accept (64*16 outputs)
 constexpr int kMaxW = 16;        // block width
 
@@ -53,67 +52,93 @@ template <> __device__ __forceinline__ float2 from_d2<float2>(double re, double 
   return make_float2((float)re, (float)im);
 }
 
-// partial[blk][i*nw + j] = sum over the block's rows of conj(V_i[r]) W_j[r]
-template <typename V>
+// partial[rb][i*nw + j] = sum over row block rb of conj(V_i[r]) W_j[r].
+// Grid: x = tile of TI basis vectors (fastest, so consecutive workgroups read
+// the same W rows and hit L2), y = row block.  Lanes take consecutive rows
+// (coalesced loads); each thread keeps a TI x NW complex tile in registers,
+// folded at the end by a butterfly within the wavefront and a fixed-order sum
+// over the four wavefronts.
+template <typename V, int TI, int NW>
 __global__ void __launch_bounds__(256)
-gram_partial(int64_t D, int nv, const V* __restrict__ Vb, int64_t ldv, int nw, const V* __restrict__ Wb,
-             int64_t ldw, int64_t rows_per_block, double2* __restrict__ partial) {
-  extern __shared__ double2 sh[];
-  const int RSP = kRS + 1;
-  double2* sv = sh;
-  double2* sw = sh + (size_t)nv * RSP;
-  const int nout = nv * nw;
-  double ar[4] = {0, 0, 0, 0}, ai[4] = {0, 0, 0, 0};
-  const int64_t rb = (int64_t)blockIdx.x * rows_per_block;
+gram_tiles(int64_t D, int nv, const V* __restrict__ Vb, int64_t ldv, int nw, const V* __restrict__ Wb,
+           int64_t ldw, int64_t rows_per_block, double2* __restrict__ partial) {
+  __shared__ double red[4][TI * NW * 2];
+  const int i0 = blockIdx.x * TI;
+  const int64_t rb = (int64_t)blockIdx.y * rows_per_block;
   const int64_t re = min(D, rb + rows_per_block);
-  const int nvec = nv + nw;
-  for (int64_t r0 = rb; r0 < re; r0 += kRS) {
-    for (int idx = threadIdx.x; idx < nvec * kRS; idx += 256) {
-      const int vec = idx / kRS, rr = idx % kRS;
-      const int64_t r = r0 + rr;
-      double2 x = make_double2(0.0, 0.0);
-      if (r < re) x = vec < nv ? to_d2(Vb[(int64_t)vec * ldv + r]) : to_d2(Wb[(int64_t)(vec - nv) * ldw + r]);
-      if (vec < nv) sv[vec * RSP + rr] = x;
-      else sw[(vec - nv) * RSP + rr] = x;
-    }
-    __syncthreads();
+  double ar[TI][NW], ai[TI][NW];
 #pragma unroll
-    for (int s = 0; s < 4; ++s) {
-      const int o = threadIdx.x + 256 * s;
-      if (o < nout) {
-        const double2* a = sv + (o / nw) * RSP;
-        const double2* b = sw + (o % nw) * RSP;
-        double xr = ar[s], xi = ai[s];
-        for (int rr = 0; rr < kRS; ++rr) {
-          const double2 u = a[rr], w = b[rr];
-          xr = fma(u.x, w.x, xr);
-          xr = fma(u.y, w.y, xr);
-          xi = fma(u.x, w.y, xi);
-          xi = fma(-u.y, w.x, xi);
+  for (int t = 0; t < TI; ++t)
+#pragma unroll
+    for (int j = 0; j < NW; ++j) ar[t][j] = ai[t][j] = 0.0;
+  for (int64_t r = rb + threadIdx.x; r < re; r += 256) {
+    double2 w[NW];
+#pragma unroll
+    for (int j = 0; j < NW; ++j) w[j] = j < nw ? to_d2(Wb[(int64_t)j * ldw + r]) : make_double2(0.0, 0.0);
+#pragma unroll
+    for (int t = 0; t < TI; ++t) {
+      if (i0 + t < nv) {
+        const double2 v = to_d2(Vb[(int64_t)(i0 + t) * ldv + r]);
+#pragma unroll
+        for (int j = 0; j < NW; ++j) {
+          ar[t][j] = fma(v.x, w[j].x, ar[t][j]);
+          ar[t][j] = fma(v.y, w[j].y, ar[t][j]);
+          ai[t][j] = fma(v.x, w[j].y, ai[t][j]);
+          ai[t][j] = fma(-v.y, w[j].x, ai[t][j]);
         }
-        ar[s] = xr;
-        ai[s] = xi;
       }
     }
-    __syncthreads();
   }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
-  for (int s = 0; s < 4; ++s) {
-    const int o = threadIdx.x + 256 * s;
-    if (o < nout) partial[(int64_t)blockIdx.x * nout + o] = make_double2(ar[s], ai[s]);
+  for (int t = 0; t < TI; ++t)
+#pragma unroll
+    for (int j = 0; j < NW; ++j) {
+      double xr = ar[t][j], xi = ai[t][j];
+#pragma unroll
+      for (int off = 32; off >= 1; off >>= 1) {
+        xr += __shfl_xor(xr, off, 64);
+        xi += __shfl_xor(xi, off, 64);
+      }
+      if (lane == 0) {
+        red[wave][(t * NW + j) * 2] = xr;
+        red[wave][(t * NW + j) * 2 + 1] = xi;
+      }
+    }
+  __syncthreads();
+  const int nout = nv * nw;
+  for (int o = threadIdx.x; o < TI * NW; o += 256) {
+    const int t = o / NW, j = o % NW;
+    if (i0 + t < nv && j < nw) {
+      double xr = 0.0, xi = 0.0;
+      for (int wv = 0; wv < 4; ++wv) {
+        xr += red[wv][o * 2];
+        xi += red[wv][o * 2 + 1];
+      }
+      partial[(int64_t)blockIdx.y * nout + (int64_t)(i0 + t) * nw + j] = make_double2(xr, xi);
+    }
   }
 }
 
-__global__ void gram_reduce(int nblk, int nout, const double2* partial, double2* out) {
-  const int o = blockIdx.x * 256 + threadIdx.x;
-  if (o >= nout) return;
+// out[o] = sum over blocks of partial[blk][o]; one wavefront per output,
+// lanes stride the blocks and the sum is finished by a fixed butterfly, so
+// the result does not depend on scheduling.
+__global__ void __launch_bounds__(256) gram_reduce(int nblk, int nout, const double2* partial, double2* out) {
+  const int o = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
   double xr = 0.0, xi = 0.0;
-  for (int b = 0; b < nblk; ++b) {
-    const double2 p = partial[(int64_t)b * nout + o];
-    xr += p.x;
-    xi += p.y;
+  if (o < nout)
+    for (int b = lane; b < nblk; b += 64) {
+      const double2 p = partial[(int64_t)b * nout + o];
+      xr += p.x;
+      xi += p.y;
+    }
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    xr += __shfl_xor(xr, off, 64);
+    xi += __shfl_xor(xi, off, 64);
   }
-  out[o] = make_double2(xr, xi);
+  if (o < nout && lane == 0) out[o] = make_double2(xr, xi);
 }
 
 // Out_k[r] = (accumulate ? Out_k[r] : 0) + sign * sum_i V_i[r] S[i][k], k < nk <= 16.
@@ -122,7 +147,7 @@ __global__ void gram_reduce(int nblk, int nout, const double2* partial, double2*
 template <typename V>
 __global__ void __launch_bounds__(256)
 small_gemm(int64_t D, int nv, const V* Vb, int64_t ldv, int nk, V* Ob, int64_t ldo, const double2* S, int ldS,
-           double sign, int accumulate) {
+           double sign, int accumulate, V* Xi) {
   const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (r >= D) return;
   double xr[kMaxW], xi[kMaxW];
@@ -151,7 +176,9 @@ small_gemm(int64_t D, int nv, const V* Vb, int64_t ldv, int nk, V* Ob, int64_t l
         orr += o.x;
         oi += o.y;
       }
-      Ob[(int64_t)k * ldo + r] = from_d2<V>(orr, oi);
+      const V o = from_d2<V>(orr, oi);
+      Ob[(int64_t)k * ldo + r] = o;
+      if (Xi) Xi[r * nk + k] = o;  // interleaved copy for the next SpMM
     }
   }
 }
@@ -194,10 +221,43 @@ struct Options {
   int nev = 1;
   int block = 1;
   int basis = 0;  // max Ritz basis size (0: automatic)
-  double tol = 1e-10;
+  double tol = 1e-10;     // relative residual ||H x - theta x|| / max(1, |theta|)
+  double eig_tol = 0.0;   // > 0: a pair is also done when its cluster's sum r^2 / gap <= eig_tol
   int max_restarts = 400;
+  int check_every = 4;    // Ritz check every this many block steps (and when the basis is full)
   uint64_t seed = 1;
 };
+
+// Convergence of the nev lowest Ritz pairs.  A pair passes on its relative
+// residual, or (eig_tol > 0) when the eigenvalue error bound of its cluster,
+// sum of squared residuals over Ritz values within 1e-4 relative of it divided
+// by the distance to the nearest Ritz value outside the cluster, is below
+// eig_tol (the residual-squared-over-gap bound for Hermitian matrices).
+bool converged(const std::vector<double>& th, const std::vector<double>& res, int nev, double tol, double eig_tol,
+               double* worst) {
+  bool ok = true;
+  double w = 0.0;
+  const int n = (int)th.size();
+  for (int i = 0; i < nev; ++i) {
+    const double sc = std::max(1.0, std::fabs(th[i]));
+    const double rel = res[i] / sc;
+    w = std::max(w, rel);
+    if (rel <= tol) continue;
+    if (!(eig_tol > 0.0)) {
+      ok = false;
+      continue;
+    }
+    double rc2 = 0.0, gap = INFINITY;
+    for (int j = 0; j < n; ++j) {
+      const double d = std::fabs(th[j] - th[i]);
+      if (d <= 1e-4 * sc) rc2 += res[j] * res[j];
+      else gap = std::min(gap, d);
+    }
+    if (!(rc2 / gap <= eig_tol)) ok = false;
+  }
+  *worst = w;
+  return ok;
+}
 
 struct Result {
   std::vector<double> vals;
@@ -208,6 +268,8 @@ struct Result {
 
 // Block Lanczos on one precision.  Vectors live in two banks (ping-pong for
 // the thick restart), each holding `cap` vectors of leading dimension ld.
+// Projection coefficients stay on the device for the update kernels; the host
+// receives them (pinned buffers) once per step to extend the projected matrix.
 class BlockLanczos {
  public:
   BlockLanczos(edx_ctx& ctx, const edx_hamiltonian& h, int prec, const Options& opt)
@@ -216,23 +278,35 @@ class BlockLanczos {
     b_ = opt.block;
     esz_ = prec == EDX_PREC_FP64 ? 16 : 8;
     ld_ = (D_ + 63) / 64 * 64;
-    int base = opt.basis > 0 ? opt.basis : std::max({2 * (opt.nev + b_), 4 * b_, 24});
+    int base = opt.basis > 0 ? opt.basis : std::max({4 * (opt.nev + b_), 6 * b_, 24});
     base = (base + b_ - 1) / b_ * b_;
-    nq_max_ = std::min(base, kMaxVec - b_);
+    nq_max_ = std::min(base, kMaxVec - 2 * b_);
     nq_max_ = std::max(nq_max_ / b_ * b_, opt.nev + b_);
     cap_ = nq_max_ + b_;
-    EDX_REQUIRE(cap_ <= kMaxVec, EDX_ERR_INVALID, "Krylov basis too large");
+    EDX_REQUIRE(cap_ + b_ <= kMaxVec, EDX_ERR_INVALID, "Krylov basis too large for nev/block");
     bank_[0].alloc((size_t)cap_ * ld_ * esz_);
     bank_[1].alloc((size_t)cap_ * ld_ * esz_);
     W_.alloc((size_t)b_ * ld_ * esz_);
     Wsave_.alloc((size_t)b_ * ld_ * esz_);
     X_.alloc((size_t)b_ * D_ * esz_);
-    n_blocks_ = (int)std::min<int64_t>(std::max<int64_t>((D_ + 255) / 256, 1), 512);
-    rows_per_block_ = ((D_ + n_blocks_ - 1) / n_blocks_ + kRS - 1) / kRS * kRS;
-    partial_.alloc((size_t)n_blocks_ * kMaxVec * kMaxW);
-    small_.alloc((size_t)kMaxVec * kMaxVec);
-    small_out_.alloc((size_t)kMaxVec * kMaxW);
+    // row blocks of >= 16 rows per thread, at most 256 of them
+    n_rowblocks_ = (int)std::min<int64_t>(std::max<int64_t>((D_ + 4095) / 4096, 1), 256);
+    rows_per_block_ = (D_ + n_rowblocks_ - 1) / n_rowblocks_;
+    partial_.alloc((size_t)n_rowblocks_ * kMaxVec * kMaxW);
+    c1_.alloc((size_t)kMaxVec * kMaxW);
+    c2_.alloc((size_t)kMaxVec * kMaxW);
+    sdev_.alloc((size_t)kSlots * kSlot);
+    EDX_HIP(hipHostMalloc((void**)&hc1_, sizeof(double2) * kMaxVec * kMaxW, hipHostMallocDefault));
+    EDX_HIP(hipHostMalloc((void**)&hc2_, sizeof(double2) * kMaxVec * kMaxW, hipHostMallocDefault));
+    EDX_HIP(hipHostMalloc((void**)&hs_, sizeof(double2) * kSlots * kSlot, hipHostMallocDefault));
   }
+  ~BlockLanczos() {
+    if (hc1_) (void)hipHostFree(hc1_);
+    if (hc2_) (void)hipHostFree(hc2_);
+    if (hs_) (void)hipHostFree(hs_);
+  }
+  BlockLanczos(const BlockLanczos&) = delete;
+  BlockLanczos& operator=(const BlockLanczos&) = delete;
 
   // Runs to convergence.  `start` (optional, device, same precision,
   // column-major with leading dimension start_ld) seeds the first block with
@@ -241,71 +315,77 @@ class BlockLanczos {
     Result res;
     cur_ = 0;
     int nq = 0;
-    // initial block
     random_block(vec(cur_, 0), b_, opt_.seed);
     if (start && n_start > 0) {
       const int ns = std::min(n_start, b_);
       EDX_HIP(hipMemcpy2DAsync(vec(cur_, 0), ld_ * esz_, start, start_ld * esz_, D_ * esz_, ns,
                                hipMemcpyDeviceToDevice, ctx_.stream));
     }
-    Mat Rdummy;
-    orthonormalize(vec(cur_, 0), 0, nullptr, Rdummy, true);
+    start_block();
     Mat M(cap_, cap_), B(b_, cap_);
     std::vector<double> theta;
     Mat Y;
-    for (int iter = 0;; ++iter) {
-      // W = H Vr
-      void* Vr = vec(cur_, nq);
-      edx::interleave(ctx_.stream, prec_, D_, b_, Vr, ld_, X_.p);
-      edx::spmm(h_, ctx_.stream, prec_, b_, X_.p, W_.p, ld_);
+    int since_check = 0;
+    for (;;) {
+      // W = H Vr   (X_ holds Vr interleaved)
+      edx::spmm(h_, ctx_, prec_, b_, X_.p, W_.p, ld_);
       res.matvecs += b_;
       const int nbasis = nq + b_;
-      Mat C1 = gram(vec(cur_, 0), nbasis, W_.p, b_);
-      gemm(vec(cur_, 0), nbasis, W_.p, b_, C1, -1.0, true);
-      Mat C2 = gram(vec(cur_, 0), nbasis, W_.p, b_);
-      gemm(vec(cur_, 0), nbasis, W_.p, b_, C2, -1.0, true);
+      // pass 1: C1 = Q^H W, W -= Q C1
+      gram_dev(vec(cur_, 0), nbasis, W_.p, b_, c1_.p);
+      update_dev(vec(cur_, 0), nbasis, W_.p, b_, c1_.p);
+      // pass 2 with the block's own Gram: [Q, W]^H W (W before this update)
+      gram_dev2(vec(cur_, 0), nbasis, W_.p, b_, c2_.p);
+      update_dev(vec(cur_, 0), nbasis, W_.p, b_, c2_.p);
+      fetch(c1_.p, hc1_, nbasis * b_);
+      fetch(c2_.p, hc2_, (nbasis + b_) * b_);
+      EDX_HIP(hipStreamSynchronize(ctx_.stream));
+      Mat C(nbasis, b_), G(b_, b_);
       for (int i = 0; i < nbasis; ++i)
-        for (int j = 0; j < b_; ++j) {
-          const cplx c = C1(i, j) + C2(i, j);
-          if (i < nq) {
-            M(i, nq + j) = c;
-            M(nq + j, i) = std::conj(c);
-          }
-        }
+        for (int j = 0; j < b_; ++j)
+          C(i, j) = cplx(hc1_[i * b_ + j].x + hc2_[i * b_ + j].x, hc1_[i * b_ + j].y + hc2_[i * b_ + j].y);
+      // Gram of the updated block: W^H W - C2^H C2 (Q orthonormal)
       for (int i = 0; i < b_; ++i)
         for (int j = 0; j < b_; ++j) {
-          const cplx cij = C1(nq + i, j) + C2(nq + i, j);
-          const cplx cji = C1(nq + j, i) + C2(nq + j, i);
-          M(nq + i, nq + j) = 0.5 * (cij + std::conj(cji));
+          cplx g(hc2_[(nbasis + i) * b_ + j].x, hc2_[(nbasis + i) * b_ + j].y);
+          for (int k = 0; k < nbasis; ++k)
+            g -= std::conj(cplx(hc2_[k * b_ + i].x, hc2_[k * b_ + i].y)) * cplx(hc2_[k * b_ + j].x, hc2_[k * b_ + j].y);
+          G(i, j) = g;
         }
+      for (int i = 0; i < nq; ++i)
+        for (int j = 0; j < b_; ++j) {
+          M(i, nq + j) = C(i, j);
+          M(nq + j, i) = std::conj(C(i, j));
+        }
+      for (int i = 0; i < b_; ++i)
+        for (int j = 0; j < b_; ++j) M(nq + i, nq + j) = 0.5 * (C(nq + i, j) + std::conj(C(nq + j, i)));
       nq += b_;
-      // next residual block
+      // next residual block: W = Vr_new R, Vr_new written at vec(nq) and into X_
       Mat R;
-      orthonormalize(W_.p, nq, vec(cur_, nq), R, false);
+      next_block(G, nq, R);
       B = Mat(b_, cap_);
       for (int i = 0; i < b_; ++i)
         for (int j = 0; j < b_; ++j) B(i, nq - b_ + j) = R(i, j);
-      // Rayleigh-Ritz
       if (nq < opt_.nev) continue;
+      const bool full = nq + b_ > nq_max_;
+      if (!full && ++since_check < opt_.check_every) continue;
+      since_check = 0;
       ritz(M, nq, theta, Y);
       const int nev = opt_.nev;
-      double worst = 0.0;
-      bool conv = true;
-      for (int c = 0; c < nev; ++c) {
+      std::vector<double> rn(nq);
+      for (int c = 0; c < nq; ++c) {
         double rn2 = 0.0;
         for (int i = 0; i < b_; ++i) {
-          cplx s(0.0, 0.0);
-          for (int j = 0; j < nq; ++j) s += B(i, j) * Y(j, c);
-          rn2 += std::norm(s);
+          cplx sum(0.0, 0.0);
+          for (int j = nq - b_; j < nq; ++j) sum += B(i, j) * Y(j, c);
+          rn2 += std::norm(sum);
         }
-        const double rel = std::sqrt(rn2) / std::max(1.0, std::fabs(theta[c]));
-        worst = std::max(worst, rel);
-        if (!(rel <= opt_.tol)) conv = false;
+        rn[c] = std::sqrt(rn2);
       }
+      double worst = 0.0;
+      const bool conv = converged(theta, rn, nev, opt_.tol, opt_.eig_tol, &worst);
       res.max_rel_res = worst;
-      const bool full = nq + b_ > nq_max_;
       if (conv) {
-        // Ritz vectors of the wanted pairs into the other bank, then back to slot 0
         Mat Yk(nq, nev);
         for (int i = 0; i < nq; ++i)
           for (int c = 0; c < nev; ++c) Yk(i, c) = Y(i, c);
@@ -333,9 +413,9 @@ class BlockLanczos {
         Mat Bn(b_, cap_);
         for (int i = 0; i < b_; ++i)
           for (int c = 0; c < k; ++c) {
-            cplx s(0.0, 0.0);
-            for (int j = 0; j < nq; ++j) s += B(i, j) * Y(j, c);
-            Bn(i, c) = s;
+            cplx sum(0.0, 0.0);
+            for (int j = nq - b_; j < nq; ++j) sum += B(i, j) * Y(j, c);
+            Bn(i, c) = sum;
           }
         M = Mat(cap_, cap_);
         for (int c = 0; c < k; ++c) M(c, c) = theta[c];
@@ -347,54 +427,87 @@ class BlockLanczos {
     }
   }
 
-  // Device address of basis vector i of the current bank.
+  // Device address of basis vector i of a bank.
   void* vec(int bank, int i) { return static_cast<char*>((void*)bank_[bank].p) + (size_t)i * ld_ * esz_; }
   int current() const { return cur_; }
   int64_t ld() const { return ld_; }
 
  private:
-  Mat gram(const void* V, int nv, const void* W, int nw) {
-    EDX_REQUIRE(nv <= kMaxVec && nw <= kMaxW, EDX_ERR_INTERNAL, "gram size");
-    const size_t smem = (size_t)(nv + nw) * (kRS + 1) * sizeof(double2);
-    if (prec_ == EDX_PREC_FP64)
-      hipLaunchKernelGGL(gram_partial<double2>, dim3(n_blocks_), dim3(256), smem, ctx_.stream, D_, nv,
-                         (const double2*)V, ld_, nw, (const double2*)W, ld_, rows_per_block_, partial_.p);
-    else
-      hipLaunchKernelGGL(gram_partial<float2>, dim3(n_blocks_), dim3(256), smem, ctx_.stream, D_, nv,
-                         (const float2*)V, ld_, nw, (const float2*)W, ld_, rows_per_block_, partial_.p);
-    EDX_HIP(hipGetLastError());
-    const int nout = nv * nw;
-    hipLaunchKernelGGL(gram_reduce, dim3(edx::grid_for(nout, 256)), dim3(256), 0, ctx_.stream, n_blocks_, nout,
-                       (const double2*)partial_.p, small_out_.p);
-    EDX_HIP(hipGetLastError());
-    std::vector<double2> h(nout);
-    EDX_HIP(hipMemcpyAsync(h.data(), small_out_.p, nout * sizeof(double2), hipMemcpyDeviceToHost, ctx_.stream));
-    EDX_HIP(hipStreamSynchronize(ctx_.stream));
-    Mat C(nv, nw);
-    for (int i = 0; i < nv; ++i)
-      for (int j = 0; j < nw; ++j) C(i, j) = cplx(h[i * nw + j].x, h[i * nw + j].y);
-    return C;
+  template <typename VT, int TI, int NW>
+  void launch_tiles(int nv, const void* V, const void* W, int nw) {
+    const dim3 grid((unsigned)((nv + TI - 1) / TI), (unsigned)n_rowblocks_);
+    hipLaunchKernelGGL((gram_tiles<VT, TI, NW>), grid, dim3(256), 0, ctx_.stream, D_, nv, (const VT*)V, ld_, nw,
+                       (const VT*)W, ld_, rows_per_block_, partial_.p);
   }
 
-  // Out[:, k] (+)= sign * V S  (S is nv x nk, nk <= 16 per launch)
-  void gemm(const void* V, int nv, void* Out, int nk, const Mat& S, double sign, bool accumulate) {
-    std::vector<double2> hs((size_t)nv * nk);
+  template <typename VT>
+  void launch_tiles_nw(int nv, const void* V, const void* W, int nw) {
+    if (nw <= 1) launch_tiles<VT, 8, 1>(nv, V, W, nw);
+    else if (nw <= 2) launch_tiles<VT, 8, 2>(nv, V, W, nw);
+    else if (nw <= 4) launch_tiles<VT, 4, 4>(nv, V, W, nw);
+    else if (nw <= 8) launch_tiles<VT, 4, 8>(nv, V, W, nw);
+    else launch_tiles<VT, 2, 16>(nv, V, W, nw);
+  }
+
+  void launch_gram(const void* V, int nv, const void* W, int nw, double2* out) {
+    EDX_REQUIRE(nv <= kMaxVec && nw <= kMaxW, EDX_ERR_INTERNAL, "gram size");
+    hipEvent_t t0 = ctx_.timer.start(ctx_.stream);
+    if (prec_ == EDX_PREC_FP64) launch_tiles_nw<double2>(nv, V, W, nw);
+    else launch_tiles_nw<float2>(nv, V, W, nw);
+    EDX_HIP(hipGetLastError());
+    const int nout = nv * nw;
+    hipLaunchKernelGGL(gram_reduce, dim3((nout + 3) / 4), dim3(256), 0, ctx_.stream, n_rowblocks_, nout,
+                       (const double2*)partial_.p, out);
+    EDX_HIP(hipGetLastError());
+    ctx_.timer.stop(ctx_.stream, t0, edx::K_GRAM, (double)(nv + nw) * D_ * esz_);
+  }
+
+  // out (device, nv x nw row-major) = V^H W
+  void gram_dev(const void* V, int nv, const void* W, int nw, double2* out) { launch_gram(V, nv, W, nw, out); }
+
+  // out = [V, W]^H W as (nv + nw) x nw.  V and W live in different buffers,
+  // so the two products are formed by two launches into one output.
+  void gram_dev2(const void* V, int nv, const void* W, int nw, double2* out) {
+    launch_gram(V, nv, W, nw, out);
+    launch_gram(W, nw, W, nw, out + (size_t)nv * nw);
+  }
+
+  void fetch(const double2* dev, double2* host, int n) {
+    EDX_HIP(hipMemcpyAsync(host, dev, (size_t)n * sizeof(double2), hipMemcpyDeviceToHost, ctx_.stream));
+  }
+
+  // W -= V C with C on the device (nv x nw, ld nw)
+  void update_dev(const void* V, int nv, void* W, int nw, const double2* C) {
+    launch_small_gemm(V, nv, W, nw, C, nw, -1.0, true, nullptr);
+  }
+
+  // Out (+)= sign * V S with S given on the host.  S goes through a ring of
+  // pinned staging slots; every step synchronises the stream at least once
+  // between two uses of a slot (at most two uploads happen between syncs).
+  void gemm_host(const void* V, int nv, void* Out, int nk, const Mat& S, double sign, bool accumulate, void* Xi) {
+    EDX_REQUIRE((size_t)nv * nk <= kSlot, EDX_ERR_INTERNAL, "staging slot too small");
+    double2* hs = hs_ + (size_t)slot_ * kSlot;
+    double2* ds = sdev_.p + (size_t)slot_ * kSlot;
+    slot_ = (slot_ + 1) % kSlots;
     for (int i = 0; i < nv; ++i)
       for (int k = 0; k < nk; ++k) hs[(size_t)i * nk + k] = make_double2(S(i, k).real(), S(i, k).imag());
-    EDX_HIP(hipMemcpyAsync(small_.p, hs.data(), hs.size() * sizeof(double2), hipMemcpyHostToDevice, ctx_.stream));
-    launch_small_gemm(V, nv, Out, nk, small_.p, nk, sign, accumulate);
-    EDX_HIP(hipStreamSynchronize(ctx_.stream));
+    EDX_HIP(hipMemcpyAsync(ds, hs, (size_t)nv * nk * sizeof(double2), hipMemcpyHostToDevice, ctx_.stream));
+    launch_small_gemm(V, nv, Out, nk, ds, nk, sign, accumulate, Xi);
   }
 
   void launch_small_gemm(const void* V, int nv, void* Out, int nk, const double2* S, int ldS, double sign,
-                         bool accumulate) {
+                         bool accumulate, void* Xi) {
+    hipEvent_t t0 = ctx_.timer.start(ctx_.stream);
     if (prec_ == EDX_PREC_FP64)
       hipLaunchKernelGGL(small_gemm<double2>, dim3(edx::grid_for(D_, 256)), dim3(256), 0, ctx_.stream, D_, nv,
-                         (const double2*)V, ld_, nk, (double2*)Out, ld_, S, ldS, sign, accumulate ? 1 : 0);
+                         (const double2*)V, ld_, nk, (double2*)Out, ld_, S, ldS, sign, accumulate ? 1 : 0,
+                         (double2*)Xi);
     else
       hipLaunchKernelGGL(small_gemm<float2>, dim3(edx::grid_for(D_, 256)), dim3(256), 0, ctx_.stream, D_, nv,
-                         (const float2*)V, ld_, nk, (float2*)Out, ld_, S, ldS, sign, accumulate ? 1 : 0);
+                         (const float2*)V, ld_, nk, (float2*)Out, ld_, S, ldS, sign, accumulate ? 1 : 0,
+                         (float2*)Xi);
     EDX_HIP(hipGetLastError());
+    ctx_.timer.stop(ctx_.stream, t0, edx::K_GEMM, (double)(nv + (accumulate ? 2 : 1) * nk) * D_ * esz_);
   }
 
   // Out[:, 0..nk) = V[:, 0..nv) Y (Out must not alias V), tiled by 16 columns.
@@ -406,7 +519,8 @@ class BlockLanczos {
     EDX_HIP(hipMemcpyAsync(S.p, hs.data(), hs.size() * sizeof(double2), hipMemcpyHostToDevice, ctx_.stream));
     for (int k0 = 0; k0 < nk; k0 += kMaxW) {
       const int nkt = std::min(kMaxW, nk - k0);
-      launch_small_gemm(V, nv, static_cast<char*>(Out) + (size_t)k0 * ld_ * esz_, nkt, S.p + k0, nk, 1.0, false);
+      launch_small_gemm(V, nv, static_cast<char*>(Out) + (size_t)k0 * ld_ * esz_, nkt, S.p + k0, nk, 1.0, false,
+                        nullptr);
     }
     EDX_HIP(hipStreamSynchronize(ctx_.stream));
   }
@@ -421,55 +535,68 @@ class BlockLanczos {
     EDX_HIP(hipGetLastError());
   }
 
-  // Orthonormalises the b vectors at W (already orthogonal to the first nq
-  // basis vectors of the current bank) into `dst` (may equal W), with
-  // W_in = dst R.  Cholesky QR twice; a rank-deficient block is replaced by
-  // random directions orthogonalised against the basis, and R is then the
-  // projection of the original block onto the new one.
-  void orthonormalize(void* W, int nq, void* dst, Mat& R, bool first) {
-    if (!dst) dst = W;
+  Mat gram_host(const void* V, int nv, const void* W, int nw) {
+    gram_dev(V, nv, W, nw, c1_.p);
+    fetch(c1_.p, hc1_, nv * nw);
+    EDX_HIP(hipStreamSynchronize(ctx_.stream));
+    Mat C(nv, nw);
+    for (int i = 0; i < nv; ++i)
+      for (int j = 0; j < nw; ++j) C(i, j) = cplx(hc1_[i * nw + j].x, hc1_[i * nw + j].y);
+    return C;
+  }
+
+  // Orthonormalises the first block in place (Cholesky QR twice) and writes
+  // its interleaved copy for the first SpMM.
+  void start_block() {
+    void* V0 = vec(cur_, 0);
+    for (int pass = 0; pass < 2; ++pass) {
+      Mat G = gram_host(V0, b_, V0, b_);
+      Mat R;
+      EDX_REQUIRE(edx::dense::cholesky_upper(G, R, 1e-24), EDX_ERR_INTERNAL, "start block is rank deficient");
+      gemm_host(V0, b_, V0, b_, edx::dense::upper_inverse(R), 1.0, false, pass == 1 ? X_.p : nullptr);
+    }
+  }
+
+  // From the orthogonalised W_ and its Gram G: Vr_new = W R^-1 at vec(nq) with
+  // W = Vr_new R (Cholesky QR, second pass on the result), and X_ = Vr_new
+  // interleaved.  A numerically rank-deficient W is replaced by fresh random
+  // directions orthogonal to the basis; R is then the projection of W on them.
+  void next_block(const Mat& G, int nq, Mat& R) {
+    void* dst = vec(cur_, nq);
     const double tiny = prec_ == EDX_PREC_FP64 ? 1e-24 : 1e-12;
-    Mat G1 = gram(W, b_, W, b_);
     Mat R1, R2;
-    bool ok = edx::dense::cholesky_upper(G1, R1, tiny);
-    if (ok) {
-      if (dst != W)
-        EDX_HIP(hipMemcpy2DAsync(dst, ld_ * esz_, W, ld_ * esz_, D_ * esz_, b_, hipMemcpyDeviceToDevice,
-                                 ctx_.stream));
-      gemm(dst, b_, dst, b_, edx::dense::upper_inverse(R1), 1.0, false);
-      Mat G2 = gram(dst, b_, dst, b_);
-      ok = edx::dense::cholesky_upper(G2, R2, 1e-8);
-      if (ok) {
-        gemm(dst, b_, dst, b_, edx::dense::upper_inverse(R2), 1.0, false);
+    if (edx::dense::cholesky_upper(G, R1, tiny)) {
+      gemm_host(W_.p, b_, dst, b_, edx::dense::upper_inverse(R1), 1.0, false, nullptr);
+      Mat G2 = gram_host(dst, b_, dst, b_);
+      if (edx::dense::cholesky_upper(G2, R2, 1e-8)) {
+        gemm_host(dst, b_, dst, b_, edx::dense::upper_inverse(R2), 1.0, false, X_.p);
         R = Mat(b_, b_);
         for (int i = 0; i < b_; ++i)
           for (int j = 0; j < b_; ++j) {
-            cplx s(0.0, 0.0);
-            for (int k = 0; k < b_; ++k) s += R2(i, k) * R1(k, j);
-            R(i, j) = s;
+            cplx sum(0.0, 0.0);
+            for (int k = 0; k < b_; ++k) sum += R2(i, k) * R1(k, j);
+            R(i, j) = sum;
           }
         return;
       }
     }
-    EDX_REQUIRE(!first || nq == 0, EDX_ERR_INTERNAL, "start block orthonormalisation failed");
-    // deflation: fresh random block orthogonal to the basis
-    EDX_HIP(hipMemcpy2DAsync(Wsave_.p, ld_ * esz_, W, ld_ * esz_, D_ * esz_, b_, hipMemcpyDeviceToDevice,
+    EDX_HIP(hipMemcpy2DAsync(Wsave_.p, ld_ * esz_, W_.p, ld_ * esz_, D_ * esz_, b_, hipMemcpyDeviceToDevice,
                              ctx_.stream));
     for (int attempt = 0; attempt < 3; ++attempt) {
       random_block(dst, b_, opt_.seed * 7919 + 104729 * (uint64_t)(++deflations_));
-      for (int pass = 0; pass < 2 && nq > 0; ++pass) {
-        Mat C = gram(vec(cur_, 0), nq, dst, b_);
-        gemm(vec(cur_, 0), nq, dst, b_, C, -1.0, true);
+      for (int pass = 0; pass < 2; ++pass) {
+        gram_dev(vec(cur_, 0), nq, dst, b_, c1_.p);
+        update_dev(vec(cur_, 0), nq, dst, b_, c1_.p);
       }
-      Mat G = gram(dst, b_, dst, b_);
-      Mat Ra;
-      if (!edx::dense::cholesky_upper(G, Ra, tiny)) continue;
-      gemm(dst, b_, dst, b_, edx::dense::upper_inverse(Ra), 1.0, false);
-      Mat G2 = gram(dst, b_, dst, b_);
-      Mat Rb;
-      if (!edx::dense::cholesky_upper(G2, Rb, 1e-8)) continue;
-      gemm(dst, b_, dst, b_, edx::dense::upper_inverse(Rb), 1.0, false);
-      R = gram(dst, b_, Wsave_.p, b_);
+      bool ok = true;
+      for (int pass = 0; pass < 2 && ok; ++pass) {
+        Mat Gd = gram_host(dst, b_, dst, b_);
+        Mat Rd;
+        ok = edx::dense::cholesky_upper(Gd, Rd, pass == 0 ? tiny : 1e-8);
+        if (ok) gemm_host(dst, b_, dst, b_, edx::dense::upper_inverse(Rd), 1.0, false, pass == 1 ? X_.p : nullptr);
+      }
+      if (!ok) continue;
+      R = gram_host(dst, b_, Wsave_.p, b_);
       return;
     }
     throw edx::Error{EDX_ERR_NOCONV, "could not extend the Krylov basis (sector dimension exhausted)"};
@@ -489,11 +616,17 @@ class BlockLanczos {
   int64_t D_ = 0, ld_ = 0;
   int b_ = 1, esz_ = 16, nq_max_ = 0, cap_ = 0, cur_ = 0, restarts_ = 0;
   uint64_t deflations_ = 0;
-  int n_blocks_ = 1;
+  int n_rowblocks_ = 1;
   int64_t rows_per_block_ = 0;
+  static constexpr int kSlots = 4;
+  static constexpr size_t kSlot = (size_t)kMaxW * kMaxW;
+  int slot_ = 0;
   DevBuf<char> bank_[2];
   DevBuf<char> W_, Wsave_, X_;
-  DevBuf<double2> partial_, small_, small_out_;
+  DevBuf<double2> partial_, c1_, c2_, sdev_;
+  double2* hc1_ = nullptr;
+  double2* hc2_ = nullptr;
+  double2* hs_ = nullptr;
 };
 
 // Dense path for small sectors: explicit block, eigen-decomposition of the
@@ -523,17 +656,60 @@ void dense_solve(edx_ctx& ctx, const edx_hamiltonian& h, int nev, double* vals, 
   }
 }
 
+void eigs_impl(edx_ctx* ctx, const edx_hamiltonian* ham, const edx_eigs_options& o, double* vals, double* vecs,
+               double* info);
+
 }  // namespace
 
 extern "C" {
 
 int edx_eigs(edx_ctx* ctx, const edx_hamiltonian* ham, int32_t nev, int32_t block, double tol, uint64_t seed,
              int32_t precision, double* vals, double* vecs, double* info) {
+  edx_eigs_options o;
+  edx_eigs_default_options(&o);
+  o.nev = nev;
+  o.block = block;
+  o.tol = tol;
+  o.seed = seed;
+  o.precision = precision;
+  return edx_eigs_ex(ctx, ham, &o, vals, vecs, info);
+}
+
+void edx_eigs_default_options(edx_eigs_options* o) {
+  if (!o) return;
+  o->nev = 1;
+  o->block = 0;
+  o->basis = 0;
+  o->precision = EDX_PREC_FP64;
+  o->max_restarts = 400;
+  o->tol = 1e-10;
+  o->eig_tol = 0.0;
+  o->mixed_tol = 1e-5;
+  o->seed = 12345;
+}
+
+int edx_eigs_ex(edx_ctx* ctx, const edx_hamiltonian* ham, const edx_eigs_options* o, double* vals, double* vecs,
+                double* info) {
   return edx::guarded(ctx, [&] {
-    EDX_REQUIRE(ctx && ham && vals, EDX_ERR_INVALID, "null argument");
+    EDX_REQUIRE(ctx && ham && o && vals, EDX_ERR_INVALID, "null argument");
+    eigs_impl(ctx, ham, *o, vals, vecs, info);
+  });
+}
+
+}  // extern "C"
+
+namespace {
+
+void eigs_impl(edx_ctx* ctx, const edx_hamiltonian* ham, const edx_eigs_options& o, double* vals, double* vecs,
+               double* info) {
+  {
+    const int nev = o.nev, block = o.block, precision = o.precision;
+    const double tol = o.tol;
+    const uint64_t seed = o.seed;
     EDX_REQUIRE(nev >= 1, EDX_ERR_INVALID, "nev must be >= 1");
     EDX_REQUIRE(precision == EDX_PREC_FP64 || precision == EDX_PREC_MIXED, EDX_ERR_INVALID, "bad precision");
     EDX_REQUIRE(tol > 0.0, EDX_ERR_INVALID, "tol must be positive");
+    EDX_REQUIRE(o.max_restarts >= 1, EDX_ERR_INVALID, "max_restarts must be >= 1");
     edx::DeviceGuard dg(ctx->device);
     const int64_t D = ham->dim;
     if (info) std::fill(info, info + 4, 0.0);
@@ -549,29 +725,39 @@ int edx_eigs(edx_ctx* ctx, const edx_hamiltonian* ham, int32_t nev, int32_t bloc
     Options opt;
     opt.nev = nev;
     opt.block = b;
+    opt.basis = o.basis;
     opt.tol = tol;
+    opt.eig_tol = o.eig_tol;
+    opt.max_restarts = o.max_restarts;
     opt.seed = seed;
     Result res;
     int64_t mv_total = 0;
     std::unique_ptr<BlockLanczos> fin;
     if (precision == EDX_PREC_MIXED) {
+      // complex64 sweeps resolve nev + 2 pairs (multiplets that straddle the
+      // nev-th eigenvalue come out whole), then an FP64 block Lanczos starts
+      // from all of those Ritz vectors and checks convergence every step
+      const int n32 = (int)std::min<int64_t>(std::min(nev + 2, kMaxW), D - 2);
       Options o32 = opt;
-      o32.tol = std::max(tol, 2e-5);
+      o32.nev = n32;
+      o32.tol = std::max(tol, o.mixed_tol > 0 ? o.mixed_tol : 1e-5);
+      o32.eig_tol = 0.0;
       BlockLanczos lz32(*ctx, *ham, EDX_PREC_MIXED, o32);
       Result r32 = lz32.run(nullptr, 0, 0);
       mv_total += r32.matvecs;
-      // FP64 continuation from the complex64 Ritz vectors
       Options o64 = opt;
-      o64.block = std::min(kMaxW, std::max(b, nev));
-      o64.basis = 3 * o64.block;
+      o64.block = std::min(kMaxW, std::max(b, n32));
+      o64.basis = 0;
+      o64.check_every = 1;
       fin.reset(new BlockLanczos(*ctx, *ham, EDX_PREC_FP64, o64));
-      DevBuf<double2> start((size_t)nev * lz32.ld());
+      DevBuf<double2> start((size_t)n32 * lz32.ld());
       hipLaunchKernelGGL((convert_vec<float2, double2>), dim3(edx::grid_for(D, 256)), dim3(256), 0, ctx->stream, D,
-                         nev, (const float2*)lz32.vec(lz32.current(), 0), lz32.ld(), start.p, lz32.ld());
+                         n32, (const float2*)lz32.vec(lz32.current(), 0), lz32.ld(), start.p, lz32.ld());
       EDX_HIP(hipGetLastError());
-      res = fin->run(start.p, nev, lz32.ld());
+      res = fin->run(start.p, n32, lz32.ld());
       res.restarts += r32.restarts;
     } else {
+      opt.check_every = 2;
       fin.reset(new BlockLanczos(*ctx, *ham, EDX_PREC_FP64, opt));
       res = fin->run(nullptr, 0, 0);
     }
@@ -587,7 +773,8 @@ int edx_eigs(edx_ctx* ctx, const edx_hamiltonian* ham, int32_t nev, int32_t bloc
       info[2] = res.max_rel_res;
       info[3] = (double)b;
     }
-  });
+  }
 }
 
-}  // extern "C"
+}  // namespace
+

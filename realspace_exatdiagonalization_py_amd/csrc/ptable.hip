@@ -318,11 +318,21 @@ void launch_spmm_nb(const edx_hamiltonian& h, hipStream_t st, int G, int nb, con
   EDX_HIP(hipGetLastError());
 }
 
+// Algorithmic bytes of one SpMM launch: per row the row pointer, diagonal,
+// the row's own x entries and the y writes; per stored entry the column
+// index, the element and one gather of nb x values.
+double spmm_bytes(const edx_hamiltonian& h, int precision, int nb) {
+  const double s = precision == EDX_PREC_FP64 ? 16.0 : 8.0;
+  return (double)h.dim * (8.0 + s + 2.0 * nb * s) + (double)h.n_entries * (4.0 + s + nb * s);
+}
+
 // Applies H to nb <= 16 interleaved vectors.  precision selects the
 // complex128 or complex64 element copy.
-void spmm(const edx_hamiltonian& h, hipStream_t st, int precision, int nb, const void* X, void* Y, int64_t ldy) {
+void spmm(const edx_hamiltonian& h, Ctx& ctx, int precision, int nb, const void* X, void* Y, int64_t ldy) {
   if (h.dim == 0) return;
   EDX_REQUIRE(nb >= 1 && nb <= 16, EDX_ERR_INVALID, "at most 16 vectors per SpMM");
+  hipStream_t st = ctx.stream;
+  hipEvent_t t0 = ctx.timer.start(st);
   const int G = h.spmv_group > 0 ? h.spmv_group : 4;
   if (precision == EDX_PREC_FP64) {
     auto v = (const double2*)h.val.p;
@@ -343,6 +353,7 @@ void spmm(const edx_hamiltonian& h, hipStream_t st, int precision, int nb, const
     else if (nb <= 8) launch_spmm_nb<float2, 8>(h, st, G, nb, v, d, x, y, ldy);
     else launch_spmm_nb<float2, 16>(h, st, G, nb, v, d, x, y, ldy);
   }
+  ctx.timer.stop(st, t0, K_SPMM, spmm_bytes(h, precision, nb));
 }
 
 // Column-major [nb][ld] -> interleaved [dim][nb].
@@ -636,7 +647,7 @@ int edx_matvec(edx_ctx* ctx, const edx_hamiltonian* ham, int32_t n_vec, const do
       DevBuf<double2> xc((size_t)nb * D), xi((size_t)nb * D), yc((size_t)nb * D);
       EDX_HIP(hipMemcpyAsync(xc.p, x + (size_t)v0 * D * 2, (size_t)nb * D * 16, hipMemcpyHostToDevice, st));
       edx::interleave(st, EDX_PREC_FP64, D, nb, xc.p, D, xi.p);
-      edx::spmm(*ham, st, EDX_PREC_FP64, nb, xi.p, yc.p, D);
+      edx::spmm(*ham, *ctx, EDX_PREC_FP64, nb, xi.p, yc.p, D);
       EDX_HIP(hipMemcpyAsync(y + (size_t)v0 * D * 2, yc.p, (size_t)nb * D * 16, hipMemcpyDeviceToHost, st));
       EDX_HIP(hipStreamSynchronize(st));
     }
@@ -649,7 +660,7 @@ int edx_matvec_device(edx_ctx* ctx, const edx_hamiltonian* ham, int32_t precisio
     EDX_REQUIRE(ctx && ham && x_dev && y_dev, EDX_ERR_INVALID, "null argument");
     EDX_REQUIRE(precision == EDX_PREC_FP64 || precision == EDX_PREC_MIXED, EDX_ERR_INVALID, "bad precision");
     edx::DeviceGuard dg(ctx->device);
-    edx::spmm(*ham, ctx->stream, precision, n_vec, x_dev, y_dev, ham->dim);
+    edx::spmm(*ham, *ctx, precision, n_vec, x_dev, y_dev, ham->dim);
     EDX_HIP(hipStreamSynchronize(ctx->stream));
   });
 }
@@ -663,7 +674,7 @@ int edx_time_matvec(edx_ctx* ctx, const edx_hamiltonian* ham, int32_t precision,
     EDX_HIP(hipEventCreate(&a));
     EDX_HIP(hipEventCreate(&b));
     EDX_HIP(hipEventRecord(a, ctx->stream));
-    for (int i = 0; i < n_rep; ++i) edx::spmm(*ham, ctx->stream, precision, n_vec, x_dev, y_dev, ham->dim);
+    for (int i = 0; i < n_rep; ++i) edx::spmm(*ham, *ctx, precision, n_vec, x_dev, y_dev, ham->dim);
     EDX_HIP(hipEventRecord(b, ctx->stream));
     EDX_HIP(hipEventSynchronize(b));
     float ms = 0.f;

@@ -259,7 +259,7 @@ def build_symmetry_orbit_catalog(
         raise ValueError(f"n_site = {n_site} exceeds the 64-bit kernel limit ({MAX_N_SITE_UINT64}); "
                          "use a smaller system.")
     assert 0 <= n_filled <= n_site
-    ctx = ctx or Context.default()
+    ctx = ctx or _edx.current_context()
     perm, phases, angles = _group_inputs(symmetry_group)
     fermionic = particle_statistics is Particle_Statistics.FERMIONIC
     n_g = group_order(symmetry_group)
@@ -286,7 +286,7 @@ def update_orbit_stabilizer_phases(
     if catalog.symmetry_group is new_group:
         return catalog
     perm, phases, angles = _group_inputs(new_group)
-    ctx = catalog._dev.ctx
+    ctx = _edx.current_context(catalog._dev.ctx)
     ctx.check(lib.edx_catalog_update_phases(ctx.handle, catalog._dev.handle, ptr(perm), ptr(phases), ptr(angles)))
     catalog.symmetry_group = new_group
     catalog._invalidate("rep_amp", "rep_rank", "rep_sym", "stab_phase", "reps", "stab_order", "stab_off", "stab_g")
@@ -360,7 +360,7 @@ class Symmetry_Sector_Basis:
 def build_symmetry_sector_basis(catalog: Symmetry_Orbit_Catalog, irrep: OneDim_Irrep, *,
                                 atol: float = 1e-12) -> Symmetry_Sector_Basis:
     """Irrep filter on the device (SRE:1515-1546)."""
-    ctx = catalog._dev.ctx
+    ctx = _edx.current_context(catalog._dev.ctx)
     chi = np.ascontiguousarray(np.asarray(irrep.values, dtype=np.complex128))
     if chi.shape[0] != group_order(catalog.symmetry_group):
         raise ValueError("irrep must have one character per group element")
@@ -409,7 +409,7 @@ class CanonicalMap:
         rep = np.empty_like(m)
         g = np.empty(m.shape[0], dtype=np.int32)
         a = np.empty(m.shape[0], dtype=np.complex128)
-        ctx = self.catalog._dev.ctx
+        ctx = _edx.current_context(self.catalog._dev.ctx)
         ctx.check(lib.edx_catalog_lookup(ctx.handle, self.catalog._dev.handle, m.shape[0], ptr(m), ptr(rep),
                                          ptr(g), ptr(a)))
         return rep, g, a
@@ -510,7 +510,7 @@ class _DeviceHamiltonian:
 def _build_device_hamiltonian(basis: Symmetry_Sector_Basis, bilinear_terms, density_terms,
                               cmap: CanonicalMap | None = None) -> _DeviceHamiltonian:
     cat = basis.catalog
-    ctx = cat._dev.ctx
+    ctx = _edx.current_context(cat._dev.ctx)
     fi, ti, t = _term_arrays(bilinear_terms)
     vi, vj, v = _term_arrays(density_terms)
     chi = np.ascontiguousarray(np.asarray(basis.irrep.values, dtype=np.complex128))
@@ -597,7 +597,9 @@ def build_ed_Hamiltonian_symmetry_block(basis, bilinear_terms, density_terms, cm
 # ---------------------------------------------------------------------------
 
 def _device_eigs(dh: _DeviceHamiltonian, nev: int, *, precision: int = _edx.PREC_FP64, tol: float = 1e-10,
-                 block: int = 0, seed: int = 12345, want_vectors: bool = True):
+                 eig_tol: float = 0.0, block: int = 0, seed: int = 12345, want_vectors: bool = True):
+    """Device eigensolve.  ``tol``: relative residual (ARPACK's criterion);
+    ``eig_tol > 0`` also accepts pairs whose eigenvalue error bound is below it."""
     n = dh.dim
     if n == 0:
         return np.zeros(0, dtype=np.float64), np.zeros((0, 0), dtype=np.complex128), np.zeros(4)
@@ -605,13 +607,13 @@ def _device_eigs(dh: _DeviceHamiltonian, nev: int, *, precision: int = _edx.PREC
     vals = np.zeros(k, dtype=np.float64)
     vecs = np.zeros((k, n), dtype=np.complex128) if want_vectors else None
     info = np.zeros(4, dtype=np.float64)
-    dh.ctx.check(lib.edx_eigs(dh.ctx.handle, dh.handle, k, block, tol, seed, precision, ptr(vals), ptr(vecs),
-                              ptr(info)))
+    opts = _edx.eigs_options(nev=k, block=block, precision=precision, tol=tol, eig_tol=eig_tol, seed=seed)
+    dh.ctx.check(lib.edx_eigs_ex(dh.ctx.handle, dh.handle, _edx.C.byref(opts), ptr(vals), ptr(vecs), ptr(info)))
     return vals, (vecs.T.copy() if want_vectors else None), info
 
 
 def _csr_to_device(H: sp.spmatrix, ctx: Context | None = None) -> _DeviceHamiltonian:
-    ctx = ctx or Context.default()
+    ctx = ctx or _edx.current_context()
     H = sp.csr_matrix(H, dtype=np.complex128)
     n = H.shape[0]
     diag = np.ascontiguousarray(H.diagonal(), dtype=np.complex128)
@@ -649,11 +651,17 @@ def diagonalize_block_arpack(H, *, nev: int = 5):
 
 
 def diagonalize_block_matrixfree(H_op: MatrixFreeHamiltonian, n: int, *, nev: int = 5,
-                                 precision: int = _edx.PREC_FP64):
-    """Lowest ``nev`` eigenpairs of the device operator (SRE:2248-2299)."""
+                                 precision: int = _edx.PREC_FP64, eig_tol: float = 0.0,
+                                 want_vectors: bool = True):
+    """Lowest ``nev`` eigenpairs of the device operator (SRE:2248-2299).
+
+    Defaults reproduce ARPACK's stopping rule (relative residual 1e-10);
+    ``precision=PREC_MIXED`` sweeps in complex64 and finishes in FP64;
+    ``eig_tol > 0`` stops once every eigenvalue is within ``eig_tol``."""
     if n == 0:
         return np.zeros(0, dtype=np.float64), np.zeros((0, 0), dtype=np.complex128)
-    vals, vecs, info = _device_eigs(H_op._dev, nev, precision=precision)
+    vals, vecs, info = _device_eigs(H_op._dev, nev, precision=precision, eig_tol=eig_tol,
+                                    want_vectors=want_vectors)
     H_op.last_info = info
     return vals, vecs
 

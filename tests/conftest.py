@@ -10,10 +10,13 @@ if str(ROOT / "tests") not in sys.path:
     sys.path.insert(0, str(ROOT / "tests"))
 
 _LIB = ROOT / "realspace_exatdiagonalization_py_amd" / "libedx.so"
-if not _LIB.exists():  # build once if nothing is built yet
-    from realspace_exatdiagonalization_py_amd.csrc.build import build
+if not _LIB.exists():  # build once if nothing is built yet (builder loaded by path)
+    import importlib.util
 
-    build()
+    _spec = importlib.util.spec_from_file_location("_edx_build", _LIB.parent / "csrc" / "build.py")
+    _mod = importlib.util.module_from_spec(_spec)
+    _spec.loader.exec_module(_mod)
+    _mod.build()
 
 
 def pytest_configure(config):
