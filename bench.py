@@ -48,34 +48,30 @@ def build_workload():
 
 
 def solve_sector(ed, data, idx, nev, precision, eig_tol):
+    """One sector solve as the reference's ed_scan_at_irrep_matrixfree does it:
+    irrep filter, projection table, eigenvalues and eigenvectors."""
     model = data.second_quantized_model
     cmap = ed.CanonicalMap(data.symmetry_group, model.particle_statistics, data.orbit_catalog)
     t0 = time.perf_counter()
     basis = ed.build_symmetry_sector_basis(data.orbit_catalog, data.irrep_list[idx])
     data.sector_dims[idx] = basis.dim
     H = ed.MatrixFreeHamiltonian(basis, model.bilinear_terms, model.density_density_terms, cmap)
-    w, _ = ed.diagonalize_block_matrixfree(H, H.n, nev=nev, precision=precision, eig_tol=eig_tol,
-                                           want_vectors=False)
+    w, v = ed.diagonalize_block_matrixfree(H, H.n, nev=nev, precision=precision, eig_tol=eig_tol,
+                                           want_vectors=True)
+    assert v.shape == (basis.dim, len(w))
     return time.perf_counter() - t0, np.asarray(w)
 
 
-def solve_all(ed, data, nev, precision, eig_tol, pool=None, ctxs=None):
-    """All sectors, sequentially or spread over worker threads (one context each)."""
+def solve_all(ed, data, nev, precision, eig_tol, workers=1):
+    """All sectors, ``workers`` at a time (one HIP stream each)."""
     n = len(data.irrep_list)
-    if pool is None:
-        out = [solve_sector(ed, data, i, nev, precision, eig_tol) for i in range(n)]
-    else:
-        from realspace_exatdiagonalization_py_amd import _edx
+    lat, vals = [0.0] * n, [None] * n
 
-        def job(i):
-            import threading
+    def one(i):
+        lat[i], vals[i] = solve_sector(ed, data, i, nev, precision, eig_tol)
 
-            ctx = ctxs[threading.get_ident()]
-            with _edx.use_context(ctx):
-                return solve_sector(ed, data, i, nev, precision, eig_tol)
-
-        out = list(pool.map(job, range(n)))
-    return [o[0] for o in out], [o[1] for o in out]
+    ed.run_concurrently(range(n), one, workers)
+    return lat, vals
 
 
 def _missing_copies(cpu, gpu, tol=1e-8):
@@ -163,43 +159,19 @@ def main() -> int:
     del data2
     log(f"catalog: {data.orbit_catalog.n_orbits} orbits, first {t_cat*1e3:.1f} ms, again {t_cat2*1e3:.1f} ms")
 
-    pool, ctxs, all_ctx = None, None, [ctx]
-    if args.workers > 1:
-        import concurrent.futures as cf
-        import threading
-
-        pool = cf.ThreadPoolExecutor(max_workers=args.workers)
-        ctxs = {}
-        lock = threading.Lock()
-
-        def init_ctx(_):
-            with lock:
-                if threading.get_ident() not in ctxs:
-                    ctxs[threading.get_ident()] = _edx.Context(ctx.device)
-            time.sleep(0.05)
-
-        list(pool.map(init_ctx, range(4 * args.workers)))
-        all_ctx = list(ctxs.values())
     for i in range(args.warmup):
-        solve_all(ed, data, args.nev, prec, args.eig_tol, pool, ctxs)
+        solve_all(ed, data, args.nev, prec, args.eig_tol, args.workers)
         log(f"warmup step {i + 1}/{args.warmup} done")
-    for c in all_ctx:
-        _edx.kernel_stats(c, enable=1, reset=True)
+    _edx.kernel_stats_all(enable=1, reset=True)
     lat_all, vals = [], None
     ctx.check(_edx.lib.edx_synchronize(ctx.handle))
     t0 = time.perf_counter()
     for i in range(args.steps):
-        lat, vals = solve_all(ed, data, args.nev, prec, args.eig_tol, pool, ctxs)
+        lat, vals = solve_all(ed, data, args.nev, prec, args.eig_tol, args.workers)
         lat_all.extend(lat)
-        log(f"step {i + 1}/{args.steps}: {sum(lat)*1e3:.1f} ms")
-    for c in all_ctx:
-        c.check(_edx.lib.edx_synchronize(c.handle))
+        log(f"step {i + 1}/{args.steps}: {sum(lat)*1e3:.1f} ms of sector latency")
     elapsed = time.perf_counter() - t0
-    ks = {k: [0.0, 0, 0.0] for k in _edx.KERNEL_CLASSES}
-    for c in all_ctx:
-        for k, v in _edx.kernel_stats(c, enable=0).items():
-            for m in range(3):
-                ks[k][m] += v[m]
+    ks = _edx.kernel_stats_all(enable=0)
     n_solves = len(data.irrep_list) * args.steps
 
     # accuracy and the plain path: the same sectors in FP64 with ARPACK's residual rule

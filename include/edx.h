@@ -165,8 +165,10 @@ int edx_eigs_ex(edx_ctx* ctx, const edx_hamiltonian* ham, const edx_eigs_options
                 double* vecs, double* info);
 
 /* ---- SpMV launch shape -------------------------------------------------------
- * Lanes of a wavefront that share one matrix row (1, 2, 4, 8 or 16).  No
- * reference counterpart (launch-shape tuning of the device product). */
+ * Lanes of a wavefront that share one matrix row in the sliced-ELL copy of the
+ * table (1, 2 or 4; slices hold 64 / lanes rows).  Setting it on a built
+ * Hamiltonian rebuilds that copy.  No reference counterpart (launch-shape
+ * tuning of the device product). */
 int edx_set_spmv_group(edx_ctx* ctx, int32_t lanes);                 /* default for new builds; 0 = auto */
 int edx_hamiltonian_set_spmv_group(edx_ctx* ctx, edx_hamiltonian* ham, int32_t lanes);
 int edx_hamiltonian_info(const edx_hamiltonian* ham, int64_t* dim, int64_t* n_entries, int32_t* spmv_group);

@@ -32,7 +32,7 @@ from .symmetry_resolved_ed import (
     ed_scan, ed_scan_at_irrep_matrix, ed_scan_at_irrep_matrixfree, ed_scan_checkpoint_filename, full_ed,
     get_canonical, get_canonical_representative, group_order, hamiltonian_linear_operator, hopping_phase_for_stats,
     load_checkpoint, populate_canonical_map, print_spectrum, project_to_sector, project_to_unnormalized_sector,
-    rank_combination, save_checkpoint, update_orbit_stabilizer_phases,
+    rank_combination, run_concurrently, save_checkpoint, update_orbit_stabilizer_phases,
 )
 from . import models
 from .models import (

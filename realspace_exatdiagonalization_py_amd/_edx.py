@@ -13,6 +13,7 @@ from __future__ import annotations
 import ctypes as C
 import os
 import threading
+import weakref
 from pathlib import Path
 
 import numpy as np
@@ -123,6 +124,8 @@ class Context:
 
     _default: "Context | None" = None
     _lock = threading.Lock()
+    _live: "weakref.WeakSet[Context]" = weakref.WeakSet()
+    _live_lock = threading.Lock()
 
     def __init__(self, device: int = 0):
         n = lib.edx_device_count()
@@ -132,6 +135,8 @@ class Context:
         raise_for(lib.edx_ctx_create(device, C.byref(h)))
         self.handle = h
         self.device = device
+        with Context._live_lock:
+            Context._live.add(self)
 
     @classmethod
     def default(cls) -> "Context":
@@ -192,6 +197,18 @@ def kernel_stats(ctx: "Context", enable: int = -1, reset: bool = False) -> dict:
     b = np.zeros(4)
     ctx.check(lib.edx_kernel_stats(ctx.handle, enable, 1 if reset else 0, ptr(ms), ptr(n), ptr(b)))
     return {k: (float(ms[i]), int(n[i]), float(b[i])) for i, k in enumerate(KERNEL_CLASSES)}
+
+
+def kernel_stats_all(enable: int = -1, reset: bool = False) -> dict:
+    """``kernel_stats`` summed over every live context (all streams of the process)."""
+    with Context._live_lock:
+        ctxs = list(Context._live)
+    tot = {k: [0.0, 0, 0.0] for k in KERNEL_CLASSES}
+    for c in ctxs:
+        for k, v in kernel_stats(c, enable, reset).items():
+            for m in range(3):
+                tot[k][m] += v[m]
+    return {k: tuple(v) for k, v in tot.items()}
 
 
 class DeviceArray:

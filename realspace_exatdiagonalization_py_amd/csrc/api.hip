@@ -71,8 +71,8 @@ int edx_synchronize(edx_ctx* ctx) {
 int edx_set_spmv_group(edx_ctx* ctx, int32_t lanes) {
   return edx::guarded(ctx, [&] {
     EDX_REQUIRE(ctx, EDX_ERR_INVALID, "null argument");
-    EDX_REQUIRE(lanes == 0 || lanes == 1 || lanes == 2 || lanes == 4 || lanes == 8 || lanes == 16, EDX_ERR_INVALID,
-                "lanes per row must be 0 (automatic) or a power of two up to 16");
+    EDX_REQUIRE(lanes == 0 || lanes == 1 || lanes == 2 || lanes == 4, EDX_ERR_INVALID,
+                "lanes per row must be 0 (automatic), 1, 2 or 4");
     ctx->autotune_group = lanes;
   });
 }
@@ -80,9 +80,9 @@ int edx_set_spmv_group(edx_ctx* ctx, int32_t lanes) {
 int edx_hamiltonian_set_spmv_group(edx_ctx* ctx, edx_hamiltonian* ham, int32_t lanes) {
   return edx::guarded(ctx, [&] {
     EDX_REQUIRE(ctx && ham, EDX_ERR_INVALID, "null argument");
-    EDX_REQUIRE(lanes == 1 || lanes == 2 || lanes == 4 || lanes == 8 || lanes == 16, EDX_ERR_INVALID,
-                "lanes per row must be a power of two up to 16");
-    ham->spmv_group = lanes;
+    EDX_REQUIRE(lanes == 1 || lanes == 2 || lanes == 4, EDX_ERR_INVALID, "lanes per row must be 1, 2 or 4");
+    edx::DeviceGuard dg(ctx->device);
+    if (lanes != ham->sell_g) edx::build_sell(*ham, *ctx, lanes);
   });
 }
 
@@ -90,7 +90,7 @@ int edx_hamiltonian_info(const edx_hamiltonian* ham, int64_t* dim, int64_t* n_en
   if (!ham) return EDX_ERR_INVALID;
   if (dim) *dim = ham->dim;
   if (n_entries) *n_entries = ham->n_entries;
-  if (spmv_group) *spmv_group = ham->spmv_group;
+  if (spmv_group) *spmv_group = ham->sell_g;
   return EDX_OK;
 }
 

@@ -52,20 +52,39 @@ template <> __device__ __forceinline__ float2 from_d2<float2>(double re, double 
   return make_float2((float)re, (float)im);
 }
 
-// partial[rb][i*nw + j] = sum over row block rb of conj(V_i[r]) W_j[r].
-// Grid: x = tile of TI basis vectors (fastest, so consecutive workgroups read
+// Column vectors taken from two arrays: vector i is A0 + i*ld0 for i < n0 and
+// A1 + (i - n0)*ld1 otherwise.  Lets one launch cover [Q, W] without a copy.
+template <typename V>
+struct VecPair {
+  const V* a0;
+  int64_t ld0;
+  int n0;
+  const V* a1;
+  int64_t ld1;
+  int n1;
+  __device__ __forceinline__ const V* vec(int i) const {
+    return i < n0 ? a0 + (int64_t)i * ld0 : a1 + (int64_t)(i - n0) * ld1;
+  }
+};
+
+// partial[rb][i*NW + j] = sum over row block rb of conj(A_i[r]) W_j[r].
+// Grid: x = tile of TI vectors of A (fastest, so consecutive workgroups read
 // the same W rows and hit L2), y = row block.  Lanes take consecutive rows
 // (coalesced loads); each thread keeps a TI x NW complex tile in registers,
 // folded at the end by a butterfly within the wavefront and a fixed-order sum
 // over the four wavefronts.
 template <typename V, int TI, int NW>
 __global__ void __launch_bounds__(256)
-gram_tiles(int64_t D, int nv, const V* __restrict__ Vb, int64_t ldv, int nw, const V* __restrict__ Wb,
-           int64_t ldw, int64_t rows_per_block, double2* __restrict__ partial) {
+gram_tiles(int64_t D, VecPair<V> A, const V* __restrict__ Wb, int64_t ldw, int64_t rows_per_block,
+           double2* __restrict__ partial) {
   __shared__ double red[4][TI * NW * 2];
+  const int nv = A.n0 + A.n1;
   const int i0 = blockIdx.x * TI;
   const int64_t rb = (int64_t)blockIdx.y * rows_per_block;
   const int64_t re = min(D, rb + rows_per_block);
+  const V* ap[TI];
+#pragma unroll
+  for (int t = 0; t < TI; ++t) ap[t] = i0 + t < nv ? A.vec(i0 + t) : nullptr;
   double ar[TI][NW], ai[TI][NW];
 #pragma unroll
   for (int t = 0; t < TI; ++t)
@@ -74,11 +93,11 @@ gram_tiles(int64_t D, int nv, const V* __restrict__ Vb, int64_t ldv, int nw, con
   for (int64_t r = rb + threadIdx.x; r < re; r += 256) {
     double2 w[NW];
 #pragma unroll
-    for (int j = 0; j < NW; ++j) w[j] = j < nw ? to_d2(Wb[(int64_t)j * ldw + r]) : make_double2(0.0, 0.0);
+    for (int j = 0; j < NW; ++j) w[j] = to_d2(Wb[(int64_t)j * ldw + r]);
 #pragma unroll
     for (int t = 0; t < TI; ++t) {
-      if (i0 + t < nv) {
-        const double2 v = to_d2(Vb[(int64_t)(i0 + t) * ldv + r]);
+      if (ap[t]) {
+        const double2 v = to_d2(ap[t][r]);
 #pragma unroll
         for (int j = 0; j < NW; ++j) {
           ar[t][j] = fma(v.x, w[j].x, ar[t][j]);
@@ -106,16 +125,16 @@ gram_tiles(int64_t D, int nv, const V* __restrict__ Vb, int64_t ldv, int nw, con
       }
     }
   __syncthreads();
-  const int nout = nv * nw;
+  const int nout = nv * NW;
   for (int o = threadIdx.x; o < TI * NW; o += 256) {
     const int t = o / NW, j = o % NW;
-    if (i0 + t < nv && j < nw) {
+    if (i0 + t < nv) {
       double xr = 0.0, xi = 0.0;
       for (int wv = 0; wv < 4; ++wv) {
         xr += red[wv][o * 2];
         xi += red[wv][o * 2 + 1];
       }
-      partial[(int64_t)blockIdx.y * nout + (int64_t)(i0 + t) * nw + j] = make_double2(xr, xi);
+      partial[(int64_t)blockIdx.y * nout + (int64_t)(i0 + t) * NW + j] = make_double2(xr, xi);
     }
   }
 }
@@ -141,20 +160,21 @@ __global__ void __launch_bounds__(256) gram_reduce(int nblk, int nout, const dou
   if (o < nout && lane == 0) out[o] = make_double2(xr, xi);
 }
 
-// Out_k[r] = (accumulate ? Out_k[r] : 0) + sign * sum_i V_i[r] S[i][k], k < nk <= 16.
-// Every input of row r is read before any output of row r is written, so Out
-// may alias V when nk <= 16.
+// Out_k[r] = sum_i A_i[r] S[i][k] for k < nk <= 16 (S row-major, leading
+// dimension ldS), FP64 accumulation.  Every input of row r is read before
+// any output of row r is written, so Out may be one of the A arrays (same
+// leading dimension).  Xi (optional) receives the interleaved copy [r][k].
 template <typename V>
 __global__ void __launch_bounds__(256)
-small_gemm(int64_t D, int nv, const V* Vb, int64_t ldv, int nk, V* Ob, int64_t ldo, const double2* S, int ldS,
-           double sign, int accumulate, V* Xi) {
+gemm_pair(int64_t D, VecPair<V> A, int nk, V* Ob, int64_t ldo, const double2* __restrict__ S, int ldS, V* Xi) {
   const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (r >= D) return;
+  const int nv = A.n0 + A.n1;
   double xr[kMaxW], xi[kMaxW];
 #pragma unroll
   for (int k = 0; k < kMaxW; ++k) xr[k] = xi[k] = 0.0;
   for (int i = 0; i < nv; ++i) {
-    const double2 v = to_d2(Vb[(int64_t)i * ldv + r]);
+    const double2 v = to_d2(A.vec(i)[r]);
     const double2* srow = S + (int64_t)i * ldS;
 #pragma unroll
     for (int k = 0; k < kMaxW; ++k) {
@@ -170,13 +190,7 @@ small_gemm(int64_t D, int nv, const V* Vb, int64_t ldv, int nk, V* Ob, int64_t l
 #pragma unroll
   for (int k = 0; k < kMaxW; ++k) {
     if (k < nk) {
-      double orr = sign * xr[k], oi = sign * xi[k];
-      if (accumulate) {
-        const double2 o = to_d2(Ob[(int64_t)k * ldo + r]);
-        orr += o.x;
-        oi += o.y;
-      }
-      const V o = from_d2<V>(orr, oi);
+      const V o = from_d2<V>(xr[k], xi[k]);
       Ob[(int64_t)k * ldo + r] = o;
       if (Xi) Xi[r * nk + k] = o;  // interleaved copy for the next SpMM
     }
@@ -268,8 +282,10 @@ struct Result {
 
 // Block Lanczos on one precision.  Vectors live in two banks (ping-pong for
 // the thick restart), each holding `cap` vectors of leading dimension ld.
-// Projection coefficients stay on the device for the update kernels; the host
-// receives them (pinned buffers) once per step to extend the projected matrix.
+// Every step makes one pass over the basis for the projections of H Vr (a
+// second one only when that pass cancels most of the block) and one for the
+// update that also forms the next block; the host receives the projections
+// (pinned buffers) to extend the projected matrix.
 class BlockLanczos {
  public:
   BlockLanczos(edx_ctx& ctx, const edx_hamiltonian& h, int prec, const Options& opt)
@@ -284,25 +300,25 @@ class BlockLanczos {
     nq_max_ = std::max(nq_max_ / b_ * b_, opt.nev + b_);
     cap_ = nq_max_ + b_;
     EDX_REQUIRE(cap_ + b_ <= kMaxVec, EDX_ERR_INVALID, "Krylov basis too large for nev/block");
+    // a single projection pass is kept while no column loses more than this
+    // factor (squared) of its norm to it: the orthogonality it leaves is
+    // about the unit roundoff times that factor
+    kappa2_ = prec == EDX_PREC_FP64 ? 1e4 : 1e2;
     bank_[0].alloc((size_t)cap_ * ld_ * esz_);
     bank_[1].alloc((size_t)cap_ * ld_ * esz_);
     W_.alloc((size_t)b_ * ld_ * esz_);
-    Wsave_.alloc((size_t)b_ * ld_ * esz_);
     X_.alloc((size_t)b_ * D_ * esz_);
     // row blocks of >= 16 rows per thread, at most 256 of them
     n_rowblocks_ = (int)std::min<int64_t>(std::max<int64_t>((D_ + 4095) / 4096, 1), 256);
     rows_per_block_ = (D_ + n_rowblocks_ - 1) / n_rowblocks_;
-    partial_.alloc((size_t)n_rowblocks_ * kMaxVec * kMaxW);
-    c1_.alloc((size_t)kMaxVec * kMaxW);
-    c2_.alloc((size_t)kMaxVec * kMaxW);
+    partial_.alloc((size_t)n_rowblocks_ * kMaxOut);
+    cdev_.alloc((size_t)kMaxOut);
     sdev_.alloc((size_t)kSlots * kSlot);
-    EDX_HIP(hipHostMalloc((void**)&hc1_, sizeof(double2) * kMaxVec * kMaxW, hipHostMallocDefault));
-    EDX_HIP(hipHostMalloc((void**)&hc2_, sizeof(double2) * kMaxVec * kMaxW, hipHostMallocDefault));
+    EDX_HIP(hipHostMalloc((void**)&hc_, sizeof(double2) * kMaxOut, hipHostMallocDefault));
     EDX_HIP(hipHostMalloc((void**)&hs_, sizeof(double2) * kSlots * kSlot, hipHostMallocDefault));
   }
   ~BlockLanczos() {
-    if (hc1_) (void)hipHostFree(hc1_);
-    if (hc2_) (void)hipHostFree(hc2_);
+    if (hc_) (void)hipHostFree(hc_);
     if (hs_) (void)hipHostFree(hs_);
   }
   BlockLanczos(const BlockLanczos&) = delete;
@@ -327,31 +343,13 @@ class BlockLanczos {
     Mat Y;
     int since_check = 0;
     for (;;) {
-      // W = H Vr   (X_ holds Vr interleaved)
+      // W = H Vr   (Vr at vec(nq), interleaved copy in X_)
       edx::spmm(h_, ctx_, prec_, b_, X_.p, W_.p, ld_);
       res.matvecs += b_;
       const int nbasis = nq + b_;
-      // pass 1: C1 = Q^H W, W -= Q C1
-      gram_dev(vec(cur_, 0), nbasis, W_.p, b_, c1_.p);
-      update_dev(vec(cur_, 0), nbasis, W_.p, b_, c1_.p);
-      // pass 2 with the block's own Gram: [Q, W]^H W (W before this update)
-      gram_dev2(vec(cur_, 0), nbasis, W_.p, b_, c2_.p);
-      update_dev(vec(cur_, 0), nbasis, W_.p, b_, c2_.p);
-      fetch(c1_.p, hc1_, nbasis * b_);
-      fetch(c2_.p, hc2_, (nbasis + b_) * b_);
-      EDX_HIP(hipStreamSynchronize(ctx_.stream));
-      Mat C(nbasis, b_), G(b_, b_);
-      for (int i = 0; i < nbasis; ++i)
-        for (int j = 0; j < b_; ++j)
-          C(i, j) = cplx(hc1_[i * b_ + j].x + hc2_[i * b_ + j].x, hc1_[i * b_ + j].y + hc2_[i * b_ + j].y);
-      // Gram of the updated block: W^H W - C2^H C2 (Q orthonormal)
-      for (int i = 0; i < b_; ++i)
-        for (int j = 0; j < b_; ++j) {
-          cplx g(hc2_[(nbasis + i) * b_ + j].x, hc2_[(nbasis + i) * b_ + j].y);
-          for (int k = 0; k < nbasis; ++k)
-            g -= std::conj(cplx(hc2_[k * b_ + i].x, hc2_[k * b_ + i].y)) * cplx(hc2_[k * b_ + j].x, hc2_[k * b_ + j].y);
-          G(i, j) = g;
-        }
+      void* Q = vec(cur_, 0);
+      Mat C, R;
+      extend(Q, nbasis, vec(cur_, nbasis), C, R);
       for (int i = 0; i < nq; ++i)
         for (int j = 0; j < b_; ++j) {
           M(i, nq + j) = C(i, j);
@@ -360,9 +358,6 @@ class BlockLanczos {
       for (int i = 0; i < b_; ++i)
         for (int j = 0; j < b_; ++j) M(nq + i, nq + j) = 0.5 * (C(nq + i, j) + std::conj(C(nq + j, i)));
       nq += b_;
-      // next residual block: W = Vr_new R, Vr_new written at vec(nq) and into X_
-      Mat R;
-      next_block(G, nq, R);
       B = Mat(b_, cap_);
       for (int i = 0; i < b_; ++i)
         for (int j = 0; j < b_; ++j) B(i, nq - b_ + j) = R(i, j);
@@ -433,81 +428,88 @@ class BlockLanczos {
   int64_t ld() const { return ld_; }
 
  private:
+  static constexpr int kMaxOut = (kMaxVec + kMaxW) * kMaxW;  // Gram outputs of [Q, W] against W
+  static constexpr int kSlots = 4;
+  static constexpr size_t kSlot = (size_t)kMaxOut;
+
   template <typename VT, int TI, int NW>
-  void launch_tiles(int nv, const void* V, const void* W, int nw) {
+  void launch_tiles(const void* A0, int n0, const void* A1, int n1, const void* W) {
+    const int nv = n0 + n1;
     const dim3 grid((unsigned)((nv + TI - 1) / TI), (unsigned)n_rowblocks_);
-    hipLaunchKernelGGL((gram_tiles<VT, TI, NW>), grid, dim3(256), 0, ctx_.stream, D_, nv, (const VT*)V, ld_, nw,
-                       (const VT*)W, ld_, rows_per_block_, partial_.p);
+    VecPair<VT> A{(const VT*)A0, ld_, n0, (const VT*)A1, ld_, n1};
+    hipLaunchKernelGGL((gram_tiles<VT, TI, NW>), grid, dim3(256), 0, ctx_.stream, D_, A, (const VT*)W, ld_,
+                       rows_per_block_, partial_.p);
   }
 
   template <typename VT>
-  void launch_tiles_nw(int nv, const void* V, const void* W, int nw) {
-    if (nw <= 1) launch_tiles<VT, 8, 1>(nv, V, W, nw);
-    else if (nw <= 2) launch_tiles<VT, 8, 2>(nv, V, W, nw);
-    else if (nw <= 4) launch_tiles<VT, 4, 4>(nv, V, W, nw);
-    else if (nw <= 8) launch_tiles<VT, 4, 8>(nv, V, W, nw);
-    else launch_tiles<VT, 2, 16>(nv, V, W, nw);
+  void launch_tiles_nw(const void* A0, int n0, const void* A1, int n1, const void* W, int nw) {
+    switch (nw) {
+#define EDX_GRAM_CASE(NWV, TIV) \
+  case NWV:                     \
+    launch_tiles<VT, TIV, NWV>(A0, n0, A1, n1, W); \
+    break;
+      EDX_GRAM_CASE(1, 8) EDX_GRAM_CASE(2, 8) EDX_GRAM_CASE(3, 8) EDX_GRAM_CASE(4, 8)
+      EDX_GRAM_CASE(5, 4) EDX_GRAM_CASE(6, 4) EDX_GRAM_CASE(7, 4) EDX_GRAM_CASE(8, 4)
+      EDX_GRAM_CASE(9, 2) EDX_GRAM_CASE(10, 2) EDX_GRAM_CASE(11, 2) EDX_GRAM_CASE(12, 2)
+      EDX_GRAM_CASE(13, 2) EDX_GRAM_CASE(14, 2) EDX_GRAM_CASE(15, 2) EDX_GRAM_CASE(16, 2)
+#undef EDX_GRAM_CASE
+      default:
+        throw edx::Error{EDX_ERR_INTERNAL, "Gram block width out of range"};
+    }
   }
 
-  void launch_gram(const void* V, int nv, const void* W, int nw, double2* out) {
-    EDX_REQUIRE(nv <= kMaxVec && nw <= kMaxW, EDX_ERR_INTERNAL, "gram size");
+  // Host copy of [A0 (n0 vectors), A1 (n1 vectors)]^H W as (n0 + n1) x nw.
+  Mat gram_host(const void* A0, int n0, const void* A1, int n1, const void* W, int nw) {
+    const int nv = n0 + n1;
+    EDX_REQUIRE(nv <= kMaxVec + kMaxW && nw <= kMaxW, EDX_ERR_INTERNAL, "gram size");
     hipEvent_t t0 = ctx_.timer.start(ctx_.stream);
-    if (prec_ == EDX_PREC_FP64) launch_tiles_nw<double2>(nv, V, W, nw);
-    else launch_tiles_nw<float2>(nv, V, W, nw);
+    if (prec_ == EDX_PREC_FP64) launch_tiles_nw<double2>(A0, n0, A1, n1, W, nw);
+    else launch_tiles_nw<float2>(A0, n0, A1, n1, W, nw);
     EDX_HIP(hipGetLastError());
     const int nout = nv * nw;
     hipLaunchKernelGGL(gram_reduce, dim3((nout + 3) / 4), dim3(256), 0, ctx_.stream, n_rowblocks_, nout,
-                       (const double2*)partial_.p, out);
+                       (const double2*)partial_.p, cdev_.p);
     EDX_HIP(hipGetLastError());
     ctx_.timer.stop(ctx_.stream, t0, edx::K_GRAM, (double)(nv + nw) * D_ * esz_);
+    EDX_HIP(hipMemcpyAsync(hc_, cdev_.p, (size_t)nout * sizeof(double2), hipMemcpyDeviceToHost, ctx_.stream));
+    EDX_HIP(hipStreamSynchronize(ctx_.stream));
+    Mat C(nv, nw);
+    for (int i = 0; i < nv; ++i)
+      for (int j = 0; j < nw; ++j) C(i, j) = cplx(hc_[i * nw + j].x, hc_[i * nw + j].y);
+    return C;
   }
 
-  // out (device, nv x nw row-major) = V^H W
-  void gram_dev(const void* V, int nv, const void* W, int nw, double2* out) { launch_gram(V, nv, W, nw, out); }
-
-  // out = [V, W]^H W as (nv + nw) x nw.  V and W live in different buffers,
-  // so the two products are formed by two launches into one output.
-  void gram_dev2(const void* V, int nv, const void* W, int nw, double2* out) {
-    launch_gram(V, nv, W, nw, out);
-    launch_gram(W, nw, W, nw, out + (size_t)nv * nw);
-  }
-
-  void fetch(const double2* dev, double2* host, int n) {
-    EDX_HIP(hipMemcpyAsync(host, dev, (size_t)n * sizeof(double2), hipMemcpyDeviceToHost, ctx_.stream));
-  }
-
-  // W -= V C with C on the device (nv x nw, ld nw)
-  void update_dev(const void* V, int nv, void* W, int nw, const double2* C) {
-    launch_small_gemm(V, nv, W, nw, C, nw, -1.0, true, nullptr);
-  }
-
-  // Out (+)= sign * V S with S given on the host.  S goes through a ring of
-  // pinned staging slots; every step synchronises the stream at least once
-  // between two uses of a slot (at most two uploads happen between syncs).
-  void gemm_host(const void* V, int nv, void* Out, int nk, const Mat& S, double sign, bool accumulate, void* Xi) {
-    EDX_REQUIRE((size_t)nv * nk <= kSlot, EDX_ERR_INTERNAL, "staging slot too small");
+  // Out[:, 0..nk) = [A0, A1] S with S ((n0 + n1) x nk) given on the host.  S
+  // goes through a ring of pinned staging slots; the stream is synchronised
+  // (by a Gram) between any four consecutive uploads, so a slot is never
+  // rewritten while its copy is pending.
+  void gemm_host(const void* A0, int n0, const void* A1, int n1, void* Out, int nk, const Mat& S, void* Xi) {
+    const int nv = n0 + n1;
+    EDX_REQUIRE((size_t)nv * nk <= kSlot && nk <= kMaxW, EDX_ERR_INTERNAL, "staging slot too small");
     double2* hs = hs_ + (size_t)slot_ * kSlot;
     double2* ds = sdev_.p + (size_t)slot_ * kSlot;
     slot_ = (slot_ + 1) % kSlots;
     for (int i = 0; i < nv; ++i)
       for (int k = 0; k < nk; ++k) hs[(size_t)i * nk + k] = make_double2(S(i, k).real(), S(i, k).imag());
     EDX_HIP(hipMemcpyAsync(ds, hs, (size_t)nv * nk * sizeof(double2), hipMemcpyHostToDevice, ctx_.stream));
-    launch_small_gemm(V, nv, Out, nk, ds, nk, sign, accumulate, Xi);
+    launch_gemm(A0, n0, A1, n1, Out, nk, ds, nk, Xi);
   }
 
-  void launch_small_gemm(const void* V, int nv, void* Out, int nk, const double2* S, int ldS, double sign,
-                         bool accumulate, void* Xi) {
+  void launch_gemm(const void* A0, int n0, const void* A1, int n1, void* Out, int nk, const double2* S, int ldS,
+                   void* Xi) {
     hipEvent_t t0 = ctx_.timer.start(ctx_.stream);
-    if (prec_ == EDX_PREC_FP64)
-      hipLaunchKernelGGL(small_gemm<double2>, dim3(edx::grid_for(D_, 256)), dim3(256), 0, ctx_.stream, D_, nv,
-                         (const double2*)V, ld_, nk, (double2*)Out, ld_, S, ldS, sign, accumulate ? 1 : 0,
-                         (double2*)Xi);
-    else
-      hipLaunchKernelGGL(small_gemm<float2>, dim3(edx::grid_for(D_, 256)), dim3(256), 0, ctx_.stream, D_, nv,
-                         (const float2*)V, ld_, nk, (float2*)Out, ld_, S, ldS, sign, accumulate ? 1 : 0,
-                         (float2*)Xi);
+    const dim3 grid(edx::grid_for(D_, 256));
+    if (prec_ == EDX_PREC_FP64) {
+      VecPair<double2> A{(const double2*)A0, ld_, n0, (const double2*)A1, ld_, n1};
+      hipLaunchKernelGGL(gemm_pair<double2>, grid, dim3(256), 0, ctx_.stream, D_, A, nk, (double2*)Out, ld_, S,
+                         ldS, (double2*)Xi);
+    } else {
+      VecPair<float2> A{(const float2*)A0, ld_, n0, (const float2*)A1, ld_, n1};
+      hipLaunchKernelGGL(gemm_pair<float2>, grid, dim3(256), 0, ctx_.stream, D_, A, nk, (float2*)Out, ld_, S,
+                         ldS, (float2*)Xi);
+    }
     EDX_HIP(hipGetLastError());
-    ctx_.timer.stop(ctx_.stream, t0, edx::K_GEMM, (double)(nv + (accumulate ? 2 : 1) * nk) * D_ * esz_);
+    ctx_.timer.stop(ctx_.stream, t0, edx::K_GEMM, (double)(n0 + n1 + (Xi ? 2 : 1) * nk) * D_ * esz_);
   }
 
   // Out[:, 0..nk) = V[:, 0..nv) Y (Out must not alias V), tiled by 16 columns.
@@ -519,8 +521,8 @@ class BlockLanczos {
     EDX_HIP(hipMemcpyAsync(S.p, hs.data(), hs.size() * sizeof(double2), hipMemcpyHostToDevice, ctx_.stream));
     for (int k0 = 0; k0 < nk; k0 += kMaxW) {
       const int nkt = std::min(kMaxW, nk - k0);
-      launch_small_gemm(V, nv, static_cast<char*>(Out) + (size_t)k0 * ld_ * esz_, nkt, S.p + k0, nk, 1.0, false,
-                        nullptr);
+      launch_gemm(V, nv, nullptr, 0, static_cast<char*>(Out) + (size_t)k0 * ld_ * esz_, nkt, S.p + k0, nk,
+                  nullptr);
     }
     EDX_HIP(hipStreamSynchronize(ctx_.stream));
   }
@@ -535,68 +537,125 @@ class BlockLanczos {
     EDX_HIP(hipGetLastError());
   }
 
-  Mat gram_host(const void* V, int nv, const void* W, int nw) {
-    gram_dev(V, nv, W, nw, c1_.p);
-    fetch(c1_.p, hc1_, nv * nw);
-    EDX_HIP(hipStreamSynchronize(ctx_.stream));
-    Mat C(nv, nw);
-    for (int i = 0; i < nv; ++i)
-      for (int j = 0; j < nw; ++j) C(i, j) = cplx(hc1_[i * nw + j].x, hc1_[i * nw + j].y);
+  // Orthonormalises the block at V in place (Cholesky QR twice) and writes
+  // its interleaved copy to X_.  False if it is numerically rank deficient.
+  bool cholqr2(void* V, double tiny, Mat* R) {
+    Mat Rt;
+    for (int pass = 0; pass < 2; ++pass) {
+      Mat G = gram_host(V, b_, nullptr, 0, V, b_);
+      Mat Rp;
+      if (!edx::dense::cholesky_upper(G, Rp, pass == 0 ? tiny : 1e-8)) return false;
+      gemm_host(nullptr, 0, V, b_, V, b_, edx::dense::upper_inverse(Rp), pass == 1 ? X_.p : nullptr);
+      Rt = pass == 0 ? Rp : mul(Rp, Rt);
+    }
+    if (R) *R = Rt;
+    return true;
+  }
+
+  void start_block() {
+    EDX_REQUIRE(cholqr2(vec(cur_, 0), 1e-24, nullptr), EDX_ERR_INTERNAL, "start block is rank deficient");
+  }
+
+  static Mat mul(const Mat& A, const Mat& B) {
+    Mat C(A.n, B.m);
+    for (int i = 0; i < A.n; ++i)
+      for (int k = 0; k < A.m; ++k) {
+        const cplx a = A(i, k);
+        for (int j = 0; j < B.m; ++j) C(i, j) += a * B(k, j);
+      }
     return C;
   }
 
-  // Orthonormalises the first block in place (Cholesky QR twice) and writes
-  // its interleaved copy for the first SpMM.
-  void start_block() {
-    void* V0 = vec(cur_, 0);
-    for (int pass = 0; pass < 2; ++pass) {
-      Mat G = gram_host(V0, b_, V0, b_);
-      Mat R;
-      EDX_REQUIRE(edx::dense::cholesky_upper(G, R, 1e-24), EDX_ERR_INTERNAL, "start block is rank deficient");
-      gemm_host(V0, b_, V0, b_, edx::dense::upper_inverse(R), 1.0, false, pass == 1 ? X_.p : nullptr);
-    }
+  // Gram of W - Q C from the Gram of [Q, W] against W (Q orthonormal):
+  // W^H W - C^H C.
+  Mat remainder_gram(const Mat& P, int nbasis) const {
+    Mat G(b_, b_);
+    for (int i = 0; i < b_; ++i)
+      for (int j = 0; j < b_; ++j) {
+        cplx g = P(nbasis + i, j);
+        for (int k = 0; k < nbasis; ++k) g -= std::conj(P(k, i)) * P(k, j);
+        G(i, j) = g;
+      }
+    return G;
   }
 
-  // From the orthogonalised W_ and its Gram G: Vr_new = W R^-1 at vec(nq) with
-  // W = Vr_new R (Cholesky QR, second pass on the result), and X_ = Vr_new
-  // interleaved.  A numerically rank-deficient W is replaced by fresh random
-  // directions orthogonal to the basis; R is then the projection of W on them.
-  void next_block(const Mat& G, int nq, Mat& R) {
-    void* dst = vec(cur_, nq);
+  // S = [-C R^-1; R^-1] so that [Q, W] S = (W - Q C) R^-1.
+  Mat update_coeffs(const Mat& C, int nbasis, const Mat* Rinv) const {
+    Mat S(nbasis + b_, b_);
+    for (int i = 0; i < nbasis; ++i)
+      for (int j = 0; j < b_; ++j) {
+        cplx sum(0.0, 0.0);
+        if (Rinv)
+          for (int k = 0; k < b_; ++k) sum += C(i, k) * (*Rinv)(k, j);
+        else
+          sum = C(i, j);
+        S(i, j) = -sum;
+      }
+    for (int i = 0; i < b_; ++i)
+      for (int j = 0; j < b_; ++j) S(nbasis + i, j) = Rinv ? (*Rinv)(i, j) : cplx(i == j ? 1.0 : 0.0, 0.0);
+    return S;
+  }
+
+  // Projects W_ = H Vr against the basis Q (nbasis vectors, Vr included):
+  // C = Q^H W (summed over passes), and writes the next block dst with
+  // W - Q C = dst R (dst orthonormal, orthogonal to Q) and its interleaved
+  // copy to X_.
+  void extend(void* Q, int nbasis, void* dst, Mat& C, Mat& R) {
     const double tiny = prec_ == EDX_PREC_FP64 ? 1e-24 : 1e-12;
-    Mat R1, R2;
-    if (edx::dense::cholesky_upper(G, R1, tiny)) {
-      gemm_host(W_.p, b_, dst, b_, edx::dense::upper_inverse(R1), 1.0, false, nullptr);
-      Mat G2 = gram_host(dst, b_, dst, b_);
-      if (edx::dense::cholesky_upper(G2, R2, 1e-8)) {
-        gemm_host(dst, b_, dst, b_, edx::dense::upper_inverse(R2), 1.0, false, X_.p);
-        R = Mat(b_, b_);
-        for (int i = 0; i < b_; ++i)
-          for (int j = 0; j < b_; ++j) {
-            cplx sum(0.0, 0.0);
-            for (int k = 0; k < b_; ++k) sum += R2(i, k) * R1(k, j);
-            R(i, j) = sum;
-          }
+    Mat P = gram_host(Q, nbasis, W_.p, b_, W_.p, b_);
+    C = Mat(nbasis, b_);
+    for (int i = 0; i < nbasis; ++i)
+      for (int j = 0; j < b_; ++j) C(i, j) = P(i, j);
+    Mat G = remainder_gram(P, nbasis);
+    bool one_pass = true;
+    for (int j = 0; j < b_; ++j)
+      if (!(G(j, j).real() * kappa2_ > P(nbasis + j, j).real())) one_pass = false;
+    Mat Cl = C;  // coefficients of the pass still to be applied to W_
+    Mat R1;
+    if (!one_pass || !edx::dense::cholesky_upper(G, R1, tiny)) {
+      // apply the first pass explicitly, then project again
+      gemm_host(Q, nbasis, W_.p, b_, W_.p, b_, update_coeffs(C, nbasis, nullptr), nullptr);
+      Mat P2 = gram_host(Q, nbasis, W_.p, b_, W_.p, b_);
+      for (int i = 0; i < nbasis; ++i)
+        for (int j = 0; j < b_; ++j) {
+          Cl(i, j) = P2(i, j);
+          C(i, j) += P2(i, j);
+        }
+      G = remainder_gram(P2, nbasis);
+      if (!edx::dense::cholesky_upper(G, R1, tiny)) {
+        // W - Q C is numerically rank deficient
+        gemm_host(Q, nbasis, W_.p, b_, W_.p, b_, update_coeffs(Cl, nbasis, nullptr), nullptr);
+        deflate(Q, nbasis, dst, R);
         return;
       }
     }
-    EDX_HIP(hipMemcpy2DAsync(Wsave_.p, ld_ * esz_, W_.p, ld_ * esz_, D_ * esz_, b_, hipMemcpyDeviceToDevice,
-                             ctx_.stream));
+    // dst = (W - Q Cl) R1^-1, then a second Cholesky QR pass on dst
+    const Mat R1i = edx::dense::upper_inverse(R1);
+    gemm_host(Q, nbasis, W_.p, b_, dst, b_, update_coeffs(Cl, nbasis, &R1i), nullptr);
+    Mat G2 = gram_host(dst, b_, nullptr, 0, dst, b_);
+    Mat R2;
+    if (edx::dense::cholesky_upper(G2, R2, 1e-8)) {
+      gemm_host(nullptr, 0, dst, b_, dst, b_, edx::dense::upper_inverse(R2), X_.p);
+      R = mul(R2, R1);
+      return;
+    }
+    gemm_host(Q, nbasis, W_.p, b_, W_.p, b_, update_coeffs(Cl, nbasis, nullptr), nullptr);
+    deflate(Q, nbasis, dst, R);
+  }
+
+  // W_ (orthogonal to Q) is numerically rank deficient: the next block is
+  // made of fresh random directions orthogonal to Q, and R is the projection
+  // of W_ on them.
+  void deflate(void* Q, int nbasis, void* dst, Mat& R) {
+    const double tiny = prec_ == EDX_PREC_FP64 ? 1e-24 : 1e-12;
     for (int attempt = 0; attempt < 3; ++attempt) {
       random_block(dst, b_, opt_.seed * 7919 + 104729 * (uint64_t)(++deflations_));
       for (int pass = 0; pass < 2; ++pass) {
-        gram_dev(vec(cur_, 0), nq, dst, b_, c1_.p);
-        update_dev(vec(cur_, 0), nq, dst, b_, c1_.p);
+        Mat P = gram_host(Q, nbasis, nullptr, 0, dst, b_);
+        gemm_host(Q, nbasis, dst, b_, dst, b_, update_coeffs(P, nbasis, nullptr), nullptr);
       }
-      bool ok = true;
-      for (int pass = 0; pass < 2 && ok; ++pass) {
-        Mat Gd = gram_host(dst, b_, dst, b_);
-        Mat Rd;
-        ok = edx::dense::cholesky_upper(Gd, Rd, pass == 0 ? tiny : 1e-8);
-        if (ok) gemm_host(dst, b_, dst, b_, edx::dense::upper_inverse(Rd), 1.0, false, pass == 1 ? X_.p : nullptr);
-      }
-      if (!ok) continue;
-      R = gram_host(dst, b_, Wsave_.p, b_);
+      if (!cholqr2(dst, tiny, nullptr)) continue;
+      R = gram_host(dst, b_, nullptr, 0, W_.p, b_);
       return;
     }
     throw edx::Error{EDX_ERR_NOCONV, "could not extend the Krylov basis (sector dimension exhausted)"};
@@ -615,17 +674,15 @@ class BlockLanczos {
   Options opt_;
   int64_t D_ = 0, ld_ = 0;
   int b_ = 1, esz_ = 16, nq_max_ = 0, cap_ = 0, cur_ = 0, restarts_ = 0;
+  double kappa2_ = 1e4;
   uint64_t deflations_ = 0;
   int n_rowblocks_ = 1;
   int64_t rows_per_block_ = 0;
-  static constexpr int kSlots = 4;
-  static constexpr size_t kSlot = (size_t)kMaxW * kMaxW;
   int slot_ = 0;
   DevBuf<char> bank_[2];
-  DevBuf<char> W_, Wsave_, X_;
-  DevBuf<double2> partial_, c1_, c2_, sdev_;
-  double2* hc1_ = nullptr;
-  double2* hc2_ = nullptr;
+  DevBuf<char> W_, X_;
+  DevBuf<double2> partial_, cdev_, sdev_;
+  double2* hc_ = nullptr;
   double2* hs_ = nullptr;
 };
 

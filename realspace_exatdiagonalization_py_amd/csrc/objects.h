@@ -52,11 +52,24 @@ struct edx_hamiltonian {
   edx::DevBuf<int32_t> col;           // [n_entries]
   edx::DevBuf<uint16_t> term;         // [n_entries] original term index
   edx::DevBuf<double2> val;           // [n_entries] complex128
-  edx::DevBuf<float2> val32;          // [n_entries] complex64 copy
   edx::DevBuf<double2> hdiag;         // [dim]
   edx::DevBuf<float2> hdiag32;        // [dim]
-  int spmv_group = 0;                 // lanes per row chosen at build
+  // sliced-ELL copy used by the SpMM: slices of 64 / sell_g rows, sell_g lanes per row
+  int sell_g = 1;
+  int64_t sell_slices = 0;
+  int64_t sell_entries = 0;           // stored entries including padding
+  edx::DevBuf<int64_t> sell_off;      // [sell_slices + 1]
+  edx::DevBuf<int32_t> sell_col;      // [sell_entries], -1 = padding
+  edx::DevBuf<double2> sell_val;      // [sell_entries]
+  edx::DevBuf<float2> sell_val32;     // [sell_entries] complex64 copy
 };
+
+namespace edx {
+// (Re)builds the sliced copy of the table with G lanes per row (1, 2 or 4).
+void build_sell(edx_hamiltonian& h, Ctx& ctx, int G);
+// Lanes per row for a table of this shape.
+int default_sell_lanes(int64_t dim, int64_t n_entries);
+}  // namespace edx
 
 namespace edx {
 

@@ -172,7 +172,7 @@ __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
 
 __global__ void __launch_bounds__(kBlock)
 ptab_values(TermView tv, SectorView sv, const double2* chi, const int64_t* row_ptr, const uint64_t* keys,
-            int32_t* col, uint16_t* term, double2* val, float2* val32) {
+            int32_t* col, uint16_t* term, double2* val) {
   __shared__ double2 chi_s[EDX_MAX_GROUP];
   __shared__ uint32_t binom_s[(EDX_MAX_SITES + 1) * (EDX_MAX_SITES + 1)];
   const int nb = (tv.n_site + 1) * (tv.n_filled + 1);
@@ -205,7 +205,6 @@ ptab_values(TermView tv, SectorView sv, const double2* chi, const int64_t* row_p
     col[e] = (int32_t)c;
     term[e] = (uint16_t)k;
     val[e] = h;
-    val32[e] = make_float2((float)h.x, (float)h.y);
   }
 }
 
@@ -239,91 +238,114 @@ template <typename V> struct Real;
 template <> struct Real<double2> { using T = double; };
 template <> struct Real<float2> { using T = float; };
 
-template <typename V, int NBMAX, int G>
+// ---------------------------------------------------------------------------
+// SpMM on the sliced-ELL copy of the table (SELL-64).  Slice s holds R = 64/G
+// consecutive rows; lane l of its wavefront serves row s*R + l/G and takes
+// every G-th entry of that row.  Entry t of lane l sits at off[s] + 64 t + l,
+// so the column and element loads of a wavefront are contiguous; padding has
+// column -1.  X is interleaved (nb values per row, one contiguous gather per
+// entry); Y is column-major with leading dimension ldy.
+// ---------------------------------------------------------------------------
+template <typename V, int NB, int G>
 __global__ void __launch_bounds__(256)
-spmm_csr(int64_t dim, int nb, const int64_t* __restrict__ row_ptr, const int32_t* __restrict__ col,
-         const V* __restrict__ val, const V* __restrict__ hdiag, const V* __restrict__ X, V* __restrict__ Y,
-         int64_t ldy) {
+spmm_sell(int64_t dim, int64_t n_slices, const int64_t* __restrict__ off, const int32_t* __restrict__ col,
+          const V* __restrict__ val, const V* __restrict__ hdiag, const V* __restrict__ X, V* __restrict__ Y,
+          int64_t ldy) {
   using T = typename Real<V>::T;
-  const int64_t row = (int64_t)blockIdx.x * (256 / G) + threadIdx.x / G;
-  const int sub = threadIdx.x % G;
-  T ar[NBMAX], ai[NBMAX];
+  const int lane = threadIdx.x & 63;
+  const int64_t s = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (s >= n_slices) return;
+  const int64_t o0 = off[s];
+  const int64_t L = (off[s + 1] - o0) >> 6;
+  const int64_t row = s * (64 / G) + lane / G;
+  T ar[NB], ai[NB];
 #pragma unroll
-  for (int j = 0; j < NBMAX; ++j) ar[j] = ai[j] = (T)0;
-  if (row < dim) {
-    const int64_t e1 = row_ptr[row + 1];
-    for (int64_t e = row_ptr[row] + sub; e < e1; e += G) {
-      const V v = val[e];
-      const V* x = X + (int64_t)col[e] * nb;
+  for (int q = 0; q < NB; ++q) ar[q] = ai[q] = (T)0;
+  const int32_t* cp = col + o0 + lane;
+  const V* vp = val + o0 + lane;
+  for (int64_t t = 0; t < L; ++t) {
+    const int c = __builtin_nontemporal_load(cp + t * 64);
+    if (c >= 0) {
+      const V v = vp[t * 64];
+      const V* x = X + (int64_t)c * NB;
 #pragma unroll
-      for (int j = 0; j < NBMAX; ++j) {
-        if (j < nb) {
-          const V xv = x[j];
-          ar[j] = fma(v.x, xv.x, ar[j]);
-          ar[j] = fma(-v.y, xv.y, ar[j]);
-          ai[j] = fma(v.x, xv.y, ai[j]);
-          ai[j] = fma(v.y, xv.x, ai[j]);
-        }
+      for (int q = 0; q < NB; ++q) {
+        const V xv = x[q];
+        ar[q] = fma(v.x, xv.x, ar[q]);
+        ar[q] = fma(-v.y, xv.y, ar[q]);
+        ai[q] = fma(v.x, xv.y, ai[q]);
+        ai[q] = fma(v.y, xv.x, ai[q]);
       }
     }
   }
   if (G > 1) {
 #pragma unroll
-    for (int off = G / 2; off >= 1; off >>= 1) {
+    for (int sh = G / 2; sh >= 1; sh >>= 1)
 #pragma unroll
-      for (int j = 0; j < NBMAX; ++j) {
-        ar[j] += __shfl_xor(ar[j], off, G);
-        ai[j] += __shfl_xor(ai[j], off, G);
+      for (int q = 0; q < NB; ++q) {
+        ar[q] += __shfl_xor(ar[q], sh, G);
+        ai[q] += __shfl_xor(ai[q], sh, G);
       }
-    }
   }
-  if (row < dim && sub == 0) {
+  if (lane % G == 0 && row < dim) {
     const V d = hdiag[row];
-    const V* xr = X + row * nb;
+    const V* xr = X + row * NB;
 #pragma unroll
-    for (int j = 0; j < NBMAX; ++j) {
-      if (j < nb) {
-        const V xv = xr[j];
-        T yr = fma(d.x, xv.x, ar[j]);
-        yr = fma(-d.y, xv.y, yr);
-        T yi = fma(d.x, xv.y, ai[j]);
-        yi = fma(d.y, xv.x, yi);
-        V out;
-        out.x = yr;
-        out.y = yi;
-        Y[(int64_t)j * ldy + row] = out;
-      }
+    for (int q = 0; q < NB; ++q) {
+      const V xv = xr[q];
+      V out;
+      out.x = fma(-d.y, xv.y, fma(d.x, xv.x, ar[q]));
+      out.y = fma(d.y, xv.x, fma(d.x, xv.y, ai[q]));
+      Y[(int64_t)q * ldy + row] = out;
     }
   }
 }
 
-template <typename V, int NBMAX>
-void launch_spmm_nb(const edx_hamiltonian& h, hipStream_t st, int G, int nb, const V* val, const V* hd, const V* X,
-                    V* Y, int64_t ldy) {
-  const int64_t dim = h.dim;
-#define EDX_SPMM_CASE(GG)                                                                                  \
-  case GG:                                                                                                 \
-    hipLaunchKernelGGL((spmm_csr<V, NBMAX, GG>), dim3(grid_for(dim * GG, 256)), dim3(256), 0, st, dim, nb, \
-                       (const int64_t*)h.row_ptr.p, (const int32_t*)h.col.p, val, hd, X, Y, ldy);          \
-    break;
-  switch (G) {
-    EDX_SPMM_CASE(1)
-    EDX_SPMM_CASE(2)
-    EDX_SPMM_CASE(4)
-    EDX_SPMM_CASE(8)
+template <typename V, int NB>
+void launch_sell_nb(const edx_hamiltonian& h, hipStream_t st, const V* val, const V* hd, const V* X, V* Y,
+                    int64_t ldy) {
+  const dim3 grid((unsigned)((h.sell_slices + 3) / 4));
+  const int64_t* off = h.sell_off.p;
+  const int32_t* col = h.sell_col.p;
+  switch (h.sell_g) {
+    case 1:
+      hipLaunchKernelGGL((spmm_sell<V, NB, 1>), grid, dim3(256), 0, st, h.dim, h.sell_slices, off, col, val, hd, X,
+                         Y, ldy);
+      break;
+    case 2:
+      hipLaunchKernelGGL((spmm_sell<V, NB, 2>), grid, dim3(256), 0, st, h.dim, h.sell_slices, off, col, val, hd, X,
+                         Y, ldy);
+      break;
     default:
-      EDX_SPMM_CASE(16)
+      hipLaunchKernelGGL((spmm_sell<V, NB, 4>), grid, dim3(256), 0, st, h.dim, h.sell_slices, off, col, val, hd, X,
+                         Y, ldy);
   }
-#undef EDX_SPMM_CASE
   EDX_HIP(hipGetLastError());
 }
 
-// Algorithmic bytes of one SpMM launch: per row the row pointer, diagonal,
-// the row's own x entries and the y writes; per stored entry the column
-// index, the element and one gather of nb x values.
+template <typename V>
+void launch_sell(const edx_hamiltonian& h, hipStream_t st, int nb, const V* val, const V* hd, const V* X, V* Y,
+                 int64_t ldy) {
+  switch (nb) {
+#define EDX_SELL_CASE(N) \
+  case N:                \
+    launch_sell_nb<V, N>(h, st, val, hd, X, Y, ldy); \
+    break;
+    EDX_SELL_CASE(1) EDX_SELL_CASE(2) EDX_SELL_CASE(3) EDX_SELL_CASE(4) EDX_SELL_CASE(5) EDX_SELL_CASE(6)
+    EDX_SELL_CASE(7) EDX_SELL_CASE(8) EDX_SELL_CASE(9) EDX_SELL_CASE(10) EDX_SELL_CASE(11) EDX_SELL_CASE(12)
+    EDX_SELL_CASE(13) EDX_SELL_CASE(14) EDX_SELL_CASE(15) EDX_SELL_CASE(16)
+#undef EDX_SELL_CASE
+    default:
+      throw Error{EDX_ERR_INTERNAL, "SpMM block width out of range"};
+  }
+}
+
+// Algorithmic bytes of one SpMM launch: per row the diagonal, the row's own x
+// entries and the y writes; per stored entry the column index, the element
+// and one gather of nb x values (padding of the sliced layout not counted).
 double spmm_bytes(const edx_hamiltonian& h, int precision, int nb) {
   const double s = precision == EDX_PREC_FP64 ? 16.0 : 8.0;
-  return (double)h.dim * (8.0 + s + 2.0 * nb * s) + (double)h.n_entries * (4.0 + s + nb * s);
+  return (double)h.dim * (s + 2.0 * nb * s) + (double)h.n_entries * (4.0 + s + nb * s);
 }
 
 // Applies H to nb <= 16 interleaved vectors.  precision selects the
@@ -333,27 +355,86 @@ void spmm(const edx_hamiltonian& h, Ctx& ctx, int precision, int nb, const void*
   EDX_REQUIRE(nb >= 1 && nb <= 16, EDX_ERR_INVALID, "at most 16 vectors per SpMM");
   hipStream_t st = ctx.stream;
   hipEvent_t t0 = ctx.timer.start(st);
-  const int G = h.spmv_group > 0 ? h.spmv_group : 4;
-  if (precision == EDX_PREC_FP64) {
-    auto v = (const double2*)h.val.p;
-    auto d = (const double2*)h.hdiag.p;
-    auto x = (const double2*)X;
-    auto y = (double2*)Y;
-    if (nb == 1) launch_spmm_nb<double2, 1>(h, st, G, nb, v, d, x, y, ldy);
-    else if (nb <= 4) launch_spmm_nb<double2, 4>(h, st, G, nb, v, d, x, y, ldy);
-    else if (nb <= 8) launch_spmm_nb<double2, 8>(h, st, G, nb, v, d, x, y, ldy);
-    else launch_spmm_nb<double2, 16>(h, st, G, nb, v, d, x, y, ldy);
-  } else {
-    auto v = (const float2*)h.val32.p;
-    auto d = (const float2*)h.hdiag32.p;
-    auto x = (const float2*)X;
-    auto y = (float2*)Y;
-    if (nb == 1) launch_spmm_nb<float2, 1>(h, st, G, nb, v, d, x, y, ldy);
-    else if (nb <= 4) launch_spmm_nb<float2, 4>(h, st, G, nb, v, d, x, y, ldy);
-    else if (nb <= 8) launch_spmm_nb<float2, 8>(h, st, G, nb, v, d, x, y, ldy);
-    else launch_spmm_nb<float2, 16>(h, st, G, nb, v, d, x, y, ldy);
-  }
+  if (precision == EDX_PREC_FP64)
+    launch_sell<double2>(h, st, nb, h.sell_val.p, h.hdiag.p, (const double2*)X, (double2*)Y, ldy);
+  else
+    launch_sell<float2>(h, st, nb, h.sell_val32.p, h.hdiag32.p, (const float2*)X, (float2*)Y, ldy);
   ctx.timer.stop(st, t0, K_SPMM, spmm_bytes(h, precision, nb));
+}
+
+// ---- construction of the sliced copy from the CSR table
+__global__ void sell_slice_len(int64_t dim, int64_t n_slices, int G, const int64_t* row_ptr, uint32_t* words) {
+  const int64_t s = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (s >= n_slices) return;
+  const int R = 64 / G;
+  int64_t L = 0;
+  for (int64_t r = s * R; r < min(dim, (s + 1) * R); ++r) {
+    const int64_t n = row_ptr[r + 1] - row_ptr[r];
+    L = max(L, (n + G - 1) / G);
+  }
+  words[s] = (uint32_t)(L * 64);
+}
+
+__global__ void sell_fill(int64_t dim, int G, const int64_t* row_ptr, const int32_t* col, const double2* val,
+                          const int64_t* off, int32_t* scol, double2* sval, float2* sval32) {
+  const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (r >= dim) return;
+  const int R = 64 / G;
+  const int64_t s = r / R;
+  const int64_t base = off[s] + (r % R) * G;
+  const int64_t e0 = row_ptr[r], e1 = row_ptr[r + 1];
+  for (int64_t e = e0; e < e1; ++e) {
+    const int64_t k = e - e0;
+    const int64_t pos = base + (k / G) * 64 + (k % G);
+    scol[pos] = col[e];
+    sval[pos] = val[e];
+    sval32[pos] = make_float2((float)val[e].x, (float)val[e].y);
+  }
+}
+
+int default_sell_lanes(int64_t dim, int64_t n_entries) {
+  // one lane per row once there are enough rows to fill the device; small
+  // sectors with long rows split each row over 2 or 4 lanes
+  const double mean_row = dim ? (double)n_entries / (double)dim : 0.0;
+  if (dim >= 65536) return 1;
+  if (mean_row >= 16) return 4;
+  if (mean_row >= 6) return 2;
+  return 1;
+}
+
+// Builds (or rebuilds with another lane count) the sliced copy of the table.
+void build_sell(edx_hamiltonian& h, Ctx& ctx, int G) {
+  EDX_REQUIRE(G == 1 || G == 2 || G == 4, EDX_ERR_INVALID, "lanes per row must be 1, 2 or 4");
+  hipStream_t st = ctx.stream;
+  const int64_t D = h.dim;
+  const int64_t ns = (D + 64 / G - 1) / (64 / G);
+  h.sell_g = G;
+  h.sell_slices = ns;
+  h.sell_off.alloc((size_t)ns + 1);
+  if (ns == 0) {
+    const int64_t zero = 0;
+    EDX_HIP(hipMemcpyAsync(h.sell_off.p, &zero, 8, hipMemcpyHostToDevice, st));
+    EDX_HIP(hipStreamSynchronize(st));
+    return;
+  }
+  DevBuf<uint32_t> words((size_t)ns);
+  hipLaunchKernelGGL(sell_slice_len, dim3(grid_for(ns, 256)), dim3(256), 0, st, D, ns, G,
+                     (const int64_t*)h.row_ptr.p, words.p);
+  EDX_HIP(hipGetLastError());
+  const int64_t total = (int64_t)exclusive_scan_u32_to_i64(ctx, words.p, h.sell_off.p, (size_t)ns);
+  EDX_HIP(hipMemcpyAsync(h.sell_off.p + ns, &total, 8, hipMemcpyHostToDevice, st));
+  h.sell_col.alloc((size_t)std::max<int64_t>(total, 1));
+  h.sell_val.alloc((size_t)std::max<int64_t>(total, 1));
+  h.sell_val32.alloc((size_t)std::max<int64_t>(total, 1));
+  EDX_HIP(hipMemsetAsync(h.sell_col.p, 0xff, h.sell_col.bytes(), st));
+  EDX_HIP(hipMemsetAsync(h.sell_val.p, 0, h.sell_val.bytes(), st));
+  EDX_HIP(hipMemsetAsync(h.sell_val32.p, 0, h.sell_val32.bytes(), st));
+  hipLaunchKernelGGL(sell_fill, dim3(grid_for(D, 256)), dim3(256), 0, st, D, G, (const int64_t*)h.row_ptr.p,
+                     (const int32_t*)h.col.p, (const double2*)h.val.p, (const int64_t*)h.sell_off.p, h.sell_col.p,
+                     h.sell_val.p, h.sell_val32.p);
+  EDX_HIP(hipGetLastError());
+  EDX_HIP(hipStreamSynchronize(st));
+  h.sell_entries = total;
 }
 
 // Column-major [nb][ld] -> interleaved [dim][nb].
@@ -404,12 +485,8 @@ void dense_matrix(const edx_hamiltonian& h, hipStream_t st, double2* out) {
 
 namespace {
 
-int pick_group(double mean_row) {
-  if (mean_row <= 6) return 1;
-  if (mean_row <= 16) return 2;
-  if (mean_row <= 40) return 4;
-  if (mean_row <= 100) return 8;
-  return 16;
+int pick_lanes(const edx::Ctx& ctx, int64_t dim, int64_t n_entries) {
+  return ctx.autotune_group > 0 ? ctx.autotune_group : edx::default_sell_lanes(dim, n_entries);
 }
 
 }  // namespace
@@ -525,10 +602,9 @@ int edx_hamiltonian_build(edx_ctx* ctx, const edx_catalog* cat, const edx_sector
         ham->col.alloc(std::max<int64_t>(E, 1));
         ham->term.alloc(std::max<int64_t>(E, 1));
         ham->val.alloc(std::max<int64_t>(E, 1));
-        ham->val32.alloc(std::max<int64_t>(E, 1));
         hipLaunchKernelGGL(ptab_values, dim3(edx::grid_for(D, kBlock)), dim3(kBlock), 0, st, tv, sv,
                            (const double2*)d_chi.p, (const int64_t*)ham->row_ptr.p, (const uint64_t*)keys.p,
-                           ham->col.p, ham->term.p, ham->val.p, ham->val32.p);
+                           ham->col.p, ham->term.p, ham->val.p);
         EDX_HIP(hipGetLastError());
         hipLaunchKernelGGL(ptab_diag, dim3(edx::grid_for(D, kBlock)), dim3(kBlock), 0, st, D,
                            (const uint64_t*)sec->reps.p, n_density, (const uint8_t*)d_vi.p,
@@ -541,7 +617,7 @@ int edx_hamiltonian_build(edx_ctx* ctx, const edx_catalog* cat, const edx_sector
         EDX_HIP(hipStreamSynchronize(st));
       }
       ham->n_entries = E;
-      ham->spmv_group = ctx->autotune_group > 0 ? ctx->autotune_group : pick_group(D ? (double)E / D : 0.0);
+      edx::build_sell(*ham, *ctx, pick_lanes(*ctx, D, E));
     } catch (...) {
       delete ham;
       throw;
@@ -609,15 +685,11 @@ int edx_hamiltonian_from_csr(edx_ctx* ctx, int64_t dim, int64_t nnz, const int64
       ham->hdiag32.alloc(dim);
       ham->col.alloc(std::max<int64_t>(nnz, 1));
       ham->val.alloc(std::max<int64_t>(nnz, 1));
-      ham->val32.alloc(std::max<int64_t>(nnz, 1));
       EDX_HIP(hipMemcpyAsync(ham->row_ptr.p, row_ptr, (dim + 1) * 8, hipMemcpyHostToDevice, st));
       if (dim) EDX_HIP(hipMemcpyAsync(ham->hdiag.p, h_diag, dim * 16, hipMemcpyHostToDevice, st));
       if (nnz) {
         EDX_HIP(hipMemcpyAsync(ham->col.p, col, nnz * 4, hipMemcpyHostToDevice, st));
         EDX_HIP(hipMemcpyAsync(ham->val.p, vals, nnz * 16, hipMemcpyHostToDevice, st));
-        hipLaunchKernelGGL(to_c64, dim3(edx::grid_for(nnz, 256)), dim3(256), 0, st, nnz, (const double2*)ham->val.p,
-                           ham->val32.p);
-        EDX_HIP(hipGetLastError());
       }
       if (dim) {
         hipLaunchKernelGGL(to_c64, dim3(edx::grid_for(dim, 256)), dim3(256), 0, st, dim, (const double2*)ham->hdiag.p,
@@ -625,7 +697,7 @@ int edx_hamiltonian_from_csr(edx_ctx* ctx, int64_t dim, int64_t nnz, const int64
         EDX_HIP(hipGetLastError());
       }
       EDX_HIP(hipStreamSynchronize(st));
-      ham->spmv_group = ctx->autotune_group > 0 ? ctx->autotune_group : pick_group(dim ? (double)nnz / dim : 0.0);
+      edx::build_sell(*ham, *ctx, pick_lanes(*ctx, dim, nnz));
     } catch (...) {
       delete ham;
       throw;

@@ -40,6 +40,9 @@ import scipy.sparse as sp
 
 from . import _edx
 from ._edx import Context, lib, ptr
+
+_WORKER_LOCK = __import__("threading").Lock()
+_WORKER_CONTEXTS: dict[int, list[Context]] = {}
 from .bitwise_operations import COMPLEX_ONE, MAX_N_SITE_UINT64, Mask
 from .groups import (
     Finite_Symmetry_Group,
@@ -685,6 +688,9 @@ class Symmetry_Resolved_ED_Data:
         # device handles do not pickle: checkpoints carry the results
         state = dict(self.__dict__)
         state["orbit_catalog"] = None
+        # snapshots, so that a checkpoint written while other sectors finish is consistent
+        state["ed_scan_res"] = dict(state["ed_scan_res"])
+        state["sector_dims"] = list(state["sector_dims"])
         return state
 
 
@@ -739,28 +745,88 @@ def ed_scan_at_irrep_matrixfree(irrep_label, ed_data: Symmetry_Resolved_ED_Data,
 _MODES = ("matrix", "matrixfree", "matrixfree_mixed")
 
 
+def _worker_contexts(n: int) -> list[Context]:
+    """``n`` contexts (HIP streams) on the default context's device, kept for reuse."""
+    with _WORKER_LOCK:
+        dev = Context.default().device
+        pool = _WORKER_CONTEXTS.setdefault(dev, [])
+        while len(pool) < n:
+            pool.append(Context(dev))
+        return pool[:n]
+
+
+def run_concurrently(items, fn, workers: int) -> None:
+    """Calls ``fn(item)`` for every item, on ``workers`` threads with one
+    context (HIP stream) each, so that independent sector solves overlap on the
+    device.  The first exception raised by any call is re-raised after every
+    thread has stopped taking work."""
+    items = list(items)
+    if workers <= 1 or len(items) <= 1:
+        for it in items:
+            fn(it)
+        return
+    import threading
+
+    ctxs = _worker_contexts(min(workers, len(items)))
+    lock = threading.Lock()
+    pending = list(reversed(items))
+    errors: list[BaseException] = []
+
+    def worker(ctx):
+        with _edx.use_context(ctx):
+            while True:
+                with lock:
+                    if errors or not pending:
+                        return
+                    it = pending.pop()
+                try:
+                    fn(it)
+                except BaseException as e:  # noqa: BLE001 - re-raised below
+                    with lock:
+                        errors.append(e)
+                    return
+
+    threads = [threading.Thread(target=worker, args=(c,), daemon=True) for c in ctxs]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    for c in ctxs:
+        c.check(lib.edx_synchronize(c.handle))
+    if errors:
+        raise errors[0]
+
+
 def _ed_scan_sectors(ed_data, *, nev=5, mode="matrix", checkpoint_path=None, scanned_sectors=None,
-                     overwrite=False):
-    """Sector loop with checkpoint resume (SRE:2462-2522)."""
+                     overwrite=False, concurrent_sectors=1):
+    """Sector loop with checkpoint resume (SRE:2462-2522).  With
+    ``concurrent_sectors > 1`` the sectors are solved that many at a time, each
+    on its own HIP stream; results and checkpoints are the same as in order."""
+    if mode not in _MODES:
+        raise ValueError(f"Unknown ED scan mode: {mode}. Use 'matrix' or 'matrixfree'.")
     if checkpoint_path is not None and os.path.isfile(checkpoint_path) and not overwrite:
         loaded = load_checkpoint(checkpoint_path)
         ed_data.ed_scan_res = loaded.ed_scan_res
         return checkpoint_path
-    for irrep_idx, irrep in enumerate(ed_data.irrep_list):
-        if irrep_idx in ed_data.ed_scan_res:
-            continue
-        if scanned_sectors is not None and irrep.label not in scanned_sectors:
-            continue
+    todo = [irrep for irrep_idx, irrep in enumerate(ed_data.irrep_list)
+            if irrep_idx not in ed_data.ed_scan_res
+            and (scanned_sectors is None or irrep.label in scanned_sectors)]
+    import threading
+
+    ckpt_lock = threading.Lock()
+
+    def solve(irrep):
         if mode == "matrixfree":
             ed_scan_at_irrep_matrixfree(irrep.label, ed_data, nev=nev)
         elif mode == "matrixfree_mixed":
             ed_scan_at_irrep_matrixfree(irrep.label, ed_data, nev=nev, precision=_edx.PREC_MIXED)
-        elif mode == "matrix":
-            ed_scan_at_irrep_matrix(irrep.label, ed_data, nev=nev)
         else:
-            raise ValueError(f"Unknown ED scan mode: {mode}. Use 'matrix' or 'matrixfree'.")
+            ed_scan_at_irrep_matrix(irrep.label, ed_data, nev=nev)
         if checkpoint_path is not None:
-            save_checkpoint(ed_data, checkpoint_path)
+            with ckpt_lock:
+                save_checkpoint(ed_data, checkpoint_path)
+
+    run_concurrently(todo, solve, concurrent_sectors)
     return checkpoint_path
 
 
@@ -793,17 +859,20 @@ def ed_scan_checkpoint_filename(model, twisted_phases_over_2π, filling_fraction
 
 def ed_scan(ed_data: Symmetry_Resolved_ED_Data, *, nev: int = 5, mode: str = "matrix", scanned_sectors=None,
             checkpoint_path: str | None = None, checkpoint_dir: str = "checkpoints", overwrite: bool = False,
-            flux_direction: int = 1, twisted_phases_over_2π_list=None):
+            flux_direction: int = 1, twisted_phases_over_2π_list=None, concurrent_sectors: int = 1):
     """Unified scan with flux points and checkpoint resume (SRE:2608-2703).
 
     Flux points reuse the catalog: only the stabilizer phases and amplitudes
     are refreshed for each twist (``update_orbit_stabilizer_phases``).
+    ``concurrent_sectors`` (not in the reference) solves that many sectors of
+    a flux point at a time on separate HIP streams.
     """
     if mode not in _MODES:
         raise ValueError(f"Unknown ED scan mode: {mode}.")
     if twisted_phases_over_2π_list is None:
         p = _ed_scan_sectors(ed_data, nev=nev, mode=mode, checkpoint_path=checkpoint_path,
-                             scanned_sectors=scanned_sectors, overwrite=overwrite)
+                             scanned_sectors=scanned_sectors, overwrite=overwrite,
+                             concurrent_sectors=concurrent_sectors)
         return None if p is None else [p]
     from .second_quantized_model import update_second_quantized_model_with_twisted_phases
 
@@ -823,7 +892,8 @@ def ed_scan(ed_data: Symmetry_Resolved_ED_Data, *, nev: int = 5, mode: str = "ma
         θ_data = Symmetry_Resolved_ED_Data(model, ed_data.n_filled, ed_data.filling_fraction, group,
                                            build_irrep_list(group, model.lattice), ed_data.orbit_catalog,
                                            [0] * len(ed_data.irrep_list))
-        _ed_scan_sectors(θ_data, nev=nev, mode=mode, scanned_sectors=scanned_sectors, overwrite=overwrite)
+        _ed_scan_sectors(θ_data, nev=nev, mode=mode, scanned_sectors=scanned_sectors, overwrite=overwrite,
+                         concurrent_sectors=concurrent_sectors)
         save_checkpoint(θ_data, ckpt)
         paths.append(ckpt)
     return paths

@@ -258,3 +258,23 @@ def test_empty_sector_and_errors():
         ed.MatrixFreeHamiltonian(b0, [(1, 1, 1.0)], [], cmap)
     with pytest.raises(ValueError):  # site out of range
         ed.MatrixFreeHamiltonian(b0, [(1, 99, 1.0)], [], cmap)
+
+
+def test_concurrent_sectors_match_sequential():
+    """Sectors solved four at a time on separate streams give the results of
+    the one-at-a-time scan (same start vectors, same arithmetic per stream)."""
+    from fractions import Fraction
+
+    model, ne, G = make_case("hub24")
+    seq = ed.build_ed_data(model, filling_fraction=Fraction(ne, model.lattice.n_site), symmetry_group=G)
+    ed.ed_scan(seq, nev=4, mode="matrixfree")
+    con = ed.build_ed_data(model, filling_fraction=Fraction(ne, model.lattice.n_site), symmetry_group=G)
+    ed.ed_scan(con, nev=4, mode="matrixfree", concurrent_sectors=4)
+    assert sorted(con.ed_scan_res) == sorted(seq.ed_scan_res) == list(range(len(seq.irrep_list)))
+    assert con.sector_dims == seq.sector_dims
+    for idx in seq.ed_scan_res:
+        np.testing.assert_allclose(con.ed_scan_res[idx][0], seq.ed_scan_res[idx][0], atol=1e-10, rtol=0)
+    mix = ed.build_ed_data(model, filling_fraction=Fraction(ne, model.lattice.n_site), symmetry_group=G)
+    ed.ed_scan(mix, nev=4, mode="matrixfree_mixed", concurrent_sectors=3)
+    for idx in seq.ed_scan_res:
+        np.testing.assert_allclose(mix.ed_scan_res[idx][0], seq.ed_scan_res[idx][0], atol=1e-10, rtol=0)

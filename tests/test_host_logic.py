@@ -1,0 +1,137 @@
+"""Host-side logic that needs no device: context routing, the concurrent
+sector runner and the ctypes binding's declarations against ``include/edx.h``."""
+
+from __future__ import annotations
+
+import re
+import threading
+import time
+from pathlib import Path
+
+import pytest
+
+import realspace_exatdiagonalization_py_amd as ed
+from realspace_exatdiagonalization_py_amd import _edx
+from realspace_exatdiagonalization_py_amd import symmetry_resolved_ed as sre
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+class _FakeLib:
+    """Stands in for the library's context calls (no device in this process)."""
+
+    def __init__(self):
+        self.created = 0
+        self.synced = []
+        self.handles = set()
+
+    def edx_device_count(self):
+        return 1
+
+    def edx_ctx_create(self, device, out):
+        self.created += 1
+        out._obj.value = 1000 + self.created
+        self.handles.add(out._obj.value)
+        return 0
+
+    def edx_ctx_destroy(self, h):
+        return 0
+
+    def edx_synchronize(self, h):
+        self.synced.append(h)
+        return 0
+
+    def edx_last_error(self, h):
+        return b"fake"
+
+
+@pytest.fixture
+def fake_lib(monkeypatch):
+    fake = _FakeLib()
+    monkeypatch.setattr(_edx, "lib", fake)
+    monkeypatch.setattr(sre, "lib", fake)
+    monkeypatch.setattr(_edx.Context, "_default", None)
+    monkeypatch.setattr(sre, "_WORKER_CONTEXTS", {})
+    yield fake
+    # contexts made here hold fake handles: keep the real library from destroying them
+    for c in list(_edx.Context._live):
+        if c.handle is not None and c.handle.value in fake.handles:
+            c.handle = None
+
+
+def test_default_context_and_routing(fake_lib):
+    c0 = _edx.Context.default()
+    assert _edx.Context.default() is c0
+    assert _edx.current_context() is c0
+    c1 = _edx.Context(0)
+    with _edx.use_context(c1):
+        assert _edx.current_context() is c1
+        assert _edx.current_context(c0) is c1
+        seen = []
+        t = threading.Thread(target=lambda: seen.append(_edx.current_context()))
+        t.start()
+        t.join()
+        assert seen == [c0]  # routing is per thread
+    assert _edx.current_context() is c0
+    assert _edx.current_context(c1) is c1
+
+
+def test_run_concurrently_uses_one_context_per_worker(fake_lib):
+    seen = {}
+    lock = threading.Lock()
+
+    def job(i):
+        time.sleep(0.01)
+        with lock:
+            seen[i] = (threading.get_ident(), _edx.current_context().handle.value)
+
+    ed.run_concurrently(range(12), job, 4)
+    assert sorted(seen) == list(range(12))
+    by_thread = {}
+    for tid, h in seen.values():
+        by_thread.setdefault(tid, set()).add(h)
+    assert 1 < len(by_thread) <= 4
+    assert all(len(hs) == 1 for hs in by_thread.values())  # a thread keeps its stream
+    handles = [h for hs in by_thread.values() for h in hs]
+    assert len(set(handles)) == len(handles)  # streams are not shared between threads
+    assert len(fake_lib.synced) >= 2  # every worker stream is drained at the end
+    # contexts are reused by the next scan
+    created = fake_lib.created
+    ed.run_concurrently(range(8), job, 4)
+    assert fake_lib.created == created
+
+
+def test_run_concurrently_reraises_and_stops(fake_lib):
+    done = []
+
+    def job(i):
+        if i == 3:
+            raise ValueError("sector 3")
+        time.sleep(0.005)
+        done.append(i)
+
+    with pytest.raises(ValueError, match="sector 3"):
+        ed.run_concurrently(range(40), job, 3)
+    assert len(done) < 39
+    with pytest.raises(KeyError):
+        ed.run_concurrently([1], lambda i: {}[i], 1)
+
+
+def _header_functions():
+    text = (ROOT / "include" / "edx.h").read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    text = re.sub(r"//[^\n]*", "", text)
+    return set(re.findall(r"\b(edx_[a-z0-9_]+)\s*\(", text))
+
+
+def test_binding_declares_every_header_function():
+    declared = _header_functions()
+    assert declared == set(_edx.SIGNATURES), (declared ^ set(_edx.SIGNATURES))
+
+
+def test_library_exports_every_header_function():
+    import ctypes
+
+    lib = ctypes.CDLL(str(_edx.LIB_PATH))
+    missing = [n for n in sorted(_header_functions()) if not hasattr(lib, n)]
+    assert not missing, missing
