@@ -132,7 +132,7 @@ def main() -> int:
     ap.add_argument("--precision", choices=("mixed", "fp64"), default="mixed")
     ap.add_argument("--eig-tol", type=float, default=1e-12,
                     help="eigenvalue error bound that ends a solve (0: ARPACK residual rule only)")
-    ap.add_argument("--workers", type=int, default=4,
+    ap.add_argument("--workers", type=int, default=8,
                     help="sectors solved concurrently, one HIP stream (context) per worker thread")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()

@@ -23,6 +23,8 @@
 // FP64 from the complex64 Ritz vectors to reach the requested tolerance.
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 
@@ -68,19 +70,25 @@ struct VecPair {
 };
 
 // partial[rb][i*NW + j] = sum over row block rb of conj(A_i[r]) W_j[r].
-// Grid: x = tile of TI vectors of A (fastest, so consecutive workgroups read
-// the same W rows and hit L2), y = row block.  Lanes take consecutive rows
-// (coalesced loads); each thread keeps a TI x NW complex tile in registers,
-// folded at the end by a butterfly within the wavefront and a fixed-order sum
-// over the four wavefronts.
+// One workgroup per (tile of TI vectors of A, row block).  The 1-D grid is
+// mapped so that all tiles of a row block run on one XCD (workgroups are
+// dispatched to the 8 XCDs round-robin): that XCD's L2 then serves the row
+// block of W to every tile, and HBM sees A and W about once.  Lanes take
+// consecutive rows (coalesced loads), two rows per iteration; each thread
+// keeps a TI x NW complex tile in registers, folded at the end by a butterfly
+// within the wavefront and a fixed-order sum over the four wavefronts.
 template <typename V, int TI, int NW>
 __global__ void __launch_bounds__(256)
-gram_tiles(int64_t D, VecPair<V> A, const V* __restrict__ Wb, int64_t ldw, int64_t rows_per_block,
-           double2* __restrict__ partial) {
+gram_tiles(int64_t D, VecPair<V> A, const V* __restrict__ Wb, int64_t ldw, int64_t rows_per_block, int n_itiles,
+           int n_tiles, double2* __restrict__ partial) {
   __shared__ double red[4][TI * NW * 2];
+  const int per_xcd = (n_tiles + 7) / 8;
+  const int logical = (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);
+  if (logical >= n_tiles) return;
+  const int itile = logical % n_itiles, rblk = logical / n_itiles;
   const int nv = A.n0 + A.n1;
-  const int i0 = blockIdx.x * TI;
-  const int64_t rb = (int64_t)blockIdx.y * rows_per_block;
+  const int i0 = itile * TI;
+  const int64_t rb = (int64_t)rblk * rows_per_block;
   const int64_t re = min(D, rb + rows_per_block);
   const V* ap[TI];
 #pragma unroll
@@ -90,21 +98,41 @@ gram_tiles(int64_t D, VecPair<V> A, const V* __restrict__ Wb, int64_t ldw, int64
   for (int t = 0; t < TI; ++t)
 #pragma unroll
     for (int j = 0; j < NW; ++j) ar[t][j] = ai[t][j] = 0.0;
-  for (int64_t r = rb + threadIdx.x; r < re; r += 256) {
+  int64_t r = rb + threadIdx.x;
+  for (; r + 256 < re; r += 512) {
+    double2 w[2][NW], v[2][TI];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+#pragma unroll
+      for (int j = 0; j < NW; ++j) w[u][j] = to_d2(Wb[(int64_t)j * ldw + r + 256 * u]);
+#pragma unroll
+      for (int t = 0; t < TI; ++t) v[u][t] = ap[t] ? to_d2(ap[t][r + 256 * u]) : make_double2(0.0, 0.0);
+    }
+#pragma unroll
+    for (int u = 0; u < 2; ++u)
+#pragma unroll
+      for (int t = 0; t < TI; ++t)
+#pragma unroll
+        for (int j = 0; j < NW; ++j) {
+          ar[t][j] = fma(v[u][t].x, w[u][j].x, ar[t][j]);
+          ar[t][j] = fma(v[u][t].y, w[u][j].y, ar[t][j]);
+          ai[t][j] = fma(v[u][t].x, w[u][j].y, ai[t][j]);
+          ai[t][j] = fma(-v[u][t].y, w[u][j].x, ai[t][j]);
+        }
+  }
+  for (; r < re; r += 256) {
     double2 w[NW];
 #pragma unroll
     for (int j = 0; j < NW; ++j) w[j] = to_d2(Wb[(int64_t)j * ldw + r]);
 #pragma unroll
     for (int t = 0; t < TI; ++t) {
-      if (ap[t]) {
-        const double2 v = to_d2(ap[t][r]);
+      const double2 v = ap[t] ? to_d2(ap[t][r]) : make_double2(0.0, 0.0);
 #pragma unroll
-        for (int j = 0; j < NW; ++j) {
-          ar[t][j] = fma(v.x, w[j].x, ar[t][j]);
-          ar[t][j] = fma(v.y, w[j].y, ar[t][j]);
-          ai[t][j] = fma(v.x, w[j].y, ai[t][j]);
-          ai[t][j] = fma(-v.y, w[j].x, ai[t][j]);
-        }
+      for (int j = 0; j < NW; ++j) {
+        ar[t][j] = fma(v.x, w[j].x, ar[t][j]);
+        ar[t][j] = fma(v.y, w[j].y, ar[t][j]);
+        ai[t][j] = fma(v.x, w[j].y, ai[t][j]);
+        ai[t][j] = fma(-v.y, w[j].x, ai[t][j]);
       }
     }
   }
@@ -134,7 +162,7 @@ gram_tiles(int64_t D, VecPair<V> A, const V* __restrict__ Wb, int64_t ldw, int64
         xr += red[wv][o * 2];
         xi += red[wv][o * 2 + 1];
       }
-      partial[(int64_t)blockIdx.y * nout + (int64_t)(i0 + t) * NW + j] = make_double2(xr, xi);
+      partial[(int64_t)rblk * nout + (int64_t)(i0 + t) * NW + j] = make_double2(xr, xi);
     }
   }
 }
@@ -173,7 +201,28 @@ gemm_pair(int64_t D, VecPair<V> A, int nk, V* Ob, int64_t ldo, const double2* __
   double xr[kMaxW], xi[kMaxW];
 #pragma unroll
   for (int k = 0; k < kMaxW; ++k) xr[k] = xi[k] = 0.0;
-  for (int i = 0; i < nv; ++i) {
+  constexpr int U = sizeof(V) == 8 ? 8 : 1;  // complex64: eight loads in flight per thread
+  int i = 0;
+  for (; i + U <= nv; i += U) {
+    double2 v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) v[u] = to_d2(A.vec(i + u)[r]);
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const double2* srow = S + (int64_t)(i + u) * ldS;
+#pragma unroll
+      for (int k = 0; k < kMaxW; ++k) {
+        if (k < nk) {
+          const double2 s = srow[k];
+          xr[k] = fma(v[u].x, s.x, xr[k]);
+          xr[k] = fma(-v[u].y, s.y, xr[k]);
+          xi[k] = fma(v[u].x, s.y, xi[k]);
+          xi[k] = fma(v[u].y, s.x, xi[k]);
+        }
+      }
+    }
+  }
+  for (; i < nv; ++i) {
     const double2 v = to_d2(A.vec(i)[r]);
     const double2* srow = S + (int64_t)i * ldS;
 #pragma unroll
@@ -304,6 +353,10 @@ class BlockLanczos {
     // factor (squared) of its norm to it: the orthogonality it leaves is
     // about the unit roundoff times that factor
     kappa2_ = prec == EDX_PREC_FP64 ? 1e4 : 1e2;
+    // one Cholesky QR pass suffices when its orthogonality error estimate is
+    // below this (FP64 only; complex64 blocks always take two passes)
+    unit_roundoff_ = prec == EDX_PREC_FP64 ? 1.1e-16 : 6e-8;
+    single_qr_tol_ = prec == EDX_PREC_FP64 ? 1e-13 : 0.0;
     bank_[0].alloc((size_t)cap_ * ld_ * esz_);
     bank_[1].alloc((size_t)cap_ * ld_ * esz_);
     W_.alloc((size_t)b_ * ld_ * esz_);
@@ -435,10 +488,12 @@ class BlockLanczos {
   template <typename VT, int TI, int NW>
   void launch_tiles(const void* A0, int n0, const void* A1, int n1, const void* W) {
     const int nv = n0 + n1;
-    const dim3 grid((unsigned)((nv + TI - 1) / TI), (unsigned)n_rowblocks_);
+    const int n_itiles = (nv + TI - 1) / TI;
+    const int n_tiles = n_itiles * n_rowblocks_;
+    const dim3 grid((unsigned)((n_tiles + 7) / 8 * 8));
     VecPair<VT> A{(const VT*)A0, ld_, n0, (const VT*)A1, ld_, n1};
     hipLaunchKernelGGL((gram_tiles<VT, TI, NW>), grid, dim3(256), 0, ctx_.stream, D_, A, (const VT*)W, ld_,
-                       rows_per_block_, partial_.p);
+                       rows_per_block_, n_itiles, n_tiles, partial_.p);
   }
 
   template <typename VT>
@@ -448,10 +503,10 @@ class BlockLanczos {
   case NWV:                     \
     launch_tiles<VT, TIV, NWV>(A0, n0, A1, n1, W); \
     break;
-      EDX_GRAM_CASE(1, 8) EDX_GRAM_CASE(2, 8) EDX_GRAM_CASE(3, 8) EDX_GRAM_CASE(4, 8)
-      EDX_GRAM_CASE(5, 4) EDX_GRAM_CASE(6, 4) EDX_GRAM_CASE(7, 4) EDX_GRAM_CASE(8, 4)
-      EDX_GRAM_CASE(9, 2) EDX_GRAM_CASE(10, 2) EDX_GRAM_CASE(11, 2) EDX_GRAM_CASE(12, 2)
-      EDX_GRAM_CASE(13, 2) EDX_GRAM_CASE(14, 2) EDX_GRAM_CASE(15, 2) EDX_GRAM_CASE(16, 2)
+      EDX_GRAM_CASE(1, 4) EDX_GRAM_CASE(2, 4) EDX_GRAM_CASE(3, 2) EDX_GRAM_CASE(4, 2)
+      EDX_GRAM_CASE(5, 2) EDX_GRAM_CASE(6, 2) EDX_GRAM_CASE(7, 2) EDX_GRAM_CASE(8, 2)
+      EDX_GRAM_CASE(9, 1) EDX_GRAM_CASE(10, 1) EDX_GRAM_CASE(11, 1) EDX_GRAM_CASE(12, 1)
+      EDX_GRAM_CASE(13, 1) EDX_GRAM_CASE(14, 1) EDX_GRAM_CASE(15, 1) EDX_GRAM_CASE(16, 1)
 #undef EDX_GRAM_CASE
       default:
         throw edx::Error{EDX_ERR_INTERNAL, "Gram block width out of range"};
@@ -607,12 +662,10 @@ class BlockLanczos {
     for (int i = 0; i < nbasis; ++i)
       for (int j = 0; j < b_; ++j) C(i, j) = P(i, j);
     Mat G = remainder_gram(P, nbasis);
-    bool one_pass = true;
-    for (int j = 0; j < b_; ++j)
-      if (!(G(j, j).real() * kappa2_ > P(nbasis + j, j).real())) one_pass = false;
+    double drop = norm_drop(P, G, nbasis);
     Mat Cl = C;  // coefficients of the pass still to be applied to W_
     Mat R1;
-    if (!one_pass || !edx::dense::cholesky_upper(G, R1, tiny)) {
+    if (!(drop < kappa2_) || !edx::dense::cholesky_upper(G, R1, tiny)) {
       // apply the first pass explicitly, then project again
       gemm_host(Q, nbasis, W_.p, b_, W_.p, b_, update_coeffs(C, nbasis, nullptr), nullptr);
       Mat P2 = gram_host(Q, nbasis, W_.p, b_, W_.p, b_);
@@ -622,6 +675,7 @@ class BlockLanczos {
           C(i, j) += P2(i, j);
         }
       G = remainder_gram(P2, nbasis);
+      drop = norm_drop(P2, G, nbasis);
       if (!edx::dense::cholesky_upper(G, R1, tiny)) {
         // W - Q C is numerically rank deficient
         gemm_host(Q, nbasis, W_.p, b_, W_.p, b_, update_coeffs(Cl, nbasis, nullptr), nullptr);
@@ -629,9 +683,18 @@ class BlockLanczos {
         return;
       }
     }
-    // dst = (W - Q Cl) R1^-1, then a second Cholesky QR pass on dst
+    // dst = (W - Q Cl) R1^-1.  Its Gram differs from the identity by about
+    // eps * drop * cond(R1)^2 (the error of the Gram G, formed by
+    // subtraction, carried through R1^-1); when that is below single_qr_tol_
+    // dst is final, otherwise a second Cholesky QR pass is made on it.
     const Mat R1i = edx::dense::upper_inverse(R1);
-    gemm_host(Q, nbasis, W_.p, b_, dst, b_, update_coeffs(Cl, nbasis, &R1i), nullptr);
+    const double cond = frobenius(R1) * frobenius(R1i);
+    const bool single = unit_roundoff_ * std::max(drop, 1.0) * cond * cond <= single_qr_tol_;
+    gemm_host(Q, nbasis, W_.p, b_, dst, b_, update_coeffs(Cl, nbasis, &R1i), single ? X_.p : nullptr);
+    if (single) {
+      R = R1;
+      return;
+    }
     Mat G2 = gram_host(dst, b_, nullptr, 0, dst, b_);
     Mat R2;
     if (edx::dense::cholesky_upper(G2, R2, 1e-8)) {
@@ -641,6 +704,22 @@ class BlockLanczos {
     }
     gemm_host(Q, nbasis, W_.p, b_, W_.p, b_, update_coeffs(Cl, nbasis, nullptr), nullptr);
     deflate(Q, nbasis, dst, R);
+  }
+
+  // Largest ratio ||w_j||^2 / ||w_j - Q c_j||^2 over the block's columns.
+  double norm_drop(const Mat& P, const Mat& G, int nbasis) const {
+    double d = 0.0;
+    for (int j = 0; j < b_; ++j) {
+      const double g = G(j, j).real();
+      d = std::max(d, g > 0.0 ? P(nbasis + j, j).real() / g : INFINITY);
+    }
+    return d;
+  }
+
+  static double frobenius(const Mat& A) {
+    double s = 0.0;
+    for (const cplx& x : A.a) s += std::norm(x);
+    return std::sqrt(s);
   }
 
   // W_ (orthogonal to Q) is numerically rank deficient: the next block is
@@ -675,6 +754,8 @@ class BlockLanczos {
   int64_t D_ = 0, ld_ = 0;
   int b_ = 1, esz_ = 16, nq_max_ = 0, cap_ = 0, cur_ = 0, restarts_ = 0;
   double kappa2_ = 1e4;
+  double unit_roundoff_ = 1.1e-16;
+  double single_qr_tol_ = 0.0;
   uint64_t deflations_ = 0;
   int n_rowblocks_ = 1;
   int64_t rows_per_block_ = 0;
@@ -789,12 +870,17 @@ void eigs_impl(edx_ctx* ctx, const edx_hamiltonian* ham, const edx_eigs_options&
     opt.seed = seed;
     Result res;
     int64_t mv_total = 0;
+    // EDX_TRACE=1 prints the work of each phase to stderr
+    static const bool trace = [] {
+      const char* e = getenv("EDX_TRACE");
+      return e && *e && *e != '0';
+    }();
     std::unique_ptr<BlockLanczos> fin;
     if (precision == EDX_PREC_MIXED) {
-      // complex64 sweeps resolve nev + 2 pairs (multiplets that straddle the
-      // nev-th eigenvalue come out whole), then an FP64 block Lanczos starts
-      // from all of those Ritz vectors and checks convergence every step
-      const int n32 = (int)std::min<int64_t>(std::min(nev + 2, kMaxW), D - 2);
+      // complex64 sweeps resolve the nev pairs to mixed_tol, then an FP64
+      // block Lanczos starts from those Ritz vectors (all in its first block)
+      // and checks convergence every step
+      const int n32 = (int)std::min<int64_t>(std::min(nev, kMaxW), D - 2);
       Options o32 = opt;
       o32.nev = n32;
       o32.tol = std::max(tol, o.mixed_tol > 0 ? o.mixed_tol : 1e-5);
@@ -802,6 +888,9 @@ void eigs_impl(edx_ctx* ctx, const edx_hamiltonian* ham, const edx_eigs_options&
       BlockLanczos lz32(*ctx, *ham, EDX_PREC_MIXED, o32);
       Result r32 = lz32.run(nullptr, 0, 0);
       mv_total += r32.matvecs;
+      if (trace)
+        fprintf(stderr, "[edx eigs] D=%lld complex64 phase: nev=%d block=%d matvecs=%lld restarts=%d res=%.2e\n",
+                (long long)D, n32, b, (long long)r32.matvecs, r32.restarts, r32.max_rel_res);
       Options o64 = opt;
       o64.block = std::min(kMaxW, std::max(b, n32));
       o64.basis = 0;
@@ -819,6 +908,9 @@ void eigs_impl(edx_ctx* ctx, const edx_hamiltonian* ham, const edx_eigs_options&
       res = fin->run(nullptr, 0, 0);
     }
     mv_total += res.matvecs;
+    if (trace)
+      fprintf(stderr, "[edx eigs] D=%lld FP64 phase: nev=%d matvecs=%lld restarts=%d res=%.2e\n", (long long)D, nev,
+              (long long)res.matvecs, res.restarts, res.max_rel_res);
     for (int c = 0; c < nev; ++c) vals[c] = res.vals[c];
     if (vecs)
       EDX_HIP(hipMemcpy2DAsync(vecs, D * 16, fin->vec(fin->current(), 0), fin->ld() * 16, D * 16, nev,
