@@ -4,21 +4,25 @@
 Workload (one GPU): spinful Fermi-Hubbard model on a 3x4 periodic square
 lattice, 24 spin-orbitals, 12 fermions (C(24,12) = 2 704 156 masks), t = 1,
 U = 4, translation group of order 12; one step solves all 12 momentum sectors
-for their 6 lowest eigenvalues.  The orbit catalog is built once outside the
+for their 6 lowest eigenpairs.  The orbit catalog is built once outside the
 timed region (the reference's benchmark convention) and timed separately; a
-sector solve covers the irrep filter, the projection-table build and the
-eigensolve.  Inputs are synthetic (the model is fully determined by its
-parameters).
+sector solve covers the irrep filter, the projection-table build, the
+eigensolve and the eigenvectors.  Inputs are synthetic (the model is fully
+determined by its parameters).
 
 Printed: one JSON line with solves/s (whole job), p50/p99 sector latency,
-|dE| against the FP64 solve, the live roofline of the SpMM kernel (HIP events
-on the launch stream over the timed region) and the CPU oracle timed on a
-bounded sample (one sector) on this host.
+|dE| against the FP64 solve, the live roofline of the dominant SpMM kernel
+(HIP events on the launch streams over the timed region) and the CPU oracle
+timed on a bounded sample (two sectors) on this host.
+
+Other modes (for profilers): ``--profile-spmm`` runs only the SpMM kernels of
+one sector; ``--copy-bytes N`` runs a plain device copy of N bytes.
 """
 
 from __future__ import annotations
 
 import argparse
+import ctypes as C
 import json
 import os
 import sys
@@ -33,6 +37,7 @@ sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "sector solves/s + p99 latency at |ΔE|≤1e-10 vs FP64; H|ψ⟩ HBM GB/s"
+TRAFFIC_FILE = ROOT / "profiles" / "spmm_hubbard3x4_traffic.json"
 
 
 def log(msg: str) -> None:
@@ -87,8 +92,9 @@ def _missing_copies(cpu, gpu, tol=1e-8):
     return miss
 
 
-def cpu_baseline(model, G, nev, gpu_vals0):
-    """The oracle (numpy/scipy restatement of the reference method) on one sector."""
+def cpu_baseline(model, G, nev, gpu_vals, sectors=(0, 1)):
+    """The oracle (numpy/scipy restatement of the reference method) on a
+    bounded sample: the catalog once (timed apart) and `sectors` solved."""
     from oracle import ed_oracle as O
     from realspace_exatdiagonalization_py_amd.groups import build_irrep_list, group_arrays
 
@@ -96,31 +102,111 @@ def cpu_baseline(model, G, nev, gpu_vals0):
     t0 = time.perf_counter()
     cat = O.build_catalog(model.lattice.n_site, 12, perm, ph, True)
     t_cat = time.perf_counter() - t0
-    irrep = build_irrep_list(G, model.lattice)[0]
-    t0 = time.perf_counter()
-    reps, stab, o2b = O.sector_basis(cat, irrep.values)
-    r, c, v, hd = O.projection_table(cat, reps, stab, o2b, model.bilinear_terms, model.density_density_terms,
-                                     irrep.values, True)
-    H = O.csr_hamiltonian(r, c, v, hd, len(reps))
-    w, _ = O.diagonalize(H, nev)
-    t_solve = time.perf_counter() - t0
+    irreps = build_irrep_list(G, model.lattice)
+    t_solve, dims, dE, dE_set, missing, cpu_vals = 0.0, [], 0.0, 0.0, 0, []
+    for idx in sectors:
+        irrep = irreps[idx]
+        t0 = time.perf_counter()
+        reps, stab, o2b = O.sector_basis(cat, irrep.values)
+        r, c, v, hd = O.projection_table(cat, reps, stab, o2b, model.bilinear_terms, model.density_density_terms,
+                                         irrep.values, True)
+        H = O.csr_hamiltonian(r, c, v, hd, len(reps))
+        w, _ = O.diagonalize(H, nev)
+        t_solve += time.perf_counter() - t0
+        w = np.sort(np.asarray(w))
+        g = np.asarray(gpu_vals[idx])
+        dims.append(len(reps))
+        dE = max(dE, float(np.max(np.abs(w - g))))
+        # distinct levels in the range both lists cover: ARPACK can return a
+        # degenerate level with fewer copies (and then levels beyond ours)
+        top = min(w[-1], g[-1]) + 1e-8
+        d1 = [np.min(np.abs(g - x)) for x in w if x <= top]
+        d2 = [np.min(np.abs(w - y)) for y in g if y <= top]
+        dE_set = max(dE_set, float(max(d1 + d2)))
+        missing += _missing_copies(w, g)
+        cpu_vals.append([float(x) for x in w])
     threads = os.environ.get("OMP_NUM_THREADS") or os.environ.get("OPENBLAS_NUM_THREADS") or str(os.cpu_count())
     return {
-        "value": round(1.0 / t_solve, 6),
+        "value": round(len(sectors) / t_solve, 6),
         "unit": "sector solves/s",
         "cores": int(threads),
         "kind": "port",
         "sample": (f"oracle (numpy/scipy restatement of the reference method: CSR projection table + ARPACK "
-                   f"eigs SR tol 1e-10) on sector k=(0,0) (dim {len(reps)}, nev={nev}); catalog timed "
-                   f"separately {t_cat:.2f}s; BLAS threads {threads}, scipy CSR matvec single-threaded"),
-        "sector_solve_s": round(t_solve, 3),
+                   f"eigs SR tol 1e-10, reference start vector) on sectors {list(sectors)} (dims {dims}, nev={nev}); "
+                   f"catalog timed apart ({t_cat:.2f} s); BLAS threads {threads}, scipy CSR matvec single-threaded"),
+        "sector_solve_s": round(t_solve / len(sectors), 3),
         "catalog_s": round(t_cat, 3),
-        "max_abs_dE_gpu_vs_cpu": float(np.max(np.abs(np.asarray(w) - gpu_vals0))),
-        "max_abs_dE_gpu_vs_cpu_setwise": float(max(np.min(np.abs(gpu_vals0 - x)) for x in w)),
-        "cpu_arpack_missing_copies": _missing_copies(np.asarray(w), gpu_vals0),
-        "cpu_vals_sector0": [float(x) for x in w],
-        "gpu_vals_sector0": [float(x) for x in gpu_vals0],
+        "max_abs_dE_gpu_vs_cpu_sorted": dE,
+        "max_abs_dE_gpu_vs_cpu_levels": dE_set,
+        "cpu_arpack_missing_copies": missing,
+        "cpu_vals": cpu_vals,
     }
+
+
+def _sector_hamiltonian(ed, data, idx=0):
+    model = data.second_quantized_model
+    cmap = ed.CanonicalMap(data.symmetry_group, model.particle_statistics, data.orbit_catalog)
+    basis = ed.build_symmetry_sector_basis(data.orbit_catalog, data.irrep_list[idx])
+    return ed.MatrixFreeHamiltonian(basis, model.bilinear_terms, model.density_density_terms, cmap)
+
+
+def time_spmm(_edx, H, precision, nb, n_rep=20):
+    """Average SpMM launch (ms) on one stream, inputs resident in HBM, and its
+    algorithmic bytes."""
+    dh = H._dev
+    ctx = dh.ctx
+    s = 16 if precision == 0 else 8
+    X = _edx.DeviceArray(ctx, dh.dim * nb * s)
+    Y = _edx.DeviceArray(ctx, dh.dim * nb * s)
+    rng = np.random.default_rng(7)
+    x = rng.standard_normal(dh.dim * nb * 2)
+    X.upload(x if precision == 0 else x.astype(np.float32))
+    ms = C.c_double()
+    ctx.check(_edx.lib.edx_time_matvec(ctx.handle, dh.handle, precision, nb, n_rep, X.ptr, Y.ptr, C.byref(ms)))
+    nbytes = dh.dim * (s + 2.0 * nb * s) + dh.n_entries * (4.0 + s + nb * s)
+    return ms.value, nbytes
+
+
+def profile_spmm(args) -> int:
+    """Only the SpMM kernels of sector k=(0,0): 20 launches each of the
+    complex64 and complex128 block-of-6 products (for counter runs)."""
+    os.environ.setdefault("EDX_QUIET", "1")
+    import realspace_exatdiagonalization_py_amd as ed
+    from realspace_exatdiagonalization_py_amd import _edx
+
+    model, G = build_workload()
+    data = ed.build_ed_data(model, filling_fraction=Fraction(12, 24), symmetry_group=G)
+    H = _sector_hamiltonian(ed, data, 0)
+    out = {"dim": H._dev.dim, "entries": H._dev.n_entries}
+    for prec, name in ((1, "spmm_sell<float2,6,1>"), (0, "spmm_sell<double2,6,1>")):
+        ms, nbytes = time_spmm(_edx, H, prec, 6)
+        out[name] = {"avg_ms": round(ms, 5), "algorithmic_bytes": int(nbytes),
+                     "GB/s": round(nbytes / ms / 1e6, 1)}
+    print(json.dumps(out))
+    return 0
+
+
+def copy_kernel(args) -> int:
+    """A plain device copy of --copy-bytes bytes, five times (counter calibration)."""
+    from realspace_exatdiagonalization_py_amd import _edx
+
+    ctx = _edx.Context.default()
+    n = int(args.copy_bytes)
+    a = _edx.DeviceArray(ctx, n)
+    b = _edx.DeviceArray(ctx, n)
+    for _ in range(5):
+        ctx.check(_edx.lib.edx_device_copy(ctx.handle, b.ptr, a.ptr, n))
+    print(json.dumps({"copy_bytes": n, "launches": 5}))
+    return 0
+
+
+def _traffic(kernel: str):
+    """HBM bytes per launch of `kernel` from the committed counter summary."""
+    try:
+        t = json.loads(TRAFFIC_FILE.read_text())
+        return t["kernels"][kernel]["hbm_bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
 
 
 def main() -> int:
@@ -135,11 +221,17 @@ def main() -> int:
     ap.add_argument("--workers", type=int, default=8,
                     help="sectors solved concurrently, one HIP stream (context) per worker thread")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--profile-spmm", action="store_true", help="run only the SpMM kernels of one sector")
+    ap.add_argument("--copy-bytes", type=float, default=0, help="run only a plain device copy of this many bytes")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}): sectors shard across GPUs "
               "without communication, so the one-GPU figure is the per-GPU rate.", file=sys.stderr)
         return 2
+    if args.profile_spmm:
+        return profile_spmm(args)
+    if args.copy_bytes:
+        return copy_kernel(args)
 
     os.environ.setdefault("EDX_QUIET", "1")
     import realspace_exatdiagonalization_py_amd as ed
@@ -174,34 +266,46 @@ def main() -> int:
     ks = _edx.kernel_stats_all(enable=0)
     n_solves = len(data.irrep_list) * args.steps
 
-    # accuracy and the plain path: the same sectors in FP64 with ARPACK's residual rule
+    # accuracy and the plain path: the same sectors in FP64 with ARPACK's
+    # residual rule, one at a time
     t0 = time.perf_counter()
     lat64, vals64 = solve_all(ed, data, args.nev, ed.PREC_FP64, 0.0)
     t_plain = time.perf_counter() - t0
     dE = max(float(np.max(np.abs(a - b))) for a, b in zip(vals, vals64))
     log(f"plain FP64 path: {t_plain*1e3:.1f} ms for {len(data.irrep_list)} sectors, max |dE| {dE:.2e}")
 
-    spmm_ms, spmm_n, spmm_bytes = ks["spmm"]
+    # roofline of the dominant SpMM (the class with the most kernel time)
+    cls = max(("spmm_c64", "spmm_c128"), key=lambda k: ks[k][0])
+    spmm_ms, spmm_n, spmm_bytes = ks[cls]
+    nb = min(args.nev, 8)
+    kname = f"spmm_sell<{'float2' if cls == 'spmm_c64' else 'double2'},{nb},1>"
     avg_ms = spmm_ms / max(spmm_n, 1)
     achieved = spmm_bytes / max(spmm_ms, 1e-12) / 1e6  # GB/s
     total_k = sum(v[0] for v in ks.values())
+    H0 = _sector_hamiltonian(ed, data, 0)
+    iso_ms, iso_bytes = time_spmm(_edx, H0, 1 if cls == "spmm_c64" else 0, nb)
+    del H0
     roofline = {
         "bound": "hbm",
-        "kernel": "spmm_csr (projection-table H|psi>, block of b vectors)",
         "achieved": round(achieved, 1),
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": None,
+        "traffic": _traffic(kname),
+        "kernel": f"{kname} (projection-table H|psi> on a block of {nb} vectors)",
         "avg_launch_ms": round(avg_ms, 5),
         "launches": spmm_n,
         "algorithmic_bytes_per_launch": round(spmm_bytes / max(spmm_n, 1)),
+        "concurrent_streams": args.workers,
+        "isolated": {"avg_launch_ms": round(iso_ms, 5), "achieved": round(iso_bytes / iso_ms / 1e6, 1),
+                     "frac": round(iso_bytes / iso_ms / 1e6 / HBM_PEAK_GBS, 4),
+                     "note": "same kernel, sector k=(0,0), one stream, 20 launches after the timed region"},
         "kernel_time_share": {k: round(v[0] / total_k, 3) for k, v in ks.items() if total_k > 0},
     }
     cpu = None
     if not args.no_cpu_baseline:
-        log("cpu baseline (oracle, one sector) ...")
-        cpu = cpu_baseline(model, G, args.nev, vals[0])
+        log("cpu baseline (oracle, two sectors) ...")
+        cpu = cpu_baseline(model, G, args.nev, vals)
     lat_ms = np.asarray(lat_all) * 1e3
     dims = list(data.sector_dims)
     line = {
@@ -215,11 +319,11 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "complex64+f64 refine" if prec == ed.PREC_MIXED else "complex128",
+        "dtype": "complex64 sweeps + complex128 refinement" if prec == ed.PREC_MIXED else "complex128",
         "data": "synthetic (model fully determined by its parameters)",
         "config": {
             "workload": "spinful Hubbard 3x4 PBC, t=1, U=4, 12 fermions on 24 orbitals, C(24,12)=2704156, "
-                        "all 12 momentum sectors, 6 lowest eigenvalues each",
+                        "all 12 momentum sectors, 6 lowest eigenpairs each",
             "sector_dims": [min(dims), max(dims)],
             "nev": args.nev,
             "precision": args.precision,

@@ -180,11 +180,13 @@ int edx_hamiltonian_info(const edx_hamiltonian* ham, int64_t* dim, int64_t* n_en
 int edx_host_herm_eig(int32_t n, const double* a, int32_t upper, double* w, double* z);
 
 /* ---- live kernel timing -------------------------------------------------------
- * No reference counterpart.  With timing enabled every SpMM (class 0), Gram
- * (1) and block-update/combine (2) launch on the ctx stream is bracketed by
- * HIP events; ms/launches/bytes (arrays of 4, NULL skips) receive the totals
- * (bytes = algorithmic bytes of each launch).  enable: 1 on, 0 off, -1 keep;
- * reset != 0 clears the totals first. */
+ * No reference counterpart.  With timing enabled every complex128 SpMM (class
+ * 0), complex64 SpMM (1), Gram (2) and block-update/combine (3) launch on the
+ * ctx stream is bracketed by HIP events; ms/launches/bytes (arrays of
+ * EDX_KERNEL_CLASSES, NULL skips) receive the totals (bytes = algorithmic
+ * bytes of each launch).  enable: 1 on, 0 off, -1 keep; reset != 0 clears the
+ * totals first. */
+#define EDX_KERNEL_CLASSES 5
 int edx_kernel_stats(edx_ctx* ctx, int32_t enable, int32_t reset, double* ms, int64_t* launches, double* bytes);
 
 /* ---- device helpers for benchmarks and tests ------------------------------ */
@@ -192,6 +194,9 @@ int edx_device_alloc(edx_ctx* ctx, uint64_t bytes, void** out);
 int edx_device_free(edx_ctx* ctx, void* p);
 int edx_memcpy_h2d(edx_ctx* ctx, void* dst, const void* src, uint64_t bytes);
 int edx_memcpy_d2h(edx_ctx* ctx, void* dst, const void* src, uint64_t bytes);
+/* Device-to-device copy on the ctx stream (runtime blit kernel); a plain copy
+ * of known size, used to calibrate memory counters. */
+int edx_device_copy(edx_ctx* ctx, void* dst, const void* src, uint64_t bytes);
 /* Average duration (ms) of the last n_rep launches of the matvec kernel timed
  * with HIP events on the ctx stream, inputs resident in device memory. */
 int edx_time_matvec(edx_ctx* ctx, const edx_hamiltonian* ham, int32_t precision, int32_t n_vec,

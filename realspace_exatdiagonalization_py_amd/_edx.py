@@ -69,6 +69,7 @@ SIGNATURES = {
     "edx_device_free": (C.c_int, [P, P]),
     "edx_memcpy_h2d": (C.c_int, [P, P, P, u64]),
     "edx_memcpy_d2h": (C.c_int, [P, P, P, u64]),
+    "edx_device_copy": (C.c_int, [P, P, P, u64]),
     "edx_time_matvec": (C.c_int, [P, P, i32, i32, i32, P, P, C.POINTER(C.c_double)]),
     "edx_kernel_stats": (C.c_int, [P, i32, i32, P, P, P]),
     "edx_eigs_ex": (C.c_int, [P, P, P, P, P, P]),
@@ -187,14 +188,15 @@ def current_context(default: "Context | None" = None) -> "Context":
     return default if default is not None else Context.default()
 
 
-KERNEL_CLASSES = ("spmm", "gram", "gemm", "other")
+KERNEL_CLASSES = ("spmm_c128", "spmm_c64", "gram", "gemm", "other")
 
 
 def kernel_stats(ctx: "Context", enable: int = -1, reset: bool = False) -> dict:
     """Totals of the live kernel timer: {class: (ms, launches, bytes)}."""
-    ms = np.zeros(4)
-    n = np.zeros(4, dtype=np.int64)
-    b = np.zeros(4)
+    k = len(KERNEL_CLASSES)
+    ms = np.zeros(k)
+    n = np.zeros(k, dtype=np.int64)
+    b = np.zeros(k)
     ctx.check(lib.edx_kernel_stats(ctx.handle, enable, 1 if reset else 0, ptr(ms), ptr(n), ptr(b)))
     return {k: (float(ms[i]), int(n[i]), float(b[i])) for i, k in enumerate(KERNEL_CLASSES)}
 

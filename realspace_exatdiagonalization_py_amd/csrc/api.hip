@@ -145,6 +145,15 @@ int edx_memcpy_d2h(edx_ctx* ctx, void* dst, const void* src, uint64_t bytes) {
   });
 }
 
+int edx_device_copy(edx_ctx* ctx, void* dst, const void* src, uint64_t bytes) {
+  return edx::guarded(ctx, [&] {
+    EDX_REQUIRE(ctx && dst && src, EDX_ERR_INVALID, "null argument");
+    edx::DeviceGuard dg(ctx->device);
+    EDX_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    EDX_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
 // Host-only: eigen-decomposition of a Hermitian n x n matrix given row-major as
 // (re, im) pairs, using its upper (upper != 0) or lower triangle.  w: n
 // ascending eigenvalues; z (optional): row-major eigenvectors as columns.

@@ -69,7 +69,7 @@ struct DevBuf {
 
 // Live per-class kernel timing with HIP events on the launch stream
 // (enabled on request; used by benchmarks to time kernels inside a solve).
-enum KernelClass { K_SPMM = 0, K_GRAM = 1, K_GEMM = 2, K_OTHER = 3, K_NCLASS = 4 };
+enum KernelClass { K_SPMM = 0, K_SPMM32 = 1, K_GRAM = 2, K_GEMM = 3, K_OTHER = 4, K_NCLASS = EDX_KERNEL_CLASSES };
 
 struct KernelTimer {
   bool on = false;
@@ -80,9 +80,9 @@ struct KernelTimer {
   };
   std::vector<Pending> pending;
   std::vector<hipEvent_t> pool;
-  double ms[K_NCLASS] = {0, 0, 0, 0};
-  int64_t launches[K_NCLASS] = {0, 0, 0, 0};
-  double bytes[K_NCLASS] = {0, 0, 0, 0};
+  double ms[K_NCLASS] = {};
+  int64_t launches[K_NCLASS] = {};
+  double bytes[K_NCLASS] = {};
 
   hipEvent_t take() {
     if (pool.empty()) {

@@ -359,7 +359,7 @@ void spmm(const edx_hamiltonian& h, Ctx& ctx, int precision, int nb, const void*
     launch_sell<double2>(h, st, nb, h.sell_val.p, h.hdiag.p, (const double2*)X, (double2*)Y, ldy);
   else
     launch_sell<float2>(h, st, nb, h.sell_val32.p, h.hdiag32.p, (const float2*)X, (float2*)Y, ldy);
-  ctx.timer.stop(st, t0, K_SPMM, spmm_bytes(h, precision, nb));
+  ctx.timer.stop(st, t0, precision == EDX_PREC_FP64 ? K_SPMM : K_SPMM32, spmm_bytes(h, precision, nb));
 }
 
 // ---- construction of the sliced copy from the CSR table
