@@ -164,6 +164,36 @@ void edx_eigs_default_options(edx_eigs_options* opts);
 int edx_eigs_ex(edx_ctx* ctx, const edx_hamiltonian* ham, const edx_eigs_options* opts, double* vals,
                 double* vecs, double* info);
 
+/* ---- eigenvector consumers -----------------------------------------------------
+ * Replaces _expand_sector_state_to_fock_amplitudes (ref
+ * observables/entanglement_spectrum.py:82-144): coeff (dim complex) expanded
+ * to Fock amplitudes, pruned (|amp| <= atol) and normalised.  convention 0 is
+ * the reference's formula, sum_g c/sqrt(|G||Stab|) conj(chi(g)) alpha_g;
+ * convention 1 uses chi(g) conj(alpha_g), under which eigenvectors of the
+ * sector block expand to eigenvectors of the Fock-space Hamiltonian (the two
+ * agree at k = 0 without twist).  Outputs the n_out nonzero amplitudes in
+ * ascending mask order; with masks or amps NULL only n_out is returned.
+ * A near-zero norm gives EDX_ERR_INVALID, as the reference's ValueError. */
+int edx_fock_expand(edx_ctx* ctx, const edx_catalog* cat, const edx_sector* sec, const double* chi,
+                    const double* coeff, double atol, int32_t convention, int64_t capacity, uint64_t* masks,
+                    double* amps, int64_t* n_out);
+/* Replaces _position_operator_matrix (ref observables/charge_pump.py:102-162):
+ * for each column of sec_from, rows[col] = row of the same representative in
+ * sec_to with vals[col] = (1/|G|) sum_g chi_to(g) conj(chi_from(g)) U(g.s),
+ * U(m) = prod over occupied sites of site_phases (n_site complex); rows[col]
+ * = -1 when the orbit is not in sec_to or |elem| <= 1e-13.  Both sectors come
+ * from cat. */
+int edx_position_operator(edx_ctx* ctx, const edx_catalog* cat, const edx_sector* sec_to,
+                          const edx_sector* sec_from, const double* chi_to, const double* chi_from,
+                          const double* site_phases, int64_t* rows, double* vals);
+/* Replaces the matrix loop of off_diagonal_long_range_order (ref
+ * observables/off_diagonal_long_range_order.py:80-140): rho (n_site x n_site
+ * complex, row-major) with rho[i][j] = <psi| a_(i+1)^dag a_(j+1) |psi> for the
+ * sector state coeff, using the table projection of project_to_sector
+ * (SRE:1660-1690) and the Jordan-Wigner sign of hopping_phase_for_stats. */
+int edx_one_body_density(edx_ctx* ctx, const edx_catalog* cat, const edx_sector* sec, const double* chi,
+                         const double* coeff, double* rho);
+
 /* ---- SpMV launch shape -------------------------------------------------------
  * Lanes of a wavefront that share one matrix row in the sliced-ELL copy of the
  * table (1, 2 or 4; slices hold 64 / lanes rows).  Setting it on a built

@@ -534,3 +534,131 @@ def sector_dims_closed_form(n_site: int, n_filled: int, perm, chis, fermionic: b
         d = np.sum(np.conj(np.asarray(chi)) * traces) / n_g
         dims.append(int(round(d.real)))
     return dims
+
+
+# ---------------------------------------------------------------------------
+# Eigenvector consumers  (observables/entanglement_spectrum.py:82-144,
+# observables/charge_pump.py:41-162, observables/off_diagonal_long_range_order.py:43-140)
+# ---------------------------------------------------------------------------
+
+def fock_expand(cat: Catalog, reps, stab, chi, c, atol: float = 1e-12, convention: str = "reference") -> dict:
+    """Sector eigenvector -> normalised Fock amplitudes ``{mask: amp}``
+    (entanglement_spectrum.py:82-144): for every basis column with
+    ``|c| > atol`` and every g ascending, add
+    ``(c / sqrt(|G| |Stab|)) * conj(chi(g)) * alpha_g`` at ``g.rep``; prune
+    ``|amp| <= atol``; renormalise.  ``convention="hamiltonian"`` uses
+    ``chi(g) * conj(alpha_g)`` instead.  Pinned: at k = 0 without twist both
+    give the Fock-space eigenvector of ``fock_hamiltonian`` with the sector
+    eigenvalue; for any sector and twist the "hamiltonian" expansion does; the
+    reference expansion of c is the conjugate of the "hamiltonian" expansion
+    of conj(c) (tests)."""
+    n_g = cat.perm.shape[0]
+    chi = np.asarray(chi, dtype=np.complex128)
+    amps: dict = {}
+    for col, s in enumerate(reps):
+        coeff = c[col]
+        if abs(coeff) <= atol:
+            continue
+        pref = coeff / np.sqrt(n_g * int(stab[col]))
+        for g in range(n_g):
+            m, ph = apply_op(int(s), cat.perm[g], cat.phases[g], cat.fermionic)
+            if convention == "reference":
+                amp = pref * np.conj(chi[g]) * ph
+            else:
+                amp = pref * chi[g] * np.conj(ph)
+            amps[m] = amps.get(m, 0.0 + 0.0j) + amp
+    for m in list(amps):
+        if abs(amps[m]) <= atol:
+            del amps[m]
+    norm2 = sum(abs(a) ** 2 for a in amps.values())
+    if norm2 <= atol:
+        raise ValueError("Expanded state has near-zero norm.")
+    inv = 1.0 / np.sqrt(norm2)
+    for m in amps:
+        amps[m] *= inv
+    return amps
+
+
+def position_phase(m: int, site_phases) -> complex:
+    """``prod over occupied sites (ascending) of site_phases`` (charge_pump.py:81-99)."""
+    z = 1.0 + 0.0j
+    while m:
+        low = m & -m
+        z *= site_phases[low.bit_length() - 1]
+        m ^= low
+    return z
+
+
+def position_operator(cat: Catalog, reps_to, reps_from, chi_to, chi_from, site_phases):
+    """``<chi_to| U |chi_from>`` as COO ``(rows, cols, vals)`` (charge_pump.py:102-162):
+    column ``col`` (rep s of the source basis) pairs with the row of s in the
+    target basis; ``elem = (1/|G|) sum_g chi_to(g) conj(chi_from(g)) U(g.s)``,
+    kept when ``|elem| > 1e-13``.  Pinned: at zero twist and ``U = 1`` it is the
+    identity on shared orbits of equal irreps and zero between different
+    irreps (character orthogonality), checked in the tests."""
+    n_g = cat.perm.shape[0]
+    where = {int(s): i for i, s in enumerate(reps_to)}
+    inv_ng = 1.0 / n_g
+    rows, cols, vals = [], [], []
+    for col, s in enumerate(reps_from):
+        row = where.get(int(s))
+        if row is None:
+            continue
+        elem = 0.0 + 0.0j
+        for g in range(n_g):
+            m, _ = apply_op(int(s), cat.perm[g], cat.phases[g], cat.fermionic)
+            elem += chi_to[g] * np.conj(chi_from[g]) * position_phase(m, site_phases)
+        elem *= inv_ng
+        if abs(elem) > 1e-13:
+            rows.append(row)
+            cols.append(col)
+            vals.append(elem)
+    return np.asarray(rows, np.int64), np.asarray(cols, np.int64), np.asarray(vals, np.complex128)
+
+
+def one_body_density_matrix(cat: Catalog, reps, stab, o2b, chi, c):
+    """``rho[i, j] = <psi| a_{i+1}^dag a_{j+1} |psi>`` of a sector state
+    (off_diagonal_long_range_order.py:80-140), restated with the table lookup
+    of project_to_sector (SRE:1660-1690) and the JW sign of
+    hopping_phase_for_stats (SRE:1777-1810).  Pinned: equals the one-body
+    density matrix of the Fock expansion of the same state (tests)."""
+    n = cat.n_site
+    n_g = cat.perm.shape[0]
+    chi = np.asarray(chi, dtype=np.complex128)
+    rho = np.zeros((n, n), dtype=np.complex128)
+    inv_ng = 1.0 / n_g
+    inv_sqrt_ng = 1.0 / np.sqrt(n_g)
+    for col, s in enumerate(reps):
+        c_col = c[col]
+        if c_col == 0:
+            continue
+        diag_w = abs(c_col) ** 2 * inv_ng
+        ket_norm = inv_sqrt_ng / np.sqrt(int(stab[col]))
+        for g in range(n_g):
+            km, kph = apply_op(int(s), cat.perm[g], cat.phases[g], cat.fermionic)
+            ket_amp = c_col * (ket_norm * np.conj(chi[g]) * kph)
+            tj = km
+            while tj:
+                lj = tj & -tj
+                j = lj.bit_length()           # 1-based
+                tj ^= lj
+                rho[j - 1, j - 1] += diag_w
+                for i in range(1, n + 1):
+                    if i == j or (km >> (i - 1)) & 1:
+                        continue
+                    nm = (km & ~(1 << (j - 1))) | (1 << (i - 1))
+                    idx = rank_colex(nm, cat.bt)
+                    o = int(cat.rep_rank[idx])
+                    row = int(o2b[o]) if o >= 0 else -1
+                    if row < 0:
+                        continue
+                    coeff = complex(cat.rep_amp[idx]) * np.conj(chi[int(cat.rep_sym[idx])])
+                    bra_amp = c[row] * (np.sqrt(int(stab[row])) * inv_sqrt_ng * np.conj(coeff))
+                    sign = 1.0
+                    if cat.fermionic:
+                        lo, hi = min(i, j), max(i, j)
+                        between = (((1 << (hi - lo - 1)) - 1) << lo) if hi - lo > 1 else 0
+                        if bin(km & between).count("1") % 2:
+                            sign = -1.0
+                    rho[i - 1, j - 1] += np.conj(bra_amp) * ket_amp * sign
+    return rho

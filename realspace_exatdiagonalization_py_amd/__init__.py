@@ -34,7 +34,10 @@ from .symmetry_resolved_ed import (
     load_checkpoint, populate_canonical_map, print_spectrum, project_to_sector, project_to_unnormalized_sector,
     rank_combination, run_concurrently, save_checkpoint, update_orbit_stabilizer_phases,
 )
-from . import models
+from . import models, observables
+from .observables import (
+    expand_sector_state_to_fock, many_body_position_phases, off_diagonal_long_range_order, one_body_density_matrix,
+)
 from .models import (
     CHECKERBOARD_TPP_DENOMINATOR, build_bosonic_fci_ed_data, build_fermionic_checkerboard_tb_model,
     build_heisenberg_ed_data, build_heisenberg_second_quantized_model, build_phase_explore_fermionic_checkerboard_model,

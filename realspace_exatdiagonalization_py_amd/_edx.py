@@ -74,6 +74,9 @@ SIGNATURES = {
     "edx_kernel_stats": (C.c_int, [P, i32, i32, P, P, P]),
     "edx_eigs_ex": (C.c_int, [P, P, P, P, P, P]),
     "edx_eigs_default_options": (None, [P]),
+    "edx_fock_expand": (C.c_int, [P, P, P, P, P, f64, i32, i64, P, P, pi64]),
+    "edx_position_operator": (C.c_int, [P, P, P, P, P, P, P, P, P]),
+    "edx_one_body_density": (C.c_int, [P, P, P, P, P, P]),
 }
 
 
