@@ -224,6 +224,9 @@ int edx_device_alloc(edx_ctx* ctx, uint64_t bytes, void** out);
 int edx_device_free(edx_ctx* ctx, void* p);
 int edx_memcpy_h2d(edx_ctx* ctx, void* dst, const void* src, uint64_t bytes);
 int edx_memcpy_d2h(edx_ctx* ctx, void* dst, const void* src, uint64_t bytes);
+/* Free and total device memory of the ctx's device (hipMemGetInfo); used by
+ * the serving loop to admit work by HBM footprint. */
+int edx_mem_info(edx_ctx* ctx, uint64_t* free_bytes, uint64_t* total_bytes);
 /* Device-to-device copy on the ctx stream (runtime blit kernel); a plain copy
  * of known size, used to calibrate memory counters. */
 int edx_device_copy(edx_ctx* ctx, void* dst, const void* src, uint64_t bytes);

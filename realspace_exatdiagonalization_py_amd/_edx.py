@@ -70,6 +70,7 @@ SIGNATURES = {
     "edx_memcpy_h2d": (C.c_int, [P, P, P, u64]),
     "edx_memcpy_d2h": (C.c_int, [P, P, P, u64]),
     "edx_device_copy": (C.c_int, [P, P, P, u64]),
+    "edx_mem_info": (C.c_int, [P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     "edx_time_matvec": (C.c_int, [P, P, i32, i32, i32, P, P, C.POINTER(C.c_double)]),
     "edx_kernel_stats": (C.c_int, [P, i32, i32, P, P, P]),
     "edx_eigs_ex": (C.c_int, [P, P, P, P, P, P]),
@@ -202,6 +203,13 @@ def kernel_stats(ctx: "Context", enable: int = -1, reset: bool = False) -> dict:
     b = np.zeros(k)
     ctx.check(lib.edx_kernel_stats(ctx.handle, enable, 1 if reset else 0, ptr(ms), ptr(n), ptr(b)))
     return {k: (float(ms[i]), int(n[i]), float(b[i])) for i, k in enumerate(KERNEL_CLASSES)}
+
+
+def mem_info(ctx: "Context") -> tuple[int, int]:
+    """(free, total) bytes of device memory."""
+    f, t = C.c_uint64(), C.c_uint64()
+    ctx.check(lib.edx_mem_info(ctx.handle, C.byref(f), C.byref(t)))
+    return int(f.value), int(t.value)
 
 
 def kernel_stats_all(enable: int = -1, reset: bool = False) -> dict:

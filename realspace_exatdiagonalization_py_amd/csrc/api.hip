@@ -145,6 +145,17 @@ int edx_memcpy_d2h(edx_ctx* ctx, void* dst, const void* src, uint64_t bytes) {
   });
 }
 
+int edx_mem_info(edx_ctx* ctx, uint64_t* free_bytes, uint64_t* total_bytes) {
+  return edx::guarded(ctx, [&] {
+    EDX_REQUIRE(ctx && free_bytes && total_bytes, EDX_ERR_INVALID, "null argument");
+    edx::DeviceGuard dg(ctx->device);
+    size_t f = 0, t = 0;
+    EDX_HIP(hipMemGetInfo(&f, &t));
+    *free_bytes = f;
+    *total_bytes = t;
+  });
+}
+
 int edx_device_copy(edx_ctx* ctx, void* dst, const void* src, uint64_t bytes) {
   return edx::guarded(ctx, [&] {
     EDX_REQUIRE(ctx && dst && src, EDX_ERR_INVALID, "null argument");

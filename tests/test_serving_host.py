@@ -1,0 +1,108 @@
+"""Host-side scheduling of the serving loop (no device): admission by
+footprint, bounded skip-ahead, error propagation, statistics."""
+
+from __future__ import annotations
+
+import threading
+import time
+
+import numpy as np
+import pytest
+
+from realspace_exatdiagonalization_py_amd.serving import SectorSolveServer, SolveRequest, poisson_arrivals, serve_stream
+
+
+class _Probe:
+    def __init__(self):
+        self.lock = threading.Lock()
+        self.in_use = 0
+        self.peak = 0
+        self.running = 0
+        self.peak_running = 0
+        self.order = []
+
+    def solve(self, server, req, ctx):
+        size, dur, fail = req.tag
+        with self.lock:
+            self.in_use += size
+            self.running += 1
+            self.peak = max(self.peak, self.in_use)
+            self.peak_running = max(self.peak_running, self.running)
+            self.order.append(req.sector)
+        time.sleep(dur)
+        with self.lock:
+            self.in_use -= size
+            self.running -= 1
+        if fail:
+            raise ValueError(f"sector {req.sector} failed")
+        return np.array([float(req.sector)]), None, 1, None
+
+
+def _server(probe, workers=4, budget=100, max_skip=8):
+    return SectorSolveServer(workers, hbm_budget=budget, catalog_cache_bytes=0, max_skip=max_skip,
+                             solve_fn=probe.solve, context_factory=lambda: None,
+                             estimate_fn=lambda r: r.tag[0])
+
+
+def _req(i, size, dur=0.01, fail=False):
+    return SolveRequest(model=None, n_filled=0, sector=i, tag=(size, dur, fail))
+
+
+def test_admission_respects_budget():
+    probe = _Probe()
+    with _server(probe, workers=6, budget=100) as srv:
+        futs = [srv.submit(_req(i, 30, 0.02)) for i in range(24)]
+        res = [f.result(timeout=30) for f in futs]
+    assert [r.eigenvalues[0] for r in res] == list(range(24))
+    assert probe.peak <= 100          # at most three 30-byte solves at once
+    assert probe.peak_running == 3
+    st = srv.stats()
+    assert st["completed"] == 24 and st["peak_admitted_bytes"] <= 100
+    assert st["p99_ms"] >= st["p50_ms"] > 0
+
+
+def test_oversized_request_runs_alone():
+    probe = _Probe()
+    with _server(probe, workers=4, budget=100) as srv:
+        futs = [srv.submit(_req(0, 500, 0.03))] + [srv.submit(_req(i, 10, 0.01)) for i in range(1, 6)]
+        for f in futs:
+            f.result(timeout=30)
+    assert probe.peak_running >= 1
+    assert probe.order[0] == 0
+
+
+def test_skip_ahead_is_bounded():
+    """A large head request lets at most max_skip smaller ones pass it."""
+    probe = _Probe()
+    with _server(probe, workers=2, budget=100, max_skip=2) as srv:
+        first = srv.submit(_req(0, 60, 0.05))        # occupies 60 of 100
+        time.sleep(0.01)
+        big = srv.submit(_req(1, 60, 0.01))          # does not fit while 0 runs
+        small = [srv.submit(_req(i, 10, 0.05)) for i in range(2, 8)]
+        for f in [first, big] + small:
+            f.result(timeout=30)
+    pos = probe.order.index(1)
+    passed = [s for s in probe.order[:pos] if s >= 2]
+    assert len(passed) <= 2
+
+
+def test_errors_reach_the_caller_and_the_loop_continues():
+    probe = _Probe()
+    with _server(probe, workers=2) as srv:
+        bad = srv.submit(_req(0, 10, 0.01, fail=True))
+        good = srv.submit(_req(1, 10, 0.01))
+        with pytest.raises(ValueError, match="sector 0 failed"):
+            bad.result(timeout=30)
+        assert good.result(timeout=30).eigenvalues[0] == 1.0
+
+
+def test_poisson_stream_and_shutdown():
+    probe = _Probe()
+    srv = _server(probe, workers=3)
+    arr = poisson_arrivals(10, 500.0, seed=1)
+    assert np.all(np.diff(arr) > 0)
+    res = serve_stream(srv, [_req(i, 10, 0.005) for i in range(10)], arr)
+    assert len(res) == 10 and all(r.latency >= r.service >= 0 for r in res)
+    srv.shutdown()
+    with pytest.raises(RuntimeError):
+        srv.submit(_req(99, 1))
