@@ -307,6 +307,38 @@ def build_catalog(n_site: int, n_filled: int, perm, phases, fermionic: bool,
     return cat
 
 
+def table_entry(m: int, n_site: int, perm, phases, fermionic: bool, tables=None):
+    """``(rep, g, alpha)`` of one mask as the catalog stores it, without the
+    catalog: rep = min image over the group, g = the largest element with
+    ``g.rep == m``, alpha = complex64 of the exp-form phase of g on rep
+    (byte-ordered angle sum, times -1 for an odd number of inversions).  For
+    spot checks of catalogs too large for ``build_catalog``.  Pinned: equals
+    ``build_catalog``'s tables on every mask of the small cases (tests)."""
+    perm = np.asarray(perm, dtype=np.int64)
+    n_g = perm.shape[0]
+    ptbl, atbl, n_bytes = tables if tables is not None else byte_tables(n_site, perm, np.asarray(phases))
+
+    def image(x, g):
+        out = 0
+        for b in range(n_bytes):
+            out |= int(ptbl[g, b, (x >> (8 * b)) & 0xFF])
+        return out
+
+    rep = min(image(m, g) for g in range(n_g))
+    g_tab = max(g for g in range(n_g) if image(rep, g) == m)
+    ang = 0.0
+    for b in range(n_bytes):
+        ang += atbl[g_tab, b, (rep >> (8 * b)) & 0xFF]
+    a = np.exp(1j * np.float64(ang))
+    if fermionic:
+        occ = [i for i in range(n_site) if (rep >> i) & 1]
+        n_inv = sum(1 for x in range(len(occ)) for y in range(x + 1, len(occ))
+                    if perm[g_tab, occ[x]] > perm[g_tab, occ[y]])
+        if n_inv & 1:
+            a = a * (-1.0 + 0.0j)
+    return rep, g_tab, np.complex64(a)
+
+
 def update_orbit_stabilizer_phases(cat: Catalog, new_phases, fermionic: bool) -> Catalog:
     """In-place flux refresh (SRE:1220-1269): stabilizer phases and table
     amplitudes recomputed as running products (``apply_op``) for the group with

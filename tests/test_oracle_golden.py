@@ -386,3 +386,21 @@ def test_closed_form_dims_all_sectors():
             assert min(dims) == 225264 and max(dims) == 225440 and dims[0] == 225432
         if name == "heis16":
             assert dims[0] == 810 and sum(dims) == math.comb(16, 8)
+
+
+@pytest.mark.parametrize("name", ["hal23", "cb23_twist", "hub23", "hal23_twist"])
+def test_table_entry_matches_catalog(name):
+    """The one-mask restatement used for spot checks of large catalogs agrees
+    with the full catalog on every mask."""
+    model, ne, G = make_case(name)
+    perm, ph = Gm.group_arrays(G)
+    N = model.lattice.n_site
+    ferm = is_fermionic(model)
+    cat = O.build_catalog(N, ne, perm, ph, ferm)
+    tables = O.byte_tables(N, np.asarray(perm, dtype=np.int64), np.asarray(ph))
+    masks = O.unrank_chunk(np.arange(cat.rep_rank.shape[0]), N, ne, cat.bt)
+    for r, m in enumerate(masks):
+        rep, g, a = O.table_entry(int(m), N, perm, ph, ferm, tables)
+        assert rep == int(cat.reps[cat.rep_rank[r]])
+        assert g == int(cat.rep_sym[r])
+        assert a == cat.rep_amp[r]
