@@ -4,7 +4,9 @@
  * Plain C types only.  Every entry point returns an int status (EDX_OK or a
  * negative EDX_ERR_* code) and records a message retrievable with
  * edx_last_error(ctx).  Device memory belongs to the objects created here;
- * host buffers passed in belong to the caller.  Calls on one ctx are
+ * host buffers passed in belong to the caller.  Device memory is allocated
+ * stream-ordered on the ctx's stream, so an object must be destroyed before
+ * the ctx it was created on.  Calls on one ctx are
  * serialised on that ctx's HIP stream; different ctxs may be used from
  * different threads.  "ref" citations are lines of the reference file
  * src/realspace_exactdiagonalization_py/symmetry_resolved_ed.py (SRE) unless

@@ -30,12 +30,24 @@ int edx_ctx_create(int32_t device, edx_ctx** out) {
     EDX_REQUIRE(n > 0, EDX_ERR_HIP, "no HIP device is visible");
     EDX_REQUIRE(device >= 0 && device < n, EDX_ERR_INVALID, "device index out of range");
     edx::DeviceGuard dg(device);
+    // freed stream-ordered memory stays in the device pool for reuse
+    hipMemPool_t pool;
+    if (hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess) {
+      uint64_t keep = UINT64_MAX;
+      (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+    }
+    (void)hipGetLastError();
     auto ctx = new edx_ctx();
     ctx->device = device;
     hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&ctx->pinned_in, edx::kPinnedIn * sizeof(double2), hipHostMallocDefault);
+    if (e == hipSuccess)
+      e = hipHostMalloc((void**)&ctx->pinned_out, edx::kPinnedOut * sizeof(double2), hipHostMallocDefault);
     if (e != hipSuccess) {
+      if (ctx->pinned_in) (void)hipHostFree(ctx->pinned_in);
+      if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
       delete ctx;
-      throw edx::Error{EDX_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e)};
+      throw edx::Error{EDX_ERR_HIP, std::string("context setup: ") + hipGetErrorString(e)};
     }
     *out = ctx;
   });
@@ -50,6 +62,8 @@ int edx_ctx_destroy(edx_ctx* ctx) {
     (void)hipStreamSynchronize(ctx->stream);
     (void)hipStreamDestroy(ctx->stream);
   }
+  if (ctx->pinned_in) (void)hipHostFree(ctx->pinned_in);
+  if (ctx->pinned_out) (void)hipHostFree(ctx->pinned_out);
   delete ctx;
   (void)hipSetDevice(prev);
   return EDX_OK;

@@ -25,6 +25,8 @@ COMMON = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
     # element formulas keep the reference's separate roundings; fused ops are explicit fma()
     "-ffp-contract=off",
+    # host dense algebra (std::complex) multiplies without the inf/NaN rescue call
+    "-fcx-limited-range",
     # no packed FP32 instructions in generated code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-command-line-argument",

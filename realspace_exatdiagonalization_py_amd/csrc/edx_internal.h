@@ -39,28 +39,52 @@ struct Error {
     if (!(cond)) throw ::edx::Error{(code), (text)};                           \
   } while (0)
 
-// Device allocation owned by an object; freed in the destructor.
+// Stream of the API call running on this thread (set by guarded() from the
+// call's ctx).  Device buffers allocated during a call are stream-ordered on
+// it: hipMallocAsync / hipFreeAsync, so that freeing memory does not
+// synchronise the whole device (hipFree does) and streams of other contexts
+// keep running.
+inline hipStream_t& tls_stream() {
+  thread_local hipStream_t s = nullptr;
+  return s;
+}
+
+struct StreamScope {
+  hipStream_t prev;
+  explicit StreamScope(hipStream_t s) : prev(tls_stream()) { tls_stream() = s; }
+  ~StreamScope() { tls_stream() = prev; }
+};
+
+// Device allocation owned by an object; freed in the destructor, on the
+// stream it was allocated on (that stream's context must outlive the buffer).
 template <class T>
 struct DevBuf {
   T* p = nullptr;
   size_t n = 0;
+  hipStream_t st = nullptr;
   DevBuf() = default;
   explicit DevBuf(size_t count) { alloc(count); }
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
-  DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+  DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n), st(o.st) { o.p = nullptr; o.n = 0; }
   DevBuf& operator=(DevBuf&& o) noexcept {
-    if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+    if (this != &o) { release(); p = o.p; n = o.n; st = o.st; o.p = nullptr; o.n = 0; }
     return *this;
   }
   ~DevBuf() { release(); }
   void alloc(size_t count) {
     release();
     n = count;
-    if (count) EDX_HIP(hipMalloc(&p, count * sizeof(T)));
+    st = tls_stream();
+    if (!count) return;
+    if (st) EDX_HIP(hipMallocAsync((void**)&p, count * sizeof(T), st));
+    else EDX_HIP(hipMalloc(&p, count * sizeof(T)));
   }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p) {
+      if (st) (void)hipFreeAsync(p, st);
+      else (void)hipFree(p);
+    }
     p = nullptr;
     n = 0;
   }
@@ -142,7 +166,16 @@ struct Ctx {
   std::string last_error;
   int autotune_group = 0;  // lanes per row in the SpMV (0 = pick by heuristics)
   KernelTimer timer;
+  // pinned host staging owned by the context (allocated once, reused by
+  // every solve on it): projection coefficients device -> host, and the
+  // ring of update coefficients host -> device
+  double2* pinned_in = nullptr;
+  double2* pinned_out = nullptr;
 };
+
+// Sizes of the context's pinned staging buffers (double2 elements).
+constexpr size_t kPinnedIn = (128 + 16) * 16;
+constexpr size_t kPinnedOut = 4 * (128 + 16) * 16;
 
 // Scoped device selection for API calls.
 struct DeviceGuard {

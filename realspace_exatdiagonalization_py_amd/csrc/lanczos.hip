@@ -43,7 +43,7 @@ void dense_matrix(const edx_hamiltonian& h, hipStream_t st, double2* out);
 
 namespace {
 
-constexpr int kMaxVec = 64;      // basis vectors the Gram/update kernels accept (64*16 outputs)
+constexpr int kMaxVec = 128;     // basis vectors the Gram/update kernels accept
 constexpr int kMaxW = 16;        // block width
 
 __device__ __forceinline__ double2 to_d2(double2 v) { return v; }
@@ -286,7 +286,7 @@ struct Options {
   int basis = 0;  // max Ritz basis size (0: automatic)
   double tol = 1e-10;     // relative residual ||H x - theta x|| / max(1, |theta|)
   double eig_tol = 0.0;   // > 0: a pair is also done when its cluster's sum r^2 / gap <= eig_tol
-  int max_restarts = 400;
+  int max_restarts = 5000;
   int check_every = 4;    // Ritz check every this many block steps (and when the basis is full)
   uint64_t seed = 1;
 };
@@ -367,12 +367,9 @@ class BlockLanczos {
     partial_.alloc((size_t)n_rowblocks_ * kMaxOut);
     cdev_.alloc((size_t)kMaxOut);
     sdev_.alloc((size_t)kSlots * kSlot);
-    EDX_HIP(hipHostMalloc((void**)&hc_, sizeof(double2) * kMaxOut, hipHostMallocDefault));
-    EDX_HIP(hipHostMalloc((void**)&hs_, sizeof(double2) * kSlots * kSlot, hipHostMallocDefault));
-  }
-  ~BlockLanczos() {
-    if (hc_) (void)hipHostFree(hc_);
-    if (hs_) (void)hipHostFree(hs_);
+    static_assert(kMaxOut <= edx::kPinnedIn && kSlots * kSlot <= edx::kPinnedOut, "pinned staging too small");
+    hc_ = ctx.pinned_in;
+    hs_ = ctx.pinned_out;
   }
   BlockLanczos(const BlockLanczos&) = delete;
   BlockLanczos& operator=(const BlockLanczos&) = delete;
@@ -819,7 +816,7 @@ void edx_eigs_default_options(edx_eigs_options* o) {
   o->block = 0;
   o->basis = 0;
   o->precision = EDX_PREC_FP64;
-  o->max_restarts = 400;
+  o->max_restarts = 5000;
   o->tol = 1e-10;
   o->eig_tol = 0.0;
   o->mixed_tol = 1e-5;

@@ -78,6 +78,7 @@ inline void set_error(edx_ctx* ctx, const std::string& msg);
 template <class F>
 int guarded(edx_ctx* ctx, F&& f) {
   try {
+    StreamScope scope(ctx ? ctx->stream : nullptr);
     f();
     return EDX_OK;
   } catch (const Error& e) {
