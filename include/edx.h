@@ -4,9 +4,9 @@
  * Plain C types only.  Every entry point returns an int status (EDX_OK or a
  * negative EDX_ERR_* code) and records a message retrievable with
  * edx_last_error(ctx).  Device memory belongs to the objects created here;
- * host buffers passed in belong to the caller.  Device memory is allocated
- * stream-ordered on the ctx's stream, so an object must be destroyed before
- * the ctx it was created on.  Calls on one ctx are
+ * host buffers passed in belong to the caller.  Work areas inside a call are
+ * allocated stream-ordered on the ctx's stream and freed before the call
+ * returns; arrays of the objects use ordinary device allocations.  Calls on one ctx are
  * serialised on that ctx's HIP stream; different ctxs may be used from
  * different threads.  "ref" citations are lines of the reference file
  * src/realspace_exactdiagonalization_py/symmetry_resolved_ed.py (SRE) unless
@@ -149,7 +149,7 @@ int edx_eigs(edx_ctx* ctx, const edx_hamiltonian* ham, int32_t nev, int32_t bloc
  *   eig_tol  > 0: a pair also counts as converged once the residual-squared-
  *            over-gap bound of its eigenvalue cluster is below eig_tol (the
  *            eigenvalue is then accurate to eig_tol; the vector to ~sqrt)
- *   mixed_tol residual at which EDX_PREC_MIXED hands over to FP64
+ *   mixed_tol residual at which EDX_PREC_MIXED hands over to FP64 (default 3e-6)
  *   basis    Ritz basis size (0: automatic), block as in edx_eigs */
 typedef struct edx_eigs_options {
   int32_t nev;

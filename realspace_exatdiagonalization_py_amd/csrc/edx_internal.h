@@ -55,9 +55,13 @@ struct StreamScope {
   ~StreamScope() { tls_stream() = prev; }
 };
 
-// Device allocation owned by an object; freed in the destructor, on the
-// stream it was allocated on (that stream's context must outlive the buffer).
-template <class T>
+// Device allocation freed in the destructor.  With Ordered (the default) a
+// buffer allocated during an API call is stream-ordered on that call's
+// stream; such buffers are used for work areas that are freed before the
+// call returns.  Arrays owned by the long-lived C-ABI objects (ObjBuf below)
+// use plain hipMalloc / hipFree, so they never refer to a stream that may be
+// destroyed before them.
+template <class T, bool Ordered = true>
 struct DevBuf {
   T* p = nullptr;
   size_t n = 0;
@@ -75,7 +79,7 @@ struct DevBuf {
   void alloc(size_t count) {
     release();
     n = count;
-    st = tls_stream();
+    st = Ordered ? tls_stream() : nullptr;
     if (!count) return;
     if (st) EDX_HIP(hipMallocAsync((void**)&p, count * sizeof(T), st));
     else EDX_HIP(hipMalloc(&p, count * sizeof(T)));
@@ -90,6 +94,9 @@ struct DevBuf {
   }
   size_t bytes() const { return n * sizeof(T); }
 };
+
+template <class T>
+using ObjBuf = DevBuf<T, false>;
 
 // Live per-class kernel timing with HIP events on the launch stream
 // (enabled on request; used by benchmarks to time kernels inside a solve).

@@ -343,7 +343,7 @@ class BlockLanczos {
     b_ = opt.block;
     esz_ = prec == EDX_PREC_FP64 ? 16 : 8;
     ld_ = (D_ + 63) / 64 * 64;
-    int base = opt.basis > 0 ? opt.basis : std::max({4 * (opt.nev + b_), 6 * b_, 24});
+    int base = opt.basis > 0 ? opt.basis : std::max({5 * (opt.nev + b_), 6 * b_, 24});
     base = (base + b_ - 1) / b_ * b_;
     nq_max_ = std::min(base, kMaxVec - 2 * b_);
     nq_max_ = std::max(nq_max_ / b_ * b_, opt.nev + b_);
@@ -819,7 +819,7 @@ void edx_eigs_default_options(edx_eigs_options* o) {
   o->max_restarts = 5000;
   o->tol = 1e-10;
   o->eig_tol = 0.0;
-  o->mixed_tol = 1e-5;
+  o->mixed_tol = 3e-6;
   o->seed = 12345;
 }
 
@@ -880,7 +880,7 @@ void eigs_impl(edx_ctx* ctx, const edx_hamiltonian* ham, const edx_eigs_options&
       const int n32 = (int)std::min<int64_t>(std::min(nev, kMaxW), D - 2);
       Options o32 = opt;
       o32.nev = n32;
-      o32.tol = std::max(tol, o.mixed_tol > 0 ? o.mixed_tol : 1e-5);
+      o32.tol = std::max(tol, o.mixed_tol > 0 ? o.mixed_tol : 3e-6);
       o32.eig_tol = 0.0;
       BlockLanczos lz32(*ctx, *ham, EDX_PREC_MIXED, o32);
       Result r32 = lz32.run(nullptr, 0, 0);

@@ -13,19 +13,19 @@ struct edx_catalog {
   int n_site = 0, n_filled = 0, n_g = 0, fermionic = 0, product_phase = 0;
   int64_t n_total = 0, n_orbits = 0, n_stab = 0;
   std::vector<int32_t> h_perm;        // host copy (1-based) to validate phase updates
-  edx::DevBuf<uint8_t> perm;          // [n_g][n_site] 0-based targets
-  edx::DevBuf<double> angle;          // [n_g][n_site]
-  edx::DevBuf<double2> phase;         // [n_g][n_site]
-  edx::DevBuf<int32_t> inv;           // [n_g]
-  edx::DevBuf<uint32_t> binom;        // [(n_site+1)][(n_filled+1)]
-  edx::DevBuf<int32_t> rep_rank;      // [C] orbit index of the mask of that rank
-  edx::DevBuf<int32_t> rep_sym;       // [C] largest g with g.rep == mask
-  edx::DevBuf<float2> rep_amp;        // [C] complex64 alpha_g(rep)
-  edx::DevBuf<uint64_t> reps;         // [n_orbits] ascending
-  edx::DevBuf<int32_t> stab_order;    // [n_orbits]
-  edx::DevBuf<uint32_t> stab_off;     // [n_orbits + 1]
-  edx::DevBuf<uint8_t> stab_g;        // [n_stab]
-  edx::DevBuf<double2> stab_phase;    // [n_stab]
+  edx::ObjBuf<uint8_t> perm;          // [n_g][n_site] 0-based targets
+  edx::ObjBuf<double> angle;          // [n_g][n_site]
+  edx::ObjBuf<double2> phase;         // [n_g][n_site]
+  edx::ObjBuf<int32_t> inv;           // [n_g]
+  edx::ObjBuf<uint32_t> binom;        // [(n_site+1)][(n_filled+1)]
+  edx::ObjBuf<int32_t> rep_rank;      // [C] orbit index of the mask of that rank
+  edx::ObjBuf<int32_t> rep_sym;       // [C] largest g with g.rep == mask
+  edx::ObjBuf<float2> rep_amp;        // [C] complex64 alpha_g(rep)
+  edx::ObjBuf<uint64_t> reps;         // [n_orbits] ascending
+  edx::ObjBuf<int32_t> stab_order;    // [n_orbits]
+  edx::ObjBuf<uint32_t> stab_off;     // [n_orbits + 1]
+  edx::ObjBuf<uint8_t> stab_g;        // [n_stab]
+  edx::ObjBuf<double2> stab_phase;    // [n_stab]
 
   edx::GroupView view() const {
     return edx::GroupView{n_site, n_filled, n_g, fermionic, product_phase, perm.p, angle.p,
@@ -37,9 +37,9 @@ struct edx_sector {
   int device = 0;
   const edx_catalog* cat = nullptr;
   int64_t dim = 0;
-  edx::DevBuf<uint64_t> reps;         // [dim]
-  edx::DevBuf<int32_t> stab_order;    // [dim]
-  edx::DevBuf<int32_t> orbit_to_basis;  // [n_orbits]
+  edx::ObjBuf<uint64_t> reps;         // [dim]
+  edx::ObjBuf<int32_t> stab_order;    // [dim]
+  edx::ObjBuf<int32_t> orbit_to_basis;  // [n_orbits]
 };
 
 struct edx_hamiltonian {
@@ -48,20 +48,20 @@ struct edx_hamiltonian {
   int64_t dim = 0;
   int64_t n_entries = 0;
   int32_t n_bilinear = 0;
-  edx::DevBuf<int64_t> row_ptr;       // [dim + 1]
-  edx::DevBuf<int32_t> col;           // [n_entries]
-  edx::DevBuf<uint16_t> term;         // [n_entries] original term index
-  edx::DevBuf<double2> val;           // [n_entries] complex128
-  edx::DevBuf<double2> hdiag;         // [dim]
-  edx::DevBuf<float2> hdiag32;        // [dim]
+  edx::ObjBuf<int64_t> row_ptr;       // [dim + 1]
+  edx::ObjBuf<int32_t> col;           // [n_entries]
+  edx::ObjBuf<uint16_t> term;         // [n_entries] original term index
+  edx::ObjBuf<double2> val;           // [n_entries] complex128
+  edx::ObjBuf<double2> hdiag;         // [dim]
+  edx::ObjBuf<float2> hdiag32;        // [dim]
   // sliced-ELL copy used by the SpMM: slices of 64 / sell_g rows, sell_g lanes per row
   int sell_g = 1;
   int64_t sell_slices = 0;
   int64_t sell_entries = 0;           // stored entries including padding
-  edx::DevBuf<int64_t> sell_off;      // [sell_slices + 1]
-  edx::DevBuf<int32_t> sell_col;      // [sell_entries], -1 = padding
-  edx::DevBuf<double2> sell_val;      // [sell_entries]
-  edx::DevBuf<float2> sell_val32;     // [sell_entries] complex64 copy
+  edx::ObjBuf<int64_t> sell_off;      // [sell_slices + 1]
+  edx::ObjBuf<int32_t> sell_col;      // [sell_entries], -1 = padding
+  edx::ObjBuf<double2> sell_val;      // [sell_entries]
+  edx::ObjBuf<float2> sell_val32;     // [sell_entries] complex64 copy
 };
 
 namespace edx {

@@ -81,13 +81,13 @@ def estimate_footprint(n_site: int, n_filled: int, n_group: int, n_bilinear: int
     D * T * p(1-p) N/(N-1) with p = Ne/N (a hop needs an occupied source and
     an empty target), projection table in CSR (22 B/entry) and sliced copy
     (28 B/entry, 20 % padding), 8 B/entry of sort keys during the build, and
-    two Krylov banks of 4(nev+b)+2b complex128 vectors."""
+    two Krylov banks of 5(nev+b)+2b complex128 vectors."""
     C = math.comb(n_site, n_filled)
     D = max(1, C // max(n_group, 1))
     p = n_filled / n_site if n_site else 0.0
     E = int(D * n_bilinear * p * (1.0 - p) * n_site / max(n_site - 1, 1))
     b = min(nev, 8)
-    nvec = 4 * (nev + b) + 2 * b
+    nvec = 5 * (nev + b) + 2 * b
     return int(E * (22 + 1.2 * 28 + 8) + D * (2 * nvec * 16 + 64))
 
 
