@@ -45,6 +45,7 @@ namespace {
 
 constexpr int kMaxVec = 128;     // basis vectors the Gram/update kernels accept
 constexpr int kMaxW = 16;        // block width
+constexpr int kMaxCombine = 48;  // outputs of one restart-combine pass
 
 __device__ __forceinline__ double2 to_d2(double2 v) { return v; }
 __device__ __forceinline__ double2 to_d2(float2 v) { return make_double2((double)v.x, (double)v.y); }
@@ -188,19 +189,21 @@ __global__ void __launch_bounds__(256) gram_reduce(int nblk, int nout, const dou
   if (o < nout && lane == 0) out[o] = make_double2(xr, xi);
 }
 
-// Out_k[r] = sum_i A_i[r] S[i][k] for k < nk <= 16 (S row-major, leading
+// Out_k[r] = sum_i A_i[r] S[i][k] for k < nk <= NKM (S row-major, leading
 // dimension ldS), FP64 accumulation.  Every input of row r is read before
 // any output of row r is written, so Out may be one of the A arrays (same
 // leading dimension).  Xi (optional) receives the interleaved copy [r][k].
-template <typename V>
+// NKM = 16 serves the block updates; the restart combine uses up to 48
+// outputs per pass so that the basis is read once.
+template <typename V, int NKM>
 __global__ void __launch_bounds__(256)
 gemm_pair(int64_t D, VecPair<V> A, int nk, V* Ob, int64_t ldo, const double2* __restrict__ S, int ldS, V* Xi) {
   const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (r >= D) return;
   const int nv = A.n0 + A.n1;
-  double xr[kMaxW], xi[kMaxW];
+  double xr[NKM], xi[NKM];
 #pragma unroll
-  for (int k = 0; k < kMaxW; ++k) xr[k] = xi[k] = 0.0;
+  for (int k = 0; k < NKM; ++k) xr[k] = xi[k] = 0.0;
   constexpr int U = sizeof(V) == 8 ? 8 : 1;  // complex64: eight loads in flight per thread
   int i = 0;
   for (; i + U <= nv; i += U) {
@@ -211,7 +214,7 @@ gemm_pair(int64_t D, VecPair<V> A, int nk, V* Ob, int64_t ldo, const double2* __
     for (int u = 0; u < U; ++u) {
       const double2* srow = S + (int64_t)(i + u) * ldS;
 #pragma unroll
-      for (int k = 0; k < kMaxW; ++k) {
+      for (int k = 0; k < NKM; ++k) {
         if (k < nk) {
           const double2 s = srow[k];
           xr[k] = fma(v[u].x, s.x, xr[k]);
@@ -226,7 +229,7 @@ gemm_pair(int64_t D, VecPair<V> A, int nk, V* Ob, int64_t ldo, const double2* __
     const double2 v = to_d2(A.vec(i)[r]);
     const double2* srow = S + (int64_t)i * ldS;
 #pragma unroll
-    for (int k = 0; k < kMaxW; ++k) {
+    for (int k = 0; k < NKM; ++k) {
       if (k < nk) {
         const double2 s = srow[k];
         xr[k] = fma(v.x, s.x, xr[k]);
@@ -237,7 +240,7 @@ gemm_pair(int64_t D, VecPair<V> A, int nk, V* Ob, int64_t ldo, const double2* __
     }
   }
 #pragma unroll
-  for (int k = 0; k < kMaxW; ++k) {
+  for (int k = 0; k < NKM; ++k) {
     if (k < nk) {
       const V o = from_d2<V>(xr[k], xi[k]);
       Ob[(int64_t)k * ldo + r] = o;
@@ -547,32 +550,42 @@ class BlockLanczos {
     launch_gemm(A0, n0, A1, n1, Out, nk, ds, nk, Xi);
   }
 
+  template <typename VT, int NKM>
+  void launch_gemm_t(const void* A0, int n0, const void* A1, int n1, void* Out, int nk, const double2* S, int ldS,
+                     void* Xi) {
+    VecPair<VT> A{(const VT*)A0, ld_, n0, (const VT*)A1, ld_, n1};
+    hipLaunchKernelGGL((gemm_pair<VT, NKM>), dim3(edx::grid_for(D_, 256)), dim3(256), 0, ctx_.stream, D_, A, nk,
+                       (VT*)Out, ld_, S, ldS, (VT*)Xi);
+  }
+
+  template <typename VT>
+  void launch_gemm_v(const void* A0, int n0, const void* A1, int n1, void* Out, int nk, const double2* S, int ldS,
+                     void* Xi) {
+    if (nk <= 16) launch_gemm_t<VT, 16>(A0, n0, A1, n1, Out, nk, S, ldS, Xi);
+    else if (nk <= 32) launch_gemm_t<VT, 32>(A0, n0, A1, n1, Out, nk, S, ldS, Xi);
+    else launch_gemm_t<VT, kMaxCombine>(A0, n0, A1, n1, Out, nk, S, ldS, Xi);
+  }
+
   void launch_gemm(const void* A0, int n0, const void* A1, int n1, void* Out, int nk, const double2* S, int ldS,
                    void* Xi) {
+    EDX_REQUIRE(nk >= 1 && nk <= kMaxCombine, EDX_ERR_INTERNAL, "update width");
     hipEvent_t t0 = ctx_.timer.start(ctx_.stream);
-    const dim3 grid(edx::grid_for(D_, 256));
-    if (prec_ == EDX_PREC_FP64) {
-      VecPair<double2> A{(const double2*)A0, ld_, n0, (const double2*)A1, ld_, n1};
-      hipLaunchKernelGGL(gemm_pair<double2>, grid, dim3(256), 0, ctx_.stream, D_, A, nk, (double2*)Out, ld_, S,
-                         ldS, (double2*)Xi);
-    } else {
-      VecPair<float2> A{(const float2*)A0, ld_, n0, (const float2*)A1, ld_, n1};
-      hipLaunchKernelGGL(gemm_pair<float2>, grid, dim3(256), 0, ctx_.stream, D_, A, nk, (float2*)Out, ld_, S,
-                         ldS, (float2*)Xi);
-    }
+    if (prec_ == EDX_PREC_FP64) launch_gemm_v<double2>(A0, n0, A1, n1, Out, nk, S, ldS, Xi);
+    else launch_gemm_v<float2>(A0, n0, A1, n1, Out, nk, S, ldS, Xi);
     EDX_HIP(hipGetLastError());
     ctx_.timer.stop(ctx_.stream, t0, edx::K_GEMM, (double)(n0 + n1 + (Xi ? 2 : 1) * nk) * D_ * esz_);
   }
 
-  // Out[:, 0..nk) = V[:, 0..nv) Y (Out must not alias V), tiled by 16 columns.
+  // Out[:, 0..nk) = V[:, 0..nv) Y (Out must not alias V), up to kMaxCombine
+  // columns per pass.
   void combine(const void* V, int nv, void* Out, int nk, const Mat& Y) {
     std::vector<double2> hs((size_t)nv * nk);
     for (int i = 0; i < nv; ++i)
       for (int k = 0; k < nk; ++k) hs[(size_t)i * nk + k] = make_double2(Y(i, k).real(), Y(i, k).imag());
     DevBuf<double2> S(hs.size());
     EDX_HIP(hipMemcpyAsync(S.p, hs.data(), hs.size() * sizeof(double2), hipMemcpyHostToDevice, ctx_.stream));
-    for (int k0 = 0; k0 < nk; k0 += kMaxW) {
-      const int nkt = std::min(kMaxW, nk - k0);
+    for (int k0 = 0; k0 < nk; k0 += kMaxCombine) {
+      const int nkt = std::min(kMaxCombine, nk - k0);
       launch_gemm(V, nv, nullptr, 0, static_cast<char*>(Out) + (size_t)k0 * ld_ * esz_, nkt, S.p + k0, nk,
                   nullptr);
     }
