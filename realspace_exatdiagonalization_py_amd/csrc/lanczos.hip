@@ -45,7 +45,6 @@ namespace {
 
 constexpr int kMaxVec = 128;     // basis vectors the Gram/update kernels accept
 constexpr int kMaxW = 16;        // block width
-constexpr int kMaxCombine = 48;  // outputs of one restart-combine pass
 
 __device__ __forceinline__ double2 to_d2(double2 v) { return v; }
 __device__ __forceinline__ double2 to_d2(float2 v) { return make_double2((double)v.x, (double)v.y); }
@@ -189,62 +188,105 @@ __global__ void __launch_bounds__(256) gram_reduce(int nblk, int nout, const dou
   if (o < nout && lane == 0) out[o] = make_double2(xr, xi);
 }
 
-// Out_k[r] = sum_i A_i[r] S[i][k] for k < nk <= NKM (S row-major, leading
-// dimension ldS), FP64 accumulation.  Every input of row r is read before
-// any output of row r is written, so Out may be one of the A arrays (same
-// leading dimension).  Xi (optional) receives the interleaved copy [r][k].
-// NKM = 16 serves the block updates; the restart combine uses up to 48
-// outputs per pass so that the basis is read once.
-template <typename V, int NKM>
+// Rows a thread of gemm_tiles owns: complex64 rows two at a time (one 16-byte
+// load per lane), complex128 rows one at a time.
+template <typename V>
+struct RowGroup;
+template <>
+struct RowGroup<float2> {
+  static constexpr int RG = 2;
+  __device__ __forceinline__ static void load(const float2* p, int64_t r, int64_t D, double2* out) {
+    if (r + 1 < D) {
+      const float4 q = *reinterpret_cast<const float4*>(p + r);
+      out[0] = make_double2(q.x, q.y);
+      out[1] = make_double2(q.z, q.w);
+    } else {
+      out[0] = r < D ? to_d2(p[r]) : make_double2(0.0, 0.0);
+      out[1] = make_double2(0.0, 0.0);
+    }
+  }
+};
+template <>
+struct RowGroup<double2> {
+  static constexpr int RG = 1;
+  __device__ __forceinline__ static void load(const double2* p, int64_t r, int64_t D, double2* out) {
+    out[0] = r < D ? p[r] : make_double2(0.0, 0.0);
+  }
+};
+
+// Out_k[r] = sum_i A_i[r] S[i][k] for k < nk (S row-major, leading dimension
+// ldS), FP64 accumulation.  One workgroup per (row block of 256 * RG rows,
+// tile of NK outputs); the 1-D grid is mapped so that the tiles of a row block
+// run on one XCD, whose L2 then serves the block's inputs to all of them.  The
+// tile's coefficients are staged in LDS (every lane reads the same entry, a
+// broadcast), and the loads of U inputs are issued before their FMAs.  Rows
+// are stored at r < D only, and vectors are 16-byte aligned with an even
+// leading dimension, so the paired complex64 loads stay inside each vector.
+// With one tile (nk <= NK) every input of row r is read before any output of
+// row r is written, so Out may be one of the A arrays (same leading
+// dimension); Xi (optional, one tile only) receives the interleaved copy
+// [r][k].
+template <typename V, int NK, int U>
 __global__ void __launch_bounds__(256)
-gemm_pair(int64_t D, VecPair<V> A, int nk, V* Ob, int64_t ldo, const double2* __restrict__ S, int ldS, V* Xi) {
-  const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (r >= D) return;
+gemm_tiles(int64_t D, VecPair<V> A, int nk, V* Ob, int64_t ldo, const double2* __restrict__ S, int ldS, V* Xi,
+           int n_kt, int n_tiles) {
+  extern __shared__ double2 s_tile[];  // [nv][NK]
+  constexpr int RG = RowGroup<V>::RG;
+  const int per_xcd = (n_tiles + 7) / 8;
+  const int logical = (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);
+  if (logical >= n_tiles) return;  // whole workgroup
+  const int kt = logical % n_kt, rb = logical / n_kt;
+  const int k0 = kt * NK;
   const int nv = A.n0 + A.n1;
-  double xr[NKM], xi[NKM];
+  for (int e = threadIdx.x; e < nv * NK; e += 256) {
+    const int i = e / NK, k = e % NK;
+    s_tile[e] = k0 + k < nk ? S[(int64_t)i * ldS + k0 + k] : make_double2(0.0, 0.0);
+  }
+  __syncthreads();
+  const int64_t r0 = (int64_t)rb * 256 * RG + (int64_t)threadIdx.x * RG;
+  double pr[RG][NK], pi[RG][NK];
 #pragma unroll
-  for (int k = 0; k < NKM; ++k) xr[k] = xi[k] = 0.0;
-  constexpr int U = sizeof(V) == 8 ? 8 : 1;  // complex64: eight loads in flight per thread
-  int i = 0;
-  for (; i + U <= nv; i += U) {
-    double2 v[U];
+  for (int q = 0; q < RG; ++q)
 #pragma unroll
-    for (int u = 0; u < U; ++u) v[u] = to_d2(A.vec(i + u)[r]);
+    for (int k = 0; k < NK; ++k) pr[q][k] = pi[q][k] = 0.0;
+  for (int i0 = 0; i0 < nv; i0 += U) {
+    double2 a[U][RG];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const double2* srow = S + (int64_t)(i + u) * ldS;
+      if (i0 + u < nv) {
+        RowGroup<V>::load(A.vec(i0 + u), r0, D, a[u]);
+      } else {
 #pragma unroll
-      for (int k = 0; k < NKM; ++k) {
-        if (k < nk) {
-          const double2 s = srow[k];
-          xr[k] = fma(v[u].x, s.x, xr[k]);
-          xr[k] = fma(-v[u].y, s.y, xr[k]);
-          xi[k] = fma(v[u].x, s.y, xi[k]);
-          xi[k] = fma(v[u].y, s.x, xi[k]);
+        for (int q = 0; q < RG; ++q) a[u][q] = make_double2(0.0, 0.0);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int i = min(i0 + u, nv - 1);  // inputs past nv carry zero amplitudes
+#pragma unroll
+      for (int k = 0; k < NK; ++k) {
+        const double2 sv = s_tile[i * NK + k];
+#pragma unroll
+        for (int q = 0; q < RG; ++q) {
+          pr[q][k] = fma(a[u][q].x, sv.x, pr[q][k]);
+          pr[q][k] = fma(-a[u][q].y, sv.y, pr[q][k]);
+          pi[q][k] = fma(a[u][q].x, sv.y, pi[q][k]);
+          pi[q][k] = fma(a[u][q].y, sv.x, pi[q][k]);
         }
       }
     }
   }
-  for (; i < nv; ++i) {
-    const double2 v = to_d2(A.vec(i)[r]);
-    const double2* srow = S + (int64_t)i * ldS;
 #pragma unroll
-    for (int k = 0; k < NKM; ++k) {
-      if (k < nk) {
-        const double2 s = srow[k];
-        xr[k] = fma(v.x, s.x, xr[k]);
-        xr[k] = fma(-v.y, s.y, xr[k]);
-        xi[k] = fma(v.x, s.y, xi[k]);
-        xi[k] = fma(v.y, s.x, xi[k]);
+  for (int q = 0; q < RG; ++q) {
+    const int64_t r = r0 + q;
+    if (r >= D) continue;
+#pragma unroll
+    for (int k = 0; k < NK; ++k) {
+      if (k0 + k < nk) {
+        const V o = from_d2<V>(pr[q][k], pi[q][k]);
+        Ob[(int64_t)(k0 + k) * ldo + r] = o;
+        if (Xi) Xi[r * nk + k0 + k] = o;  // interleaved copy for the next SpMM
       }
-    }
-  }
-#pragma unroll
-  for (int k = 0; k < NKM; ++k) {
-    if (k < nk) {
-      const V o = from_d2<V>(xr[k], xi[k]);
-      Ob[(int64_t)k * ldo + r] = o;
-      if (Xi) Xi[r * nk + k] = o;  // interleaved copy for the next SpMM
     }
   }
 }
@@ -550,25 +592,52 @@ class BlockLanczos {
     launch_gemm(A0, n0, A1, n1, Out, nk, ds, nk, Xi);
   }
 
-  template <typename VT, int NKM>
+  template <typename VT, int NK>
   void launch_gemm_t(const void* A0, int n0, const void* A1, int n1, void* Out, int nk, const double2* S, int ldS,
                      void* Xi) {
     VecPair<VT> A{(const VT*)A0, ld_, n0, (const VT*)A1, ld_, n1};
-    hipLaunchKernelGGL((gemm_pair<VT, NKM>), dim3(edx::grid_for(D_, 256)), dim3(256), 0, ctx_.stream, D_, A, nk,
-                       (VT*)Out, ld_, S, ldS, (VT*)Xi);
+    const int nv = n0 + n1;
+    const int n_kt = (nk + NK - 1) / NK;
+    const int64_t rows = 256 * RowGroup<VT>::RG;
+    const int64_t n_tiles = (int64_t)n_kt * ((D_ + rows - 1) / rows);
+    EDX_REQUIRE(n_tiles < (1ll << 30), EDX_ERR_INTERNAL, "update grid");
+    hipLaunchKernelGGL((gemm_tiles<VT, NK, 4>), dim3((unsigned)((n_tiles + 7) / 8 * 8)), dim3(256),
+                       (size_t)nv * NK * sizeof(double2), ctx_.stream, D_, A, nk, (VT*)Out, ld_, S, ldS, (VT*)Xi,
+                       n_kt, (int)n_tiles);
   }
 
+  // One tile of the smallest width that holds nk (nk <= 16), otherwise tiles
+  // of 6 or 8 outputs, whichever pads less.
   template <typename VT>
   void launch_gemm_v(const void* A0, int n0, const void* A1, int n1, void* Out, int nk, const double2* S, int ldS,
                      void* Xi) {
-    if (nk <= 16) launch_gemm_t<VT, 16>(A0, n0, A1, n1, Out, nk, S, ldS, Xi);
-    else if (nk <= 32) launch_gemm_t<VT, 32>(A0, n0, A1, n1, Out, nk, S, ldS, Xi);
-    else launch_gemm_t<VT, kMaxCombine>(A0, n0, A1, n1, Out, nk, S, ldS, Xi);
+    if (nk <= 2) launch_gemm_t<VT, 2>(A0, n0, A1, n1, Out, nk, S, ldS, Xi);
+    else if (nk <= 4) launch_gemm_t<VT, 4>(A0, n0, A1, n1, Out, nk, S, ldS, Xi);
+    else if (nk <= 6) launch_gemm_t<VT, 6>(A0, n0, A1, n1, Out, nk, S, ldS, Xi);
+    else if (nk <= 8) launch_gemm_t<VT, 8>(A0, n0, A1, n1, Out, nk, S, ldS, Xi);
+    else if (nk <= 12) launch_gemm_t<VT, 12>(A0, n0, A1, n1, Out, nk, S, ldS, Xi);
+    else if (nk <= 16) launch_gemm_t<VT, 16>(A0, n0, A1, n1, Out, nk, S, ldS, Xi);
+    else if ((nk + 5) / 6 * 6 < (nk + 7) / 8 * 8) launch_gemm_t<VT, 6>(A0, n0, A1, n1, Out, nk, S, ldS, Xi);
+    else launch_gemm_t<VT, 8>(A0, n0, A1, n1, Out, nk, S, ldS, Xi);
+  }
+
+  // True when the nk output vectors at Out overlap the n vectors at A.
+  bool overlaps(const void* A, int n, const void* Out, int nk) const {
+    if (!A || n == 0) return false;
+    const char* a = static_cast<const char*>(A);
+    const char* o = static_cast<const char*>(Out);
+    const size_t vb = (size_t)ld_ * esz_;
+    return o < a + n * vb && a < o + nk * vb;
   }
 
   void launch_gemm(const void* A0, int n0, const void* A1, int n1, void* Out, int nk, const double2* S, int ldS,
                    void* Xi) {
-    EDX_REQUIRE(nk >= 1 && nk <= kMaxCombine, EDX_ERR_INTERNAL, "update width");
+    EDX_REQUIRE(nk >= 1 && nk <= kMaxVec + kMaxW && n0 + n1 >= 1 && n0 + n1 <= kMaxVec + kMaxW, EDX_ERR_INTERNAL,
+                "update width");
+    // several output tiles run in separate workgroups: the outputs must not
+    // be inputs, and the interleaved copy is written by single-tile updates
+    EDX_REQUIRE(nk <= 16 || (!Xi && !overlaps(A0, n0, Out, nk) && !overlaps(A1, n1, Out, nk)), EDX_ERR_INTERNAL,
+                "multi-tile update aliases its inputs");
     hipEvent_t t0 = ctx_.timer.start(ctx_.stream);
     if (prec_ == EDX_PREC_FP64) launch_gemm_v<double2>(A0, n0, A1, n1, Out, nk, S, ldS, Xi);
     else launch_gemm_v<float2>(A0, n0, A1, n1, Out, nk, S, ldS, Xi);
@@ -576,19 +645,15 @@ class BlockLanczos {
     ctx_.timer.stop(ctx_.stream, t0, edx::K_GEMM, (double)(n0 + n1 + (Xi ? 2 : 1) * nk) * D_ * esz_);
   }
 
-  // Out[:, 0..nk) = V[:, 0..nv) Y (Out must not alias V), up to kMaxCombine
-  // columns per pass.
+  // Out[:, 0..nk) = V[:, 0..nv) Y (Out must not alias V), one launch over
+  // all output tiles, so the basis is read from HBM about once.
   void combine(const void* V, int nv, void* Out, int nk, const Mat& Y) {
     std::vector<double2> hs((size_t)nv * nk);
     for (int i = 0; i < nv; ++i)
       for (int k = 0; k < nk; ++k) hs[(size_t)i * nk + k] = make_double2(Y(i, k).real(), Y(i, k).imag());
     DevBuf<double2> S(hs.size());
     EDX_HIP(hipMemcpyAsync(S.p, hs.data(), hs.size() * sizeof(double2), hipMemcpyHostToDevice, ctx_.stream));
-    for (int k0 = 0; k0 < nk; k0 += kMaxCombine) {
-      const int nkt = std::min(kMaxCombine, nk - k0);
-      launch_gemm(V, nv, nullptr, 0, static_cast<char*>(Out) + (size_t)k0 * ld_ * esz_, nkt, S.p + k0, nk,
-                  nullptr);
-    }
+    launch_gemm(V, nv, nullptr, 0, Out, nk, S.p, nk, nullptr);
     EDX_HIP(hipStreamSynchronize(ctx_.stream));
   }
 

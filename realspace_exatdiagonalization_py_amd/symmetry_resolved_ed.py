@@ -608,11 +608,13 @@ def _device_eigs(dh: _DeviceHamiltonian, nev: int, *, precision: int = _edx.PREC
         return np.zeros(0, dtype=np.float64), np.zeros((0, 0), dtype=np.complex128), np.zeros(4)
     k = min(nev, n) if n <= DENSE_CUTOFF else nev
     vals = np.zeros(k, dtype=np.float64)
-    vecs = np.zeros((k, n), dtype=np.complex128) if want_vectors else None
+    # the device writes the vectors one after another; the (n, k) result is
+    # the transposed (Fortran-ordered) view, so each column is contiguous
+    vecs = np.empty((k, n), dtype=np.complex128) if want_vectors else None
     info = np.zeros(4, dtype=np.float64)
     opts = _edx.eigs_options(nev=k, block=block, precision=precision, tol=tol, eig_tol=eig_tol, seed=seed)
     dh.ctx.check(lib.edx_eigs_ex(dh.ctx.handle, dh.handle, _edx.C.byref(opts), ptr(vals), ptr(vecs), ptr(info)))
-    return vals, (vecs.T.copy() if want_vectors else None), info
+    return vals, (vecs.T if want_vectors else None), info
 
 
 def _csr_to_device(H: sp.spmatrix, ctx: Context | None = None) -> _DeviceHamiltonian:
