@@ -168,17 +168,21 @@ def time_spmm(_edx, H, precision, nb, n_rep=20):
 
 
 def profile_spmm(args) -> int:
-    """Only the SpMM kernels of sector k=(0,0): 20 launches each of the
-    complex64 and complex128 block-of-6 products (for counter runs)."""
+    """Only the SpMM kernels of sector k=(0,0) in the layout the benchmark
+    uses (lanes per row tuned for complex64, block of 6, as the solve step
+    tunes them): 20 launches each of the complex64 and complex128 products
+    (for counter runs)."""
     os.environ.setdefault("EDX_QUIET", "1")
     import realspace_exatdiagonalization_py_amd as ed
     from realspace_exatdiagonalization_py_amd import _edx
 
+    _edx.set_autotune(not args.no_autotune, ed.PREC_MIXED, 6)
     model, G = build_workload()
     data = ed.build_ed_data(model, filling_fraction=Fraction(12, 24), symmetry_group=G)
     H = _sector_hamiltonian(ed, data, 0)
-    out = {"dim": H._dev.dim, "entries": H._dev.n_entries}
-    for prec, name in ((1, "spmm_sell<float2,6,1>"), (0, "spmm_sell<double2,6,1>")):
+    g = H._dev.spmv_group
+    out = {"dim": H._dev.dim, "entries": H._dev.n_entries, "lanes": g}
+    for prec, name in ((1, f"spmm_sell<float2,6,{g}>"), (0, f"spmm_sell<double2,6,{g}>")):
         ms, nbytes = time_spmm(_edx, H, prec, 6)
         out[name] = {"avg_ms": round(ms, 5), "algorithmic_bytes": int(nbytes),
                      "GB/s": round(nbytes / ms / 1e6, 1)}
@@ -221,6 +225,9 @@ def main() -> int:
     ap.add_argument("--workers", type=int, default=8,
                     help="sectors solved concurrently, one HIP stream (context) per worker thread")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="heuristic SpMM lanes instead of timed ones")
+    ap.add_argument("--no-kernel-timer", action="store_true",
+                    help="no live per-kernel HIP-event timing (for runs under a profiler, which times kernels itself)")
     ap.add_argument("--profile-spmm", action="store_true", help="run only the SpMM kernels of one sector")
     ap.add_argument("--copy-bytes", type=float, default=0, help="run only a plain device copy of this many bytes")
     args = ap.parse_args()
@@ -239,6 +246,8 @@ def main() -> int:
 
     prec = ed.PREC_MIXED if args.precision == "mixed" else ed.PREC_FP64
     ctx = _edx.Context.default()
+    # SpMM lanes per row timed once per sector shape (the warmup step pays)
+    _edx.set_autotune(not args.no_autotune, prec, min(args.nev, 8))
     model, G = build_workload()
     log("building catalog ...")
     t0 = time.perf_counter()
@@ -254,7 +263,7 @@ def main() -> int:
     for i in range(args.warmup):
         solve_all(ed, data, args.nev, prec, args.eig_tol, args.workers)
         log(f"warmup step {i + 1}/{args.warmup} done")
-    _edx.kernel_stats_all(enable=1, reset=True)
+    _edx.kernel_stats_all(enable=0 if args.no_kernel_timer else 1, reset=True)
     lat_all, vals = [], None
     ctx.check(_edx.lib.edx_synchronize(ctx.handle))
     t0 = time.perf_counter()
@@ -278,11 +287,12 @@ def main() -> int:
     cls = max(("spmm_c64", "spmm_c128"), key=lambda k: ks[k][0])
     spmm_ms, spmm_n, spmm_bytes = ks[cls]
     nb = min(args.nev, 8)
-    kname = f"spmm_sell<{'float2' if cls == 'spmm_c64' else 'double2'},{nb},1>"
-    avg_ms = spmm_ms / max(spmm_n, 1)
-    achieved = spmm_bytes / max(spmm_ms, 1e-12) / 1e6  # GB/s
-    total_k = sum(v[0] for v in ks.values())
     H0 = _sector_hamiltonian(ed, data, 0)
+    lanes = H0._dev.spmv_group
+    kname = f"spmm_sell<{'float2' if cls == 'spmm_c64' else 'double2'},{nb},{lanes}>"
+    avg_ms = spmm_ms / max(spmm_n, 1)
+    achieved = spmm_bytes / spmm_ms / 1e6 if spmm_ms > 0 else 0.0  # GB/s (0: timer off)
+    total_k = sum(v[0] for v in ks.values())
     iso_ms, iso_bytes = time_spmm(_edx, H0, 1 if cls == "spmm_c64" else 0, nb)
     del H0
     roofline = {
@@ -338,6 +348,9 @@ def main() -> int:
         "catalog_ms": round(t_cat2 * 1e3, 3),
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "autotune": [{"dim_log2": e["dim_log2"], "block": e["block"], "lanes": e["lanes"],
+                      "ms_by_lanes": {str(k): round(v, 5) for k, v in e["ms"].items()}}
+                     for e in _edx.autotune_table()],
     }
     print(json.dumps(line))
     return 0

@@ -205,6 +205,27 @@ int edx_set_spmv_group(edx_ctx* ctx, int32_t lanes);                 /* default 
 int edx_hamiltonian_set_spmv_group(edx_ctx* ctx, edx_hamiltonian* ham, int32_t lanes);
 int edx_hamiltonian_info(const edx_hamiltonian* ham, int64_t* dim, int64_t* n_entries, int32_t* spmv_group);
 
+/* ---- SpMM autotuner --------------------------------------------------------
+ * With tuning on, every Hamiltonian build (edx_hamiltonian_build,
+ * edx_hamiltonian_from_csr) picks the lanes per row of its sliced copy from a
+ * process-wide table keyed by (precision, block width nb, floor(log2 dim),
+ * mean row length in quarter octaves); on a miss it times the 1-, 2- and
+ * 4-lane layouts (warm-up plus five launches of the SpMM at that precision
+ * and width, on the build's stream), records the three times and keeps the
+ * fastest.  A fixed edx_set_spmv_group choice overrides tuning; sectors with
+ * dim <= 500 (dense path) are not tuned.  edx_autotune_table returns, per
+ * entry, keys[5*i..] = (precision, nb, dim_log2, row_quarter_octaves, lanes)
+ * and ms[3*i..] = average launch times of the 1/2/4-lane layouts; n is the
+ * table size (entries beyond capacity are not written).
+ * edx_hamiltonian_autotune times and rebuilds one Hamiltonian without the
+ * table (the Hamiltonian must not be in use by another call meanwhile).
+ * No reference counterpart (launch-shape tuning of the device product). */
+int edx_set_autotune(int32_t enable, int32_t precision, int32_t nb);
+int edx_autotune_table(int32_t capacity, int32_t* keys, float* ms, int32_t* n);
+int edx_autotune_clear(void);
+int edx_hamiltonian_autotune(edx_ctx* ctx, edx_hamiltonian* ham, int32_t precision, int32_t nb, int32_t* lanes,
+                             float* ms);
+
 /* ---- host-only helper (no device needed) ------------------------------------
  * Eigen-decomposition of a Hermitian n x n matrix (row-major (re, im) pairs)
  * from its upper (upper != 0, as LAPACK eigh(UPLO="U") in ref SRE:2281) or

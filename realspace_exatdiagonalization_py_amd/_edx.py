@@ -64,6 +64,10 @@ SIGNATURES = {
     "edx_set_spmv_group": (C.c_int, [P, i32]),
     "edx_hamiltonian_set_spmv_group": (C.c_int, [P, P, i32]),
     "edx_hamiltonian_info": (C.c_int, [P, pi64, pi64, pi32]),
+    "edx_set_autotune": (C.c_int, [i32, i32, i32]),
+    "edx_autotune_table": (C.c_int, [i32, P, P, pi32]),
+    "edx_autotune_clear": (C.c_int, []),
+    "edx_hamiltonian_autotune": (C.c_int, [P, P, i32, i32, pi32, P]),
     "edx_host_herm_eig": (C.c_int, [i32, P, i32, P, P]),
     "edx_device_alloc": (C.c_int, [P, u64, PP]),
     "edx_device_free": (C.c_int, [P, P]),
@@ -203,6 +207,35 @@ def kernel_stats(ctx: "Context", enable: int = -1, reset: bool = False) -> dict:
     b = np.zeros(k)
     ctx.check(lib.edx_kernel_stats(ctx.handle, enable, 1 if reset else 0, ptr(ms), ptr(n), ptr(b)))
     return {k: (float(ms[i]), int(n[i]), float(b[i])) for i, k in enumerate(KERNEL_CLASSES)}
+
+
+def set_autotune(enable: bool = True, precision: int = PREC_MIXED, block: int = 6) -> None:
+    """Turns the SpMM autotuner on or off for Hamiltonian builds of this
+    process: each new build's lanes per row come from the tuning table for
+    (precision, block width, size, row length), timed on a miss."""
+    raise_for(lib.edx_set_autotune(1 if enable else 0, int(precision), int(block)))
+
+
+def autotune_table() -> list[dict]:
+    """Entries of the tuning table: key, chosen lanes and the average SpMM
+    launch time (ms) of the 1-, 2- and 4-lane layouts."""
+    n = C.c_int32()
+    raise_for(lib.edx_autotune_table(0, None, None, C.byref(n)))
+    cap = n.value
+    keys = np.zeros(5 * max(cap, 1), dtype=np.int32)
+    ms = np.zeros(3 * max(cap, 1), dtype=np.float32)
+    raise_for(lib.edx_autotune_table(cap, ptr(keys), ptr(ms), C.byref(n)))
+    out = []
+    for i in range(min(cap, n.value)):
+        k = keys[5 * i:5 * i + 5]
+        out.append({"precision": int(k[0]), "block": int(k[1]), "dim_log2": int(k[2]),
+                    "row_quarter_octaves": int(k[3]), "lanes": int(k[4]),
+                    "ms": {1: float(ms[3 * i]), 2: float(ms[3 * i + 1]), 4: float(ms[3 * i + 2])}})
+    return out
+
+
+def autotune_clear() -> None:
+    raise_for(lib.edx_autotune_clear())
 
 
 def mem_info(ctx: "Context") -> tuple[int, int]:

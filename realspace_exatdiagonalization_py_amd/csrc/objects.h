@@ -69,6 +69,9 @@ namespace edx {
 void build_sell(edx_hamiltonian& h, Ctx& ctx, int G);
 // Lanes per row for a table of this shape.
 int default_sell_lanes(int64_t dim, int64_t n_entries);
+// Builds the sliced copy with the context's fixed lane count, the autotuned
+// one (timing the layouts on a miss of the tuning table) or the heuristic.
+void build_sell_for(edx_hamiltonian& h, Ctx& ctx);
 }  // namespace edx
 
 namespace edx {

@@ -483,14 +483,6 @@ void dense_matrix(const edx_hamiltonian& h, hipStream_t st, double2* out) {
 
 }  // namespace edx
 
-namespace {
-
-int pick_lanes(const edx::Ctx& ctx, int64_t dim, int64_t n_entries) {
-  return ctx.autotune_group > 0 ? ctx.autotune_group : edx::default_sell_lanes(dim, n_entries);
-}
-
-}  // namespace
-
 extern "C" {
 
 int edx_hamiltonian_build(edx_ctx* ctx, const edx_catalog* cat, const edx_sector* sec, const double* chi,
@@ -617,7 +609,7 @@ int edx_hamiltonian_build(edx_ctx* ctx, const edx_catalog* cat, const edx_sector
         EDX_HIP(hipStreamSynchronize(st));
       }
       ham->n_entries = E;
-      edx::build_sell(*ham, *ctx, pick_lanes(*ctx, D, E));
+      edx::build_sell_for(*ham, *ctx);
     } catch (...) {
       delete ham;
       throw;
@@ -697,7 +689,7 @@ int edx_hamiltonian_from_csr(edx_ctx* ctx, int64_t dim, int64_t nnz, const int64
         EDX_HIP(hipGetLastError());
       }
       EDX_HIP(hipStreamSynchronize(st));
-      edx::build_sell(*ham, *ctx, pick_lanes(*ctx, dim, nnz));
+      edx::build_sell_for(*ham, *ctx);
     } catch (...) {
       delete ham;
       throw;

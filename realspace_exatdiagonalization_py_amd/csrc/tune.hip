@@ -1,0 +1,202 @@
+// SpMM launch-shape autotuner.
+//
+// The sliced copy of a sector Hamiltonian can give each matrix row 1, 2 or 4
+// lanes of a wavefront (spmm_sell<V, NB, G>: slices of 64 / G rows).  Which
+// is fastest depends on the sector dimension (one lane per row fills the 256
+// CUs only for large sectors), on the mean row length (gathers per lane) and
+// on the block width and element precision the solver will use.  With tuning
+// on, every Hamiltonian build looks its shape up in a process-wide table
+// keyed by (precision, block width, floor(log2 dim), mean row length rounded
+// to a quarter octave); on a miss it builds the three layouts one after the
+// other, times each on the build's stream (one warm-up and kRuns timed
+// launches on zero vectors), records the timings and keeps the fastest.
+// Sectors of one scan share a key, so one build per scan pays for the timing.
+// Timings taken while other streams load the device are noisier; the table
+// keeps the first measurement of a key.
+#include <cmath>
+#include <map>
+#include <mutex>
+
+#include "objects.h"
+
+namespace edx {
+void spmm(const edx_hamiltonian& h, Ctx& ctx, int precision, int nb, const void* X, void* Y, int64_t ldy);
+
+namespace {
+
+constexpr int kRuns = 5;
+constexpr int kLanes[3] = {1, 2, 4};
+
+struct TuneKey {
+  int prec, nb, dim_log2, row_q;
+  bool operator<(const TuneKey& o) const {
+    if (prec != o.prec) return prec < o.prec;
+    if (nb != o.nb) return nb < o.nb;
+    if (dim_log2 != o.dim_log2) return dim_log2 < o.dim_log2;
+    return row_q < o.row_q;
+  }
+};
+
+struct TuneEntry {
+  int lanes;
+  float ms[3];
+};
+
+struct TuneState {
+  std::mutex mu;
+  bool enabled = false;
+  int prec = EDX_PREC_MIXED;
+  int nb = 6;
+  std::map<TuneKey, TuneEntry> table;
+};
+
+TuneState& state() {
+  static TuneState s;
+  return s;
+}
+
+TuneKey key_for(const edx_hamiltonian& h, int prec, int nb) {
+  const double row = h.dim ? (double)h.n_entries / (double)h.dim : 0.0;
+  TuneKey k;
+  k.prec = prec;
+  k.nb = nb;
+  k.dim_log2 = h.dim > 0 ? (int)std::floor(std::log2((double)h.dim)) : 0;
+  k.row_q = row > 0 ? (int)std::lround(4.0 * std::log2(row)) : -1000;
+  return k;
+}
+
+// Average time of one SpMM launch with the layout h currently has.
+float time_spmm(const edx_hamiltonian& h, Ctx& ctx, int prec, int nb, const void* X, void* Y) {
+  spmm(h, ctx, prec, nb, X, Y, h.dim);  // warm-up
+  hipEvent_t a, b;
+  EDX_HIP(hipEventCreate(&a));
+  EDX_HIP(hipEventCreate(&b));
+  EDX_HIP(hipEventRecord(a, ctx.stream));
+  for (int r = 0; r < kRuns; ++r) spmm(h, ctx, prec, nb, X, Y, h.dim);
+  EDX_HIP(hipEventRecord(b, ctx.stream));
+  EDX_HIP(hipEventSynchronize(b));
+  float ms = 0.0f;
+  EDX_HIP(hipEventElapsedTime(&ms, a, b));
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  return ms / kRuns;
+}
+
+}  // namespace
+
+// Times the three layouts of h at (prec, nb), leaves h built with the
+// fastest and returns the entry.
+TuneEntry tune_spmm(edx_hamiltonian& h, Ctx& ctx, int prec, int nb) {
+  const size_t esz = prec == EDX_PREC_FP64 ? 16 : 8;
+  DevBuf<char> X((size_t)h.dim * nb * esz), Y((size_t)h.dim * nb * esz);
+  EDX_HIP(hipMemsetAsync(X.p, 0, X.bytes(), ctx.stream));
+  TuneEntry e{1, {0.0f, 0.0f, 0.0f}};
+  float best = INFINITY;
+  for (int c = 0; c < 3; ++c) {
+    if (h.sell_g != kLanes[c] || h.sell_slices == 0) build_sell(h, ctx, kLanes[c]);
+    e.ms[c] = time_spmm(h, ctx, prec, nb, X.p, Y.p);
+    if (e.ms[c] < best) {
+      best = e.ms[c];
+      e.lanes = kLanes[c];
+    }
+  }
+  if (h.sell_g != e.lanes) build_sell(h, ctx, e.lanes);
+  return e;
+}
+
+// Lane count for a new build: the context's fixed choice, else the tuned
+// one (timing on a table miss), else the heuristic.  Leaves h built.
+void build_sell_for(edx_hamiltonian& h, Ctx& ctx) {
+  if (ctx.autotune_group > 0) {
+    build_sell(h, ctx, ctx.autotune_group);
+    return;
+  }
+  TuneState& s = state();
+  bool on;
+  int prec, nb;
+  {
+    std::lock_guard<std::mutex> lk(s.mu);
+    on = s.enabled;
+    prec = s.prec;
+    nb = s.nb;
+  }
+  if (!on || h.dim <= 500) {
+    build_sell(h, ctx, default_sell_lanes(h.dim, h.n_entries));
+    return;
+  }
+  const TuneKey k = key_for(h, prec, nb);
+  int hit = 0;
+  {
+    std::lock_guard<std::mutex> lk(s.mu);
+    auto it = s.table.find(k);
+    if (it != s.table.end()) hit = it->second.lanes;
+  }
+  if (hit) {
+    build_sell(h, ctx, hit);
+    return;
+  }
+  build_sell(h, ctx, default_sell_lanes(h.dim, h.n_entries));
+  const TuneEntry e = tune_spmm(h, ctx, prec, nb);
+  std::lock_guard<std::mutex> lk(s.mu);
+  s.table.emplace(k, e);  // the first measurement of a key is kept
+}
+
+}  // namespace edx
+
+extern "C" {
+
+int edx_set_autotune(int32_t enable, int32_t precision, int32_t nb) {
+  if (precision != EDX_PREC_FP64 && precision != EDX_PREC_MIXED) return EDX_ERR_INVALID;
+  if (nb < 1 || nb > 16) return EDX_ERR_INVALID;
+  auto& s = edx::state();
+  std::lock_guard<std::mutex> lk(s.mu);
+  s.enabled = enable != 0;
+  s.prec = precision;
+  s.nb = nb;
+  return EDX_OK;
+}
+
+int edx_autotune_table(int32_t capacity, int32_t* keys, float* ms, int32_t* n) {
+  if (!n || (capacity > 0 && (!keys || !ms))) return EDX_ERR_INVALID;
+  auto& s = edx::state();
+  std::lock_guard<std::mutex> lk(s.mu);
+  int i = 0;
+  for (const auto& kv : s.table) {
+    if (i < capacity) {
+      int32_t* r = keys + 5 * i;
+      r[0] = kv.first.prec;
+      r[1] = kv.first.nb;
+      r[2] = kv.first.dim_log2;
+      r[3] = kv.first.row_q;
+      r[4] = kv.second.lanes;
+      for (int c = 0; c < 3; ++c) ms[3 * i + c] = kv.second.ms[c];
+    }
+    ++i;
+  }
+  *n = i;
+  return EDX_OK;
+}
+
+int edx_autotune_clear(void) {
+  auto& s = edx::state();
+  std::lock_guard<std::mutex> lk(s.mu);
+  s.table.clear();
+  return EDX_OK;
+}
+
+int edx_hamiltonian_autotune(edx_ctx* ctx, edx_hamiltonian* ham, int32_t precision, int32_t nb, int32_t* lanes,
+                             float* ms) {
+  return edx::guarded(ctx, [&] {
+    EDX_REQUIRE(ctx && ham, EDX_ERR_INVALID, "null argument");
+    EDX_REQUIRE(precision == EDX_PREC_FP64 || precision == EDX_PREC_MIXED, EDX_ERR_INVALID, "unknown precision");
+    EDX_REQUIRE(nb >= 1 && nb <= 16, EDX_ERR_INVALID, "block width must be 1..16");
+    EDX_REQUIRE(ham->dim > 0, EDX_ERR_INVALID, "empty Hamiltonian");
+    edx::DeviceGuard dg(ctx->device);
+    const edx::TuneEntry e = edx::tune_spmm(*ham, *ctx, precision, nb);
+    if (lanes) *lanes = e.lanes;
+    if (ms)
+      for (int c = 0; c < 3; ++c) ms[c] = e.ms[c];
+  });
+}
+
+}  // extern "C"

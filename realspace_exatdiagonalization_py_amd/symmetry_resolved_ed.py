@@ -487,6 +487,17 @@ class _DeviceHamiltonian:
             lib.edx_hamiltonian_destroy(self.handle)
             self.handle = None
 
+    def autotune(self, precision: int = _edx.PREC_MIXED, block: int = 6) -> dict:
+        """Times the 1-, 2- and 4-lane SpMM layouts at (precision, block),
+        keeps the fastest; returns ``{"lanes": g, "ms": {1: t1, 2: t2, 4: t4}}``.
+        Not concurrent with other calls on this Hamiltonian."""
+        lanes = _edx.C.c_int32()
+        ms = np.zeros(3, dtype=np.float32)
+        self.ctx.check(lib.edx_hamiltonian_autotune(self.ctx.handle, self.handle, int(precision), int(block),
+                                                    _edx.C.byref(lanes), ptr(ms)))
+        self.spmv_group = lanes.value
+        return {"lanes": lanes.value, "ms": {1: float(ms[0]), 2: float(ms[1]), 4: float(ms[2])}}
+
     def matvec(self, x: np.ndarray) -> np.ndarray:
         """``H x`` for a vector (n,) or a block (n, k) of complex128."""
         x = np.asarray(x, dtype=np.complex128)
