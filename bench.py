@@ -168,24 +168,25 @@ def time_spmm(_edx, H, precision, nb, n_rep=20):
 
 
 def profile_spmm(args) -> int:
-    """Only the SpMM kernels of sector k=(0,0) in the layout the benchmark
-    uses (lanes per row tuned for complex64, block of 6, as the solve step
-    tunes them): 20 launches each of the complex64 and complex128 products
-    (for counter runs)."""
+    """Only the SpMM kernels of sector k=(0,0), each sliced layout in turn
+    (1, 2 and 4 lanes per row, the candidates of the autotuner): 20 launches
+    each of the complex64 and complex128 block-of-6 products (for counter
+    runs)."""
     os.environ.setdefault("EDX_QUIET", "1")
     import realspace_exatdiagonalization_py_amd as ed
     from realspace_exatdiagonalization_py_amd import _edx
 
-    _edx.set_autotune(not args.no_autotune, ed.PREC_MIXED, 6)
     model, G = build_workload()
     data = ed.build_ed_data(model, filling_fraction=Fraction(12, 24), symmetry_group=G)
     H = _sector_hamiltonian(ed, data, 0)
-    g = H._dev.spmv_group
-    out = {"dim": H._dev.dim, "entries": H._dev.n_entries, "lanes": g}
-    for prec, name in ((1, f"spmm_sell<float2,6,{g}>"), (0, f"spmm_sell<double2,6,{g}>")):
-        ms, nbytes = time_spmm(_edx, H, prec, 6)
-        out[name] = {"avg_ms": round(ms, 5), "algorithmic_bytes": int(nbytes),
-                     "GB/s": round(nbytes / ms / 1e6, 1)}
+    dh = H._dev
+    out = {"dim": dh.dim, "entries": dh.n_entries}
+    for g in (1, 2, 4):
+        dh.ctx.check(_edx.lib.edx_hamiltonian_set_spmv_group(dh.ctx.handle, dh.handle, g))
+        for prec, v in ((1, "float2"), (0, "double2")):
+            ms, nbytes = time_spmm(_edx, H, prec, 6)
+            out[f"spmm_sell<{v},6,{g}>"] = {"avg_ms": round(ms, 5), "algorithmic_bytes": int(nbytes),
+                                            "GB/s": round(nbytes / ms / 1e6, 1)}
     print(json.dumps(out))
     return 0
 
