@@ -40,7 +40,12 @@ int edx_ctx_create(int32_t device, edx_ctx** out) {
     auto ctx = new edx_ctx();
     ctx->device = device;
     hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipHostMalloc((void**)&ctx->pinned_in, edx::kPinnedIn * sizeof(double2), hipHostMallocDefault);
+    // Gram results are written by kernels straight into pinned_in: coherent
+    // (fine-grained) host memory, read by the host after a stream sync
+    if (e == hipSuccess)
+      e = hipHostMalloc((void**)&ctx->pinned_in, edx::kPinnedIn * sizeof(double2),
+                        hipHostMallocMapped | hipHostMallocCoherent);
+    if (e == hipSuccess) e = hipHostGetDevicePointer((void**)&ctx->pinned_in_dev, ctx->pinned_in, 0);
     if (e == hipSuccess)
       e = hipHostMalloc((void**)&ctx->pinned_out, edx::kPinnedOut * sizeof(double2), hipHostMallocDefault);
     if (e != hipSuccess) {

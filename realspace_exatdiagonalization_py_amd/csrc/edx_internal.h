@@ -176,7 +176,8 @@ struct Ctx {
   // pinned host staging owned by the context (allocated once, reused by
   // every solve on it): projection coefficients device -> host, and the
   // ring of update coefficients host -> device
-  double2* pinned_in = nullptr;
+  double2* pinned_in = nullptr;      // coherent: kernels write Gram results into it directly
+  double2* pinned_in_dev = nullptr;  // its device address
   double2* pinned_out = nullptr;
 };
 

@@ -410,10 +410,10 @@ class BlockLanczos {
     n_rowblocks_ = (int)std::min<int64_t>(std::max<int64_t>((D_ + 4095) / 4096, 1), 256);
     rows_per_block_ = (D_ + n_rowblocks_ - 1) / n_rowblocks_;
     partial_.alloc((size_t)n_rowblocks_ * kMaxOut);
-    cdev_.alloc((size_t)kMaxOut);
     sdev_.alloc((size_t)kSlots * kSlot);
     static_assert(kMaxOut <= edx::kPinnedIn && kSlots * kSlot <= edx::kPinnedOut, "pinned staging too small");
     hc_ = ctx.pinned_in;
+    hc_dev_ = ctx.pinned_in_dev;
     hs_ = ctx.pinned_out;
   }
   BlockLanczos(const BlockLanczos&) = delete;
@@ -564,11 +564,11 @@ class BlockLanczos {
     else launch_tiles_nw<float2>(A0, n0, A1, n1, W, nw);
     EDX_HIP(hipGetLastError());
     const int nout = nv * nw;
+    // the reduction writes the result straight into the pinned host buffer
     hipLaunchKernelGGL(gram_reduce, dim3((nout + 3) / 4), dim3(256), 0, ctx_.stream, n_rowblocks_, nout,
-                       (const double2*)partial_.p, cdev_.p);
+                       (const double2*)partial_.p, hc_dev_);
     EDX_HIP(hipGetLastError());
     ctx_.timer.stop(ctx_.stream, t0, edx::K_GRAM, (double)(nv + nw) * D_ * esz_);
-    EDX_HIP(hipMemcpyAsync(hc_, cdev_.p, (size_t)nout * sizeof(double2), hipMemcpyDeviceToHost, ctx_.stream));
     EDX_HIP(hipStreamSynchronize(ctx_.stream));
     Mat C(nv, nw);
     for (int i = 0; i < nv; ++i)
@@ -837,8 +837,9 @@ class BlockLanczos {
   int slot_ = 0;
   DevBuf<char> bank_[2];
   DevBuf<char> W_, X_;
-  DevBuf<double2> partial_, cdev_, sdev_;
-  double2* hc_ = nullptr;
+  DevBuf<double2> partial_, sdev_;
+  double2* hc_ = nullptr;      // pinned Gram results (host address)
+  double2* hc_dev_ = nullptr;  // the same buffer's device address
   double2* hs_ = nullptr;
 };
 
