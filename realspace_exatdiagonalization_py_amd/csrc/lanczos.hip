@@ -69,19 +69,72 @@ struct VecPair {
   }
 };
 
+// Sum of NV per-lane values over the 64 lanes of a wavefront by halving: at
+// each lane distance O (32, 16, ..., 1) a lane pair splits its current set of
+// values in two, each lane sends the half it drops and adds the partner's copy
+// of the half it keeps (N values take N/2 shuffles); a set of odd size is
+// summed whole at that distance instead.  Afterwards a lane holds NF sums, for
+// value indices base .. base + NF; lanes that differ only in the bits of `dup`
+// hold the same sums.  The order of additions is fixed, so results do not
+// depend on scheduling.
+template <int N, int O>
+struct WaveSplitSum {
+  static constexpr int NF = (N % 2 == 0) ? WaveSplitSum<N / 2, O / 2>::NF : WaveSplitSum<N, O / 2>::NF;
+  __device__ __forceinline__ static void run(double* v, int lane, int& base, int& dup) {
+    if constexpr (N % 2 == 0) {
+      constexpr int H = N / 2;
+      const bool up = (lane & O) != 0;
+#pragma unroll
+      for (int i = 0; i < H; ++i) {
+        const double send = up ? v[i] : v[H + i];
+        const double keep = up ? v[H + i] : v[i];
+        v[i] = keep + __shfl_xor(send, O, 64);
+      }
+      if (up) base += H;
+      WaveSplitSum<H, O / 2>::run(v, lane, base, dup);
+    } else {
+#pragma unroll
+      for (int i = 0; i < N; ++i) v[i] += __shfl_xor(v[i], O, 64);
+      dup |= O;
+      WaveSplitSum<N, O / 2>::run(v, lane, base, dup);
+    }
+  }
+};
+template <int N>
+struct WaveSplitSum<N, 0> {
+  static constexpr int NF = N;
+  __device__ __forceinline__ static void run(double*, int, int&, int&) {}
+};
+
+// Keeps the loads issued just before it in place: no code motion moves them
+// past this point (towards their uses), and the scheduler does not start the
+// following arithmetic (which waits on earlier loads) before them.  A loop
+// that loads the next stage and then computes the current one thereby keeps
+// the next stage's loads in flight during the arithmetic.
+#define EDX_KEEP_PREFETCH()          \
+  do {                               \
+    asm volatile("" ::: "memory");   \
+    __builtin_amdgcn_sched_barrier(0); \
+  } while (0)
+
 // partial[rb][i*NW + j] = sum over row block rb of conj(A_i[r]) W_j[r].
-// One workgroup per (tile of TI vectors of A, row block).  The 1-D grid is
+// One workgroup per (tile of TI vectors of A, row block); the 1-D grid is
 // mapped so that all tiles of a row block run on one XCD (workgroups are
-// dispatched to the 8 XCDs round-robin): that XCD's L2 then serves the row
-// block of W to every tile, and HBM sees A and W about once.  Lanes take
-// consecutive rows (coalesced loads), two rows per iteration; each thread
-// keeps a TI x NW complex tile in registers, folded at the end by a butterfly
-// within the wavefront and a fixed-order sum over the four wavefronts.
+// dispatched to the 8 XCDs round-robin), whose L2 then serves that block of W
+// to every tile.  Lanes take consecutive rows (coalesced loads).  The loop
+// keeps two register stages: the loads of the next row are in flight while the
+// current one is multiplied.  Loads are unconditional: rows past the block
+// read row ld - 1 and vector indices past nv read vector nv - 1; every vector
+// is zero in rows [D, ld) with ld > D, so the extra rows add nothing, and the
+// sums of padded vector slots are never written.  Each thread keeps a TI x NW
+// complex tile, summed over the wavefront by WaveSplitSum and over the four
+// wavefronts in a fixed order.
 template <typename V, int TI, int NW>
 __global__ void __launch_bounds__(256)
-gram_tiles(int64_t D, VecPair<V> A, const V* __restrict__ Wb, int64_t ldw, int64_t rows_per_block, int n_itiles,
-           int n_tiles, double2* __restrict__ partial) {
-  __shared__ double red[4][TI * NW * 2];
+gram_tiles(int64_t D, int64_t ld, VecPair<V> A, const V* __restrict__ Wb, int64_t ldw, int64_t rows_per_block,
+           int n_itiles, int n_tiles, double2* __restrict__ partial) {
+  constexpr int NV = TI * NW * 2;
+  __shared__ double red[4][NV];
   const int per_xcd = (n_tiles + 7) / 8;
   const int logical = (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);
   if (logical >= n_tiles) return;
@@ -92,66 +145,54 @@ gram_tiles(int64_t D, VecPair<V> A, const V* __restrict__ Wb, int64_t ldw, int64
   const int64_t re = min(D, rb + rows_per_block);
   const V* ap[TI];
 #pragma unroll
-  for (int t = 0; t < TI; ++t) ap[t] = i0 + t < nv ? A.vec(i0 + t) : nullptr;
-  double ar[TI][NW], ai[TI][NW];
+  for (int t = 0; t < TI; ++t) ap[t] = A.vec(min(i0 + t, nv - 1));
+  const V* wp[NW];
 #pragma unroll
-  for (int t = 0; t < TI; ++t)
+  for (int j = 0; j < NW; ++j) wp[j] = Wb + (int64_t)j * ldw;
+  double acc[NV];
 #pragma unroll
-    for (int j = 0; j < NW; ++j) ar[t][j] = ai[t][j] = 0.0;
-  int64_t r = rb + threadIdx.x;
-  for (; r + 256 < re; r += 512) {
-    double2 w[2][NW], v[2][TI];
-#pragma unroll
-    for (int u = 0; u < 2; ++u) {
-#pragma unroll
-      for (int j = 0; j < NW; ++j) w[u][j] = to_d2(Wb[(int64_t)j * ldw + r + 256 * u]);
-#pragma unroll
-      for (int t = 0; t < TI; ++t) v[u][t] = ap[t] ? to_d2(ap[t][r + 256 * u]) : make_double2(0.0, 0.0);
-    }
-#pragma unroll
-    for (int u = 0; u < 2; ++u)
-#pragma unroll
-      for (int t = 0; t < TI; ++t)
-#pragma unroll
-        for (int j = 0; j < NW; ++j) {
-          ar[t][j] = fma(v[u][t].x, w[u][j].x, ar[t][j]);
-          ar[t][j] = fma(v[u][t].y, w[u][j].y, ar[t][j]);
-          ai[t][j] = fma(v[u][t].x, w[u][j].y, ai[t][j]);
-          ai[t][j] = fma(-v[u][t].y, w[u][j].x, ai[t][j]);
-        }
+  for (int i = 0; i < NV; ++i) acc[i] = 0.0;
+  const int64_t nstage = (re - rb + 255) / 256;
+  V w0[NW], a0[TI], w1[NW], a1[TI];
+#define EDX_GRAM_LOAD(WB, AB, K)                                     \
+  {                                                                  \
+    int64_t rr = rb + (K) * 256 + (int64_t)threadIdx.x;              \
+    rr = rr < re ? rr : ld - 1;                                      \
+    _Pragma("unroll") for (int j = 0; j < NW; ++j) WB[j] = wp[j][rr]; \
+    _Pragma("unroll") for (int t = 0; t < TI; ++t) AB[t] = ap[t][rr]; \
+    EDX_KEEP_PREFETCH();                                             \
   }
-  for (; r < re; r += 256) {
-    double2 w[NW];
-#pragma unroll
-    for (int j = 0; j < NW; ++j) w[j] = to_d2(Wb[(int64_t)j * ldw + r]);
-#pragma unroll
-    for (int t = 0; t < TI; ++t) {
-      const double2 v = ap[t] ? to_d2(ap[t][r]) : make_double2(0.0, 0.0);
-#pragma unroll
-      for (int j = 0; j < NW; ++j) {
-        ar[t][j] = fma(v.x, w[j].x, ar[t][j]);
-        ar[t][j] = fma(v.y, w[j].y, ar[t][j]);
-        ai[t][j] = fma(v.x, w[j].y, ai[t][j]);
-        ai[t][j] = fma(-v.y, w[j].x, ai[t][j]);
-      }
-    }
+#define EDX_GRAM_FMA(WB, AB)                         \
+  {                                                  \
+    _Pragma("unroll") for (int j = 0; j < NW; ++j) { \
+      const double2 w = to_d2(WB[j]);                \
+      _Pragma("unroll") for (int t = 0; t < TI; ++t) { \
+        const double2 v = to_d2(AB[t]);              \
+        double& xr = acc[(t * NW + j) * 2];          \
+        double& xi = acc[(t * NW + j) * 2 + 1];      \
+        xr = fma(v.x, w.x, xr);                      \
+        xr = fma(v.y, w.y, xr);                      \
+        xi = fma(v.x, w.y, xi);                      \
+        xi = fma(-v.y, w.x, xi);                     \
+      }                                              \
+    }                                                \
   }
+  EDX_GRAM_LOAD(w0, a0, 0)
+  for (int64_t k = 0; k < nstage; k += 2) {
+    EDX_GRAM_LOAD(w1, a1, k + 1)
+    EDX_GRAM_FMA(w0, a0)
+    EDX_GRAM_LOAD(w0, a0, k + 2)
+    EDX_GRAM_FMA(w1, a1)
+  }
+#undef EDX_GRAM_LOAD
+#undef EDX_GRAM_FMA
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  int base = 0, dup = 0;
+  WaveSplitSum<NV, 32>::run(acc, lane, base, dup);
+  if ((lane & dup) == 0) {
 #pragma unroll
-  for (int t = 0; t < TI; ++t)
-#pragma unroll
-    for (int j = 0; j < NW; ++j) {
-      double xr = ar[t][j], xi = ai[t][j];
-#pragma unroll
-      for (int off = 32; off >= 1; off >>= 1) {
-        xr += __shfl_xor(xr, off, 64);
-        xi += __shfl_xor(xi, off, 64);
-      }
-      if (lane == 0) {
-        red[wave][(t * NW + j) * 2] = xr;
-        red[wave][(t * NW + j) * 2 + 1] = xi;
-      }
-    }
+    for (int i = 0; i < WaveSplitSum<NV, 32>::NF; ++i) red[wave][base + i] = acc[i];
+  }
   __syncthreads();
   const int nout = nv * NW;
   for (int o = threadIdx.x; o < TI * NW; o += 256) {
@@ -188,105 +229,81 @@ __global__ void __launch_bounds__(256) gram_reduce(int nblk, int nout, const dou
   if (o < nout && lane == 0) out[o] = make_double2(xr, xi);
 }
 
-// Rows a thread of gemm_tiles owns: complex64 rows two at a time (one 16-byte
-// load per lane), complex128 rows one at a time.
-template <typename V>
-struct RowGroup;
-template <>
-struct RowGroup<float2> {
-  static constexpr int RG = 2;
-  __device__ __forceinline__ static void load(const float2* p, int64_t r, int64_t D, double2* out) {
-    if (r + 1 < D) {
-      const float4 q = *reinterpret_cast<const float4*>(p + r);
-      out[0] = make_double2(q.x, q.y);
-      out[1] = make_double2(q.z, q.w);
-    } else {
-      out[0] = r < D ? to_d2(p[r]) : make_double2(0.0, 0.0);
-      out[1] = make_double2(0.0, 0.0);
-    }
-  }
-};
-template <>
-struct RowGroup<double2> {
-  static constexpr int RG = 1;
-  __device__ __forceinline__ static void load(const double2* p, int64_t r, int64_t D, double2* out) {
-    out[0] = r < D ? p[r] : make_double2(0.0, 0.0);
-  }
-};
+// Vectors loaded per stage of gemm_tiles (two stages in flight).
+constexpr int kGemmStage = 4;
 
 // Out_k[r] = sum_i A_i[r] S[i][k] for k < nk (S row-major, leading dimension
-// ldS), FP64 accumulation.  One workgroup per (row block of 256 * RG rows,
-// tile of NK outputs); the 1-D grid is mapped so that the tiles of a row block
-// run on one XCD, whose L2 then serves the block's inputs to all of them.  The
-// tile's coefficients are staged in LDS (every lane reads the same entry, a
-// broadcast), and the loads of U inputs are issued before their FMAs.  Rows
-// are stored at r < D only, and vectors are 16-byte aligned with an even
-// leading dimension, so the paired complex64 loads stay inside each vector.
-// With one tile (nk <= NK) every input of row r is read before any output of
-// row r is written, so Out may be one of the A arrays (same leading
-// dimension); Xi (optional, one tile only) receives the interleaved copy
-// [r][k].
-template <typename V, int NK, int U>
+// ldS), FP64 accumulation.  One workgroup per (256 rows, tile of NK outputs);
+// the 1-D grid is mapped so that the tiles of a row block run on one XCD,
+// whose L2 then serves the block's inputs to all of them.  Each lane owns one
+// row.  The tile's coefficients are staged in LDS (every lane reads the same
+// entry, a broadcast), with zero rows up to a multiple of two stages, so the
+// input loop needs no bounds test: it loads vector min(i, nv - 1) and the zero
+// coefficient cancels it.  Loads of the next kGemmStage vectors are in flight
+// while the current ones are multiplied.  Lanes past D read row D (zero
+// padding, never written) and store nothing.  With one tile (nk <= NK) every
+// input of row r is read by the lane of row r before it writes any output of
+// row r, so Out may be one of the A arrays (same leading dimension); Xi
+// (optional, one tile only) receives the interleaved copy [r][k].
+template <typename V, int NK>
 __global__ void __launch_bounds__(256)
 gemm_tiles(int64_t D, VecPair<V> A, int nk, V* Ob, int64_t ldo, const double2* __restrict__ S, int ldS, V* Xi,
            int n_kt, int n_tiles) {
-  extern __shared__ double2 s_tile[];  // [nv][NK]
-  constexpr int RG = RowGroup<V>::RG;
+  constexpr int U = kGemmStage;
+  extern __shared__ double2 s_tile[];  // [nvp][NK]
   const int per_xcd = (n_tiles + 7) / 8;
   const int logical = (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);
   if (logical >= n_tiles) return;  // whole workgroup
   const int kt = logical % n_kt, rb = logical / n_kt;
   const int k0 = kt * NK;
   const int nv = A.n0 + A.n1;
-  for (int e = threadIdx.x; e < nv * NK; e += 256) {
+  const int nvp = (nv + 2 * U - 1) / (2 * U) * (2 * U);
+  for (int e = threadIdx.x; e < nvp * NK; e += 256) {
     const int i = e / NK, k = e % NK;
-    s_tile[e] = k0 + k < nk ? S[(int64_t)i * ldS + k0 + k] : make_double2(0.0, 0.0);
+    s_tile[e] = (i < nv && k0 + k < nk) ? S[(int64_t)i * ldS + k0 + k] : make_double2(0.0, 0.0);
   }
   __syncthreads();
-  const int64_t r0 = (int64_t)rb * 256 * RG + (int64_t)threadIdx.x * RG;
-  double pr[RG][NK], pi[RG][NK];
+  const int64_t r = (int64_t)rb * 256 + threadIdx.x;
+  const bool live = r < D;
+  const int64_t rc = live ? r : D;
+  double pr[NK], pi[NK];
 #pragma unroll
-  for (int q = 0; q < RG; ++q)
-#pragma unroll
-    for (int k = 0; k < NK; ++k) pr[q][k] = pi[q][k] = 0.0;
-  for (int i0 = 0; i0 < nv; i0 += U) {
-    double2 a[U][RG];
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      if (i0 + u < nv) {
-        RowGroup<V>::load(A.vec(i0 + u), r0, D, a[u]);
-      } else {
-#pragma unroll
-        for (int q = 0; q < RG; ++q) a[u][q] = make_double2(0.0, 0.0);
-      }
-    }
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int i = min(i0 + u, nv - 1);  // inputs past nv carry zero amplitudes
-#pragma unroll
-      for (int k = 0; k < NK; ++k) {
-        const double2 sv = s_tile[i * NK + k];
-#pragma unroll
-        for (int q = 0; q < RG; ++q) {
-          pr[q][k] = fma(a[u][q].x, sv.x, pr[q][k]);
-          pr[q][k] = fma(-a[u][q].y, sv.y, pr[q][k]);
-          pi[q][k] = fma(a[u][q].x, sv.y, pi[q][k]);
-          pi[q][k] = fma(a[u][q].y, sv.x, pi[q][k]);
-        }
-      }
-    }
+  for (int k = 0; k < NK; ++k) pr[k] = pi[k] = 0.0;
+  V b0[U], b1[U];
+#define EDX_GEMM_LOAD(B, I0)                                                          \
+  {                                                                                   \
+    _Pragma("unroll") for (int u = 0; u < U; ++u) B[u] = A.vec(min((I0) + u, nv - 1))[rc]; \
+    EDX_KEEP_PREFETCH();                                                              \
   }
+#define EDX_GEMM_FMA(B, I0)                                \
+  {                                                        \
+    _Pragma("unroll") for (int u = 0; u < U; ++u) {        \
+      const double2 x = to_d2(B[u]);                       \
+      _Pragma("unroll") for (int k = 0; k < NK; ++k) {     \
+        const double2 sv = s_tile[((I0) + u) * NK + k];    \
+        pr[k] = fma(x.x, sv.x, pr[k]);                     \
+        pr[k] = fma(-x.y, sv.y, pr[k]);                    \
+        pi[k] = fma(x.x, sv.y, pi[k]);                     \
+        pi[k] = fma(x.y, sv.x, pi[k]);                     \
+      }                                                    \
+    }                                                      \
+  }
+  EDX_GEMM_LOAD(b0, 0)
+  for (int i0 = 0; i0 < nvp; i0 += 2 * U) {
+    EDX_GEMM_LOAD(b1, i0 + U)
+    EDX_GEMM_FMA(b0, i0)
+    EDX_GEMM_LOAD(b0, i0 + 2 * U)
+    EDX_GEMM_FMA(b1, i0 + U)
+  }
+#undef EDX_GEMM_LOAD
+#undef EDX_GEMM_FMA
+  if (!live) return;
 #pragma unroll
-  for (int q = 0; q < RG; ++q) {
-    const int64_t r = r0 + q;
-    if (r >= D) continue;
-#pragma unroll
-    for (int k = 0; k < NK; ++k) {
-      if (k0 + k < nk) {
-        const V o = from_d2<V>(pr[q][k], pi[q][k]);
-        Ob[(int64_t)(k0 + k) * ldo + r] = o;
-        if (Xi) Xi[r * nk + k0 + k] = o;  // interleaved copy for the next SpMM
-      }
+  for (int k = 0; k < NK; ++k) {
+    if (k0 + k < nk) {
+      const V o = from_d2<V>(pr[k], pi[k]);
+      Ob[(int64_t)(k0 + k) * ldo + r] = o;
+      if (Xi) Xi[r * nk + k0 + k] = o;  // interleaved copy for the next SpMM
     }
   }
 }
@@ -387,7 +404,11 @@ class BlockLanczos {
     D_ = h.dim;
     b_ = opt.block;
     esz_ = prec == EDX_PREC_FP64 ? 16 : 8;
-    ld_ = (D_ + 63) / 64 * 64;
+    // leading dimension: a multiple of 64 rows with at least one row past D;
+    // rows [D, ld) of every vector are zero (cleared below, never written:
+    // kernels store rows < D only), which the Gram kernel's unconditional
+    // loads rely on
+    ld_ = (D_ + 64) / 64 * 64;
     int base = opt.basis > 0 ? opt.basis : std::max({5 * (opt.nev + b_), 6 * b_, 24});
     base = (base + b_ - 1) / b_ * b_;
     nq_max_ = std::min(base, kMaxVec - 2 * b_);
@@ -406,9 +427,17 @@ class BlockLanczos {
     bank_[1].alloc((size_t)cap_ * ld_ * esz_);
     W_.alloc((size_t)b_ * ld_ * esz_);
     X_.alloc((size_t)b_ * D_ * esz_);
-    // row blocks of >= 16 rows per thread, at most 256 of them
-    n_rowblocks_ = (int)std::min<int64_t>(std::max<int64_t>((D_ + 4095) / 4096, 1), 256);
-    rows_per_block_ = (D_ + n_rowblocks_ - 1) / n_rowblocks_;
+    const size_t pad = (size_t)(ld_ - D_) * esz_;
+    EDX_HIP(hipMemset2DAsync(static_cast<char*>((void*)bank_[0].p) + D_ * esz_, ld_ * esz_, 0, pad, cap_,
+                             ctx_.stream));
+    EDX_HIP(hipMemset2DAsync(static_cast<char*>((void*)bank_[1].p) + D_ * esz_, ld_ * esz_, 0, pad, cap_,
+                             ctx_.stream));
+    EDX_HIP(hipMemset2DAsync(static_cast<char*>((void*)W_.p) + D_ * esz_, ld_ * esz_, 0, pad, b_, ctx_.stream));
+    // Gram row blocks of about 8192 rows (32 per lane), a multiple of 256
+    // rows each, at most 1024 of them
+    n_rowblocks_ = (int)std::min<int64_t>(std::max<int64_t>((D_ + 8191) / 8192, 1), 1024);
+    rows_per_block_ = ((D_ + n_rowblocks_ - 1) / n_rowblocks_ + 255) / 256 * 256;
+    n_rowblocks_ = (int)((D_ + rows_per_block_ - 1) / rows_per_block_);
     partial_.alloc((size_t)n_rowblocks_ * kMaxOut);
     sdev_.alloc((size_t)kSlots * kSlot);
     static_assert(kMaxOut <= edx::kPinnedIn && kSlots * kSlot <= edx::kPinnedOut, "pinned staging too small");
@@ -534,7 +563,7 @@ class BlockLanczos {
     const int n_tiles = n_itiles * n_rowblocks_;
     const dim3 grid((unsigned)((n_tiles + 7) / 8 * 8));
     VecPair<VT> A{(const VT*)A0, ld_, n0, (const VT*)A1, ld_, n1};
-    hipLaunchKernelGGL((gram_tiles<VT, TI, NW>), grid, dim3(256), 0, ctx_.stream, D_, A, (const VT*)W, ld_,
+    hipLaunchKernelGGL((gram_tiles<VT, TI, NW>), grid, dim3(256), 0, ctx_.stream, D_, ld_, A, (const VT*)W, ld_,
                        rows_per_block_, n_itiles, n_tiles, partial_.p);
   }
 
@@ -545,9 +574,10 @@ class BlockLanczos {
   case NWV:                     \
     launch_tiles<VT, TIV, NWV>(A0, n0, A1, n1, W); \
     break;
-      EDX_GRAM_CASE(1, 4) EDX_GRAM_CASE(2, 4) EDX_GRAM_CASE(3, 2) EDX_GRAM_CASE(4, 2)
-      EDX_GRAM_CASE(5, 2) EDX_GRAM_CASE(6, 2) EDX_GRAM_CASE(7, 2) EDX_GRAM_CASE(8, 2)
-      EDX_GRAM_CASE(9, 1) EDX_GRAM_CASE(10, 1) EDX_GRAM_CASE(11, 1) EDX_GRAM_CASE(12, 1)
+      // TI x NW tiles of 16 to 32 complex sums per thread
+      EDX_GRAM_CASE(1, 8) EDX_GRAM_CASE(2, 8) EDX_GRAM_CASE(3, 6) EDX_GRAM_CASE(4, 6)
+      EDX_GRAM_CASE(5, 4) EDX_GRAM_CASE(6, 4) EDX_GRAM_CASE(7, 3) EDX_GRAM_CASE(8, 3)
+      EDX_GRAM_CASE(9, 2) EDX_GRAM_CASE(10, 2) EDX_GRAM_CASE(11, 2) EDX_GRAM_CASE(12, 2)
       EDX_GRAM_CASE(13, 1) EDX_GRAM_CASE(14, 1) EDX_GRAM_CASE(15, 1) EDX_GRAM_CASE(16, 1)
 #undef EDX_GRAM_CASE
       default:
@@ -598,11 +628,11 @@ class BlockLanczos {
     VecPair<VT> A{(const VT*)A0, ld_, n0, (const VT*)A1, ld_, n1};
     const int nv = n0 + n1;
     const int n_kt = (nk + NK - 1) / NK;
-    const int64_t rows = 256 * RowGroup<VT>::RG;
-    const int64_t n_tiles = (int64_t)n_kt * ((D_ + rows - 1) / rows);
+    const int64_t n_tiles = (int64_t)n_kt * ((D_ + 255) / 256);
     EDX_REQUIRE(n_tiles < (1ll << 30), EDX_ERR_INTERNAL, "update grid");
-    hipLaunchKernelGGL((gemm_tiles<VT, NK, 4>), dim3((unsigned)((n_tiles + 7) / 8 * 8)), dim3(256),
-                       (size_t)nv * NK * sizeof(double2), ctx_.stream, D_, A, nk, (VT*)Out, ld_, S, ldS, (VT*)Xi,
+    const int nvp = (nv + 2 * kGemmStage - 1) / (2 * kGemmStage) * (2 * kGemmStage);
+    hipLaunchKernelGGL((gemm_tiles<VT, NK>), dim3((unsigned)((n_tiles + 7) / 8 * 8)), dim3(256),
+                       (size_t)nvp * NK * sizeof(double2), ctx_.stream, D_, A, nk, (VT*)Out, ld_, S, ldS, (VT*)Xi,
                        n_kt, (int)n_tiles);
   }
 
