@@ -1,6 +1,14 @@
 // Context management and device-memory helpers of the edx C ABI.
 #include <cstring>
 
+#include <atomic>
+#include <cstdlib>
+#include <map>
+#include <mutex>
+#include <unordered_map>
+#include <utility>
+#include <vector>
+
 #include "dense.h"
 #include "objects.h"
 
@@ -9,6 +17,189 @@ std::string& tls_error() {
   thread_local std::string s;
   return s;
 }
+}  // namespace edx
+
+namespace edx {
+namespace {
+// Live context streams per device (the memory cache orders the reuse of
+// shared blocks after their work).
+// Immortal (never destroyed): objects may release memory while the process
+// exits, after static destructors have run.
+std::mutex& g_streams_mu = *new std::mutex;
+std::vector<std::pair<int, hipStream_t>>& g_streams = *new std::vector<std::pair<int, hipStream_t>>;
+
+void register_stream(int device, hipStream_t s) {
+  std::lock_guard<std::mutex> lk(g_streams_mu);
+  g_streams.emplace_back(device, s);
+}
+
+void unregister_stream(hipStream_t s) {
+  std::lock_guard<std::mutex> lk(g_streams_mu);
+  for (size_t i = 0; i < g_streams.size(); ++i)
+    if (g_streams[i].second == s) {
+      g_streams.erase(g_streams.begin() + (long)i);
+      break;
+    }
+}
+}  // namespace
+
+namespace {
+struct CachedBlock {
+  void* p;
+  int device;
+  std::vector<hipEvent_t> deps;  // to complete before the block is reused
+};
+std::mutex& g_cache_mu = *new std::mutex;
+std::multimap<size_t, CachedBlock>& g_cache = *new std::multimap<size_t, CachedBlock>;  // free blocks by size
+std::unordered_map<void*, size_t>& g_sizes = *new std::unordered_map<void*, size_t>;  // every block owned -> size
+std::vector<hipEvent_t>& g_events = *new std::vector<hipEvent_t>;                       // spare events
+// Set when the process starts exiting: blocks released afterwards are left to
+// the process teardown (the runtime may already be shutting down).
+std::atomic<bool> g_exiting{false};
+void mark_exiting() { g_exiting.store(true); }
+
+hipEvent_t take_event() {
+  if (!g_events.empty()) {
+    hipEvent_t e = g_events.back();
+    g_events.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+    (void)hipGetLastError();
+    return nullptr;
+  }
+  return e;
+}
+
+size_t round_bytes(size_t b) {
+  if (b <= (1u << 20)) return (b + 4095) / 4096 * 4096;
+  return (b + (2u << 20) - 1) / (2u << 20) * (2u << 20);
+}
+
+// Returns every cached block of the device to the runtime (after its events).
+void drain_cache(int device) {
+  for (auto it = g_cache.begin(); it != g_cache.end();) {
+    if (it->second.device != device) {
+      ++it;
+      continue;
+    }
+    for (hipEvent_t e : it->second.deps) {
+      (void)hipEventSynchronize(e);
+      g_events.push_back(e);
+    }
+    (void)hipFree(it->second.p);
+    g_sizes.erase(it->second.p);
+    it = g_cache.erase(it);
+  }
+}
+}  // namespace
+
+void* cache_alloc(size_t bytes, int device, hipStream_t s) {
+  static const bool registered = [] { return std::atexit(mark_exiting) == 0; }();
+  (void)registered;
+  std::lock_guard<std::mutex> lk(g_cache_mu);
+  const size_t want = round_bytes(bytes);
+  const size_t limit = std::max(want + want / 4, want + (1u << 20));
+  for (auto it = g_cache.lower_bound(want); it != g_cache.end() && it->first <= limit; ++it) {
+    if (it->second.device != device) continue;
+    CachedBlock blk = std::move(it->second);
+    g_cache.erase(it);
+    for (hipEvent_t e : blk.deps) {
+      if (s) EDX_HIP(hipStreamWaitEvent(s, e, 0));
+      else EDX_HIP(hipEventSynchronize(e));
+      g_events.push_back(e);
+    }
+    return blk.p;
+  }
+  void* p = nullptr;
+  hipError_t e = hipMalloc(&p, want);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    drain_cache(device);
+    e = hipMalloc(&p, want);
+  }
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    throw Error{EDX_ERR_OOM, "device allocation of " + std::to_string(bytes) + " bytes failed"};
+  }
+  g_sizes[p] = want;
+  return p;
+}
+
+void cache_free(void* p, int device, hipStream_t s, bool shared) {
+  if (!p || g_exiting.load()) return;
+  std::lock_guard<std::mutex> lk(g_cache_mu);
+  auto sz = g_sizes.find(p);
+  if (sz == g_sizes.end()) return;  // not a cache block
+  CachedBlock blk{p, device, {}};
+  int prev = 0;
+  (void)hipGetDevice(&prev);
+  if (prev != device) (void)hipSetDevice(device);
+  auto record_on = [&](hipStream_t st) {
+    hipEvent_t e = take_event();
+    if (e && hipEventRecord(e, st) == hipSuccess) blk.deps.push_back(e);
+    else if (e) g_events.push_back(e);
+  };
+  if (shared || !s) {
+    // every live context stream (the allocating one may be gone already)
+    std::lock_guard<std::mutex> lk2(g_streams_mu);
+    for (auto& st : g_streams)
+      if (st.first == device) record_on(st.second);
+  } else {
+    record_on(s);
+  }
+  (void)hipGetLastError();
+  if (prev != device) (void)hipSetDevice(prev);
+  g_cache.emplace(sz->second, std::move(blk));
+}
+
+void copy_to_host(Ctx& ctx, void* dst, size_t dpitch, const void* src, size_t spitch, size_t width, size_t rows) {
+  if (!rows || !width) return;
+  for (int i = 0; i < 2; ++i) {
+    if (!ctx.stage[i]) EDX_HIP(hipHostMalloc((void**)&ctx.stage[i], kStageBytes, hipHostMallocDefault));
+    if (!ctx.stage_ev[i]) EDX_HIP(hipEventCreateWithFlags(&ctx.stage_ev[i], hipEventDisableTiming));
+  }
+  // pieces of at most kStageBytes: whole rows when a row fits, else row parts
+  struct Piece {
+    size_t row, off, bytes;
+  };
+  std::vector<Piece> pieces;
+  if (width <= kStageBytes) {
+    const size_t per = kStageBytes / width;
+    for (size_t r = 0; r < rows; r += per) pieces.push_back({r, 0, std::min(per, rows - r) * width});
+  } else {
+    for (size_t r = 0; r < rows; ++r)
+      for (size_t o = 0; o < width; o += kStageBytes) pieces.push_back({r, o, std::min(kStageBytes, width - o)});
+  }
+  auto issue = [&](size_t k) {
+    const Piece& pc = pieces[k];
+    char* buf = ctx.stage[k % 2];
+    if (width <= kStageBytes) {
+      const size_t nr = pc.bytes / width;
+      EDX_HIP(hipMemcpy2DAsync(buf, width, static_cast<const char*>(src) + pc.row * spitch, spitch, width, nr,
+                               hipMemcpyDeviceToHost, ctx.stream));
+    } else {
+      EDX_HIP(hipMemcpyAsync(buf, static_cast<const char*>(src) + pc.row * spitch + pc.off, pc.bytes,
+                             hipMemcpyDeviceToHost, ctx.stream));
+    }
+    EDX_HIP(hipEventRecord(ctx.stage_ev[k % 2], ctx.stream));
+  };
+  issue(0);
+  for (size_t k = 0; k < pieces.size(); ++k) {
+    EDX_HIP(hipEventSynchronize(ctx.stage_ev[k % 2]));
+    if (k + 1 < pieces.size()) issue(k + 1);  // the other buffer was emptied by the previous iteration
+    const Piece& pc = pieces[k];
+    const char* buf = ctx.stage[k % 2];
+    if (width <= kStageBytes) {
+      const size_t nr = pc.bytes / width;
+      for (size_t r = 0; r < nr; ++r) std::memcpy(static_cast<char*>(dst) + (pc.row + r) * dpitch, buf + r * width, width);
+    } else {
+      std::memcpy(static_cast<char*>(dst) + pc.row * dpitch + pc.off, buf, pc.bytes);
+    }
+  }
+}
+
 }  // namespace edx
 
 extern "C" {
@@ -54,6 +245,7 @@ int edx_ctx_create(int32_t device, edx_ctx** out) {
       delete ctx;
       throw edx::Error{EDX_ERR_HIP, std::string("context setup: ") + hipGetErrorString(e)};
     }
+    edx::register_stream(device, ctx->stream);
     *out = ctx;
   });
 }
@@ -64,11 +256,16 @@ int edx_ctx_destroy(edx_ctx* ctx) {
   (void)hipGetDevice(&prev);
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) {
+    edx::unregister_stream(ctx->stream);
     (void)hipStreamSynchronize(ctx->stream);
     (void)hipStreamDestroy(ctx->stream);
   }
   if (ctx->pinned_in) (void)hipHostFree(ctx->pinned_in);
   if (ctx->pinned_out) (void)hipHostFree(ctx->pinned_out);
+  for (int i = 0; i < 2; ++i) {
+    if (ctx->stage[i]) (void)hipHostFree(ctx->stage[i]);
+    if (ctx->stage_ev[i]) (void)hipEventDestroy(ctx->stage_ev[i]);
+  }
   delete ctx;
   (void)hipSetDevice(prev);
   return EDX_OK;

@@ -28,141 +28,33 @@ struct Mat {
   const cplx& operator()(int i, int j) const { return a[(size_t)i * m + j]; }
 };
 
-// Implicit QL on the symmetric tridiagonal (d, e), e[i] coupling i and i+1,
-// rotating the columns of the row-major real matrix z (n x n).
-inline void tridiag_ql(int n, std::vector<double>& d, std::vector<double>& e, std::vector<double>& z) {
-  const double eps = std::numeric_limits<double>::epsilon();
-  if (n > 0) e[n - 1] = 0.0;
-  for (int l = 0; l < n; ++l) {
-    for (int iter = 0;; ++iter) {
-      int m = l;
-      for (; m < n - 1; ++m) {
-        const double dd = std::fabs(d[m]) + std::fabs(d[m + 1]);
-        if (std::fabs(e[m]) <= eps * dd) break;
-      }
-      if (m == l) break;
-      if (iter > 60) throw std::runtime_error("tridiagonal QL did not converge");
-      double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
-      double r = std::hypot(g, 1.0);
-      g = d[m] - d[l] + e[l] / (g + std::copysign(r, g));
-      double s = 1.0, c = 1.0, p = 0.0;
-      bool split = false;
-      for (int i = m - 1; i >= l; --i) {
-        double f = s * e[i];
-        const double b = c * e[i];
-        r = std::hypot(f, g);
-        e[i + 1] = r;
-        if (r == 0.0) {
-          d[i + 1] -= p;
-          e[m] = 0.0;
-          split = true;
-          break;
-        }
-        s = f / r;
-        c = g / r;
-        g = d[i + 1] - p;
-        r = (d[i] - g) * s + 2.0 * c * b;
-        p = s * r;
-        d[i + 1] = g + p;
-        g = c * r - b;
-        for (int k = 0; k < n; ++k) {
-          double* zk = &z[(size_t)k * n];
-          f = zk[i + 1];
-          zk[i + 1] = s * zk[i] + c * f;
-          zk[i] = c * zk[i] - s * f;
-        }
-      }
-      if (split) continue;
-      d[l] -= p;
-      e[l] = g;
-      e[m] = 0.0;
-    }
-  }
-}
+// Selected parts of the eigen-decomposition of the Hermitian matrix given by
+// one triangle of A0 (the lower one, or the upper one with lower=false).
+// Eigenvalues ascending in w.  With ncol > 0, Zc (n x ncol) receives the
+// eigenvectors of the ncol lowest eigenvalues.  With row0 in [0, n), Zr
+// ((n - row0) x n) receives rows row0 .. n-1 of every eigenvector (column c
+// belongs to w[c]); this is what residual norms of a Krylov basis need, and
+// without Zc it is obtained at O(n^2 (n - row0)) cost instead of O(n^3).
+//
+// Method: Householder reduction to a complex tridiagonal matrix (reflectors
+// kept, not multiplied out), a diagonal unitary scaling to a real symmetric
+// tridiagonal one, implicit-shift QL on that with the plane rotations applied
+// only to what is requested (all of z, stored transposed so a rotation touches
+// two contiguous rows, or just the requested rows of Q D), then the reflectors
+// applied to the requested eigenvector columns.  Real and imaginary parts are
+// kept in separate arrays.
+void herm_eig_sel(const Mat& A0, bool lower, std::vector<double>& w, int ncol, Mat* Zc, int row0, Mat* Zr);
 
 // Eigen-decomposition of the Hermitian matrix given by the lower triangle of
-// A (the upper triangle is ignored; with lower=false the upper triangle is
-// used instead).  Eigenvalues ascending in w, eigenvectors as the columns of
-// Z (row-major n x n).
+// A (the upper triangle with lower=false): eigenvalues ascending in w,
+// eigenvectors as the columns of Z (row-major n x n).
 inline void herm_eig(const Mat& A0, std::vector<double>& w, Mat& Z, bool lower = true) {
-  const int n = A0.n;
-  Mat A(n, n);
-  for (int i = 0; i < n; ++i)
-    for (int j = 0; j < n; ++j) {
-      if (i == j) A(i, j) = cplx(A0(i, i).real(), 0.0);
-      else if ((i > j) == lower) A(i, j) = A0(i, j);
-      else A(i, j) = std::conj(A0(j, i));
-    }
-  Mat Q(n, n);
-  for (int i = 0; i < n; ++i) Q(i, i) = 1.0;
-  std::vector<cplx> v(n), p(n), q(n), qv(n);
-  for (int k = 0; k + 2 < n; ++k) {
-    double xn2 = 0.0;
-    for (int i = k + 1; i < n; ++i) xn2 += std::norm(A(i, k));
-    const double xn = std::sqrt(xn2);
-    if (xn == 0.0) continue;
-    const cplx x0 = A(k + 1, k);
-    const cplx ph = std::abs(x0) > 0.0 ? x0 / std::abs(x0) : cplx(1.0, 0.0);
-    const cplx alpha = -ph * xn;
-    std::fill(v.begin(), v.end(), cplx(0.0, 0.0));
-    for (int i = k + 1; i < n; ++i) v[i] = A(i, k);
-    v[k + 1] -= alpha;
-    double vn2 = 0.0;
-    for (int i = k + 1; i < n; ++i) vn2 += std::norm(v[i]);
-    const double vn = std::sqrt(vn2);
-    if (vn == 0.0) continue;
-    for (int i = k + 1; i < n; ++i) v[i] /= vn;
-    // trailing block B = A[k+1.., k+1..]: p = B v, K = v^H p, q = p - K v,
-    // B <- B - 2 (v q^H + q v^H)
-    for (int i = k + 1; i < n; ++i) {
-      cplx s(0.0, 0.0);
-      for (int j = k + 1; j < n; ++j) s += A(i, j) * v[j];
-      p[i] = s;
-    }
-    cplx K(0.0, 0.0);
-    for (int i = k + 1; i < n; ++i) K += std::conj(v[i]) * p[i];
-    for (int i = k + 1; i < n; ++i) q[i] = p[i] - K.real() * v[i];
-    for (int i = k + 1; i < n; ++i)
-      for (int j = k + 1; j < n; ++j)
-        A(i, j) -= 2.0 * (v[i] * std::conj(q[j]) + q[i] * std::conj(v[j]));
-    A(k + 1, k) = alpha;
-    A(k, k + 1) = std::conj(alpha);
-    for (int i = k + 2; i < n; ++i) A(i, k) = A(k, i) = 0.0;
-    // Q <- Q (I - 2 v v^H)
-    for (int r = 0; r < n; ++r) {
-      cplx s(0.0, 0.0);
-      for (int j = k + 1; j < n; ++j) s += Q(r, j) * v[j];
-      qv[r] = s;
-    }
-    for (int r = 0; r < n; ++r)
-      for (int j = k + 1; j < n; ++j) Q(r, j) -= 2.0 * qv[r] * std::conj(v[j]);
+  if (A0.n == 0) {
+    w.clear();
+    Z = Mat();
+    return;
   }
-  std::vector<double> d(n), e(n, 0.0);
-  std::vector<cplx> phi(n, cplx(1.0, 0.0));
-  for (int i = 0; i < n; ++i) d[i] = A(i, i).real();
-  for (int i = 0; i + 1 < n; ++i) {
-    const cplx ei = A(i + 1, i);
-    const double ae = std::abs(ei);
-    e[i] = ae;
-    phi[i + 1] = ae > 0.0 ? phi[i] * (ei / ae) : phi[i];
-  }
-  std::vector<double> z((size_t)n * n, 0.0);
-  for (int i = 0; i < n; ++i) z[(size_t)i * n + i] = 1.0;
-  tridiag_ql(n, d, e, z);
-  std::vector<int> order(n);
-  std::iota(order.begin(), order.end(), 0);
-  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return d[a] < d[b]; });
-  w.resize(n);
-  Z = Mat(n, n);
-  for (int c = 0; c < n; ++c) {
-    const int src = order[c];
-    w[c] = d[src];
-    for (int r = 0; r < n; ++r) {
-      cplx s(0.0, 0.0);
-      for (int i = 0; i < n; ++i) s += Q(r, i) * phi[i] * z[(size_t)i * n + src];
-      Z(r, c) = s;
-    }
-  }
+  herm_eig_sel(A0, lower, w, A0.n, &Z, -1, nullptr);
 }
 
 // Upper Cholesky factor R with A = R^H R.  Returns false when a pivot is not

@@ -55,40 +55,59 @@ struct StreamScope {
   ~StreamScope() { tls_stream() = prev; }
 };
 
-// Device allocation freed in the destructor.  With Ordered (the default) a
-// buffer allocated during an API call is stream-ordered on that call's
-// stream; such buffers are used for work areas that are freed before the
-// call returns.  Arrays owned by the long-lived C-ABI objects (ObjBuf below)
-// use plain hipMalloc / hipFree, so they never refer to a stream that may be
-// destroyed before them.
+struct Ctx;
+// Copies `rows` rows of `width` bytes from device memory (row pitch spitch)
+// to host memory (row pitch dpitch) through the context's pinned staging
+// buffers, on the context's stream; returns when the host copy is complete.
+// Unlike a runtime copy into pageable memory it holds no runtime lock while
+// the host side copies.
+void copy_to_host(Ctx& ctx, void* dst, size_t dpitch, const void* src, size_t spitch, size_t width, size_t rows);
+
+// Device memory cache.  Freeing device memory through the runtime
+// (hipFree, and also hipFreeAsync) takes a runtime-wide lock for
+// milliseconds, during which no thread can launch or synchronise, so
+// concurrent solves stall each other.  Released blocks are therefore kept:
+// cache_free records, on the stream that used the block (or, for arrays that
+// may be shared between contexts, on every live context stream of the
+// device), the events after which it may be reused; cache_alloc hands out the
+// smallest cached block that fits (at most 1/4 larger, or 1 MiB for small
+// requests) and makes the allocating stream wait on those events (the host
+// waits instead when there is no stream).  Only a miss calls hipMalloc; when
+// that fails the cached blocks are returned to the runtime and it is retried.
+void* cache_alloc(size_t bytes, int device, hipStream_t s);
+void cache_free(void* p, int device, hipStream_t s, bool shared);
+
+// Device allocation freed in the destructor, from the memory cache.  Work
+// areas (Ordered, the default) are ordered on the stream of the API call that
+// allocates them; arrays owned by the long-lived C-ABI objects (ObjBuf below)
+// may be used by other contexts and freed from any thread, so their blocks
+// become reusable only after the work of every live stream at release time.
 template <class T, bool Ordered = true>
 struct DevBuf {
   T* p = nullptr;
   size_t n = 0;
   hipStream_t st = nullptr;
+  int dev = 0;
   DevBuf() = default;
   explicit DevBuf(size_t count) { alloc(count); }
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
-  DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n), st(o.st) { o.p = nullptr; o.n = 0; }
+  DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n), st(o.st), dev(o.dev) { o.p = nullptr; o.n = 0; }
   DevBuf& operator=(DevBuf&& o) noexcept {
-    if (this != &o) { release(); p = o.p; n = o.n; st = o.st; o.p = nullptr; o.n = 0; }
+    if (this != &o) { release(); p = o.p; n = o.n; st = o.st; dev = o.dev; o.p = nullptr; o.n = 0; }
     return *this;
   }
   ~DevBuf() { release(); }
   void alloc(size_t count) {
     release();
     n = count;
-    st = Ordered ? tls_stream() : nullptr;
+    st = tls_stream();
     if (!count) return;
-    if (st) EDX_HIP(hipMallocAsync((void**)&p, count * sizeof(T), st));
-    else EDX_HIP(hipMalloc(&p, count * sizeof(T)));
+    EDX_HIP(hipGetDevice(&dev));
+    p = static_cast<T*>(cache_alloc(count * sizeof(T), dev, st));
   }
   void release() {
-    if (p) {
-      if (st) (void)hipFreeAsync(p, st);
-      else (void)hipFree(p);
-    }
+    if (p) cache_free(p, dev, st, !Ordered);
     p = nullptr;
     n = 0;
   }
@@ -179,11 +198,30 @@ struct Ctx {
   double2* pinned_in = nullptr;      // coherent: kernels write Gram results into it directly
   double2* pinned_in_dev = nullptr;  // its device address
   double2* pinned_out = nullptr;
+  char* stage[2] = {nullptr, nullptr};  // pinned staging for device -> host copies (lazy)
+  hipEvent_t stage_ev[2] = {nullptr, nullptr};
 };
 
 // Sizes of the context's pinned staging buffers (double2 elements).
-constexpr size_t kPinnedIn = (128 + 16) * 16;
-constexpr size_t kPinnedOut = 4 * (128 + 16) * 16;
+// pinned_in holds a Gram result ((128 + 16) x 16), the factor written by the
+// device Cholesky of a block (16 x 16) and its status word, then a ring of
+// kLogSlots block-step logs: projection coefficients ((128 + 16) x 16), the
+// block coupling R (16 x 16) and four words of status.
+constexpr size_t kPinnedGram = (128 + 16) * 16;
+constexpr size_t kPinnedFactor = 16 * 16 + 1;
+constexpr int kLogSlots = 16;
+constexpr size_t kLogC = (128 + 16) * 16;
+constexpr size_t kLogSlot = kLogC + 16 * 16 + 4;
+constexpr size_t kPinnedIn = kPinnedGram + kPinnedFactor + kLogSlots * kLogSlot;
+// pinned_out: a ring of four update-coefficient slots ((128 + 16) x 16
+// each), then the coefficients of a restart combination (up to 144 x 144).
+constexpr size_t kPinnedRing = 4 * (128 + 16) * 16;
+constexpr size_t kPinnedCombine = (128 + 16) * (128 + 16);
+constexpr size_t kPinnedOut = kPinnedRing + kPinnedCombine;
+// Device -> host copies into pageable memory go through two pinned staging
+// buffers of this many bytes each (owned by the context, allocated on first
+// use): DMA into one while the host copies the other out.
+constexpr size_t kStageBytes = 8u << 20;
 
 // Scoped device selection for API calls.
 struct DeviceGuard {

@@ -22,6 +22,7 @@
 // FP64 projections, small-matrix algebra and Ritz problems, then restarts in
 // FP64 from the complex64 Ritz vectors to reach the requested tolerance.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -45,6 +46,7 @@ namespace {
 
 constexpr int kMaxVec = 128;     // basis vectors the Gram/update kernels accept
 constexpr int kMaxW = 16;        // block width
+constexpr size_t kLogC = edx::kLogC;
 
 __device__ __forceinline__ double2 to_d2(double2 v) { return v; }
 __device__ __forceinline__ double2 to_d2(float2 v) { return make_double2((double)v.x, (double)v.y); }
@@ -67,6 +69,24 @@ struct VecPair {
   __device__ __forceinline__ const V* vec(int i) const {
     return i < n0 ? a0 + (int64_t)i * ld0 : a1 + (int64_t)(i - n0) * ld1;
   }
+};
+
+// Launch predicate of the kernels of a device block step.  ctl[0] is the
+// step's status (0: one Cholesky QR pass, 1: a second pass follows on the
+// device, 2: the host takes over the step), ctl[1] is set once a step has
+// been handed to the host: every later predicated kernel then returns at once
+// until the host clears it.  mode 0 always runs; mode 1 runs unless halted;
+// mode 2 (second Cholesky QR pass) runs when not halted and ctl[0] == 1.
+struct StepPred {
+  const int* ctl = nullptr;
+  int mode = 0;
+  __device__ __forceinline__ bool run() const {
+    if (!ctl || mode == 0) return true;
+    if (ctl[1]) return false;
+    return mode == 1 || ctl[0] == 1;
+  }
+  // the first update writes the interleaved copy only when it is final
+  __device__ __forceinline__ bool final_update() const { return !ctl || mode != 1 || ctl[0] == 0; }
 };
 
 // Sum of NV per-lane values over the 64 lanes of a wavefront by halving: at
@@ -132,9 +152,10 @@ struct WaveSplitSum<N, 0> {
 template <typename V, int TI, int NW>
 __global__ void __launch_bounds__(256)
 gram_tiles(int64_t D, int64_t ld, VecPair<V> A, const V* __restrict__ Wb, int64_t ldw, int64_t rows_per_block,
-           int n_itiles, int n_tiles, double2* __restrict__ partial) {
+           int n_itiles, int n_tiles, double2* __restrict__ partial, StepPred pred) {
   constexpr int NV = TI * NW * 2;
   __shared__ double red[4][NV];
+  if (!pred.run()) return;
   const int per_xcd = (n_tiles + 7) / 8;
   const int logical = (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);
   if (logical >= n_tiles) return;
@@ -211,7 +232,9 @@ gram_tiles(int64_t D, int64_t ld, VecPair<V> A, const V* __restrict__ Wb, int64_
 // out[o] = sum over blocks of partial[blk][o]; one wavefront per output,
 // lanes stride the blocks and the sum is finished by a fixed butterfly, so
 // the result does not depend on scheduling.
-__global__ void __launch_bounds__(256) gram_reduce(int nblk, int nout, const double2* partial, double2* out) {
+__global__ void __launch_bounds__(256) gram_reduce(int nblk, int nout, const double2* partial, double2* out,
+                                                   StepPred pred) {
+  if (!pred.run()) return;
   const int o = blockIdx.x * 4 + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   double xr = 0.0, xi = 0.0;
@@ -248,9 +271,11 @@ constexpr int kGemmStage = 4;
 template <typename V, int NK>
 __global__ void __launch_bounds__(256)
 gemm_tiles(int64_t D, VecPair<V> A, int nk, V* Ob, int64_t ldo, const double2* __restrict__ S, int ldS, V* Xi,
-           int n_kt, int n_tiles) {
+           int n_kt, int n_tiles, StepPred pred) {
   constexpr int U = kGemmStage;
   extern __shared__ double2 s_tile[];  // [nvp][NK]
+  if (!pred.run()) return;
+  if (!pred.final_update()) Xi = nullptr;
   const int per_xcd = (n_tiles + 7) / 8;
   const int logical = (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);
   if (logical >= n_tiles) return;  // whole workgroup
@@ -305,6 +330,198 @@ gemm_tiles(int64_t D, VecPair<V> A, int nk, V* Ob, int64_t ldo, const double2* _
       Ob[(int64_t)(k0 + k) * ldo + r] = o;
       if (Xi) Xi[r * nk + k0 + k] = o;  // interleaved copy for the next SpMM
     }
+  }
+}
+
+// Second Cholesky QR pass of a block, on the device: G (b x b, row-major,
+// Hermitian) = R^H R with R upper triangular, S = R^-1 (row-major, the
+// coefficients of the update dst <- dst S) and R copied to Rout (host-visible)
+// for the projected matrix.  One wavefront; column j of R is formed after
+// column j - 1 (lanes take the entries of a row), then lane j forms column j
+// of R^-1.  The formulas are those of the host factorisation
+// (edx::dense::cholesky_upper, upper_inverse).  A pivot at or below
+// tiny * max diag sets status[0] = 1 and S = I (the update then leaves dst
+// unchanged); the caller only takes this path when its error estimate makes
+// the factorisation safe, and treats status 1 as an internal error.
+__global__ void __launch_bounds__(64) block_cholinv(int b, const double2* __restrict__ G, double tiny,
+                                                   double2* __restrict__ S, double2* __restrict__ Rout,
+                                                   int* __restrict__ status, const double2* __restrict__ R1,
+                                                   StepPred pred) {
+  __shared__ double2 A[kMaxW][kMaxW], R[kMaxW][kMaxW], X[kMaxW][kMaxW];
+  if (!pred.run()) return;
+  const int lane = threadIdx.x;
+  for (int e = lane; e < kMaxW * kMaxW; e += 64) {
+    const int i = e / kMaxW, j = e % kMaxW;
+    A[i][j] = (i < b && j < b) ? G[i * b + j] : make_double2(0.0, 0.0);
+    R[i][j] = X[i][j] = make_double2(0.0, 0.0);
+  }
+  __syncthreads();
+  double dmax = 0.0;
+  for (int i = 0; i < b; ++i) dmax = fmax(dmax, A[i][i].x);
+  bool ok = dmax > 0.0;
+  for (int j = 0; j < b; ++j) {
+    double s = A[j][j].x;
+    for (int k = 0; k < j; ++k) s -= R[k][j].x * R[k][j].x + R[k][j].y * R[k][j].y;
+    if (!(s > tiny * dmax)) ok = false;
+    const double rjj = sqrt(fmax(s, tiny * dmax > 0.0 ? tiny * dmax : 1.0));
+    const int i = lane;
+    if (i > j && i < b) {
+      double tr = A[j][i].x, ti = A[j][i].y;
+      for (int k = 0; k < j; ++k) {  // t -= conj(R(k,j)) R(k,i)
+        const double2 a = R[k][j], c = R[k][i];
+        tr -= a.x * c.x + a.y * c.y;
+        ti -= a.x * c.y - a.y * c.x;
+      }
+      R[j][i] = make_double2(tr / rjj, ti / rjj);
+    }
+    if (lane == 0) R[j][j] = make_double2(rjj, 0.0);
+    __syncthreads();
+  }
+  if (lane < b) {
+    const int j = lane;
+    X[j][j] = make_double2(1.0 / R[j][j].x, 0.0);
+    for (int i = j - 1; i >= 0; --i) {
+      double sr = 0.0, si = 0.0;
+      for (int k = i + 1; k <= j; ++k) {
+        const double2 a = R[i][k], c = X[k][j];
+        sr += a.x * c.x - a.y * c.y;
+        si += a.x * c.y + a.y * c.x;
+      }
+      X[i][j] = make_double2(-sr / R[i][i].x, -si / R[i][i].x);
+    }
+  }
+  __syncthreads();
+  for (int e = lane; e < b * b; e += 64) {
+    const int i = e / b, j = e % b;
+    S[e] = ok ? X[i][j] : make_double2(i == j ? 1.0 : 0.0, 0.0);
+    if (R1) {  // Rout = R R1 (both upper triangular)
+      double sr = 0.0, si = 0.0;
+      for (int k = i; k <= j; ++k) {
+        const double2 a = R[i][k], c = R1[k * b + j];
+        sr += a.x * c.x - a.y * c.y;
+        si += a.x * c.y + a.y * c.x;
+      }
+      Rout[e] = make_double2(sr, si);
+    } else {
+      Rout[e] = R[i][j];
+    }
+  }
+  if (lane == 0) status[0] = ok ? 0 : 1;
+}
+
+// Coefficients of a block step from its Gram P = [Q, W]^H W ((nbasis + b) x b,
+// row-major; Q the nbasis basis vectors, W = H Vr), as the host computes them
+// (extend): C = P[0, nbasis), G = W^H W - C^H C, drop = max_j (W^H W)_jj /
+// G_jj, G = R1^H R1, and S = [-C R1^-1; R1^-1] so that [Q, W] S =
+// (W - Q C) R1^-1.  The status (ctl[0]) is 0 when the error estimate
+// unit_roundoff * max(drop, 1) * cond(R1)^2 is within single_tol (one pass
+// suffices), 1 when it is within 1/4 (a second pass on the device is safe),
+// and 2 otherwise (also instead of 1 when no device second pass follows,
+// second_pass = 0), or when drop >= kappa2 (a second projection is needed)
+// or G is not positive definite; status 2 also sets the halt word ctl[1], and
+// the host finishes the step.  The log receives C, R1 and the status.
+__global__ void __launch_bounds__(256)
+step_coeffs(const double2* __restrict__ P, int nbasis, int b, double kappa2, double tiny, double uround,
+            double single_tol, int second_pass, double2* __restrict__ S, double2* __restrict__ R1out,
+            double2* __restrict__ log, int* __restrict__ ctl) {
+  __shared__ double2 G[kMaxW][kMaxW], R[kMaxW][kMaxW], X[kMaxW][kMaxW];
+  __shared__ int status_s;
+  if (ctl[1]) return;  // halted: the host finishes an earlier step first
+  const int tid = threadIdx.x;
+  for (int e = tid; e < nbasis * b; e += 256) log[e] = P[e];
+  for (int e = tid; e < kMaxW * kMaxW; e += 256) {
+    const int i = e / kMaxW, j = e % kMaxW;
+    double2 g = make_double2(0.0, 0.0);
+    if (i < b && j < b) {
+      g = P[(nbasis + i) * b + j];
+      for (int k = 0; k < nbasis; ++k) {  // g -= conj(P(k,i)) P(k,j)
+        const double2 a = P[k * b + i], c = P[k * b + j];
+        g.x -= a.x * c.x + a.y * c.y;
+        g.y -= a.x * c.y - a.y * c.x;
+      }
+    }
+    G[i][j] = g;
+    R[i][j] = X[i][j] = make_double2(0.0, 0.0);
+  }
+  __syncthreads();
+  double drop = 0.0, dmax = 0.0;
+  for (int j = 0; j < b; ++j) {
+    const double g = G[j][j].x;
+    drop = fmax(drop, g > 0.0 ? P[(nbasis + j) * b + j].x / g : INFINITY);
+    dmax = fmax(dmax, g);
+  }
+  bool ok = dmax > 0.0;
+  for (int j = 0; j < b; ++j) {
+    double sd = G[j][j].x;
+    for (int k = 0; k < j; ++k) sd -= R[k][j].x * R[k][j].x + R[k][j].y * R[k][j].y;
+    if (!(sd > tiny * dmax)) ok = false;
+    const double rjj = sqrt(fmax(sd, dmax > 0.0 ? tiny * dmax : 1.0));
+    const int i = tid;
+    if (i > j && i < b) {
+      double tr = G[j][i].x, ti = G[j][i].y;
+      for (int k = 0; k < j; ++k) {
+        const double2 a = R[k][j], c = R[k][i];
+        tr -= a.x * c.x + a.y * c.y;
+        ti -= a.x * c.y - a.y * c.x;
+      }
+      R[j][i] = make_double2(tr / rjj, ti / rjj);
+    }
+    if (tid == 0) R[j][j] = make_double2(rjj, 0.0);
+    __syncthreads();
+  }
+  if (tid < b) {
+    const int j = tid;
+    X[j][j] = make_double2(1.0 / R[j][j].x, 0.0);
+    for (int i = j - 1; i >= 0; --i) {
+      double sr = 0.0, si = 0.0;
+      for (int k = i + 1; k <= j; ++k) {
+        const double2 a = R[i][k], c = X[k][j];
+        sr += a.x * c.x - a.y * c.y;
+        si += a.x * c.y + a.y * c.x;
+      }
+      X[i][j] = make_double2(-sr / R[i][i].x, -si / R[i][i].x);
+    }
+  }
+  __syncthreads();
+  if (tid == 0) {
+    double fr = 0.0, fx = 0.0;
+    for (int i = 0; i < b; ++i)
+      for (int j = 0; j < b; ++j) {
+        fr += R[i][j].x * R[i][j].x + R[i][j].y * R[i][j].y;
+        fx += X[i][j].x * X[i][j].x + X[i][j].y * X[i][j].y;
+      }
+    const double cond = sqrt(fr) * sqrt(fx);
+    const double est = uround * fmax(drop, 1.0) * cond * cond;
+    int st = 2;
+    if (ok && drop < kappa2) st = est <= single_tol ? 0 : (est <= 0.25 && second_pass ? 1 : 2);
+    status_s = st;
+    double2* tail = log + kLogC + kMaxW * kMaxW;
+    tail[0] = make_double2((double)st, 0.0);
+    tail[1] = make_double2(drop, cond);
+    ctl[0] = st;
+    if (st == 2) ctl[1] = 1;
+  }
+  __syncthreads();
+  if (status_s == 2) return;
+  for (int e = tid; e < b * b; e += 256) {
+    R1out[e] = R[e / b][e % b];
+    log[kLogC + e] = R[e / b][e % b];
+  }
+  for (int e = tid; e < (nbasis + b) * b; e += 256) {
+    const int i = e / b, j = e % b;
+    double2 v;
+    if (i < nbasis) {  // -sum_k C(i,k) R1^-1(k,j)
+      double sr = 0.0, si = 0.0;
+      for (int k = 0; k <= j; ++k) {
+        const double2 a = P[i * b + k], c = X[k][j];
+        sr += a.x * c.x - a.y * c.y;
+        si += a.x * c.y + a.y * c.x;
+      }
+      v = make_double2(-sr, -si);
+    } else {
+      v = X[i - nbasis][j];
+    }
+    S[e] = v;
   }
 }
 
@@ -389,7 +606,15 @@ struct Result {
   int64_t matvecs = 0;
   int restarts = 0;
   double max_rel_res = 0.0;
+  // host time (ms): whole run, waiting on the stream, Ritz problems
+  double ms_total = 0.0, ms_wait = 0.0, ms_ritz = 0.0;
+  int syncs = 0, checks = 0;
 };
+
+// Wall-clock milliseconds since an arbitrary origin.
+inline double now_ms() {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
 
 // Block Lanczos on one precision.  Vectors live in two banks (ping-pong for
 // the thick restart), each holding `cap` vectors of leading dimension ld.
@@ -440,10 +665,23 @@ class BlockLanczos {
     n_rowblocks_ = (int)((D_ + rows_per_block_ - 1) / rows_per_block_);
     partial_.alloc((size_t)n_rowblocks_ * kMaxOut);
     sdev_.alloc((size_t)kSlots * kSlot);
-    static_assert(kMaxOut <= edx::kPinnedIn && kSlots * kSlot <= edx::kPinnedOut, "pinned staging too small");
+    static_assert(kMaxOut <= edx::kPinnedGram && kSlots * kSlot <= edx::kPinnedRing, "pinned staging too small");
+    static_assert(kMaxW * kMaxW + 1 <= edx::kPinnedFactor, "pinned factor area too small");
+    static_assert((size_t)(kMaxVec + kMaxW) * kMaxW <= edx::kLogC && edx::kLogSlot >= edx::kLogC + kMaxW * kMaxW + 4,
+                  "block step log slot too small");
     hc_ = ctx.pinned_in;
     hc_dev_ = ctx.pinned_in_dev;
+    hf_ = ctx.pinned_in + edx::kPinnedGram;
+    hf_dev_ = ctx.pinned_in_dev + edx::kPinnedGram;
     hs_ = ctx.pinned_out;
+    fdev_.alloc((size_t)2 * kMaxW * kMaxW);
+    pbuf_.alloc((size_t)kMaxOut);
+    sbuf_.alloc((size_t)kMaxOut);
+    r1buf_.alloc((size_t)kMaxW * kMaxW);
+    g2buf_.alloc((size_t)kMaxW * kMaxW);
+    s2buf_.alloc((size_t)kMaxW * kMaxW);
+    ctl_.alloc(4);
+    EDX_HIP(hipMemsetAsync(ctl_.p, 0, 4 * sizeof(int), ctx_.stream));
   }
   BlockLanczos(const BlockLanczos&) = delete;
   BlockLanczos& operator=(const BlockLanczos&) = delete;
@@ -452,6 +690,7 @@ class BlockLanczos {
   // column-major with leading dimension start_ld) seeds the first block with
   // n_start vectors (rest random).  Ritz vectors are left in vec(0..nev).
   Result run(const void* start, int n_start, int64_t start_ld) {
+    const double t_start = now_ms();
     Result res;
     cur_ = 0;
     int nq = 0;
@@ -464,39 +703,52 @@ class BlockLanczos {
     start_block();
     Mat M(cap_, cap_), B(b_, cap_);
     std::vector<double> theta;
-    Mat Y;
     int since_check = 0;
+    std::vector<int> pending;  // basis size before each block step enqueued since the last sync
     for (;;) {
-      // W = H Vr   (Vr at vec(nq), interleaved copy in X_)
+      // one block step on the device: W = H Vr (Vr at vec(nq), interleaved
+      // copy in X_), projection, coefficients, update; the host does not wait
       edx::spmm(h_, ctx_, prec_, b_, X_.p, W_.p, ld_);
-      res.matvecs += b_;
-      const int nbasis = nq + b_;
-      void* Q = vec(cur_, 0);
-      Mat C, R;
-      extend(Q, nbasis, vec(cur_, nbasis), C, R);
-      for (int i = 0; i < nq; ++i)
-        for (int j = 0; j < b_; ++j) {
-          M(i, nq + j) = C(i, j);
-          M(nq + j, i) = std::conj(C(i, j));
-        }
-      for (int i = 0; i < b_; ++i)
-        for (int j = 0; j < b_; ++j) M(nq + i, nq + j) = 0.5 * (C(nq + i, j) + std::conj(C(nq + j, i)));
+      enqueue_step(nq, (int)pending.size());
+      pending.push_back(nq);
       nq += b_;
+      bool full = nq + b_ > nq_max_;
+      bool due = nq >= opt_.nev && (full || ++since_check >= opt_.check_every);
+      if (!due && (int)pending.size() < edx::kLogSlots) continue;
+      wait();
+      settle(true);
+      const int n_enq = (int)pending.size();
+      int n_acc = 0;
+      nq = absorb(pending, M, res, n_acc);
+      pending.clear();
+      if (n_acc < n_enq) {  // a step was finished on the host; the later ones were skipped
+        since_check = std::max(0, since_check - (n_enq - n_acc));
+        full = nq + b_ > nq_max_;
+        due = nq >= opt_.nev && (full || since_check >= opt_.check_every);
+      }
+      if (!due) continue;
+      since_check = 0;
+      // coupling of the newest block: H Q = Q M + Vr B with B = [0, R]
+      settle(false);
       B = Mat(b_, cap_);
       for (int i = 0; i < b_; ++i)
-        for (int j = 0; j < b_; ++j) B(i, nq - b_ + j) = R(i, j);
-      if (nq < opt_.nev) continue;
-      const bool full = nq + b_ > nq_max_;
-      if (!full && ++since_check < opt_.check_every) continue;
-      since_check = 0;
-      ritz(M, nq, theta, Y);
+        for (int j = 0; j < b_; ++j) B(i, nq - b_ + j) = Rstep_(i, j);
       const int nev = opt_.nev;
+      // vectors kept by a thick restart
+      int k = nev + (nq - nev) / 2;
+      k = std::max(k, nev + std::min(b_, nq - nev));
+      k = std::min(k, nq - b_);
+      k = std::max(k, nev);
+      // Ritz values, rows nq-b .. nq-1 of every Ritz vector (the residual
+      // norms need only those) and, when restarting, the k leading vectors
+      Mat Yl, Yk;
+      ritz(M, nq, theta, full ? k : 0, full ? &Yk : nullptr, &Yl);
       std::vector<double> rn(nq);
       for (int c = 0; c < nq; ++c) {
         double rn2 = 0.0;
         for (int i = 0; i < b_; ++i) {
           cplx sum(0.0, 0.0);
-          for (int j = nq - b_; j < nq; ++j) sum += B(i, j) * Y(j, c);
+          for (int j = 0; j < b_; ++j) sum += B(i, nq - b_ + j) * Yl(j, c);
           rn2 += std::norm(sum);
         }
         rn[c] = std::sqrt(rn2);
@@ -505,27 +757,26 @@ class BlockLanczos {
       const bool conv = converged(theta, rn, nev, opt_.tol, opt_.eig_tol, &worst);
       res.max_rel_res = worst;
       if (conv) {
-        Mat Yk(nq, nev);
-        for (int i = 0; i < nq; ++i)
-          for (int c = 0; c < nev; ++c) Yk(i, c) = Y(i, c);
-        combine(vec(cur_, 0), nq, vec(1 - cur_, 0), nev, Yk);
+        Mat Yn;
+        if (full) {
+          Yn = Mat(nq, nev);
+          for (int i = 0; i < nq; ++i)
+            for (int c = 0; c < nev; ++c) Yn(i, c) = Yk(i, c);
+        } else {
+          std::vector<double> th2;
+          ritz(M, nq, th2, nev, &Yn, nullptr);
+        }
+        combine(vec(cur_, 0), nq, vec(1 - cur_, 0), nev, Yn);
         cur_ = 1 - cur_;
         res.vals.assign(theta.begin(), theta.begin() + nev);
         res.restarts = restarts_;
-        EDX_HIP(hipStreamSynchronize(ctx_.stream));
+        finish(res, t_start);
         return res;
       }
       if (full) {
         EDX_REQUIRE(restarts_ < opt_.max_restarts, EDX_ERR_NOCONV,
                     "block Lanczos did not converge within " + std::to_string(opt_.max_restarts) +
                         " restarts (max relative residual " + std::to_string(worst) + ")");
-        int k = nev + (nq - nev) / 2;
-        k = std::max(k, nev + std::min(b_, nq - nev));
-        k = std::min(k, nq - b_);
-        k = std::max(k, nev);
-        Mat Yk(nq, k);
-        for (int i = 0; i < nq; ++i)
-          for (int c = 0; c < k; ++c) Yk(i, c) = Y(i, c);
         combine(vec(cur_, 0), nq, vec(1 - cur_, 0), k, Yk);
         EDX_HIP(hipMemcpy2DAsync(vec(1 - cur_, k), ld_ * esz_, vec(cur_, nq), ld_ * esz_, D_ * esz_, b_,
                                  hipMemcpyDeviceToDevice, ctx_.stream));
@@ -533,7 +784,7 @@ class BlockLanczos {
         for (int i = 0; i < b_; ++i)
           for (int c = 0; c < k; ++c) {
             cplx sum(0.0, 0.0);
-            for (int j = nq - b_; j < nq; ++j) sum += B(i, j) * Y(j, c);
+            for (int j = 0; j < b_; ++j) sum += B(i, nq - b_ + j) * Yl(j, c);
             Bn(i, c) = sum;
           }
         M = Mat(cap_, cap_);
@@ -557,22 +808,22 @@ class BlockLanczos {
   static constexpr size_t kSlot = (size_t)kMaxOut;
 
   template <typename VT, int TI, int NW>
-  void launch_tiles(const void* A0, int n0, const void* A1, int n1, const void* W) {
+  void launch_tiles(const void* A0, int n0, const void* A1, int n1, const void* W, StepPred pred) {
     const int nv = n0 + n1;
     const int n_itiles = (nv + TI - 1) / TI;
     const int n_tiles = n_itiles * n_rowblocks_;
     const dim3 grid((unsigned)((n_tiles + 7) / 8 * 8));
     VecPair<VT> A{(const VT*)A0, ld_, n0, (const VT*)A1, ld_, n1};
     hipLaunchKernelGGL((gram_tiles<VT, TI, NW>), grid, dim3(256), 0, ctx_.stream, D_, ld_, A, (const VT*)W, ld_,
-                       rows_per_block_, n_itiles, n_tiles, partial_.p);
+                       rows_per_block_, n_itiles, n_tiles, partial_.p, pred);
   }
 
   template <typename VT>
-  void launch_tiles_nw(const void* A0, int n0, const void* A1, int n1, const void* W, int nw) {
+  void launch_tiles_nw(const void* A0, int n0, const void* A1, int n1, const void* W, int nw, StepPred pred) {
     switch (nw) {
 #define EDX_GRAM_CASE(NWV, TIV) \
   case NWV:                     \
-    launch_tiles<VT, TIV, NWV>(A0, n0, A1, n1, W); \
+    launch_tiles<VT, TIV, NWV>(A0, n0, A1, n1, W, pred); \
     break;
       // TI x NW tiles of 16 to 32 complex sums per thread
       EDX_GRAM_CASE(1, 8) EDX_GRAM_CASE(2, 8) EDX_GRAM_CASE(3, 6) EDX_GRAM_CASE(4, 6)
@@ -590,22 +841,72 @@ class BlockLanczos {
     const int nv = n0 + n1;
     EDX_REQUIRE(nv <= kMaxVec + kMaxW && nw <= kMaxW, EDX_ERR_INTERNAL, "gram size");
     hipEvent_t t0 = ctx_.timer.start(ctx_.stream);
-    if (prec_ == EDX_PREC_FP64) launch_tiles_nw<double2>(A0, n0, A1, n1, W, nw);
-    else launch_tiles_nw<float2>(A0, n0, A1, n1, W, nw);
+    if (prec_ == EDX_PREC_FP64) launch_tiles_nw<double2>(A0, n0, A1, n1, W, nw, StepPred{});
+    else launch_tiles_nw<float2>(A0, n0, A1, n1, W, nw, StepPred{});
     EDX_HIP(hipGetLastError());
     const int nout = nv * nw;
     // the reduction writes the result straight into the pinned host buffer
     hipLaunchKernelGGL(gram_reduce, dim3((nout + 3) / 4), dim3(256), 0, ctx_.stream, n_rowblocks_, nout,
-                       (const double2*)partial_.p, hc_dev_);
+                       (const double2*)partial_.p, hc_dev_, StepPred{});
     EDX_HIP(hipGetLastError());
     ctx_.timer.stop(ctx_.stream, t0, edx::K_GRAM, (double)(nv + nw) * D_ * esz_);
-    EDX_HIP(hipStreamSynchronize(ctx_.stream));
+    wait();
+    settle(true);
     Mat C(nv, nw);
     for (int i = 0; i < nv; ++i)
       for (int j = 0; j < nw; ++j) C(i, j) = cplx(hc_[i * nw + j].x, hc_[i * nw + j].y);
     return C;
   }
 
+  // Device copy of [A0, A1]^H W ((n0 + n1) x nw, row-major) at out; no sync.
+  void gram_device(const void* A0, int n0, const void* A1, int n1, const void* W, int nw, double2* out,
+                   StepPred pred = StepPred{}) {
+    const int nv = n0 + n1;
+    EDX_REQUIRE(nv <= kMaxVec + kMaxW && nw <= kMaxW, EDX_ERR_INTERNAL, "gram size");
+    hipEvent_t t0 = ctx_.timer.start(ctx_.stream);
+    if (prec_ == EDX_PREC_FP64) launch_tiles_nw<double2>(A0, n0, A1, n1, W, nw, pred);
+    else launch_tiles_nw<float2>(A0, n0, A1, n1, W, nw, pred);
+    EDX_HIP(hipGetLastError());
+    const int nout = nv * nw;
+    hipLaunchKernelGGL(gram_reduce, dim3((nout + 3) / 4), dim3(256), 0, ctx_.stream, n_rowblocks_, nout,
+                       (const double2*)partial_.p, out, pred);
+    EDX_HIP(hipGetLastError());
+    ctx_.timer.stop(ctx_.stream, t0, edx::K_GRAM, (double)(nv + nw) * D_ * esz_);
+  }
+
+  // Second Cholesky QR pass of the block at dst without a host round trip:
+  // Gram, factorisation (block_cholinv) and update dst <- dst R2^-1 with its
+  // interleaved copy, all enqueued; R2 lands in the pinned factor area and
+  // R = R2 R1 is formed by settle() once the stream has passed it.
+  void cholqr_pass_device(void* dst, const Mat& R1, Mat* Rtarget) {
+    double2* g2 = fdev_.p;
+    double2* s2 = fdev_.p + kMaxW * kMaxW;
+    gram_device(dst, b_, nullptr, 0, dst, b_, g2);
+    hipLaunchKernelGGL(block_cholinv, dim3(1), dim3(64), 0, ctx_.stream, b_, (const double2*)g2, 1e-8, s2, hf_dev_,
+                       reinterpret_cast<int*>(hf_dev_ + kMaxW * kMaxW), (const double2*)nullptr, StepPred{});
+    EDX_HIP(hipGetLastError());
+    launch_gemm(nullptr, 0, dst, b_, dst, b_, s2, b_, X_.p);
+    r_pending_ = true;
+    r1_pending_ = R1;
+    r_target_ = Rtarget;
+  }
+
+ public:
+  // Completes a pending device Cholesky pass (stream synchronised first
+  // unless the caller just did): R = R2 R1 into the step's coupling.
+  void settle(bool synced) {
+    if (!r_pending_) return;
+    if (!synced) wait();
+    r_pending_ = false;
+    const int st = *reinterpret_cast<volatile int*>(hf_ + kMaxW * kMaxW);
+    EDX_REQUIRE(st == 0, EDX_ERR_INTERNAL, "device Cholesky QR pass met a non-positive pivot");
+    Mat R2(b_, b_);
+    for (int i = 0; i < b_; ++i)
+      for (int j = 0; j < b_; ++j) R2(i, j) = cplx(hf_[i * b_ + j].x, hf_[i * b_ + j].y);
+    *r_target_ = mul(R2, r1_pending_);
+  }
+
+ private:
   // Out[:, 0..nk) = [A0, A1] S with S ((n0 + n1) x nk) given on the host.  S
   // goes through a ring of pinned staging slots; the stream is synchronised
   // (by a Gram) between any four consecutive uploads, so a slot is never
@@ -624,7 +925,7 @@ class BlockLanczos {
 
   template <typename VT, int NK>
   void launch_gemm_t(const void* A0, int n0, const void* A1, int n1, void* Out, int nk, const double2* S, int ldS,
-                     void* Xi) {
+                     void* Xi, StepPred pred) {
     VecPair<VT> A{(const VT*)A0, ld_, n0, (const VT*)A1, ld_, n1};
     const int nv = n0 + n1;
     const int n_kt = (nk + NK - 1) / NK;
@@ -633,22 +934,22 @@ class BlockLanczos {
     const int nvp = (nv + 2 * kGemmStage - 1) / (2 * kGemmStage) * (2 * kGemmStage);
     hipLaunchKernelGGL((gemm_tiles<VT, NK>), dim3((unsigned)((n_tiles + 7) / 8 * 8)), dim3(256),
                        (size_t)nvp * NK * sizeof(double2), ctx_.stream, D_, A, nk, (VT*)Out, ld_, S, ldS, (VT*)Xi,
-                       n_kt, (int)n_tiles);
+                       n_kt, (int)n_tiles, pred);
   }
 
   // One tile of the smallest width that holds nk (nk <= 16), otherwise tiles
   // of 6 or 8 outputs, whichever pads less.
   template <typename VT>
   void launch_gemm_v(const void* A0, int n0, const void* A1, int n1, void* Out, int nk, const double2* S, int ldS,
-                     void* Xi) {
-    if (nk <= 2) launch_gemm_t<VT, 2>(A0, n0, A1, n1, Out, nk, S, ldS, Xi);
-    else if (nk <= 4) launch_gemm_t<VT, 4>(A0, n0, A1, n1, Out, nk, S, ldS, Xi);
-    else if (nk <= 6) launch_gemm_t<VT, 6>(A0, n0, A1, n1, Out, nk, S, ldS, Xi);
-    else if (nk <= 8) launch_gemm_t<VT, 8>(A0, n0, A1, n1, Out, nk, S, ldS, Xi);
-    else if (nk <= 12) launch_gemm_t<VT, 12>(A0, n0, A1, n1, Out, nk, S, ldS, Xi);
-    else if (nk <= 16) launch_gemm_t<VT, 16>(A0, n0, A1, n1, Out, nk, S, ldS, Xi);
-    else if ((nk + 5) / 6 * 6 < (nk + 7) / 8 * 8) launch_gemm_t<VT, 6>(A0, n0, A1, n1, Out, nk, S, ldS, Xi);
-    else launch_gemm_t<VT, 8>(A0, n0, A1, n1, Out, nk, S, ldS, Xi);
+                     void* Xi, StepPred pd) {
+    if (nk <= 2) launch_gemm_t<VT, 2>(A0, n0, A1, n1, Out, nk, S, ldS, Xi, pd);
+    else if (nk <= 4) launch_gemm_t<VT, 4>(A0, n0, A1, n1, Out, nk, S, ldS, Xi, pd);
+    else if (nk <= 6) launch_gemm_t<VT, 6>(A0, n0, A1, n1, Out, nk, S, ldS, Xi, pd);
+    else if (nk <= 8) launch_gemm_t<VT, 8>(A0, n0, A1, n1, Out, nk, S, ldS, Xi, pd);
+    else if (nk <= 12) launch_gemm_t<VT, 12>(A0, n0, A1, n1, Out, nk, S, ldS, Xi, pd);
+    else if (nk <= 16) launch_gemm_t<VT, 16>(A0, n0, A1, n1, Out, nk, S, ldS, Xi, pd);
+    else if ((nk + 5) / 6 * 6 < (nk + 7) / 8 * 8) launch_gemm_t<VT, 6>(A0, n0, A1, n1, Out, nk, S, ldS, Xi, pd);
+    else launch_gemm_t<VT, 8>(A0, n0, A1, n1, Out, nk, S, ldS, Xi, pd);
   }
 
   // True when the nk output vectors at Out overlap the n vectors at A.
@@ -661,7 +962,7 @@ class BlockLanczos {
   }
 
   void launch_gemm(const void* A0, int n0, const void* A1, int n1, void* Out, int nk, const double2* S, int ldS,
-                   void* Xi) {
+                   void* Xi, StepPred pred = StepPred{}) {
     EDX_REQUIRE(nk >= 1 && nk <= kMaxVec + kMaxW && n0 + n1 >= 1 && n0 + n1 <= kMaxVec + kMaxW, EDX_ERR_INTERNAL,
                 "update width");
     // several output tiles run in separate workgroups: the outputs must not
@@ -669,22 +970,25 @@ class BlockLanczos {
     EDX_REQUIRE(nk <= 16 || (!Xi && !overlaps(A0, n0, Out, nk) && !overlaps(A1, n1, Out, nk)), EDX_ERR_INTERNAL,
                 "multi-tile update aliases its inputs");
     hipEvent_t t0 = ctx_.timer.start(ctx_.stream);
-    if (prec_ == EDX_PREC_FP64) launch_gemm_v<double2>(A0, n0, A1, n1, Out, nk, S, ldS, Xi);
-    else launch_gemm_v<float2>(A0, n0, A1, n1, Out, nk, S, ldS, Xi);
+    if (prec_ == EDX_PREC_FP64) launch_gemm_v<double2>(A0, n0, A1, n1, Out, nk, S, ldS, Xi, pred);
+    else launch_gemm_v<float2>(A0, n0, A1, n1, Out, nk, S, ldS, Xi, pred);
     EDX_HIP(hipGetLastError());
     ctx_.timer.stop(ctx_.stream, t0, edx::K_GEMM, (double)(n0 + n1 + (Xi ? 2 : 1) * nk) * D_ * esz_);
   }
 
   // Out[:, 0..nk) = V[:, 0..nv) Y (Out must not alias V), one launch over
   // all output tiles, so the basis is read from HBM about once.
+  // The coefficients go through the context's pinned combine area (the
+  // stream is synchronised at the end, so the area is free again).
   void combine(const void* V, int nv, void* Out, int nk, const Mat& Y) {
-    std::vector<double2> hs((size_t)nv * nk);
+    EDX_REQUIRE((size_t)nv * nk <= edx::kPinnedCombine, EDX_ERR_INTERNAL, "restart combination too large");
+    double2* hs = ctx_.pinned_out + edx::kPinnedRing;
     for (int i = 0; i < nv; ++i)
       for (int k = 0; k < nk; ++k) hs[(size_t)i * nk + k] = make_double2(Y(i, k).real(), Y(i, k).imag());
-    DevBuf<double2> S(hs.size());
-    EDX_HIP(hipMemcpyAsync(S.p, hs.data(), hs.size() * sizeof(double2), hipMemcpyHostToDevice, ctx_.stream));
+    DevBuf<double2> S((size_t)nv * nk);
+    EDX_HIP(hipMemcpyAsync(S.p, hs, (size_t)nv * nk * sizeof(double2), hipMemcpyHostToDevice, ctx_.stream));
     launch_gemm(V, nv, nullptr, 0, Out, nk, S.p, nk, nullptr);
-    EDX_HIP(hipStreamSynchronize(ctx_.stream));
+    wait();
   }
 
   void random_block(void* out, int nvec, uint64_t seed) {
@@ -756,13 +1060,108 @@ class BlockLanczos {
     return S;
   }
 
-  // Projects W_ = H Vr against the basis Q (nbasis vectors, Vr included):
-  // C = Q^H W (summed over passes), and writes the next block dst with
-  // W - Q C = dst R (dst orthonormal, orthogonal to Q) and its interleaved
-  // copy to X_.
-  void extend(void* Q, int nbasis, void* dst, Mat& C, Mat& R) {
+  // Log slot s of the pinned ring (host and device addresses).
+  const double2* log_host(int s) const { return hc_ + edx::kPinnedGram + edx::kPinnedFactor + (size_t)s * edx::kLogSlot; }
+  double2* log_dev(int s) const { return hc_dev_ + edx::kPinnedGram + edx::kPinnedFactor + (size_t)s * edx::kLogSlot; }
+
+  // Enqueues the block step that follows a basis of q vectors (the SpMM has
+  // been enqueued): Gram of [Q, W] against W, the coefficients (step_coeffs,
+  // log slot `slot`), the update into vec(q + b) and, when the step's status
+  // asks for it, the second Cholesky QR pass; nothing waits for the host.
+  void enqueue_step(int q, int slot) {
+    const int nbasis = q + b_;
+    void* Q = vec(cur_, 0);
+    void* dst = vec(cur_, nbasis);
+    const StepPred p1{ctl_.p, 1}, p2{ctl_.p, 2};
     const double tiny = prec_ == EDX_PREC_FP64 ? 1e-24 : 1e-12;
-    Mat P = gram_host(Q, nbasis, W_.p, b_, W_.p, b_);
+    double2* lg = log_dev(slot);
+    gram_device(Q, nbasis, W_.p, b_, W_.p, b_, pbuf_.p, p1);
+    // complex64 steps always take a second Cholesky QR pass; FP64 steps rarely
+    // do, so theirs is not enqueued and such a step goes to the host
+    const int second = prec_ == EDX_PREC_FP64 ? 0 : 1;
+    hipLaunchKernelGGL(step_coeffs, dim3(1), dim3(256), 0, ctx_.stream, (const double2*)pbuf_.p, nbasis, b_, kappa2_,
+                       tiny, unit_roundoff_, single_qr_tol_, second, sbuf_.p, r1buf_.p, lg, ctl_.p);
+    EDX_HIP(hipGetLastError());
+    launch_gemm(Q, nbasis, W_.p, b_, dst, b_, sbuf_.p, b_, X_.p, p1);
+    if (!second) return;
+    gram_device(dst, b_, nullptr, 0, dst, b_, g2buf_.p, p2);
+    hipLaunchKernelGGL(block_cholinv, dim3(1), dim3(64), 0, ctx_.stream, b_, (const double2*)g2buf_.p, 1e-8,
+                       s2buf_.p, lg + kLogC, reinterpret_cast<int*>(lg + kLogC + kMaxW * kMaxW + 2),
+                       (const double2*)r1buf_.p, p2);
+    EDX_HIP(hipGetLastError());
+    launch_gemm(nullptr, 0, dst, b_, dst, b_, s2buf_.p, b_, X_.p, p2);
+  }
+
+  // Adds block step q's projections C ((q + b) x b) to the projected matrix.
+  void add_block(Mat& M, const Mat& C, int q) const {
+    for (int i = 0; i < q; ++i)
+      for (int j = 0; j < b_; ++j) {
+        M(i, q + j) = C(i, j);
+        M(q + j, i) = std::conj(C(i, j));
+      }
+    for (int i = 0; i < b_; ++i)
+      for (int j = 0; j < b_; ++j) M(q + i, q + j) = 0.5 * (C(q + i, j) + std::conj(C(q + j, i)));
+  }
+
+  // Folds the logs of the enqueued steps (stream synchronised) into M and the
+  // newest coupling; a step the device handed over (status 2) is finished on
+  // the host and ends the list (the steps after it were skipped).  Returns the
+  // basis size; n_acc = steps accepted.
+  int absorb(const std::vector<int>& pending, Mat& M, Result& res, int& n_acc) {
+    int nq = pending.empty() ? 0 : pending[0];
+    n_acc = 0;
+    for (size_t s = 0; s < pending.size(); ++s) {
+      const int q = pending[s];
+      const int nbasis = q + b_;
+      const double2* lg = log_host((int)s);
+      const double2* tail = lg + kLogC + kMaxW * kMaxW;
+      const int st = (int)tail[0].x;
+      Mat C(nbasis, b_);
+      if (st == 2) {
+        resolve(q, C, Rstep_);
+      } else {
+        EDX_REQUIRE(st == 0 || st == 1, EDX_ERR_INTERNAL, "block step log without a status");
+        if (st == 1)
+          EDX_REQUIRE(*reinterpret_cast<const volatile int*>(tail + 2) == 0, EDX_ERR_INTERNAL,
+                      "device Cholesky QR pass met a non-positive pivot");
+        for (int i = 0; i < nbasis; ++i)
+          for (int j = 0; j < b_; ++j) C(i, j) = cplx(lg[i * b_ + j].x, lg[i * b_ + j].y);
+        Rstep_ = Mat(b_, b_);
+        for (int i = 0; i < b_; ++i)
+          for (int j = 0; j < b_; ++j) Rstep_(i, j) = cplx(lg[kLogC + i * b_ + j].x, lg[kLogC + i * b_ + j].y);
+      }
+      add_block(M, C, q);
+      res.matvecs += b_;
+      nq = nbasis;
+      ++n_acc;
+      if (st == 2) break;
+    }
+    return nq;
+  }
+
+  // Finishes on the host the block step after a basis of q vectors that the
+  // device handed over: its Gram is still in pbuf_ (the predicated kernels
+  // after it did nothing, and W_ = H Vr is unchanged), then the device steps
+  // are released.
+  void resolve(int q, Mat& C, Mat& R) {
+    const int nbasis = q + b_;
+    const int nv = nbasis + b_;
+    std::vector<double2> h((size_t)nv * b_);
+    EDX_HIP(hipMemcpyAsync(h.data(), pbuf_.p, h.size() * sizeof(double2), hipMemcpyDeviceToHost, ctx_.stream));
+    wait();
+    Mat P(nv, b_);
+    for (int i = 0; i < nv; ++i)
+      for (int j = 0; j < b_; ++j) P(i, j) = cplx(h[(size_t)i * b_ + j].x, h[(size_t)i * b_ + j].y);
+    extend(P, vec(cur_, 0), nbasis, vec(cur_, nbasis), C, R);
+    EDX_HIP(hipMemsetAsync(ctl_.p, 0, 4 * sizeof(int), ctx_.stream));
+  }
+
+  // Projects W_ = H Vr against the basis Q (nbasis vectors, Vr included),
+  // given the Gram P = [Q, W]^H W: C = Q^H W (summed over passes), and writes
+  // the next block dst with W - Q C = dst R (dst orthonormal, orthogonal to
+  // Q) and its interleaved copy to X_.  Host-driven (waits on the stream).
+  void extend(const Mat& P, void* Q, int nbasis, void* dst, Mat& C, Mat& R) {
+    const double tiny = prec_ == EDX_PREC_FP64 ? 1e-24 : 1e-12;
     C = Mat(nbasis, b_);
     for (int i = 0; i < nbasis; ++i)
       for (int j = 0; j < b_; ++j) C(i, j) = P(i, j);
@@ -798,6 +1197,13 @@ class BlockLanczos {
     gemm_host(Q, nbasis, W_.p, b_, dst, b_, update_coeffs(Cl, nbasis, &R1i), single ? X_.p : nullptr);
     if (single) {
       R = R1;
+      return;
+    }
+    // With the Gram of dst within 1/4 of the identity (by the estimate) its
+    // Cholesky factorisation cannot fail: the second pass runs on the device
+    // without a host round trip
+    if (unit_roundoff_ * std::max(drop, 1.0) * cond * cond <= 0.25) {
+      cholqr_pass_device(dst, R1, &R);
       return;
     }
     Mat G2 = gram_host(dst, b_, nullptr, 0, dst, b_);
@@ -845,11 +1251,32 @@ class BlockLanczos {
     throw edx::Error{EDX_ERR_NOCONV, "could not extend the Krylov basis (sector dimension exhausted)"};
   }
 
-  void ritz(const Mat& M, int nq, std::vector<double>& theta, Mat& Y) {
+  // Ritz values of the projected matrix M (nq x nq), the ncol leading Ritz
+  // vectors (Yc, nq x ncol) and rows nq-b .. nq-1 of every Ritz vector (Yl).
+  void ritz(const Mat& M, int nq, std::vector<double>& theta, int ncol, Mat* Yc, Mat* Yl) {
+    const double t0 = now_ms();
     Mat A(nq, nq);
     for (int i = 0; i < nq; ++i)
       for (int j = 0; j < nq; ++j) A(i, j) = M(i, j);
-    edx::dense::herm_eig(A, theta, Y, true);
+    edx::dense::herm_eig_sel(A, true, theta, ncol, Yc, Yl ? nq - b_ : -1, Yl);
+    ms_ritz_ += now_ms() - t0;
+    ++checks_;
+  }
+
+  // Blocks until the stream has finished its work (time and count kept).
+  void wait() {
+    const double t0 = now_ms();
+    EDX_HIP(hipStreamSynchronize(ctx_.stream));
+    ms_wait_ += now_ms() - t0;
+    ++syncs_;
+  }
+
+  void finish(Result& res, double t_start) {
+    res.ms_total = now_ms() - t_start;
+    res.ms_wait = ms_wait_;
+    res.ms_ritz = ms_ritz_;
+    res.syncs = syncs_;
+    res.checks = checks_;
   }
 
   edx_ctx& ctx_;
@@ -870,6 +1297,21 @@ class BlockLanczos {
   DevBuf<double2> partial_, sdev_;
   double2* hc_ = nullptr;      // pinned Gram results (host address)
   double2* hc_dev_ = nullptr;  // the same buffer's device address
+  double2* hf_ = nullptr;      // pinned factor of the device Cholesky pass (host address)
+  double2* hf_dev_ = nullptr;  // its device address
+  DevBuf<double2> fdev_;       // device Gram (b x b) and coefficients of the device Cholesky pass
+  // device block steps: Gram, update coefficients, R1, second-pass Gram and
+  // coefficients, and the status / halt words
+  DevBuf<double2> pbuf_, sbuf_, r1buf_, g2buf_, s2buf_;
+  DevBuf<int> ctl_;
+  // a block step whose second Cholesky QR pass runs on the device leaves its
+  // coupling R = R2 R1 pending until the stream has passed the factorisation
+  bool r_pending_ = false;
+  Mat r1_pending_;
+  Mat* r_target_ = nullptr;
+  Mat Rstep_;  // R of the newest block step
+  double ms_wait_ = 0.0, ms_ritz_ = 0.0;
+  int syncs_ = 0, checks_ = 0;
   double2* hs_ = nullptr;
 };
 
@@ -958,6 +1400,7 @@ void eigs_impl(edx_ctx* ctx, const edx_hamiltonian* ham, const edx_eigs_options&
     const int64_t D = ham->dim;
     if (info) std::fill(info, info + 4, 0.0);
     if (D == 0) return;
+    const double t_all = now_ms();
     if (D <= 500) {
       dense_solve(*ctx, *ham, nev, vals, vecs);
       if (info) info[3] = 0;
@@ -995,8 +1438,11 @@ void eigs_impl(edx_ctx* ctx, const edx_hamiltonian* ham, const edx_eigs_options&
       Result r32 = lz32.run(nullptr, 0, 0);
       mv_total += r32.matvecs;
       if (trace)
-        fprintf(stderr, "[edx eigs] D=%lld complex64 phase: nev=%d block=%d matvecs=%lld restarts=%d res=%.2e\n",
-                (long long)D, n32, b, (long long)r32.matvecs, r32.restarts, r32.max_rel_res);
+        fprintf(stderr,
+                "[edx eigs] D=%lld complex64 phase: nev=%d block=%d matvecs=%lld restarts=%d res=%.2e | host %.1f ms: "
+                "wait %.1f (%d syncs), ritz %.1f (%d checks)\n",
+                (long long)D, n32, b, (long long)r32.matvecs, r32.restarts, r32.max_rel_res, r32.ms_total, r32.ms_wait,
+                r32.syncs, r32.ms_ritz, r32.checks);
       Options o64 = opt;
       o64.block = std::min(kMaxW, std::max(b, n32));
       o64.basis = 0;
@@ -1015,13 +1461,18 @@ void eigs_impl(edx_ctx* ctx, const edx_hamiltonian* ham, const edx_eigs_options&
     }
     mv_total += res.matvecs;
     if (trace)
-      fprintf(stderr, "[edx eigs] D=%lld FP64 phase: nev=%d matvecs=%lld restarts=%d res=%.2e\n", (long long)D, nev,
-              (long long)res.matvecs, res.restarts, res.max_rel_res);
+      fprintf(stderr,
+              "[edx eigs] D=%lld FP64 phase: nev=%d matvecs=%lld restarts=%d res=%.2e | host %.1f ms: wait %.1f "
+              "(%d syncs), ritz %.1f (%d checks)\n",
+              (long long)D, nev, (long long)res.matvecs, res.restarts, res.max_rel_res, res.ms_total, res.ms_wait,
+              res.syncs, res.ms_ritz, res.checks);
     for (int c = 0; c < nev; ++c) vals[c] = res.vals[c];
-    if (vecs)
-      EDX_HIP(hipMemcpy2DAsync(vecs, D * 16, fin->vec(fin->current(), 0), fin->ld() * 16, D * 16, nev,
-                               hipMemcpyDeviceToHost, ctx->stream));
+    const double t_copy = now_ms();
+    if (vecs) edx::copy_to_host(*ctx, vecs, D * 16, fin->vec(fin->current(), 0), fin->ld() * 16, D * 16, nev);
     EDX_HIP(hipStreamSynchronize(ctx->stream));
+    if (trace)
+      fprintf(stderr, "[edx eigs] D=%lld total %.1f ms, eigenvector copy %.1f ms\n", (long long)D, now_ms() - t_all,
+              now_ms() - t_copy);
     if (info) {
       info[0] = (double)mv_total;
       info[1] = (double)res.restarts;

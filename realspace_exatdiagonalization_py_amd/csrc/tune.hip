@@ -9,13 +9,19 @@
 // keyed by (precision, block width, floor(log2 dim), mean row length rounded
 // to a quarter octave); on a miss it builds the three layouts one after the
 // other, times each on the build's stream (one warm-up and kRuns timed
-// launches on zero vectors), records the timings and keeps the fastest.
-// Sectors of one scan share a key, so one build per scan pays for the timing.
-// Timings taken while other streams load the device are noisier; the table
-// keeps the first measurement of a key.
+// launches on zero vectors, each launch timed on its own and the fastest
+// kept, since work of other streams only ever slows a launch down), records
+// the timings and keeps the fastest layout.  Sectors of one scan share a key,
+// so one build per scan pays for the timing: builds of a key that is being
+// timed wait for that result instead of timing it again (so concurrent
+// solves do not all time at once and disturb each other's measurements).
+// The table keeps the first measurement of a key.
+#include <chrono>
 #include <cmath>
+#include <condition_variable>
 #include <map>
 #include <mutex>
+#include <set>
 
 #include "objects.h"
 
@@ -24,7 +30,7 @@ void spmm(const edx_hamiltonian& h, Ctx& ctx, int precision, int nb, const void*
 
 namespace {
 
-constexpr int kRuns = 5;
+constexpr int kRuns = 7;
 constexpr int kLanes[3] = {1, 2, 4};
 
 struct TuneKey {
@@ -44,10 +50,12 @@ struct TuneEntry {
 
 struct TuneState {
   std::mutex mu;
+  std::condition_variable cv;  // a key's timing finished
   bool enabled = false;
   int prec = EDX_PREC_MIXED;
   int nb = 6;
   std::map<TuneKey, TuneEntry> table;
+  std::set<TuneKey> busy;  // keys being timed
 };
 
 TuneState& state() {
@@ -65,21 +73,26 @@ TuneKey key_for(const edx_hamiltonian& h, int prec, int nb) {
   return k;
 }
 
-// Average time of one SpMM launch with the layout h currently has.
+// Fastest of kRuns individually timed SpMM launches with the layout h
+// currently has (after one warm-up launch).
 float time_spmm(const edx_hamiltonian& h, Ctx& ctx, int prec, int nb, const void* X, void* Y) {
   spmm(h, ctx, prec, nb, X, Y, h.dim);  // warm-up
-  hipEvent_t a, b;
-  EDX_HIP(hipEventCreate(&a));
-  EDX_HIP(hipEventCreate(&b));
-  EDX_HIP(hipEventRecord(a, ctx.stream));
-  for (int r = 0; r < kRuns; ++r) spmm(h, ctx, prec, nb, X, Y, h.dim);
-  EDX_HIP(hipEventRecord(b, ctx.stream));
-  EDX_HIP(hipEventSynchronize(b));
-  float ms = 0.0f;
-  EDX_HIP(hipEventElapsedTime(&ms, a, b));
-  (void)hipEventDestroy(a);
-  (void)hipEventDestroy(b);
-  return ms / kRuns;
+  hipEvent_t ev[kRuns + 1];
+  for (auto& e : ev) EDX_HIP(hipEventCreate(&e));
+  EDX_HIP(hipEventRecord(ev[0], ctx.stream));
+  for (int r = 0; r < kRuns; ++r) {
+    spmm(h, ctx, prec, nb, X, Y, h.dim);
+    EDX_HIP(hipEventRecord(ev[r + 1], ctx.stream));
+  }
+  EDX_HIP(hipEventSynchronize(ev[kRuns]));
+  float best = INFINITY;
+  for (int r = 0; r < kRuns; ++r) {
+    float ms = 0.0f;
+    EDX_HIP(hipEventElapsedTime(&ms, ev[r], ev[r + 1]));
+    best = std::min(best, ms);
+  }
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  return best;
 }
 
 }  // namespace
@@ -127,18 +140,32 @@ void build_sell_for(edx_hamiltonian& h, Ctx& ctx) {
   const TuneKey k = key_for(h, prec, nb);
   int hit = 0;
   {
-    std::lock_guard<std::mutex> lk(s.mu);
+    std::unique_lock<std::mutex> lk(s.mu);
+    // another build is timing this key: wait for its result (bounded)
+    s.cv.wait_for(lk, std::chrono::seconds(30), [&] { return !s.busy.count(k); });
     auto it = s.table.find(k);
     if (it != s.table.end()) hit = it->second.lanes;
+    else if (s.busy.count(k)) hit = default_sell_lanes(h.dim, h.n_entries);  // still timing: heuristic
+    else s.busy.insert(k);
   }
   if (hit) {
     build_sell(h, ctx, hit);
     return;
   }
-  build_sell(h, ctx, default_sell_lanes(h.dim, h.n_entries));
-  const TuneEntry e = tune_spmm(h, ctx, prec, nb);
+  TuneEntry e{0, {0.0f, 0.0f, 0.0f}};
+  try {
+    build_sell(h, ctx, default_sell_lanes(h.dim, h.n_entries));
+    e = tune_spmm(h, ctx, prec, nb);
+  } catch (...) {
+    std::lock_guard<std::mutex> lk(s.mu);
+    s.busy.erase(k);
+    s.cv.notify_all();
+    throw;
+  }
   std::lock_guard<std::mutex> lk(s.mu);
   s.table.emplace(k, e);  // the first measurement of a key is kept
+  s.busy.erase(k);
+  s.cv.notify_all();
 }
 
 }  // namespace edx
