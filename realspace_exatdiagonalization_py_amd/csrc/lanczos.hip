@@ -252,8 +252,12 @@ __global__ void __launch_bounds__(256) gram_reduce(int nblk, int nout, const dou
   if (o < nout && lane == 0) out[o] = make_double2(xr, xi);
 }
 
-// Vectors loaded per stage of gemm_tiles (two stages in flight).
+// Vectors loaded per stage of gemm_tiles (two stages in flight): 4 for the
+// block updates (nk <= 16 outputs, one tile), 2 for the wide restart
+// combinations (tiles of 12 outputs), the fastest measured at those shapes.
 constexpr int kGemmStage = 4;
+constexpr int kCombineStage = 2;
+constexpr int kCombineTile = 12;
 
 // Out_k[r] = sum_i A_i[r] S[i][k] for k < nk (S row-major, leading dimension
 // ldS), FP64 accumulation.  One workgroup per (256 rows, tile of NK outputs);
@@ -262,17 +266,16 @@ constexpr int kGemmStage = 4;
 // row.  The tile's coefficients are staged in LDS (every lane reads the same
 // entry, a broadcast), with zero rows up to a multiple of two stages, so the
 // input loop needs no bounds test: it loads vector min(i, nv - 1) and the zero
-// coefficient cancels it.  Loads of the next kGemmStage vectors are in flight
-// while the current ones are multiplied.  Lanes past D read row D (zero
+// coefficient cancels it.  Loads of the next U vectors are in flight while the
+// current ones are multiplied.  Lanes past D read row D (zero
 // padding, never written) and store nothing.  With one tile (nk <= NK) every
 // input of row r is read by the lane of row r before it writes any output of
 // row r, so Out may be one of the A arrays (same leading dimension); Xi
 // (optional, one tile only) receives the interleaved copy [r][k].
-template <typename V, int NK>
+template <typename V, int NK, int U>
 __global__ void __launch_bounds__(256)
 gemm_tiles(int64_t D, VecPair<V> A, int nk, V* Ob, int64_t ldo, const double2* __restrict__ S, int ldS, V* Xi,
            int n_kt, int n_tiles, StepPred pred) {
-  constexpr int U = kGemmStage;
   extern __shared__ double2 s_tile[];  // [nvp][NK]
   if (!pred.run()) return;
   if (!pred.final_update()) Xi = nullptr;
@@ -923,7 +926,7 @@ class BlockLanczos {
     launch_gemm(A0, n0, A1, n1, Out, nk, ds, nk, Xi);
   }
 
-  template <typename VT, int NK>
+  template <typename VT, int NK, int U = kGemmStage>
   void launch_gemm_t(const void* A0, int n0, const void* A1, int n1, void* Out, int nk, const double2* S, int ldS,
                      void* Xi, StepPred pred) {
     VecPair<VT> A{(const VT*)A0, ld_, n0, (const VT*)A1, ld_, n1};
@@ -931,14 +934,15 @@ class BlockLanczos {
     const int n_kt = (nk + NK - 1) / NK;
     const int64_t n_tiles = (int64_t)n_kt * ((D_ + 255) / 256);
     EDX_REQUIRE(n_tiles < (1ll << 30), EDX_ERR_INTERNAL, "update grid");
-    const int nvp = (nv + 2 * kGemmStage - 1) / (2 * kGemmStage) * (2 * kGemmStage);
-    hipLaunchKernelGGL((gemm_tiles<VT, NK>), dim3((unsigned)((n_tiles + 7) / 8 * 8)), dim3(256),
+    const int nvp = (nv + 2 * U - 1) / (2 * U) * (2 * U);
+    hipLaunchKernelGGL((gemm_tiles<VT, NK, U>), dim3((unsigned)((n_tiles + 7) / 8 * 8)), dim3(256),
                        (size_t)nvp * NK * sizeof(double2), ctx_.stream, D_, A, nk, (VT*)Out, ld_, S, ldS, (VT*)Xi,
                        n_kt, (int)n_tiles, pred);
   }
 
   // One tile of the smallest width that holds nk (nk <= 16), otherwise tiles
-  // of 6 or 8 outputs, whichever pads less.
+  // of 12 outputs (restart combinations: 97 us against 129 us with tiles of
+  // 6 at 60 -> 33 vectors, complex64, D = 225k).
   template <typename VT>
   void launch_gemm_v(const void* A0, int n0, const void* A1, int n1, void* Out, int nk, const double2* S, int ldS,
                      void* Xi, StepPred pd) {
@@ -948,8 +952,7 @@ class BlockLanczos {
     else if (nk <= 8) launch_gemm_t<VT, 8>(A0, n0, A1, n1, Out, nk, S, ldS, Xi, pd);
     else if (nk <= 12) launch_gemm_t<VT, 12>(A0, n0, A1, n1, Out, nk, S, ldS, Xi, pd);
     else if (nk <= 16) launch_gemm_t<VT, 16>(A0, n0, A1, n1, Out, nk, S, ldS, Xi, pd);
-    else if ((nk + 5) / 6 * 6 < (nk + 7) / 8 * 8) launch_gemm_t<VT, 6>(A0, n0, A1, n1, Out, nk, S, ldS, Xi, pd);
-    else launch_gemm_t<VT, 8>(A0, n0, A1, n1, Out, nk, S, ldS, Xi, pd);
+    else launch_gemm_t<VT, kCombineTile, kCombineStage>(A0, n0, A1, n1, Out, nk, S, ldS, Xi, pd);
   }
 
   // True when the nk output vectors at Out overlap the n vectors at A.
