@@ -130,7 +130,12 @@ def build_spinful_hubbard_second_quantized_model(
                 for s in (1, 2):
                     i = sv(x, y, s)
                     density.append((i, i, e))
+    # the spinful tight-binding description (each spin hops as the spinless
+    # model does), from which twisted-boundary terms are generated
     tb = initialize_real_space_tightbinding_model(lat, model_name="Hubbard")
+    for s in (1, 2):
+        add_hopping_term(tb, ((((0, 0), s), ((1, 0), s)), complex(-t)))
+        add_hopping_term(tb, ((((0, 0), s), ((0, 1), s)), complex(-t)))
     return Real_Space_Second_Quantized_Model(
         params={"t": t, "U": U}, lattice=lat, tb_model=tb,
         particle_statistics=Particle_Statistics.FERMIONIC,

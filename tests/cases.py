@@ -40,6 +40,14 @@ def make_case(name: str):
     if name == "hal23":
         m = M.build_zero_flux_bosonic_fci_second_quantized_model(sample_size=[2, 3])
         return m, 3, build_translation_group(m.lattice)
+    if name == "hub23_twist":
+        m = M.build_spinful_hubbard_second_quantized_model([2, 3], 1.0, 4.0)
+        m, G = _twisted(m, (0.3, 0.21))
+        return m, 6, G
+    if name == "heis12_twist":
+        m = M.build_heisenberg_second_quantized_model(12)
+        m, G = _twisted(m, (0.27, 0.0))
+        return m, 6, G
     if name == "hal34":
         m = M.build_zero_flux_bosonic_fci_second_quantized_model(sample_size=[3, 4])
         return m, 6, build_translation_group(m.lattice)

@@ -277,12 +277,41 @@ def _union_vs_full(model, ne, G, projection=O.projection_table):
 
 
 @pytest.mark.parametrize("name,tol", [("hal23", 1e-12), ("hub23", 1e-12), ("hal23_twist", 1e-7),
-                                      ("cb23_twist", 1e-7), ("heis12", 1e-12)])
+                                      ("cb23_twist", 1e-7), ("heis12", 1e-12), ("hub23_twist", 1e-7),
+                                      ("heis12_twist", 1e-7)])
 def test_union_of_sectors_is_full_spectrum(name, tol):
     model, ne, G = make_case(name)
     err, nonherm = _union_vs_full(model, ne, G)
     assert err <= tol
     assert nonherm <= 1e-7  # Hermitian up to the complex64 rounding of the stored amplitudes
+
+
+@pytest.mark.parametrize("builder", [
+    lambda: M.build_spinful_hubbard_second_quantized_model([3, 4], 1.0, 4.0),
+    lambda: M.build_spinful_hubbard_second_quantized_model([2, 5], 1.0, 4.0),
+    lambda: M.build_spinful_hubbard_second_quantized_model([2, 2], 1.0, 8.0),
+    lambda: M.build_heisenberg_second_quantized_model(10),
+    lambda: M.build_zero_flux_bosonic_fci_second_quantized_model(sample_size=[2, 3]),
+    lambda: M.build_checkerboard_second_quantized_model([2, 3], dict(M.my_optimal_param)),
+])
+def test_tight_binding_description_reproduces_the_terms(builder):
+    """Twisted-boundary terms are generated from a model's tight-binding
+    description; at zero twist they must give the model's own operator: the
+    same amplitude for every ordered pair of sites (bonds that join the same
+    pair, as for a periodic direction of length 2, add up)."""
+    from collections import defaultdict
+
+    from realspace_exatdiagonalization_py_amd.lattice import generate_bilinear_terms
+
+    m = builder()
+    gen = generate_bilinear_terms(m.tb_model, twisted_phases_over_2π=[0.0] * m.lattice.dim)
+    a, b = defaultdict(complex), defaultdict(complex)
+    for i, j, t in m.bilinear_terms:
+        a[(i, j)] += t
+    for i, j, t in gen:
+        b[(i, j)] += t
+    assert set(a) == set(b)
+    assert max(abs(a[k] - b[k]) for k in a) <= 1e-12
 
 
 def _projector_check(name, projection=O.projection_table, n_irreps=3):
