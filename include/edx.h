@@ -126,9 +126,13 @@ int edx_hamiltonian_from_csr(edx_ctx* ctx, int64_t dim, int64_t nnz, const int64
  * [n_vec][dim]); replaces MatrixFreeHamiltonian.__call__ and
  * apply_hamiltonian (ref SRE:1965-1976, :1979-2069). */
 int edx_matvec(edx_ctx* ctx, const edx_hamiltonian* ham, int32_t n_vec, const double* x, double* y);
-/* Same on device-resident complex128 buffers (row-major [dim][n_vec]
- * interleaved input, column-major [n_vec][dim] output): the HBM-resident
- * kernel the benchmarks time.  Pointers are device addresses. */
+/* Same on device-resident buffers (row-major [dim][n_vec] interleaved input,
+ * column-major [n_vec][dim] output; complex128, or complex64 with
+ * EDX_PREC_MIXED).  Pointers are device addresses.  Both apply the table
+ * itself; for a table built on a twisted catalog (site phases other than +-1,
+ * whose alpha the catalog rounds to complex64) the eigensolver's sliced copy
+ * holds the Hermitian part (H + H^H) / 2 instead, and these two read the CSR
+ * arrays. */
 int edx_matvec_device(edx_ctx* ctx, const edx_hamiltonian* ham, int32_t precision, int32_t n_vec,
                       const void* x_dev, void* y_dev);
 int edx_hamiltonian_destroy(edx_hamiltonian* ham);
@@ -136,7 +140,10 @@ int edx_hamiltonian_destroy(edx_hamiltonian* ham);
 /* ---- eigensolver ----------------------------------------------------------
  * Replaces diagonalize_block_matrixfree (ref SRE:2248-2299): the nev lowest
  * eigenvalues (ascending, real) of the sector block, by thick-restarted block
- * Lanczos with full reorthogonalisation.  block = 0 picks min(nev, 8).
+ * Lanczos with full reorthogonalisation.  The operator is the Hermitian part
+ * of the block (the block itself unless its catalog is twisted; the
+ * reference's eigs returns the real parts of the block's eigenvalues, the same
+ * numbers up to the complex64 rounding of alpha).  block = 0 picks min(nev, 8).
  * precision: EDX_PREC_FP64 or EDX_PREC_MIXED (complex64 sweeps then FP64
  * refinement).  vecs (optional, may be NULL): complex128 [nev][dim].
  * info (optional): [0] matvecs (vector count), [1] restarts,
@@ -253,8 +260,9 @@ int edx_mem_info(edx_ctx* ctx, uint64_t* free_bytes, uint64_t* total_bytes);
 /* Device-to-device copy on the ctx stream (runtime blit kernel); a plain copy
  * of known size, used to calibrate memory counters. */
 int edx_device_copy(edx_ctx* ctx, void* dst, const void* src, uint64_t bytes);
-/* Average duration (ms) of the last n_rep launches of the matvec kernel timed
- * with HIP events on the ctx stream, inputs resident in device memory. */
+/* Average duration (ms) of the last n_rep launches of the eigensolver's SpMM
+ * (y_dev = its operator x_dev, layouts as edx_matvec_device) timed with HIP
+ * events on the ctx stream, inputs resident in device memory. */
 int edx_time_matvec(edx_ctx* ctx, const edx_hamiltonian* ham, int32_t precision, int32_t n_vec,
                     int32_t n_rep, const void* x_dev, void* y_dev, double* avg_ms);
 

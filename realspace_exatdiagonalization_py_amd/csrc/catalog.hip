@@ -199,11 +199,21 @@ uint64_t binom_u64(int n, int k) {
 }
 
 // Validates the group and uploads perm / phases / angles / inverse / binomials.
+// Every site phase +1 or -1 (no twist): alpha is a sign, exact in complex64.
+bool exact_site_phases(const double* phases, size_t n) {
+  for (size_t i = 0; i < n; ++i) {
+    const double re = phases[2 * i], im = phases[2 * i + 1];
+    if (im != 0.0 || (re != 1.0 && re != -1.0)) return false;
+  }
+  return true;
+}
+
 void upload_group(edx_catalog& cat, edx_ctx& ctx, const int32_t* perm, const double* phases,
                   const double* angles) {
   const int N = cat.n_site, G = cat.n_g;
   std::vector<uint8_t> p0((size_t)G * N);
   cat.h_perm.assign(perm, perm + (size_t)G * N);
+  cat.exact_alpha = exact_site_phases(phases, (size_t)G * N);
   for (int g = 0; g < G; ++g) {
     std::vector<int> seen(N, 0);
     for (int i = 0; i < N; ++i) {
@@ -377,6 +387,7 @@ int edx_catalog_update_phases(edx_ctx* ctx, edx_catalog* cat, const int32_t* per
     edx::DeviceGuard dg(ctx->device);
     EDX_HIP(hipMemcpyAsync(cat->phase.p, perm_phases, n * 16, hipMemcpyHostToDevice, ctx->stream));
     EDX_HIP(hipMemcpyAsync(cat->angle.p, perm_angles, n * 8, hipMemcpyHostToDevice, ctx->stream));
+    cat->exact_alpha = exact_site_phases(perm_phases, n);
     const edx::GroupView gv = cat->view();
     const size_t smem = group_smem_bytes(cat->n_site, cat->n_filled, cat->n_g);
     hipLaunchKernelGGL(catalog_update_stab, dim3(edx::grid_for(cat->n_orbits, kBlock)), dim3(kBlock), smem,

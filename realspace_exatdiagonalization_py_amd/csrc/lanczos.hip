@@ -4,11 +4,12 @@
 // Reference: diagonalize_block_matrixfree (SRE:2248-2299) forms the dense
 // block for n <= 500 and calls LAPACK eigh on its upper triangle, and for
 // larger blocks runs ARPACK eigs(which="SR", tol=1e-10) on the CSR operator,
-// returning the real parts in ascending order.  The blocks are Hermitian up to
-// the complex64 rounding of the stored amplitudes, so the real parts of their
-// eigenvalues equal the eigenvalues of the Hermitian part to second order in
-// that rounding; this solver works with the Hermitian part (the projected
-// matrix is symmetrised) and reproduces ARPACK's real parts.
+// returning the real parts in ascending order.  With twisted boundaries the
+// blocks are Hermitian only up to the complex64 rounding of the stored alpha;
+// the sliced operator this solver applies then holds the Hermitian part
+// (H + H^H) / 2, whose eigenvalues are ARPACK's real parts to second order in
+// that rounding, except inside multiplets the rounding splits, where the
+// real parts themselves are fixed only to the rounding (~1e-8).
 //
 // Krylov state: basis Q (nq vectors whose H-images are known), residual block
 // Vr (b vectors) and coupling B with H Q = Q M + Vr B.  Each step applies H to
@@ -568,37 +569,65 @@ struct Options {
   int basis = 0;  // max Ritz basis size (0: automatic)
   double tol = 1e-10;     // relative residual ||H x - theta x|| / max(1, |theta|)
   double eig_tol = 0.0;   // > 0: a pair is also done when its cluster's sum r^2 / gap <= eig_tol
+  // Ritz values of an earlier, larger basis of the same operator (or of a
+  // lower-precision copy): the gaps of the eig_tol rule are measured to these
+  // too, so a small basis after a restart cannot overestimate them
+  std::vector<double> gap_ref, gap_ref_res;  // values and residual norms
+  // return, past the nev-th pair, the Ritz vectors of its cluster too (values
+  // within 1e-4 relative of it), so that a restart from them holds the cluster
+  bool out_cluster = false;
   int max_restarts = 5000;
   int check_every = 4;    // Ritz check every this many block steps (and when the basis is full)
   uint64_t seed = 1;
 };
 
-// Convergence of the nev lowest Ritz pairs.  A pair passes on its relative
-// residual, or (eig_tol > 0) when the eigenvalue error bound of its cluster,
-// sum of squared residuals over Ritz values within 1e-4 relative of it divided
-// by the distance to the nearest Ritz value outside the cluster, is below
-// eig_tol (the residual-squared-over-gap bound for Hermitian matrices).
+// Convergence of the nev lowest Ritz pairs.  A pair passes only when its
+// cluster (the Ritz values within 1e-4 relative of it, which may reach past
+// the nev-th) is resolved as a whole: until then the error of each of its Ritz
+// values is first order in the residuals, not residual^2 / gap.  The cluster
+// then passes when all its members meet the relative residual tol, or
+// (eig_tol > 0) when its eigenvalue error bound, the sum of its squared
+// residuals divided by the distance to the nearest Ritz value outside it, is
+// below eig_tol (the residual-squared-over-gap bound for Hermitian matrices).
+// `ref` / `ref_res` hold the Ritz values and residual norms of an earlier,
+// larger basis.  Those whose residual places an eigenvalue within half the
+// window of the pair are copies of the cluster: the current basis must hold
+// as many (else a copy has not been captured yet).  The distance is also
+// taken to the values outside the window, since a basis of a few vectors has
+// its higher Ritz values far up the spectrum.
 bool converged(const std::vector<double>& th, const std::vector<double>& res, int nev, double tol, double eig_tol,
-               double* worst) {
+               const std::vector<double>& ref, const std::vector<double>& ref_res, double* worst) {
   bool ok = true;
   double w = 0.0;
   const int n = (int)th.size();
   for (int i = 0; i < nev; ++i) {
     const double sc = std::max(1.0, std::fabs(th[i]));
-    const double rel = res[i] / sc;
-    w = std::max(w, rel);
-    if (rel <= tol) continue;
-    if (!(eig_tol > 0.0)) {
-      ok = false;
-      continue;
-    }
+    w = std::max(w, res[i] / sc);
+    int members = 0;
+    bool all_tol = true;
     double rc2 = 0.0, gap = INFINITY;
     for (int j = 0; j < n; ++j) {
       const double d = std::fabs(th[j] - th[i]);
-      if (d <= 1e-4 * sc) rc2 += res[j] * res[j];
-      else gap = std::min(gap, d);
+      if (d <= 1e-4 * sc) {
+        ++members;
+        rc2 += res[j] * res[j];
+        if (res[j] / std::max(1.0, std::fabs(th[j])) > tol) all_tol = false;
+      } else {
+        gap = std::min(gap, d);
+      }
     }
-    if (!(rc2 / gap <= eig_tol)) ok = false;
+    int ref_members = 0;
+    for (size_t j = 0; j < ref.size(); ++j) {
+      const double d = std::fabs(ref[j] - th[i]);
+      if (d > 1e-4 * sc) gap = std::min(gap, d);
+      else if (d + ref_res[j] <= 0.5e-4 * sc) ++ref_members;
+    }
+    if (ref_members > members) {
+      ok = false;
+      continue;
+    }
+    if (all_tol) continue;
+    if (!(eig_tol > 0.0 && rc2 / gap <= eig_tol)) ok = false;
   }
   *worst = w;
   return ok;
@@ -606,6 +635,9 @@ bool converged(const std::vector<double>& th, const std::vector<double>& res, in
 
 struct Result {
   std::vector<double> vals;
+  std::vector<double> ritz;  // every Ritz value of the final basis, ascending
+  std::vector<double> ritz_res;  // and their residual norms
+  int nvec = 0;              // Ritz vectors left in vec(0..nvec): nev, or more with out_cluster
   int64_t matvecs = 0;
   int restarts = 0;
   double max_rel_res = 0.0;
@@ -757,21 +789,29 @@ class BlockLanczos {
         rn[c] = std::sqrt(rn2);
       }
       double worst = 0.0;
-      const bool conv = converged(theta, rn, nev, opt_.tol, opt_.eig_tol, &worst);
+      const bool conv = converged(theta, rn, nev, opt_.tol, opt_.eig_tol, opt_.gap_ref, opt_.gap_ref_res, &worst);
       res.max_rel_res = worst;
       if (conv) {
+        int nout = nev;
+        if (opt_.out_cluster) {
+          const double sc = std::max(1.0, std::fabs(theta[nev - 1]));
+          while (nout < nq && nout < kMaxW && std::fabs(theta[nout] - theta[nev - 1]) <= 1e-4 * sc) ++nout;
+        }
         Mat Yn;
-        if (full) {
-          Yn = Mat(nq, nev);
+        if (full && nout <= k) {
+          Yn = Mat(nq, nout);
           for (int i = 0; i < nq; ++i)
-            for (int c = 0; c < nev; ++c) Yn(i, c) = Yk(i, c);
+            for (int c = 0; c < nout; ++c) Yn(i, c) = Yk(i, c);
         } else {
           std::vector<double> th2;
-          ritz(M, nq, th2, nev, &Yn, nullptr);
+          ritz(M, nq, th2, nout, &Yn, nullptr);
         }
-        combine(vec(cur_, 0), nq, vec(1 - cur_, 0), nev, Yn);
+        combine(vec(cur_, 0), nq, vec(1 - cur_, 0), nout, Yn);
         cur_ = 1 - cur_;
         res.vals.assign(theta.begin(), theta.begin() + nev);
+        res.nvec = nout;
+        res.ritz = theta;
+        res.ritz_res = rn;
         res.restarts = restarts_;
         finish(res, t_start);
         return res;
@@ -1428,15 +1468,18 @@ void eigs_impl(edx_ctx* ctx, const edx_hamiltonian* ham, const edx_eigs_options&
       return e && *e && *e != '0';
     }();
     std::unique_ptr<BlockLanczos> fin;
+    int fin_block = b;
     if (precision == EDX_PREC_MIXED) {
       // complex64 sweeps resolve the nev pairs to mixed_tol, then an FP64
-      // block Lanczos starts from those Ritz vectors (all in its first block)
-      // and checks convergence every step
+      // block Lanczos starts from those Ritz vectors and the rest of the
+      // nev-th pair's cluster (all in its first block), checks convergence
+      // every step and judges clusters and gaps against the complex64 basis
       const int n32 = (int)std::min<int64_t>(std::min(nev, kMaxW), D - 2);
       Options o32 = opt;
       o32.nev = n32;
       o32.tol = std::max(tol, o.mixed_tol > 0 ? o.mixed_tol : 3e-6);
       o32.eig_tol = 0.0;
+      o32.out_cluster = true;
       BlockLanczos lz32(*ctx, *ham, EDX_PREC_MIXED, o32);
       Result r32 = lz32.run(nullptr, 0, 0);
       mv_total += r32.matvecs;
@@ -1446,16 +1489,20 @@ void eigs_impl(edx_ctx* ctx, const edx_hamiltonian* ham, const edx_eigs_options&
                 "wait %.1f (%d syncs), ritz %.1f (%d checks)\n",
                 (long long)D, n32, b, (long long)r32.matvecs, r32.restarts, r32.max_rel_res, r32.ms_total, r32.ms_wait,
                 r32.syncs, r32.ms_ritz, r32.checks);
+      const int n_start = std::max(n32, std::min(r32.nvec, kMaxW));
       Options o64 = opt;
-      o64.block = std::min(kMaxW, std::max(b, n32));
+      o64.block = std::min(kMaxW, std::max(b, n_start));
       o64.basis = 0;
       o64.check_every = 1;
+      o64.gap_ref = r32.ritz;
+      o64.gap_ref_res = r32.ritz_res;
+      fin_block = o64.block;
       fin.reset(new BlockLanczos(*ctx, *ham, EDX_PREC_FP64, o64));
-      DevBuf<double2> start((size_t)n32 * lz32.ld());
+      DevBuf<double2> start((size_t)n_start * lz32.ld());
       hipLaunchKernelGGL((convert_vec<float2, double2>), dim3(edx::grid_for(D, 256)), dim3(256), 0, ctx->stream, D,
-                         n32, (const float2*)lz32.vec(lz32.current(), 0), lz32.ld(), start.p, lz32.ld());
+                         n_start, (const float2*)lz32.vec(lz32.current(), 0), lz32.ld(), start.p, lz32.ld());
       EDX_HIP(hipGetLastError());
-      res = fin->run(start.p, n32, lz32.ld());
+      res = fin->run(start.p, n_start, lz32.ld());
       res.restarts += r32.restarts;
     } else {
       opt.check_every = 2;
@@ -1465,9 +1512,10 @@ void eigs_impl(edx_ctx* ctx, const edx_hamiltonian* ham, const edx_eigs_options&
     mv_total += res.matvecs;
     if (trace)
       fprintf(stderr,
-              "[edx eigs] D=%lld FP64 phase: nev=%d matvecs=%lld restarts=%d res=%.2e | host %.1f ms: wait %.1f "
-              "(%d syncs), ritz %.1f (%d checks)\n",
-              (long long)D, nev, (long long)res.matvecs, res.restarts, res.max_rel_res, res.ms_total, res.ms_wait,
+              "[edx eigs] D=%lld FP64 phase: nev=%d block=%d matvecs=%lld restarts=%d res=%.2e | host %.1f ms: "
+              "wait %.1f (%d syncs), ritz %.1f (%d checks)\n",
+              (long long)D, nev, fin_block, (long long)res.matvecs, res.restarts, res.max_rel_res, res.ms_total,
+              res.ms_wait,
               res.syncs, res.ms_ritz, res.checks);
     for (int c = 0; c < nev; ++c) vals[c] = res.vals[c];
     const double t_copy = now_ms();

@@ -11,6 +11,9 @@ struct edx_ctx : public edx::Ctx {};
 struct edx_catalog {
   int device = 0;
   int n_site = 0, n_filled = 0, n_g = 0, fermionic = 0, product_phase = 0;
+  // 1 when every site phase is +1 or -1: alpha is then +-1, exact in the
+  // complex64 rep_amp, and the projection table is Hermitian to FP64 rounding
+  int exact_alpha = 1;
   int64_t n_total = 0, n_orbits = 0, n_stab = 0;
   std::vector<int32_t> h_perm;        // host copy (1-based) to validate phase updates
   edx::ObjBuf<uint8_t> perm;          // [n_g][n_site] 0-based targets
@@ -54,6 +57,10 @@ struct edx_hamiltonian {
   edx::ObjBuf<double2> val;           // [n_entries] complex128
   edx::ObjBuf<double2> hdiag;         // [dim]
   edx::ObjBuf<float2> hdiag32;        // [dim]
+  // 1: the sliced copy holds the Hermitian part (H + H^H) / 2 of the table
+  // (tables whose alpha were rounded to complex64 are Hermitian only to that
+  // rounding); edx_matvec then applies the table itself from the CSR arrays
+  int herm_part = 0;
   // sliced-ELL copy used by the SpMM: slices of 64 / sell_g rows, sell_g lanes per row
   int sell_g = 1;
   int64_t sell_slices = 0;

@@ -375,8 +375,27 @@ __global__ void sell_slice_len(int64_t dim, int64_t n_slices, int G, const int64
   words[s] = (uint32_t)(L * 64);
 }
 
-__global__ void sell_fill(int64_t dim, int G, const int64_t* row_ptr, const int32_t* col, const double2* val,
-                          const int64_t* off, int32_t* scol, double2* sval, float2* sval32) {
+// Sum of the table entries of row c in column r (columns ascend within a row).
+__device__ double2 entry_sum(const int64_t* row_ptr, const int32_t* col, const double2* val, int64_t c, int32_t r) {
+  int64_t lo = row_ptr[c], hi = row_ptr[c + 1];
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (col[mid] < r) lo = mid + 1;
+    else hi = mid;
+  }
+  double2 s = make_double2(0.0, 0.0);
+  for (int64_t e = lo; e < row_ptr[c + 1] && col[e] == r; ++e) {
+    s.x += val[e].x;
+    s.y += val[e].y;
+  }
+  return s;
+}
+
+// herm = 1 stores the Hermitian part: every entry of the run (r, c) is halved
+// and the run's first entry also carries conj(sum of the run (c, r)) / 2, so the
+// run sums to (H_rc + conj(H_cr)) / 2.
+__global__ void sell_fill(int64_t dim, int G, int herm, const int64_t* row_ptr, const int32_t* col,
+                          const double2* val, const int64_t* off, int32_t* scol, double2* sval, float2* sval32) {
   const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (r >= dim) return;
   const int R = 64 / G;
@@ -386,10 +405,62 @@ __global__ void sell_fill(int64_t dim, int G, const int64_t* row_ptr, const int3
   for (int64_t e = e0; e < e1; ++e) {
     const int64_t k = e - e0;
     const int64_t pos = base + (k / G) * 64 + (k % G);
-    scol[pos] = col[e];
-    sval[pos] = val[e];
-    sval32[pos] = make_float2((float)val[e].x, (float)val[e].y);
+    const int32_t c = col[e];
+    double2 v = val[e];
+    if (herm) {
+      v = make_double2(0.5 * v.x, 0.5 * v.y);
+      if (e == e0 || col[e - 1] != c) {
+        const double2 t = entry_sum(row_ptr, col, val, c, (int32_t)r);
+        v.x += 0.5 * t.x;
+        v.y -= 0.5 * t.y;
+      }
+    }
+    scol[pos] = c;
+    sval[pos] = v;
+    sval32[pos] = make_float2((float)v.x, (float)v.y);
   }
+}
+
+// Y[j][r] = hdiag[r] X[r][j] + sum_e val[e] X[col[e]][j] straight from the CSR
+// table (X interleaved [dim][nb], Y column-major with leading dimension ldy):
+// the table's own operator where the sliced copy holds its Hermitian part.
+template <typename V>
+__global__ void csr_apply(int64_t dim, int nb, const int64_t* row_ptr, const int32_t* col, const double2* val,
+                          const double2* hd, const V* X, V* Y, int64_t ldy) {
+  const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (r >= dim) return;
+  for (int j = 0; j < nb; ++j) {
+    const V x = X[r * nb + j];
+    double yr = hd[r].x * x.x - hd[r].y * x.y, yi = hd[r].x * x.y + hd[r].y * x.x;
+    for (int64_t e = row_ptr[r]; e < row_ptr[r + 1]; ++e) {
+      const V xc = X[(int64_t)col[e] * nb + j];
+      const double2 a = val[e];
+      yr += a.x * xc.x - a.y * xc.y;
+      yi += a.x * xc.y + a.y * xc.x;
+    }
+    V y;
+    y.x = (typename Real<V>::T)yr;
+    y.y = (typename Real<V>::T)yi;
+    Y[(int64_t)j * ldy + r] = y;
+  }
+}
+
+// The table's operator: the sliced SpMM, or the CSR arrays when the sliced
+// copy holds the Hermitian part.
+void apply_table(const edx_hamiltonian& h, Ctx& ctx, int precision, int nb, const void* X, void* Y, int64_t ldy) {
+  if (!h.herm_part) {
+    spmm(h, ctx, precision, nb, X, Y, ldy);
+    return;
+  }
+  if (precision == EDX_PREC_FP64)
+    hipLaunchKernelGGL(csr_apply<double2>, dim3(grid_for(h.dim, 256)), dim3(256), 0, ctx.stream, h.dim, nb,
+                       (const int64_t*)h.row_ptr.p, (const int32_t*)h.col.p, (const double2*)h.val.p,
+                       (const double2*)h.hdiag.p, (const double2*)X, (double2*)Y, ldy);
+  else
+    hipLaunchKernelGGL(csr_apply<float2>, dim3(grid_for(h.dim, 256)), dim3(256), 0, ctx.stream, h.dim, nb,
+                       (const int64_t*)h.row_ptr.p, (const int32_t*)h.col.p, (const double2*)h.val.p,
+                       (const double2*)h.hdiag.p, (const float2*)X, (float2*)Y, ldy);
+  EDX_HIP(hipGetLastError());
 }
 
 int default_sell_lanes(int64_t dim, int64_t n_entries) {
@@ -429,9 +500,9 @@ void build_sell(edx_hamiltonian& h, Ctx& ctx, int G) {
   EDX_HIP(hipMemsetAsync(h.sell_col.p, 0xff, h.sell_col.bytes(), st));
   EDX_HIP(hipMemsetAsync(h.sell_val.p, 0, h.sell_val.bytes(), st));
   EDX_HIP(hipMemsetAsync(h.sell_val32.p, 0, h.sell_val32.bytes(), st));
-  hipLaunchKernelGGL(sell_fill, dim3(grid_for(D, 256)), dim3(256), 0, st, D, G, (const int64_t*)h.row_ptr.p,
-                     (const int32_t*)h.col.p, (const double2*)h.val.p, (const int64_t*)h.sell_off.p, h.sell_col.p,
-                     h.sell_val.p, h.sell_val32.p);
+  hipLaunchKernelGGL(sell_fill, dim3(grid_for(D, 256)), dim3(256), 0, st, D, G, h.herm_part,
+                     (const int64_t*)h.row_ptr.p, (const int32_t*)h.col.p, (const double2*)h.val.p,
+                     (const int64_t*)h.sell_off.p, h.sell_col.p, h.sell_val.p, h.sell_val32.p);
   EDX_HIP(hipGetLastError());
   EDX_HIP(hipStreamSynchronize(st));
   h.sell_entries = total;
@@ -571,6 +642,7 @@ int edx_hamiltonian_build(edx_ctx* ctx, const edx_catalog* cat, const edx_sector
       ham->sec = sec;
       ham->dim = D;
       ham->n_bilinear = T;
+      ham->herm_part = cat->exact_alpha ? 0 : 1;
       ham->row_ptr.alloc(D + 1);
       ham->hdiag.alloc(D);
       ham->hdiag32.alloc(D);
@@ -711,7 +783,7 @@ int edx_matvec(edx_ctx* ctx, const edx_hamiltonian* ham, int32_t n_vec, const do
       DevBuf<double2> xc((size_t)nb * D), xi((size_t)nb * D), yc((size_t)nb * D);
       EDX_HIP(hipMemcpyAsync(xc.p, x + (size_t)v0 * D * 2, (size_t)nb * D * 16, hipMemcpyHostToDevice, st));
       edx::interleave(st, EDX_PREC_FP64, D, nb, xc.p, D, xi.p);
-      edx::spmm(*ham, *ctx, EDX_PREC_FP64, nb, xi.p, yc.p, D);
+      edx::apply_table(*ham, *ctx, EDX_PREC_FP64, nb, xi.p, yc.p, D);
       EDX_HIP(hipMemcpyAsync(y + (size_t)v0 * D * 2, yc.p, (size_t)nb * D * 16, hipMemcpyDeviceToHost, st));
       EDX_HIP(hipStreamSynchronize(st));
     }
@@ -724,7 +796,7 @@ int edx_matvec_device(edx_ctx* ctx, const edx_hamiltonian* ham, int32_t precisio
     EDX_REQUIRE(ctx && ham && x_dev && y_dev, EDX_ERR_INVALID, "null argument");
     EDX_REQUIRE(precision == EDX_PREC_FP64 || precision == EDX_PREC_MIXED, EDX_ERR_INVALID, "bad precision");
     edx::DeviceGuard dg(ctx->device);
-    edx::spmm(*ham, *ctx, precision, n_vec, x_dev, y_dev, ham->dim);
+    edx::apply_table(*ham, *ctx, precision, n_vec, x_dev, y_dev, ham->dim);
     EDX_HIP(hipStreamSynchronize(ctx->stream));
   });
 }

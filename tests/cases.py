@@ -44,6 +44,10 @@ def make_case(name: str):
         m = M.build_spinful_hubbard_second_quantized_model([2, 3], 1.0, 4.0)
         m, G = _twisted(m, (0.3, 0.21))
         return m, 6, G
+    if name == "hub24_twist":  # sectors of ~1.6k states: past the dense cutoff
+        m = M.build_spinful_hubbard_second_quantized_model([2, 4], 1.0, 4.0)
+        m, G = _twisted(m, (0.31, 0.17))
+        return m, 8, G
     if name == "heis12_twist":
         m = M.build_heisenberg_second_quantized_model(12)
         m, G = _twisted(m, (0.27, 0.0))

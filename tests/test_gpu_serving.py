@@ -70,3 +70,31 @@ def test_serving_twist_uses_its_own_catalog():
     vals, _ = ed.ed_scan_at_irrep_matrixfree(lab, data, nev=3)
     np.testing.assert_allclose(rb.eigenvalues, vals, atol=1e-10, rtol=0)
     assert np.max(np.abs(ra.eigenvalues - rb.eigenvalues)) > 1e-6
+
+
+@pytest.mark.parametrize("family,size,ne,sector,twist,nev", [
+    ("heisenberg", 22, 11, (1, 0), 0.6961189834824061, 1),
+    ("hubbard", [2, 5], 10, (0, 0), 0.25882866282148365, 6),
+    ("hubbard", [2, 5], 10, (1, 3), 0.5968223667041186, 8),
+])
+def test_twisted_mixed_requests_match_fp64(family, size, ne, sector, twist, nev):
+    """Twisted requests of the serving workload (seed 2024) in mixed precision
+    against FP64, within 1e-10.  The first: the FP64 phase restarts on a basis
+    of a few vectors whose higher Ritz values lie far up the spectrum, so the
+    eigenvalue error bound also measures gaps to the complex64 phase's Ritz
+    values.  The Hubbard ones: spin multiplets split by ~1e-11 straddle the
+    nev-th level, so a pair passes on its residual only with its whole
+    cluster."""
+    import realspace_exatdiagonalization_py_amd as ed
+
+    if family == "heisenberg":
+        model = ed.build_heisenberg_second_quantized_model(size)
+    else:
+        model = ed.build_spinful_hubbard_second_quantized_model(size, 1.0, 4.0)
+    print(f"[mixed-twist] {family} {size} k={sector} twist={twist} nev={nev}", flush=True)
+    kw = dict(nev=nev, twist=(twist, 0.0), eig_tol=1e-12)
+    with SectorSolveServer(workers=1) as srv:
+        a = srv.solve_alone(SolveRequest(model, ne, sector, precision="mixed", **kw))
+        b = srv.solve_alone(SolveRequest(model, ne, sector, precision="fp64", **kw))
+    assert a.dim == b.dim > 500
+    assert np.max(np.abs(a.eigenvalues - b.eigenvalues)) <= 1e-10, (a.eigenvalues, b.eigenvalues)
