@@ -15,8 +15,12 @@ Printed: one JSON line with solves/s (whole job), p50/p99 sector latency,
 (HIP events on the launch streams over the timed region) and the CPU oracle
 timed on a bounded sample (two sectors) on this host.
 
-Other modes (for profilers): ``--profile-spmm`` runs only the SpMM kernels of
-one sector; ``--copy-bytes N`` runs a plain device copy of N bytes.
+Other modes: ``--serve`` sends a seeded Poisson stream of mixed requests
+(Heisenberg chains, Hubbard and checkerboard lattices at several sizes, random
+momentum sector, twist angle and number of eigenvalues) through the serving
+loop and prints its own JSON line (latency percentiles, solves/s, |dE| of a
+sample against FP64); ``--profile-spmm`` runs only the SpMM kernels of one
+sector; ``--copy-bytes N`` runs a plain device copy of N bytes.
 """
 
 from __future__ import annotations
@@ -224,6 +228,135 @@ def copy_kernel(args) -> int:
     return 0
 
 
+# Serving workload (cfg5): families of the three model configurations at
+# several sizes; the approximate sector dimensions are C(N, Ne) / |G|.
+SERVE_FAMILIES = {
+    # spin-1/2 Heisenberg chain, N sites, Sz = 0: dims 9.2e3 .. 1.4e6
+    "heisenberg": [20, 22, 24, 26, 28],
+    # spinful Hubbard, half filling, t = 1, U = 4: dims 1.8e4 .. 2.9e6
+    "hubbard": [[2, 5], [2, 6], [3, 4], [2, 7]],
+    # checkerboard flat band, 8 fermions, the 48-site configuration's couplings:
+    # dims 6.1e4 .. 1.6e7
+    "checkerboard": [[3, 4], [4, 4], [4, 5], [4, 6]],
+}
+
+
+def _serve_model(ed, family, size):
+    from realspace_exatdiagonalization_py_amd import models as M
+
+    if family == "heisenberg":
+        return ed.build_heisenberg_second_quantized_model(size), size // 2
+    if family == "hubbard":
+        return ed.build_spinful_hubbard_second_quantized_model(list(size), 1.0, 4.0), size[0] * size[1]
+    p = {"t": 1.0, "t'_1": 0.3, "t'_2": -0.3, "t''": -0.2, "phi_over_2pi": 1 / 8, "V1": 1.0, "V2": 0.0, "V3": 0.0}
+    return M.build_zero_flux_fermionic_fci_second_quantized_model(sample_size=list(size), params=p), 8
+
+
+def _approx_dim(family, size):
+    import math
+
+    if family == "heisenberg":
+        return math.comb(size, size // 2) / size
+    n_cells = size[0] * size[1]
+    if family == "hubbard":
+        return math.comb(2 * n_cells, n_cells) / n_cells
+    return math.comb(2 * n_cells, 8) / n_cells
+
+
+def serve_requests(ed, n_req, max_dim, seed, eig_tol):
+    """The seeded request list of the serving workload and (family, size) of each."""
+    from realspace_exatdiagonalization_py_amd.serving import SolveRequest
+
+    rng = np.random.default_rng(seed)
+    models = {}
+    reqs, meta = [], []
+    fams = list(SERVE_FAMILIES)
+    for n in range(n_req):
+        fam = fams[int(rng.integers(len(fams)))]
+        sizes = [z for z in SERVE_FAMILIES[fam] if _approx_dim(fam, z) <= max_dim]
+        size = sizes[int(rng.integers(len(sizes)))]
+        key = (fam, str(size))
+        if key not in models:
+            models[key] = _serve_model(ed, fam, size)
+        model, ne = models[key]
+        L = model.lattice.sample_size
+        sector = (int(rng.integers(L[0])), int(rng.integers(L[1])))
+        theta = float(rng.uniform(0.0, 1.0))
+        nev = int(rng.integers(1, 11))
+        reqs.append(SolveRequest(model, ne, sector, nev=nev, twist=(theta, 0.0), precision="mixed",
+                                 eig_tol=eig_tol, tag=n))
+        meta.append((fam, size))
+    return reqs, meta
+
+
+def serve_bench(args) -> int:
+    """A seeded Poisson stream of sector-solve requests drawn uniformly from
+    the three model families (random size, momentum sector, twist angle and
+    number of eigenvalues) through the serving loop; prints one JSON line."""
+    os.environ.setdefault("EDX_QUIET", "1")
+    import realspace_exatdiagonalization_py_amd as ed
+    from realspace_exatdiagonalization_py_amd.serving import (SectorSolveServer, SolveRequest, poisson_arrivals,
+                                                              serve_stream)
+
+    reqs, meta = serve_requests(ed, args.serve_requests, args.serve_max_dim, args.serve_seed, args.eig_tol)
+    rng = np.random.default_rng(args.serve_seed + 1)
+    arrivals = poisson_arrivals(len(reqs), args.serve_rate, seed=args.serve_seed)
+    log(f"serving {len(reqs)} requests (Poisson {args.serve_rate}/s, dims up to {args.serve_max_dim:.2g}) ...")
+    with SectorSolveServer(workers=args.workers) as server:
+        t0 = time.perf_counter()
+        results = serve_stream(server, reqs, arrivals)
+        wall = time.perf_counter() - t0
+        st = server.stats()
+        # accuracy: a sample of the requests (dimension <= 4e6, so that the
+        # FP64 re-solves stay short) solved alone in FP64
+        small = [j for j, r in enumerate(results) if r.dim <= 4e6]
+        sample = rng.choice(small, size=min(args.serve_check, len(small)), replace=False) if small else []
+        dE, checks = 0.0, []
+        for j in sample:
+            r = reqs[int(j)]
+            ref = server.solve_alone(SolveRequest(r.model, r.n_filled, r.sector, nev=r.nev, twist=r.twist,
+                                                  precision="fp64", eig_tol=r.eig_tol))
+            d = float(np.max(np.abs(ref.eigenvalues - results[int(j)].eigenvalues)))
+            dE = max(dE, d)
+            checks.append({"request": int(j), "family": meta[int(j)][0], "size": meta[int(j)][1],
+                           "sector": list(r.sector), "twist": r.twist[0], "nev": r.nev, "dim": ref.dim, "dE": d})
+    dims = [r.dim for r in results]
+    lat = np.asarray([r.latency for r in results]) * 1e3
+    svc = np.asarray([r.service for r in results]) * 1e3
+    by_fam = {}
+    for (fam, size), r in zip(meta, results):
+        d = by_fam.setdefault(fam, {"n": 0, "dims": [r.dim, r.dim], "service_ms": 0.0})
+        d["n"] += 1
+        d["dims"] = [min(d["dims"][0], r.dim), max(d["dims"][1], r.dim)]
+        d["service_ms"] += r.service * 1e3
+    line = {
+        "metric": METRIC + " (serving)",
+        "value": round(len(results) / wall, 3),
+        "unit": "sector solves/s",
+        "n_gpus": 1,
+        "higher_is_better": True,
+        "data": "synthetic, seeded request stream",
+        "dtype": "complex64 sweeps + complex128 refinement",
+        "config": {"workload": "Poisson stream over the Heisenberg, Hubbard and checkerboard families: random "
+                               "size, momentum sector, twist angle in [0, 2pi) and 1-10 eigenvalues",
+                   "requests": len(reqs), "offered_rate_per_s": args.serve_rate, "workers": args.workers,
+                   "seed": args.serve_seed, "max_dim": args.serve_max_dim},
+        "wall_s": round(wall, 3),
+        "p50_ms": round(float(np.percentile(lat, 50)), 3),
+        "p99_ms": round(float(np.percentile(lat, 99)), 3),
+        "mean_queue_ms": round(st.get("mean_queue_ms", 0.0), 3),
+        "service_p50_ms": round(float(np.percentile(svc, 50)), 3),
+        "dims": [int(min(dims)), int(max(dims))],
+        "families": by_fam,
+        "max_abs_dE_vs_fp64_sample": dE,
+        "fp64_checks": checks,
+        "peak_admitted_bytes": st.get("peak_admitted_bytes"),
+        "hbm_budget_bytes": st.get("hbm_budget_bytes"),
+    }
+    print(json.dumps(line))
+    return 0
+
+
 def _traffic(kernel: str):
     """HBM bytes per launch of `kernel` from the committed counter summary."""
     try:
@@ -250,6 +383,12 @@ def main() -> int:
                     help="no live per-kernel HIP-event timing (for runs under a profiler, which times kernels itself)")
     ap.add_argument("--profile-spmm", action="store_true", help="run only the SpMM kernels of one sector")
     ap.add_argument("--copy-bytes", type=float, default=0, help="run only a plain device copy of this many bytes")
+    ap.add_argument("--serve", action="store_true", help="serving workload: a Poisson stream of mixed requests")
+    ap.add_argument("--serve-requests", type=int, default=48)
+    ap.add_argument("--serve-rate", type=float, default=200.0, help="mean arrivals per second")
+    ap.add_argument("--serve-max-dim", type=float, default=4e6, help="largest approximate sector dimension drawn")
+    ap.add_argument("--serve-seed", type=int, default=2024)
+    ap.add_argument("--serve-check", type=int, default=6, help="requests re-solved alone in FP64 for |dE|")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}): sectors shard across GPUs "
@@ -259,6 +398,8 @@ def main() -> int:
         return profile_spmm(args)
     if args.copy_bytes:
         return copy_kernel(args)
+    if args.serve:
+        return serve_bench(args)
 
     os.environ.setdefault("EDX_QUIET", "1")
     import realspace_exatdiagonalization_py_amd as ed
