@@ -553,6 +553,14 @@ __global__ void fill_gaussian(int64_t D, int nvec, V* out, int64_t ld, uint64_t 
   }
 }
 
+// dst[:, j] = src[:, j] for j < gridDim.y, rows < D (one element per thread).
+template <typename V>
+__global__ void copy_vecs(int64_t D, const V* __restrict__ src, int64_t lds, V* __restrict__ dst, int64_t ldd) {
+  const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (r >= D) return;
+  dst[(int64_t)blockIdx.y * ldd + r] = src[(int64_t)blockIdx.y * lds + r];
+}
+
 template <typename A, typename B>
 __global__ void convert_vec(int64_t D, int nvec, const A* src, int64_t lds, B* dst, int64_t ldd) {
   const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
@@ -576,6 +584,11 @@ struct Options {
   // return, past the nev-th pair, the Ritz vectors of its cluster too (values
   // within 1e-4 relative of it), so that a restart from them holds the cluster
   bool out_cluster = false;
+  // > 0: also stop once the worst relative residual of the nev pairs is
+  // within stall_ceiling and has not dropped by 10 % over this many checks
+  // (a lower-precision phase that has reached its rounding floor)
+  int stall_checks = 0;
+  double stall_ceiling = 0.0;
   int max_restarts = 5000;
   int check_every = 4;    // Ritz check every this many block steps (and when the basis is full)
   uint64_t seed = 1;
@@ -732,13 +745,14 @@ class BlockLanczos {
     random_block(vec(cur_, 0), b_, opt_.seed);
     if (start && n_start > 0) {
       const int ns = std::min(n_start, b_);
-      EDX_HIP(hipMemcpy2DAsync(vec(cur_, 0), ld_ * esz_, start, start_ld * esz_, D_ * esz_, ns,
-                               hipMemcpyDeviceToDevice, ctx_.stream));
+      copy_block(vec(cur_, 0), ld_, start, start_ld, ns);
     }
     start_block();
     Mat M(cap_, cap_), B(b_, cap_);
     std::vector<double> theta;
     int since_check = 0;
+    double best_worst = INFINITY;  // stall detection
+    int since_best = 0;
     std::vector<int> pending;  // basis size before each block step enqueued since the last sync
     for (;;) {
       // one block step on the device: W = H Vr (Vr at vec(nq), interleaved
@@ -789,8 +803,16 @@ class BlockLanczos {
         rn[c] = std::sqrt(rn2);
       }
       double worst = 0.0;
-      const bool conv = converged(theta, rn, nev, opt_.tol, opt_.eig_tol, opt_.gap_ref, opt_.gap_ref_res, &worst);
+      bool conv = converged(theta, rn, nev, opt_.tol, opt_.eig_tol, opt_.gap_ref, opt_.gap_ref_res, &worst);
       res.max_rel_res = worst;
+      if (!conv && opt_.stall_checks > 0) {
+        if (worst < 0.9 * best_worst) {
+          best_worst = worst;
+          since_best = 0;
+        } else if (++since_best >= opt_.stall_checks && worst <= opt_.stall_ceiling) {
+          conv = true;
+        }
+      }
       if (conv) {
         int nout = nev;
         if (opt_.out_cluster) {
@@ -821,8 +843,7 @@ class BlockLanczos {
                     "block Lanczos did not converge within " + std::to_string(opt_.max_restarts) +
                         " restarts (max relative residual " + std::to_string(worst) + ")");
         combine(vec(cur_, 0), nq, vec(1 - cur_, 0), k, Yk);
-        EDX_HIP(hipMemcpy2DAsync(vec(1 - cur_, k), ld_ * esz_, vec(cur_, nq), ld_ * esz_, D_ * esz_, b_,
-                                 hipMemcpyDeviceToDevice, ctx_.stream));
+        copy_block(vec(1 - cur_, k), ld_, vec(cur_, nq), ld_, b_);
         Mat Bn(b_, cap_);
         for (int i = 0; i < b_; ++i)
           for (int c = 0; c < k; ++c) {
@@ -838,6 +859,19 @@ class BlockLanczos {
         ++restarts_;
       }
     }
+  }
+
+  // n vectors (rows < D) from src to dst, both of this run's precision.
+  void copy_block(void* dst, int64_t ldd, const void* src, int64_t lds, int n) {
+    if (n <= 0) return;
+    const dim3 grid((unsigned)edx::grid_for(D_, 256), (unsigned)n);
+    if (prec_ == EDX_PREC_FP64)
+      hipLaunchKernelGGL(copy_vecs<double2>, grid, dim3(256), 0, ctx_.stream, D_, (const double2*)src, lds,
+                         (double2*)dst, ldd);
+    else
+      hipLaunchKernelGGL(copy_vecs<float2>, grid, dim3(256), 0, ctx_.stream, D_, (const float2*)src, lds,
+                         (float2*)dst, ldd);
+    EDX_HIP(hipGetLastError());
   }
 
   // Device address of basis vector i of a bank.
@@ -1413,7 +1447,7 @@ void edx_eigs_default_options(edx_eigs_options* o) {
   o->max_restarts = 5000;
   o->tol = 1e-10;
   o->eig_tol = 0.0;
-  o->mixed_tol = 3e-6;
+  o->mixed_tol = 1e-6;
   o->seed = 12345;
 }
 
@@ -1477,9 +1511,11 @@ void eigs_impl(edx_ctx* ctx, const edx_hamiltonian* ham, const edx_eigs_options&
       const int n32 = (int)std::min<int64_t>(std::min(nev, kMaxW), D - 2);
       Options o32 = opt;
       o32.nev = n32;
-      o32.tol = std::max(tol, o.mixed_tol > 0 ? o.mixed_tol : 3e-6);
+      o32.tol = std::max(tol, o.mixed_tol > 0 ? o.mixed_tol : 1e-6);
       o32.eig_tol = 0.0;
       o32.out_cluster = true;
+      o32.stall_checks = 8;
+      o32.stall_ceiling = 30.0 * o32.tol;
       BlockLanczos lz32(*ctx, *ham, EDX_PREC_MIXED, o32);
       Result r32 = lz32.run(nullptr, 0, 0);
       mv_total += r32.matvecs;
