@@ -157,7 +157,8 @@ int edx_eigs(edx_ctx* ctx, const edx_hamiltonian* ham, int32_t nev, int32_t bloc
  *            over-gap bound of its eigenvalue cluster is below eig_tol (the
  *            eigenvalue is then accurate to eig_tol; the vector to ~sqrt)
  *   mixed_tol residual at which EDX_PREC_MIXED hands over to FP64 (default 1e-6;
- *            earlier when the complex64 sweeps stall within 30x of it)
+ *            earlier when the complex64 sweeps stall below max(30 x mixed_tol,
+ *            1e-4), as they do under a target below their rounding floor)
  *   basis    Ritz basis size (0: automatic), block as in edx_eigs */
 typedef struct edx_eigs_options {
   int32_t nev;

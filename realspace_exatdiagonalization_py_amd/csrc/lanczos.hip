@@ -1514,8 +1514,10 @@ void eigs_impl(edx_ctx* ctx, const edx_hamiltonian* ham, const edx_eigs_options&
       o32.tol = std::max(tol, o.mixed_tol > 0 ? o.mixed_tol : 1e-6);
       o32.eig_tol = 0.0;
       o32.out_cluster = true;
+      // a target below the complex64 rounding floor still hands over: the
+      // sweeps stop once they stall anywhere below 1e-4
       o32.stall_checks = 8;
-      o32.stall_ceiling = 30.0 * o32.tol;
+      o32.stall_ceiling = std::max(30.0 * o32.tol, 1e-4);
       BlockLanczos lz32(*ctx, *ham, EDX_PREC_MIXED, o32);
       Result r32 = lz32.run(nullptr, 0, 0);
       mv_total += r32.matvecs;

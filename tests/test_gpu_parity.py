@@ -239,6 +239,31 @@ def test_twisted_block_solves_its_hermitian_part():
     assert np.max(np.abs(w64 - w_ref)) <= 1e-7, (w64, w_ref)
 
 
+def test_mixed_handover_below_the_complex64_floor():
+    """mixed_tol far below what complex64 sweeps can reach: they stall, hand
+    over to FP64 anyway, and the result still matches FP64 within 1e-10."""
+    import ctypes as C
+    from realspace_exatdiagonalization_py_amd import _edx
+
+    model, ne, G, oc, gc = _both("hub24")
+    irrep = build_irrep_list(G, model.lattice)[1]
+    basis = ed.build_symmetry_sector_basis(gc, irrep)
+    H = ed.MatrixFreeHamiltonian(basis, model.bilinear_terms, model.density_density_terms,
+                                 ed.CanonicalMap(G, model.particle_statistics, gc))
+    dh = H._dev
+    assert dh.dim > 500
+    out = {}
+    for prec, mixed_tol in ((ed.PREC_FP64, 1e-6), (ed.PREC_MIXED, 1e-9)):
+        vals, info = np.zeros(6), np.zeros(4)
+        o = _edx.eigs_options(nev=6, precision=prec, tol=1e-10, mixed_tol=mixed_tol, seed=12345, max_restarts=2000)
+        dh.ctx.check(_edx.lib.edx_eigs_ex(dh.ctx.handle, dh.handle, C.byref(o), _edx.ptr(vals), None,
+                                          _edx.ptr(info)))
+        out[prec] = (vals, info)
+    print(f"[handover] matvecs fp64 {out[ed.PREC_FP64][1][0]:.0f} mixed {out[ed.PREC_MIXED][1][0]:.0f}", flush=True)
+    assert np.max(np.abs(out[ed.PREC_MIXED][0] - out[ed.PREC_FP64][0])) <= 1e-10
+    assert out[ed.PREC_MIXED][1][1] < 200  # restarts: the stall guard ended the sweeps
+
+
 def test_eigs_mixed_precision_cfg1():
     w64, _, w_ref, _ = _solve_both("heis16", 0, 4)
     w32, _, _, _ = _solve_both("heis16", 0, 4, precision=ed.PREC_MIXED)
