@@ -656,12 +656,28 @@ def diagonalize_block_dense(H, *, nev: int = 5):
     return w[:k], z[:, :k]
 
 
+def _hermitian_part(H: sp.spmatrix) -> sp.spmatrix:
+    """(H + H^H) / 2 when H is not Hermitian beyond FP64 rounding, else H.
+
+    Twisted blocks are Hermitian only to the complex64 rounding of the
+    stored alpha; the reference's eigs returns the real parts of their
+    eigenvalues, which equal those of the Hermitian part to second order in
+    that rounding, and the device Lanczos needs a Hermitian operator."""
+    H = sp.csr_matrix(H, dtype=np.complex128)
+    scale = float(np.abs(H.data).max(initial=0.0))
+    if scale == 0.0 or float(np.abs((H - H.getH()).data).max(initial=0.0)) <= 1e-14 * scale:
+        return H
+    return ((H + H.getH()) * 0.5).tocsr()
+
+
 def diagonalize_block_arpack(H, *, nev: int = 5):
-    """Lowest ``nev`` eigenpairs of a sparse block on the device (SRE:2214-2245)."""
+    """Lowest ``nev`` eigenpairs of a sparse block on the device (SRE:2214-2245);
+    a block that is not Hermitian beyond FP64 rounding is solved through its
+    Hermitian part."""
     n = H.shape[0]
     if n == 0:
         return np.zeros(0, dtype=np.float64), np.zeros((0, 0), dtype=np.complex128)
-    dh = _csr_to_device(H)
+    dh = _csr_to_device(_hermitian_part(H))
     vals, vecs, _ = _device_eigs(dh, nev)
     return vals, vecs
 

@@ -239,6 +239,20 @@ def test_twisted_block_solves_its_hermitian_part():
     assert np.max(np.abs(w64 - w_ref)) <= 1e-7, (w64, w_ref)
 
 
+def test_twisted_matrix_mode_matches_matrixfree():
+    """Matrix mode (scipy CSR block back to the device) solves the Hermitian
+    part of a twisted block too, so both modes agree within 1e-10."""
+    from fractions import Fraction
+
+    model, ne, G = make_case("hub24_twist")
+    data = ed.build_ed_data(model, filling_fraction=Fraction(ne, model.lattice.n_site), symmetry_group=G)
+    label = data.irrep_list[1].label
+    w_mat, v_mat = ed.ed_scan_at_irrep_matrix(label, data, nev=5)
+    w_mf, _ = ed.ed_scan_at_irrep_matrixfree(label, data, nev=5)
+    assert data.sector_dims[1] > 500
+    assert np.max(np.abs(w_mat - w_mf)) <= 1e-10, (w_mat, w_mf)
+
+
 def test_mixed_handover_below_the_complex64_floor():
     """mixed_tol far below what complex64 sweeps can reach: they stall, hand
     over to FP64 anyway, and the result still matches FP64 within 1e-10."""

@@ -135,3 +135,20 @@ def test_library_exports_every_header_function():
     lib = ctypes.CDLL(str(_edx.LIB_PATH))
     missing = [n for n in sorted(_header_functions()) if not hasattr(lib, n)]
     assert not missing, missing
+
+
+def test_hermitian_part_of_a_block():
+    """Matrix mode hands the device Lanczos (H + H^H) / 2 when a block is not
+    Hermitian beyond FP64 rounding, and the block itself otherwise."""
+    import numpy as np
+    import scipy.sparse as sp
+
+    rng = np.random.default_rng(3)
+    A = sp.random(40, 40, density=0.2, random_state=4, dtype=np.float64) * (1 + 0.5j)
+    H = (A + A.getH()).tocsr()
+    np.testing.assert_array_equal(sre._hermitian_part(H).toarray(), H.toarray())  # Hermitian: unchanged
+    E = H.copy()
+    E.data = E.data * (1 + 1e-8 * rng.standard_normal(E.data.shape))  # rounding-sized asymmetry
+    Hh = sre._hermitian_part(E)
+    np.testing.assert_allclose(Hh.toarray(), (E.toarray() + E.toarray().conj().T) / 2, rtol=0, atol=1e-15)
+    assert abs(Hh - Hh.getH()).max() == 0.0
