@@ -79,6 +79,22 @@ int default_sell_lanes(int64_t dim, int64_t n_entries);
 // Builds the sliced copy with the context's fixed lane count, the autotuned
 // one (timing the layouts on a miss of the tuning table) or the heuristic.
 void build_sell_for(edx_hamiltonian& h, Ctx& ctx);
+// The same in steps, for builds that write the sliced copy themselves:
+// plan_sell picks the lanes (and on a miss of the tuning table marks the key
+// as being timed, tune = true); once the table and its sliced copy are
+// complete, finish_sell times the layouts if asked to; abandon_sell releases
+// the key of a build that failed before that.
+struct SellPlan {
+  int lanes = 1;
+  bool tune = false;
+  int prec = 0, nb = 0;
+};
+SellPlan plan_sell(const edx_hamiltonian& h, Ctx& ctx);
+void finish_sell(edx_hamiltonian& h, Ctx& ctx, SellPlan& p);
+void abandon_sell(const edx_hamiltonian& h, SellPlan& p);
+// Slice offsets and storage of a sliced copy with G lanes per row (contents
+// not written); returns the stored entry count.
+int64_t prepare_sell(edx_hamiltonian& h, Ctx& ctx, int G);
 }  // namespace edx
 
 namespace edx {

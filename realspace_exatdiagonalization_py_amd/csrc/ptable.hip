@@ -124,8 +124,19 @@ ptab_fill(TermView tv, SectorView sv, const int64_t* row_ptr, uint32_t* cursor, 
   });
 }
 
-constexpr int kSortThreads = 64;
-constexpr int kSortCap = 128;
+// Longest row (entry count) of the table: one atomic max per workgroup.
+__global__ void __launch_bounds__(kBlock) row_len_max(int64_t dim, const uint32_t* cnt, uint32_t* out) {
+  __shared__ uint32_t wmax[kBlock / 64];
+  const int64_t r = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  uint32_t m = r < dim ? cnt[r] : 0u;
+  for (int sh = 32; sh >= 1; sh >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, sh, 64));
+  if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < kBlock / 64; ++w) m = max(m, wmax[w]);
+    atomicMax(out, m);
+  }
+}
 
 __device__ __forceinline__ void insertion_sort_global(uint64_t* a, int64_t n) {
   for (int64_t i = 1; i < n; ++i) {
@@ -139,29 +150,45 @@ __device__ __forceinline__ void insertion_sort_global(uint64_t* a, int64_t n) {
   }
 }
 
-__global__ void __launch_bounds__(kSortThreads) ptab_sort(int64_t dim, const int64_t* row_ptr, uint64_t* keys) {
-  __shared__ uint64_t buf[kSortCap * kSortThreads];
-  const int64_t r = (int64_t)blockIdx.x * kSortThreads + threadIdx.x;
+// Per row: keys ascending.  NT threads per block, each sorting its row in
+// CAP slots of LDS (rows longer than CAP are sorted in place in global
+// memory); CAP and NT are chosen from the longest row so that blocks stay
+// small in LDS and many of them run per CU.
+template <int CAP, int NT>
+__global__ void __launch_bounds__(NT) ptab_sort(int64_t dim, const int64_t* row_ptr, uint64_t* keys) {
+  __shared__ uint64_t buf[CAP * NT];
+  const int64_t r = (int64_t)blockIdx.x * NT + threadIdx.x;
   if (r >= dim) return;
   const int64_t b = row_ptr[r];
   const int64_t n = row_ptr[r + 1] - b;
   if (n <= 1) return;
-  if (n > kSortCap) {
+  if (n > CAP) {
     insertion_sort_global(keys + b, n);
     return;
   }
-  uint64_t* s = buf + threadIdx.x;  // element i at s[i * kSortThreads]
-  for (int i = 0; i < n; ++i) s[i * kSortThreads] = keys[b + i];
+  uint64_t* s = buf + threadIdx.x;  // element i at s[i * NT]
+  for (int i = 0; i < n; ++i) s[i * NT] = keys[b + i];
   for (int i = 1; i < n; ++i) {
-    const uint64_t v = s[i * kSortThreads];
+    const uint64_t v = s[i * NT];
     int j = i - 1;
-    while (j >= 0 && s[j * kSortThreads] > v) {
-      s[(j + 1) * kSortThreads] = s[j * kSortThreads];
+    while (j >= 0 && s[j * NT] > v) {
+      s[(j + 1) * NT] = s[j * NT];
       --j;
     }
-    s[(j + 1) * kSortThreads] = v;
+    s[(j + 1) * NT] = v;
   }
-  for (int i = 0; i < n; ++i) keys[b + i] = s[i * kSortThreads];
+  for (int i = 0; i < n; ++i) keys[b + i] = s[i * NT];
+}
+
+// 32 KB of LDS per block up to rows of 64 entries, 64 KB above.
+void launch_sort(hipStream_t st, int64_t dim, uint32_t max_row, const int64_t* row_ptr, uint64_t* keys) {
+  if (max_row <= 32)
+    hipLaunchKernelGGL((ptab_sort<32, 128>), dim3(edx::grid_for(dim, 128)), dim3(128), 0, st, dim, row_ptr, keys);
+  else if (max_row <= 64)
+    hipLaunchKernelGGL((ptab_sort<64, 64>), dim3(edx::grid_for(dim, 64)), dim3(64), 0, st, dim, row_ptr, keys);
+  else
+    hipLaunchKernelGGL((ptab_sort<128, 64>), dim3(edx::grid_for(dim, 64)), dim3(64), 0, st, dim, row_ptr, keys);
+  EDX_HIP(hipGetLastError());
 }
 
 // Complex product formed as (a.re b.re - a.im b.im) + (a.re b.im + a.im b.re) i
@@ -170,9 +197,20 @@ __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
   return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
+// Sliced copy written by ptab_values itself (col == nullptr: not written):
+// G lanes per row, slices of 64 / G rows, `rows` = slices * (64 / G).
+struct SellOut {
+  int G;
+  int64_t rows;
+  const int64_t* off;
+  int32_t* col;
+  double2* val;
+  float2* val32;
+};
+
 __global__ void __launch_bounds__(kBlock)
 ptab_values(TermView tv, SectorView sv, const double2* chi, const int64_t* row_ptr, const uint64_t* keys,
-            int32_t* col, uint16_t* term, double2* val) {
+            int32_t* col, uint16_t* term, double2* val, SellOut so) {
   __shared__ double2 chi_s[EDX_MAX_GROUP];
   __shared__ uint32_t binom_s[(EDX_MAX_SITES + 1) * (EDX_MAX_SITES + 1)];
   const int nb = (tv.n_site + 1) * (tv.n_filled + 1);
@@ -180,31 +218,53 @@ ptab_values(TermView tv, SectorView sv, const double2* chi, const int64_t* row_p
   for (int i = threadIdx.x; i < EDX_MAX_GROUP; i += blockDim.x) chi_s[i] = chi[i];
   __syncthreads();
   const int64_t r = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (r >= sv.dim) return;
-  const int nf1 = tv.n_filled + 1;
-  const double stab_row = (double)sv.stab[r];
-  for (int64_t e = row_ptr[r]; e < row_ptr[r + 1]; ++e) {
-    const uint64_t key = keys[e];
-    const int64_t c = (int64_t)(key >> 16);
-    const int k = (int)(key & 0xffffu);
-    const uint64_t m = sv.reps[c];
-    const uint64_t nm = m ^ (1ull << tv.from_bit[k]) ^ (1ull << tv.to_bit[k]);
-    const uint64_t idx = edxd::rank_colex(nm, binom_s, nf1);
-    const int g = sv.rep_sym[idx];
-    const float2 a32 = sv.rep_amp[idx];
-    const double2 a = make_double2((double)a32.x, (double)a32.y);
-    const double s = (tv.fermionic && (edxd::popc64(m & tv.between[k]) & 1)) ? -1.0 : 1.0;
-    const double2 t = tv.amp[k];
-    // t * hop_phase, with hop_phase the complex (s + 0i)
-    double2 h = make_double2(t.x * s - t.y * 0.0, t.x * 0.0 + t.y * s);
-    h = cmul(h, a);
-    const double2 cc = make_double2(chi_s[g].x, -chi_s[g].y);
-    h = cmul(h, cc);
-    const double q = sqrt(stab_row / (double)sv.stab[c]);
-    h = make_double2(h.x * q - h.y * 0.0, h.x * 0.0 + h.y * q);
-    col[e] = (int32_t)c;
-    term[e] = (uint16_t)k;
-    val[e] = h;
+  // slots of this row in the sliced copy: entry k at base + (k / G) * 64 + k % G
+  int64_t sbase = 0, slots = 0;
+  if (so.col && r < so.rows) {
+    const int R = 64 / so.G;
+    const int64_t sl = r / R;
+    sbase = so.off[sl] + (r % R) * so.G;
+    slots = (so.off[sl + 1] - so.off[sl]) / 64 * so.G;
+  }
+  int64_t k = 0;
+  if (r < sv.dim) {
+    const int nf1 = tv.n_filled + 1;
+    const double stab_row = (double)sv.stab[r];
+    for (int64_t e = row_ptr[r]; e < row_ptr[r + 1]; ++e, ++k) {
+      const uint64_t key = keys[e];
+      const int64_t c = (int64_t)(key >> 16);
+      const int kt = (int)(key & 0xffffu);
+      const uint64_t m = sv.reps[c];
+      const uint64_t nm = m ^ (1ull << tv.from_bit[kt]) ^ (1ull << tv.to_bit[kt]);
+      const uint64_t idx = edxd::rank_colex(nm, binom_s, nf1);
+      const int g = sv.rep_sym[idx];
+      const float2 a32 = sv.rep_amp[idx];
+      const double2 a = make_double2((double)a32.x, (double)a32.y);
+      const double s = (tv.fermionic && (edxd::popc64(m & tv.between[kt]) & 1)) ? -1.0 : 1.0;
+      const double2 t = tv.amp[kt];
+      // t * hop_phase, with hop_phase the complex (s + 0i)
+      double2 h = make_double2(t.x * s - t.y * 0.0, t.x * 0.0 + t.y * s);
+      h = cmul(h, a);
+      const double2 cc = make_double2(chi_s[g].x, -chi_s[g].y);
+      h = cmul(h, cc);
+      const double q = sqrt(stab_row / (double)sv.stab[c]);
+      h = make_double2(h.x * q - h.y * 0.0, h.x * 0.0 + h.y * q);
+      col[e] = (int32_t)c;
+      term[e] = (uint16_t)kt;
+      val[e] = h;
+      if (slots) {
+        const int64_t pos = sbase + (k / so.G) * 64 + (k % so.G);
+        so.col[pos] = (int32_t)c;
+        so.val[pos] = h;
+        so.val32[pos] = make_float2((float)h.x, (float)h.y);
+      }
+    }
+  }
+  for (; k < slots; ++k) {  // padding, and the rows past the last one
+    const int64_t pos = sbase + (k / so.G) * 64 + (k % so.G);
+    so.col[pos] = -1;
+    so.val[pos] = make_double2(0.0, 0.0);
+    so.val32[pos] = make_float2(0.0f, 0.0f);
   }
 }
 
@@ -473,20 +533,21 @@ int default_sell_lanes(int64_t dim, int64_t n_entries) {
   return 1;
 }
 
-// Builds (or rebuilds with another lane count) the sliced copy of the table.
-void build_sell(edx_hamiltonian& h, Ctx& ctx, int G) {
+// Slice offsets and storage of the sliced copy with G lanes per row.
+int64_t prepare_sell(edx_hamiltonian& h, Ctx& ctx, int G) {
   EDX_REQUIRE(G == 1 || G == 2 || G == 4, EDX_ERR_INVALID, "lanes per row must be 1, 2 or 4");
   hipStream_t st = ctx.stream;
   const int64_t D = h.dim;
   const int64_t ns = (D + 64 / G - 1) / (64 / G);
   h.sell_g = G;
   h.sell_slices = ns;
+  h.sell_entries = 0;
   h.sell_off.alloc((size_t)ns + 1);
   if (ns == 0) {
     const int64_t zero = 0;
     EDX_HIP(hipMemcpyAsync(h.sell_off.p, &zero, 8, hipMemcpyHostToDevice, st));
     EDX_HIP(hipStreamSynchronize(st));
-    return;
+    return 0;
   }
   DevBuf<uint32_t> words((size_t)ns);
   hipLaunchKernelGGL(sell_slice_len, dim3(grid_for(ns, 256)), dim3(256), 0, st, D, ns, G,
@@ -497,6 +558,17 @@ void build_sell(edx_hamiltonian& h, Ctx& ctx, int G) {
   h.sell_col.alloc((size_t)std::max<int64_t>(total, 1));
   h.sell_val.alloc((size_t)std::max<int64_t>(total, 1));
   h.sell_val32.alloc((size_t)std::max<int64_t>(total, 1));
+  h.sell_entries = total;
+  return total;
+}
+
+// Builds (or rebuilds with another lane count) the sliced copy of the table
+// from its CSR arrays.
+void build_sell(edx_hamiltonian& h, Ctx& ctx, int G) {
+  prepare_sell(h, ctx, G);
+  if (h.sell_slices == 0) return;
+  hipStream_t st = ctx.stream;
+  const int64_t D = h.dim;
   EDX_HIP(hipMemsetAsync(h.sell_col.p, 0xff, h.sell_col.bytes(), st));
   EDX_HIP(hipMemsetAsync(h.sell_val.p, 0, h.sell_val.bytes(), st));
   EDX_HIP(hipMemsetAsync(h.sell_val32.p, 0, h.sell_val32.bytes(), st));
@@ -505,7 +577,6 @@ void build_sell(edx_hamiltonian& h, Ctx& ctx, int G) {
                      (const int64_t*)h.sell_off.p, h.sell_col.p, h.sell_val.p, h.sell_val32.p);
   EDX_HIP(hipGetLastError());
   EDX_HIP(hipStreamSynchronize(st));
-  h.sell_entries = total;
 }
 
 // Column-major [nb][ld] -> interleaved [dim][nb].
@@ -637,6 +708,8 @@ int edx_hamiltonian_build(edx_ctx* ctx, const edx_catalog* cat, const edx_sector
     SectorView sv{D, sec->reps.p, sec->stab_order.p, sec->orbit_to_basis.p, cat->rep_rank.p, cat->rep_sym.p,
                   cat->rep_amp.p};
     auto ham = new edx_hamiltonian();
+    edx::SellPlan plan;  // sliced copy written by ptab_values (not for twisted tables)
+    bool sell_written = false;
     try {
       ham->device = ctx->device;
       ham->sec = sec;
@@ -649,26 +722,43 @@ int edx_hamiltonian_build(edx_ctx* ctx, const edx_catalog* cat, const edx_sector
       const size_t smem = terms_smem_bytes(N, cat->n_filled, T);
       int64_t E = 0;
       if (D > 0) {
-        DevBuf<uint32_t> cnt(D);
+        DevBuf<uint32_t> cnt(D), rmax(1);
         EDX_HIP(hipMemsetAsync(cnt.p, 0, D * 4, st));
+        EDX_HIP(hipMemsetAsync(rmax.p, 0, 4, st));
         hipLaunchKernelGGL(ptab_count, dim3(edx::grid_for(D, kBlock)), dim3(kBlock), smem, st, tv, sv, cnt.p);
         EDX_HIP(hipGetLastError());
+        hipLaunchKernelGGL(row_len_max, dim3(edx::grid_for(D, kBlock)), dim3(kBlock), 0, st, D,
+                           (const uint32_t*)cnt.p, rmax.p);
+        EDX_HIP(hipGetLastError());
         E = (int64_t)edx::exclusive_scan_u32_to_i64(*ctx, cnt.p, ham->row_ptr.p, (size_t)D);
+        uint32_t max_row = 0;
+        EDX_HIP(hipMemcpyAsync(&max_row, rmax.p, 4, hipMemcpyDeviceToHost, st));
+        EDX_HIP(hipStreamSynchronize(st));
         EDX_HIP(hipMemcpyAsync(ham->row_ptr.p + D, &E, 8, hipMemcpyHostToDevice, st));
         DevBuf<uint64_t> keys(std::max<int64_t>(E, 1));
         EDX_HIP(hipMemsetAsync(cnt.p, 0, D * 4, st));
         hipLaunchKernelGGL(ptab_fill, dim3(edx::grid_for(D, kBlock)), dim3(kBlock), smem, st, tv, sv,
                            (const int64_t*)ham->row_ptr.p, cnt.p, keys.p);
         EDX_HIP(hipGetLastError());
-        hipLaunchKernelGGL(ptab_sort, dim3(edx::grid_for(D, kSortThreads)), dim3(kSortThreads), 0, st, D,
-                           (const int64_t*)ham->row_ptr.p, keys.p);
-        EDX_HIP(hipGetLastError());
+        launch_sort(st, D, max_row, (const int64_t*)ham->row_ptr.p, keys.p);
         ham->col.alloc(std::max<int64_t>(E, 1));
         ham->term.alloc(std::max<int64_t>(E, 1));
         ham->val.alloc(std::max<int64_t>(E, 1));
-        hipLaunchKernelGGL(ptab_values, dim3(edx::grid_for(D, kBlock)), dim3(kBlock), 0, st, tv, sv,
+        ham->n_entries = E;
+        // the sliced copy is written in the same pass, except for twisted
+        // tables (their copy needs the finished table's transposed entries)
+        SellOut so{1, 0, nullptr, nullptr, nullptr, nullptr};
+        if (!ham->herm_part) {
+          plan = edx::plan_sell(*ham, *ctx);
+          edx::prepare_sell(*ham, *ctx, plan.lanes);
+          sell_written = true;
+          so = SellOut{plan.lanes, ham->sell_slices * (64 / plan.lanes), (const int64_t*)ham->sell_off.p,
+                       ham->sell_col.p, ham->sell_val.p, ham->sell_val32.p};
+        }
+        const int64_t value_rows = so.col ? std::max<int64_t>(so.rows, D) : D;
+        hipLaunchKernelGGL(ptab_values, dim3(edx::grid_for(value_rows, kBlock)), dim3(kBlock), 0, st, tv, sv,
                            (const double2*)d_chi.p, (const int64_t*)ham->row_ptr.p, (const uint64_t*)keys.p,
-                           ham->col.p, ham->term.p, ham->val.p);
+                           ham->col.p, ham->term.p, ham->val.p, so);
         EDX_HIP(hipGetLastError());
         hipLaunchKernelGGL(ptab_diag, dim3(edx::grid_for(D, kBlock)), dim3(kBlock), 0, st, D,
                            (const uint64_t*)sec->reps.p, n_density, (const uint8_t*)d_vi.p,
@@ -681,8 +771,10 @@ int edx_hamiltonian_build(edx_ctx* ctx, const edx_catalog* cat, const edx_sector
         EDX_HIP(hipStreamSynchronize(st));
       }
       ham->n_entries = E;
-      edx::build_sell_for(*ham, *ctx);
+      if (sell_written) edx::finish_sell(*ham, *ctx, plan);
+      else edx::build_sell_for(*ham, *ctx);
     } catch (...) {
+      edx::abandon_sell(*ham, plan);
       delete ham;
       throw;
     }
@@ -740,6 +832,8 @@ int edx_hamiltonian_from_csr(edx_ctx* ctx, int64_t dim, int64_t nnz, const int64
     edx::DeviceGuard dg(ctx->device);
     hipStream_t st = ctx->stream;
     auto ham = new edx_hamiltonian();
+    edx::SellPlan plan;  // sliced copy written by ptab_values (not for twisted tables)
+    bool sell_written = false;
     try {
       ham->device = ctx->device;
       ham->dim = dim;

@@ -118,54 +118,75 @@ TuneEntry tune_spmm(edx_hamiltonian& h, Ctx& ctx, int prec, int nb) {
 }
 
 // Lane count for a new build: the context's fixed choice, else the tuned
-// one (timing on a table miss), else the heuristic.  Leaves h built.
-void build_sell_for(edx_hamiltonian& h, Ctx& ctx) {
+// one, else the heuristic.  On a miss of the tuning table the plan says so
+// (and marks the key as being timed); finish_sell then times the layouts
+// once the table is complete.
+SellPlan plan_sell(const edx_hamiltonian& h, Ctx& ctx) {
+  SellPlan p;
   if (ctx.autotune_group > 0) {
-    build_sell(h, ctx, ctx.autotune_group);
-    return;
+    p.lanes = ctx.autotune_group;
+    return p;
   }
   TuneState& s = state();
   bool on;
-  int prec, nb;
   {
     std::lock_guard<std::mutex> lk(s.mu);
     on = s.enabled;
-    prec = s.prec;
-    nb = s.nb;
+    p.prec = s.prec;
+    p.nb = s.nb;
   }
-  if (!on || h.dim <= 500) {
-    build_sell(h, ctx, default_sell_lanes(h.dim, h.n_entries));
-    return;
+  p.lanes = default_sell_lanes(h.dim, h.n_entries);
+  if (!on || h.dim <= 500) return p;
+  const TuneKey k = key_for(h, p.prec, p.nb);
+  std::unique_lock<std::mutex> lk(s.mu);
+  // another build is timing this key: wait for its result (bounded)
+  s.cv.wait_for(lk, std::chrono::seconds(30), [&] { return !s.busy.count(k); });
+  auto it = s.table.find(k);
+  if (it != s.table.end()) {
+    p.lanes = it->second.lanes;
+  } else if (!s.busy.count(k)) {  // still being timed elsewhere: keep the heuristic
+    s.busy.insert(k);
+    p.tune = true;
   }
-  const TuneKey k = key_for(h, prec, nb);
-  int hit = 0;
-  {
-    std::unique_lock<std::mutex> lk(s.mu);
-    // another build is timing this key: wait for its result (bounded)
-    s.cv.wait_for(lk, std::chrono::seconds(30), [&] { return !s.busy.count(k); });
-    auto it = s.table.find(k);
-    if (it != s.table.end()) hit = it->second.lanes;
-    else if (s.busy.count(k)) hit = default_sell_lanes(h.dim, h.n_entries);  // still timing: heuristic
-    else s.busy.insert(k);
-  }
-  if (hit) {
-    build_sell(h, ctx, hit);
-    return;
-  }
+  return p;
+}
+
+void finish_sell(edx_hamiltonian& h, Ctx& ctx, SellPlan& p) {
+  if (!p.tune) return;
+  TuneState& s = state();
+  const TuneKey k = key_for(h, p.prec, p.nb);
   TuneEntry e{0, {0.0f, 0.0f, 0.0f}};
   try {
-    build_sell(h, ctx, default_sell_lanes(h.dim, h.n_entries));
-    e = tune_spmm(h, ctx, prec, nb);
+    e = tune_spmm(h, ctx, p.prec, p.nb);
   } catch (...) {
-    std::lock_guard<std::mutex> lk(s.mu);
-    s.busy.erase(k);
-    s.cv.notify_all();
+    abandon_sell(h, p);
     throw;
   }
   std::lock_guard<std::mutex> lk(s.mu);
   s.table.emplace(k, e);  // the first measurement of a key is kept
   s.busy.erase(k);
   s.cv.notify_all();
+  p.tune = false;
+}
+
+void abandon_sell(const edx_hamiltonian& h, SellPlan& p) {
+  if (!p.tune) return;
+  TuneState& s = state();
+  std::lock_guard<std::mutex> lk(s.mu);
+  s.busy.erase(key_for(h, p.prec, p.nb));
+  s.cv.notify_all();
+  p.tune = false;
+}
+
+void build_sell_for(edx_hamiltonian& h, Ctx& ctx) {
+  SellPlan p = plan_sell(h, ctx);
+  try {
+    build_sell(h, ctx, p.lanes);
+  } catch (...) {
+    abandon_sell(h, p);
+    throw;
+  }
+  finish_sell(h, ctx, p);
 }
 
 }  // namespace edx

@@ -203,6 +203,31 @@ def _sliced_spmm(H, X):
     return Yd.download(np.empty((nb, n), np.complex128)).T
 
 
+@pytest.mark.parametrize("name,irrep", [("heis16", 0), ("hub24", 1), ("hal23", 0), ("hub34", 2)])
+def test_sliced_copy_written_with_the_table(name, irrep):
+    """The sliced copy that ptab_values writes while building the table (with
+    its padding) is the one rebuilt from the CSR arrays: the same layout gives
+    bitwise the same SpMM."""
+    from realspace_exatdiagonalization_py_amd import _edx
+
+    model, ne, G, oc, gc = _both(name)
+    ir = build_irrep_list(G, model.lattice)[irrep]
+    basis = ed.build_symmetry_sector_basis(gc, ir)
+    H = ed.MatrixFreeHamiltonian(basis, model.bilinear_terms, model.density_density_terms,
+                                 ed.CanonicalMap(G, model.particle_statistics, gc))
+    dh = H._dev
+    n = basis.dim
+    print(f"[sell] {name} irrep={irrep} dim={n}", flush=True)
+    X = np.stack([random_vector(n, s) for s in range(6)], axis=1)
+    g0 = dh.spmv_group
+    y_built = _sliced_spmm(H, X)
+    for g in [x for x in (1, 2, 4) if x != g0] + [g0]:  # rebuilt from the CSR arrays, ending on g0
+        dh.ctx.check(_edx.lib.edx_hamiltonian_set_spmv_group(dh.ctx.handle, dh.handle, g))
+        y_g = _sliced_spmm(H, X)
+        assert np.linalg.norm(y_g - y_built) <= 1e-13 * np.linalg.norm(y_built), g
+    np.testing.assert_array_equal(y_g, y_built)
+
+
 def test_twisted_block_solves_its_hermitian_part():
     """A twisted catalog rounds alpha to complex64 (as the reference stores
     it), so the block is Hermitian only to that rounding.  H x applies the
