@@ -14,9 +14,11 @@
 //   P2 scan    row counts -> row_ptr (int64)
 //   P3 fill    same walk; key (col << 16 | term) at row_ptr[row] + atomic slot
 //   P4 sort    per row: keys ascending, so the row order is (col, term) and the
-//              result never depends on atomic ordering
-//   P5 values  per entry: element recomputed from (col, term) in the reference's
-//              multiplication order; complex128 and complex64 copies
+//              result never depends on atomic ordering (LDS sized by the
+//              longest row)
+//   P5 values  4 lanes per row: element recomputed from (col, term) in the
+//              reference's multiplication order, written to the CSR arrays
+//              and (untwisted tables) straight into the sliced copy
 //   P6 diag    per row: density terms in term order
 // Hops are enumerated per occupied site over terms grouped by source site,
 // so every lane walks the same number of candidates.
@@ -83,7 +85,12 @@ size_t terms_smem_bytes(int n_site, int n_filled, int n_terms) {
   return (size_t)(n_site + 1) * (n_filled + 1) * 4 + (size_t)(n_site + 1) * 4 + (size_t)n_terms * 3 + 16;
 }
 
-// Visits every valid hop of column c: f(row, term).
+// Lanes that share one row of the value pass: each takes every kItemLanes-th
+// entry, so more catalog lookups are in flight (the count and fill passes
+// keep one lane per column, which measured faster).
+constexpr int kItemLanes = 4;
+
+// Visits every valid hop of column rep m: f(row, term).
 template <class F>
 __device__ __forceinline__ void for_each_hop(const TermView& tv, const SectorView& sv, const SharedTerms& st,
                                              uint64_t m, F&& f) {
@@ -217,7 +224,9 @@ ptab_values(TermView tv, SectorView sv, const double2* chi, const int64_t* row_p
   for (int i = threadIdx.x; i < nb; i += blockDim.x) binom_s[i] = tv.binom[i];
   for (int i = threadIdx.x; i < EDX_MAX_GROUP; i += blockDim.x) chi_s[i] = chi[i];
   __syncthreads();
-  const int64_t r = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const int64_t r = t / kItemLanes;
+  const int sub = (int)(t % kItemLanes);  // this lane takes entries k with k % kItemLanes == sub
   // slots of this row in the sliced copy: entry k at base + (k / G) * 64 + k % G
   int64_t sbase = 0, slots = 0;
   if (so.col && r < so.rows) {
@@ -226,11 +235,14 @@ ptab_values(TermView tv, SectorView sv, const double2* chi, const int64_t* row_p
     sbase = so.off[sl] + (r % R) * so.G;
     slots = (so.off[sl + 1] - so.off[sl]) / 64 * so.G;
   }
-  int64_t k = 0;
+  int64_t n_row = 0;
   if (r < sv.dim) {
     const int nf1 = tv.n_filled + 1;
     const double stab_row = (double)sv.stab[r];
-    for (int64_t e = row_ptr[r]; e < row_ptr[r + 1]; ++e, ++k) {
+    const int64_t e0 = row_ptr[r], e1 = row_ptr[r + 1];
+    n_row = e1 - e0;
+    for (int64_t e = e0 + sub; e < e1; e += kItemLanes) {
+      const int64_t k = e - e0;
       const uint64_t key = keys[e];
       const int64_t c = (int64_t)(key >> 16);
       const int kt = (int)(key & 0xffffu);
@@ -241,9 +253,9 @@ ptab_values(TermView tv, SectorView sv, const double2* chi, const int64_t* row_p
       const float2 a32 = sv.rep_amp[idx];
       const double2 a = make_double2((double)a32.x, (double)a32.y);
       const double s = (tv.fermionic && (edxd::popc64(m & tv.between[kt]) & 1)) ? -1.0 : 1.0;
-      const double2 t = tv.amp[kt];
+      const double2 amp = tv.amp[kt];
       // t * hop_phase, with hop_phase the complex (s + 0i)
-      double2 h = make_double2(t.x * s - t.y * 0.0, t.x * 0.0 + t.y * s);
+      double2 h = make_double2(amp.x * s - amp.y * 0.0, amp.x * 0.0 + amp.y * s);
       h = cmul(h, a);
       const double2 cc = make_double2(chi_s[g].x, -chi_s[g].y);
       h = cmul(h, cc);
@@ -260,7 +272,8 @@ ptab_values(TermView tv, SectorView sv, const double2* chi, const int64_t* row_p
       }
     }
   }
-  for (; k < slots; ++k) {  // padding, and the rows past the last one
+  for (int64_t k = sub; k < slots; k += kItemLanes) {  // padding, and the rows past the last one
+    if (k < n_row) continue;
     const int64_t pos = sbase + (k / so.G) * 64 + (k % so.G);
     so.col[pos] = -1;
     so.val[pos] = make_double2(0.0, 0.0);
@@ -756,7 +769,8 @@ int edx_hamiltonian_build(edx_ctx* ctx, const edx_catalog* cat, const edx_sector
                        ham->sell_col.p, ham->sell_val.p, ham->sell_val32.p};
         }
         const int64_t value_rows = so.col ? std::max<int64_t>(so.rows, D) : D;
-        hipLaunchKernelGGL(ptab_values, dim3(edx::grid_for(value_rows, kBlock)), dim3(kBlock), 0, st, tv, sv,
+        hipLaunchKernelGGL(ptab_values, dim3(edx::grid_for(value_rows * kItemLanes, kBlock)), dim3(kBlock), 0, st,
+                           tv, sv,
                            (const double2*)d_chi.p, (const int64_t*)ham->row_ptr.p, (const uint64_t*)keys.p,
                            ham->col.p, ham->term.p, ham->val.p, so);
         EDX_HIP(hipGetLastError());
