@@ -429,17 +429,23 @@ step_coeffs(const double2* __restrict__ P, int nbasis, int b, double kappa2, dou
             double single_tol, int second_pass, double2* __restrict__ S, double2* __restrict__ R1out,
             double2* __restrict__ log, int* __restrict__ ctl) {
   __shared__ double2 G[kMaxW][kMaxW], R[kMaxW][kMaxW], X[kMaxW][kMaxW];
+  __shared__ double2 Ps[(kMaxVec + kMaxW) * kMaxW];  // P, read once with all lanes
   __shared__ int status_s;
   if (ctl[1]) return;  // halted: the host finishes an earlier step first
   const int tid = threadIdx.x;
-  for (int e = tid; e < nbasis * b; e += 256) log[e] = P[e];
+  for (int e = tid; e < (nbasis + b) * b; e += 256) {
+    const double2 v = P[e];
+    Ps[e] = v;
+    if (e < nbasis * b) log[e] = v;
+  }
+  __syncthreads();
   for (int e = tid; e < kMaxW * kMaxW; e += 256) {
     const int i = e / kMaxW, j = e % kMaxW;
     double2 g = make_double2(0.0, 0.0);
     if (i < b && j < b) {
-      g = P[(nbasis + i) * b + j];
+      g = Ps[(nbasis + i) * b + j];
       for (int k = 0; k < nbasis; ++k) {  // g -= conj(P(k,i)) P(k,j)
-        const double2 a = P[k * b + i], c = P[k * b + j];
+        const double2 a = Ps[k * b + i], c = Ps[k * b + j];
         g.x -= a.x * c.x + a.y * c.y;
         g.y -= a.x * c.y - a.y * c.x;
       }
@@ -451,7 +457,7 @@ step_coeffs(const double2* __restrict__ P, int nbasis, int b, double kappa2, dou
   double drop = 0.0, dmax = 0.0;
   for (int j = 0; j < b; ++j) {
     const double g = G[j][j].x;
-    drop = fmax(drop, g > 0.0 ? P[(nbasis + j) * b + j].x / g : INFINITY);
+    drop = fmax(drop, g > 0.0 ? Ps[(nbasis + j) * b + j].x / g : INFINITY);
     dmax = fmax(dmax, g);
   }
   bool ok = dmax > 0.0;
@@ -517,7 +523,7 @@ step_coeffs(const double2* __restrict__ P, int nbasis, int b, double kappa2, dou
     if (i < nbasis) {  // -sum_k C(i,k) R1^-1(k,j)
       double sr = 0.0, si = 0.0;
       for (int k = 0; k <= j; ++k) {
-        const double2 a = P[i * b + k], c = X[k][j];
+        const double2 a = Ps[i * b + k], c = X[k][j];
         sr += a.x * c.x - a.y * c.y;
         si += a.x * c.y + a.y * c.x;
       }
