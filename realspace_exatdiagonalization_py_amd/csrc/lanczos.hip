@@ -351,12 +351,13 @@ __global__ void __launch_bounds__(64) block_cholinv(int b, const double2* __rest
                                                    double2* __restrict__ S, double2* __restrict__ Rout,
                                                    int* __restrict__ status, const double2* __restrict__ R1,
                                                    StepPred pred) {
-  __shared__ double2 A[kMaxW][kMaxW], R[kMaxW][kMaxW], X[kMaxW][kMaxW];
+  __shared__ double2 A[kMaxW][kMaxW], R[kMaxW][kMaxW], X[kMaxW][kMaxW], P1[kMaxW][kMaxW];
   if (!pred.run()) return;
   const int lane = threadIdx.x;
   for (int e = lane; e < kMaxW * kMaxW; e += 64) {
     const int i = e / kMaxW, j = e % kMaxW;
     A[i][j] = (i < b && j < b) ? G[i * b + j] : make_double2(0.0, 0.0);
+    P1[i][j] = (R1 && i < b && j < b) ? R1[i * b + j] : make_double2(0.0, 0.0);
     R[i][j] = X[i][j] = make_double2(0.0, 0.0);
   }
   __syncthreads();
@@ -401,7 +402,7 @@ __global__ void __launch_bounds__(64) block_cholinv(int b, const double2* __rest
     if (R1) {  // Rout = R R1 (both upper triangular)
       double sr = 0.0, si = 0.0;
       for (int k = i; k <= j; ++k) {
-        const double2 a = R[i][k], c = R1[k * b + j];
+        const double2 a = R[i][k], c = P1[k][j];
         sr += a.x * c.x - a.y * c.y;
         si += a.x * c.y + a.y * c.x;
       }
