@@ -253,10 +253,11 @@ __global__ void __launch_bounds__(256) gram_reduce(int nblk, int nout, const dou
   if (o < nout && lane == 0) out[o] = make_double2(xr, xi);
 }
 
-// Vectors loaded per stage of gemm_tiles (two stages in flight): 4 for the
-// block updates (nk <= 16 outputs, one tile), 2 for the wide restart
-// combinations (tiles of 12 outputs), the fastest measured at those shapes.
-constexpr int kGemmStage = 4;
+// Vectors loaded per stage of gemm_tiles (two stages in flight): 2 for the
+// block updates (nk <= 16 outputs, one tile; 1-7 % faster than 4 in
+// complex64 at D = 2.3e5 and 3.8e6, equal in complex128) and for the wide
+// restart combinations (tiles of 12 outputs).
+constexpr int kGemmStage = 2;
 constexpr int kCombineStage = 2;
 constexpr int kCombineTile = 12;
 
@@ -909,11 +910,14 @@ class BlockLanczos {
   case NWV:                     \
     launch_tiles<VT, TIV, NWV>(A0, n0, A1, n1, W, pred); \
     break;
-      // TI x NW tiles of 16 to 32 complex sums per thread
+      // TI x NW tiles of 16 to 32 complex sums per thread.  Measured at
+      // D = 3.8e6 and 2.3e5, 48 and 88 basis vectors, complex64 and
+      // complex128: TI = 4 beats 3 at NW = 7, 8 (10-22 %), TI = 3 beats 2 at
+      // NW = 9, 10 (17-21 %), TI = 2 beats 1 at NW = 16 (about 40 %)
       EDX_GRAM_CASE(1, 8) EDX_GRAM_CASE(2, 8) EDX_GRAM_CASE(3, 6) EDX_GRAM_CASE(4, 6)
-      EDX_GRAM_CASE(5, 4) EDX_GRAM_CASE(6, 4) EDX_GRAM_CASE(7, 3) EDX_GRAM_CASE(8, 3)
-      EDX_GRAM_CASE(9, 2) EDX_GRAM_CASE(10, 2) EDX_GRAM_CASE(11, 2) EDX_GRAM_CASE(12, 2)
-      EDX_GRAM_CASE(13, 1) EDX_GRAM_CASE(14, 1) EDX_GRAM_CASE(15, 1) EDX_GRAM_CASE(16, 1)
+      EDX_GRAM_CASE(5, 4) EDX_GRAM_CASE(6, 4) EDX_GRAM_CASE(7, 4) EDX_GRAM_CASE(8, 4)
+      EDX_GRAM_CASE(9, 3) EDX_GRAM_CASE(10, 3) EDX_GRAM_CASE(11, 2) EDX_GRAM_CASE(12, 2)
+      EDX_GRAM_CASE(13, 2) EDX_GRAM_CASE(14, 2) EDX_GRAM_CASE(15, 2) EDX_GRAM_CASE(16, 2)
 #undef EDX_GRAM_CASE
       default:
         throw edx::Error{EDX_ERR_INTERNAL, "Gram block width out of range"};
@@ -1022,8 +1026,8 @@ class BlockLanczos {
   }
 
   // One tile of the smallest width that holds nk (nk <= 16), otherwise tiles
-  // of 12 outputs (restart combinations: 97 us against 129 us with tiles of
-  // 6 at 60 -> 33 vectors, complex64, D = 225k).
+  // of up to 12 outputs (restart combinations: 97 us against 129 us with
+  // tiles of 6 at 60 -> 33 vectors, complex64, D = 225k).
   template <typename VT>
   void launch_gemm_v(const void* A0, int n0, const void* A1, int n1, void* Out, int nk, const double2* S, int ldS,
                      void* Xi, StepPred pd) {
@@ -1033,7 +1037,19 @@ class BlockLanczos {
     else if (nk <= 8) launch_gemm_t<VT, 8>(A0, n0, A1, n1, Out, nk, S, ldS, Xi, pd);
     else if (nk <= 12) launch_gemm_t<VT, 12>(A0, n0, A1, n1, Out, nk, S, ldS, Xi, pd);
     else if (nk <= 16) launch_gemm_t<VT, 16>(A0, n0, A1, n1, Out, nk, S, ldS, Xi, pd);
-    else launch_gemm_t<VT, kCombineTile, kCombineStage>(A0, n0, A1, n1, Out, nk, S, ldS, Xi, pd);
+    else {
+      // as many tiles as with kCombineTile outputs each, but of equal width
+      // (9 to 12 outputs), so the last one carries no padding: the
+      // combination is bound by FP64 FMAs (60 -> 33: 84.8 us in tiles of 11
+      // against 91.4 us in tiles of 12, complex64, D = 225k)
+      const int n_kt = (nk + kCombineTile - 1) / kCombineTile;
+      switch ((nk + n_kt - 1) / n_kt) {
+        case 9: launch_gemm_t<VT, 9, kCombineStage>(A0, n0, A1, n1, Out, nk, S, ldS, Xi, pd); break;
+        case 10: launch_gemm_t<VT, 10, kCombineStage>(A0, n0, A1, n1, Out, nk, S, ldS, Xi, pd); break;
+        case 11: launch_gemm_t<VT, 11, kCombineStage>(A0, n0, A1, n1, Out, nk, S, ldS, Xi, pd); break;
+        default: launch_gemm_t<VT, kCombineTile, kCombineStage>(A0, n0, A1, n1, Out, nk, S, ldS, Xi, pd);
+      }
+    }
   }
 
   // True when the nk output vectors at Out overlap the n vectors at A.
