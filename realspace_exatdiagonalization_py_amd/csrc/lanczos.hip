@@ -914,8 +914,19 @@ class BlockLanczos {
       // D = 3.8e6 and 2.3e5, 48 and 88 basis vectors, complex64 and
       // complex128: TI = 4 beats 3 at NW = 7, 8 (10-22 %), TI = 3 beats 2 at
       // NW = 9, 10 (17-21 %), TI = 2 beats 1 at NW = 16 (about 40 %)
-      EDX_GRAM_CASE(1, 8) EDX_GRAM_CASE(2, 8) EDX_GRAM_CASE(3, 6) EDX_GRAM_CASE(4, 6)
-      EDX_GRAM_CASE(5, 4) EDX_GRAM_CASE(6, 4) EDX_GRAM_CASE(7, 4) EDX_GRAM_CASE(8, 4)
+      EDX_GRAM_CASE(1, 8) EDX_GRAM_CASE(2, 8) EDX_GRAM_CASE(3, 6)
+      EDX_GRAM_CASE(5, 4) EDX_GRAM_CASE(7, 4) EDX_GRAM_CASE(8, 4)
+      // from 2^20 rows the larger tile wins (fewer re-reads of W; D = 3.8e6:
+      // 6-9 % at NW = 6, 3-11 % at NW = 4); below, the smaller one keeps more
+      // workgroups per row block (D = 2.3e5: TI = 5 10-15 % slower at NW = 6)
+      case 4:
+        if (D_ >= (int64_t(1) << 20)) launch_tiles<VT, 8, 4>(A0, n0, A1, n1, W, pred);
+        else launch_tiles<VT, 6, 4>(A0, n0, A1, n1, W, pred);
+        break;
+      case 6:
+        if (D_ >= (int64_t(1) << 20)) launch_tiles<VT, 5, 6>(A0, n0, A1, n1, W, pred);
+        else launch_tiles<VT, 4, 6>(A0, n0, A1, n1, W, pred);
+        break;
       EDX_GRAM_CASE(9, 3) EDX_GRAM_CASE(10, 3) EDX_GRAM_CASE(11, 2) EDX_GRAM_CASE(12, 2)
       EDX_GRAM_CASE(13, 2) EDX_GRAM_CASE(14, 2) EDX_GRAM_CASE(15, 2) EDX_GRAM_CASE(16, 2)
 #undef EDX_GRAM_CASE

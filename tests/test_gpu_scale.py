@@ -134,3 +134,26 @@ def test_checkerboard48_sector_solve():
     assert np.max(np.abs(w32 - w64)) <= 1e-10
     R = H(v32) - v32 * w32
     assert np.max(np.linalg.norm(R, axis=0)) <= 1e-5
+
+
+@pytest.mark.parametrize("nev", [6, 4])
+def test_heisenberg28_blocks_above_a_million_rows(nev):
+    """A sector of the 28-site Heisenberg chain (D ~ 1.43 M, above the 2^20
+    rows from which the Gram uses its larger tiles at block widths 4 and 6):
+    mixed precision against FP64 within 1e-10, and the true residuals of the
+    returned FP64 vectors."""
+    model = M.build_heisenberg_second_quantized_model(28)
+    G = build_translation_group(model.lattice)
+    gc = ed.build_symmetry_orbit_catalog(second_quantized_model=model, n_filled=14, symmetry_group=G,
+                                         particle_statistics=model.particle_statistics)
+    irrep = build_irrep_list(G, model.lattice)[3]
+    basis = ed.build_symmetry_sector_basis(gc, irrep)
+    H = ed.MatrixFreeHamiltonian(basis, model.bilinear_terms, model.density_density_terms, None)
+    assert basis.dim >= 1 << 20
+    print(f"[scale] heisenberg28 k=3 D={basis.dim} nev={nev}", flush=True)
+    w32, _ = ed.diagonalize_block_matrixfree(H, H.n, nev=nev, precision=ed.PREC_MIXED, eig_tol=1e-12,
+                                             want_vectors=False)
+    w64, v64 = ed.diagonalize_block_matrixfree(H, H.n, nev=nev)
+    assert np.max(np.abs(w32 - w64)) <= 1e-10, (w32, w64)
+    R = H._dev.matvec(np.ascontiguousarray(v64)) - v64 * w64
+    assert np.max(np.linalg.norm(R, axis=0)) <= 1e-9 * max(1.0, np.abs(w64).max())
