@@ -303,6 +303,25 @@ def test_mixed_handover_below_the_complex64_floor():
     assert out[ed.PREC_MIXED][1][1] < 200  # restarts: the stall guard ended the sweeps
 
 
+@pytest.mark.parametrize("nev", [12, 20])
+def test_many_levels_wide_blocks(nev):
+    """nev above the block width and above 16 (the complex64 sweeps resolve 16
+    pairs, the FP64 phase starts from them in a block of up to 16): both
+    precisions against the dense spectrum of the oracle's block."""
+    model, ne, G, oc, gc = _both("hub24")
+    irrep = build_irrep_list(G, model.lattice)[1]
+    reps, r, c, v, hd, k = _oracle_block(model, oc, irrep, True)
+    dense = O.csr_hamiltonian(r, c, v, hd, len(reps)).toarray()
+    exact = np.linalg.eigvalsh((dense + dense.conj().T) / 2)[:nev]
+    basis = ed.build_symmetry_sector_basis(gc, irrep)
+    H = ed.MatrixFreeHamiltonian(basis, model.bilinear_terms, model.density_density_terms,
+                                 ed.CanonicalMap(G, model.particle_statistics, gc))
+    print(f"[wide] hub24 irrep=1 dim={basis.dim} nev={nev}", flush=True)
+    for prec in (ed.PREC_FP64, ed.PREC_MIXED):
+        w, _ = ed.diagonalize_block_matrixfree(H, H.n, nev=nev, precision=prec)
+        assert np.max(np.abs(w - exact)) <= 1e-10, (prec, w, exact)
+
+
 def test_eigs_mixed_precision_cfg1():
     w64, _, w_ref, _ = _solve_both("heis16", 0, 4)
     w32, _, _, _ = _solve_both("heis16", 0, 4, precision=ed.PREC_MIXED)
