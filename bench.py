@@ -302,7 +302,7 @@ def serve_bench(args) -> int:
     rng = np.random.default_rng(args.serve_seed + 1)
     arrivals = poisson_arrivals(len(reqs), args.serve_rate, seed=args.serve_seed)
     log(f"serving {len(reqs)} requests (Poisson {args.serve_rate}/s, dims up to {args.serve_max_dim:.2g}) ...")
-    with SectorSolveServer(workers=args.workers) as server:
+    with SectorSolveServer(workers=args.workers, policy=args.serve_policy) as server:
         t0 = time.perf_counter()
         results = serve_stream(server, reqs, arrivals)
         wall = time.perf_counter() - t0
@@ -340,6 +340,7 @@ def serve_bench(args) -> int:
         "config": {"workload": "Poisson stream over the Heisenberg, Hubbard and checkerboard families: random "
                                "size, momentum sector, twist angle in [0, 2pi) and 1-10 eigenvalues",
                    "requests": len(reqs), "offered_rate_per_s": args.serve_rate, "workers": args.workers,
+                   "policy": args.serve_policy,
                    "seed": args.serve_seed, "max_dim": args.serve_max_dim},
         "wall_s": round(wall, 3),
         "p50_ms": round(float(np.percentile(lat, 50)), 3),
@@ -389,6 +390,8 @@ def main() -> int:
     ap.add_argument("--serve-max-dim", type=float, default=4e6, help="largest approximate sector dimension drawn")
     ap.add_argument("--serve-seed", type=int, default=2024)
     ap.add_argument("--serve-check", type=int, default=6, help="requests re-solved alone in FP64 for |dE|")
+    ap.add_argument("--serve-policy", choices=("fifo", "sjf"), default="fifo",
+                    help="queue order: arrival order, or least estimated footprint first (with aging)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}): sectors shard across GPUs "

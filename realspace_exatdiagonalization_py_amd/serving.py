@@ -107,17 +107,26 @@ class SectorSolveServer:
 
     hbm_budget: bytes that admitted requests may occupy together (default:
     80 % of the free device memory at start).  A request larger than the
-    whole budget runs alone.  Requests behind the head of the queue may start
-    first when they fit and the head does not, at most ``max_skip`` times for
-    any one head request.
+    whole budget runs alone.
+
+    policy: ``"fifo"`` serves in arrival order; requests behind the head may
+    start first when they fit and the head does not, at most ``max_skip``
+    times for any one head request.  ``"sjf"`` serves the admissible request
+    of least estimated footprint (which grows with the sector dimension, so
+    roughly the shortest job) first; every older request it passes ages by
+    one, and a request passed ``max_skip`` times goes next (waiting for room
+    if it must), so none starves.
     """
 
     def __init__(self, workers: int = 8, *, hbm_budget: int | None = None, catalog_cache_bytes: int | None = None,
-                 max_skip: int = 8, solve_fn: Callable | None = None,
+                 max_skip: int = 8, policy: str = "fifo", solve_fn: Callable | None = None,
                  context_factory: Callable[[], Any] | None = None,
                  estimate_fn: Callable[[SolveRequest], int] | None = None):
         if workers < 1:
             raise ValueError("workers must be >= 1")
+        if policy not in ("fifo", "sjf"):
+            raise ValueError("policy must be 'fifo' or 'sjf'")
+        self._policy = policy
         self._cv = threading.Condition()
         self._pending: collections.deque = collections.deque()
         self._in_use = 0
@@ -194,10 +203,15 @@ class SectorSolveServer:
         }
 
     # ---- scheduling ------------------------------------------------------------
+    def _fits(self, ent) -> bool:
+        return self._in_use + ent[3] <= self._budget or self._running == 0
+
     def _take(self):
         """Next admissible entry (called with the lock held), or None."""
         if not self._pending:
             return None
+        if self._policy == "sjf":
+            return self._take_sjf()
         head = self._pending[0]
         limit = 1 if head[4] >= self._max_skip else len(self._pending)
         for idx in range(min(limit, len(self._pending))):
@@ -208,6 +222,25 @@ class SectorSolveServer:
                     head[4] += 1
                 return ent
         return None
+
+    def _take_sjf(self):
+        for idx, ent in enumerate(self._pending):  # a request passed max_skip times goes next
+            if ent[4] >= self._max_skip:
+                if not self._fits(ent):
+                    return None
+                del self._pending[idx]
+                return ent
+        best = None
+        for idx, ent in enumerate(self._pending):
+            if self._fits(ent) and (best is None or ent[3] < self._pending[best][3]):
+                best = idx
+        if best is None:
+            return None
+        for idx in range(best):  # the older requests passed over
+            self._pending[idx][4] += 1
+        ent = self._pending[best]
+        del self._pending[best]
+        return ent
 
     def _worker(self, ctx) -> None:
         while True:

@@ -38,8 +38,8 @@ class _Probe:
         return np.array([float(req.sector)]), None, 1, None
 
 
-def _server(probe, workers=4, budget=100, max_skip=8):
-    return SectorSolveServer(workers, hbm_budget=budget, catalog_cache_bytes=0, max_skip=max_skip,
+def _server(probe, workers=4, budget=100, max_skip=8, policy="fifo"):
+    return SectorSolveServer(workers, hbm_budget=budget, catalog_cache_bytes=0, max_skip=max_skip, policy=policy,
                              solve_fn=probe.solve, context_factory=lambda: None,
                              estimate_fn=lambda r: r.tag[0])
 
@@ -106,3 +106,32 @@ def test_poisson_stream_and_shutdown():
     srv.shutdown()
     with pytest.raises(RuntimeError):
         srv.submit(_req(99, 1))
+
+
+def test_sjf_serves_the_smallest_admissible_first():
+    probe = _Probe()
+    with _server(probe, workers=1, budget=1000, policy="sjf") as srv:
+        blocker = srv.submit(_req(0, 5, 0.05))  # queued requests wait behind it
+        time.sleep(0.01)
+        futs = [srv.submit(_req(i, size, 0.001)) for i, size in ((1, 50), (2, 10), (3, 30), (4, 20))]
+        for f in [blocker] + futs:
+            f.result(timeout=30)
+    assert probe.order == [0, 2, 4, 3, 1]
+
+
+def test_sjf_aging_bounds_the_wait():
+    """A large request is passed over by smaller ones at most max_skip times."""
+    probe = _Probe()
+    with _server(probe, workers=1, budget=1000, max_skip=2, policy="sjf") as srv:
+        blocker = srv.submit(_req(0, 5, 0.05))
+        time.sleep(0.01)
+        big = srv.submit(_req(1, 90, 0.001))
+        small = [srv.submit(_req(i, 10, 0.001)) for i in range(2, 8)]
+        for f in [blocker, big] + small:
+            f.result(timeout=30)
+    assert probe.order.index(1) <= 3  # blocker, then at most two small ones
+
+
+def test_unknown_policy_is_rejected():
+    with pytest.raises(ValueError):
+        SectorSolveServer(1, hbm_budget=10, policy="lifo", solve_fn=lambda *a: None, context_factory=lambda: None)
