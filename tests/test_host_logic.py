@@ -152,3 +152,13 @@ def test_hermitian_part_of_a_block():
     Hh = sre._hermitian_part(E)
     np.testing.assert_allclose(Hh.toarray(), (E.toarray() + E.toarray().conj().T) / 2, rtol=0, atol=1e-15)
     assert abs(Hh - Hh.getH()).max() == 0.0
+
+
+def test_more_than_63_sites_is_refused_before_any_device_work():
+    """The 64-bit mask limit (the reference's N <= 63 check, SRE:1328-1332)
+    raises ValueError before the library is asked for anything."""
+    model = ed.build_heisenberg_second_quantized_model(64)
+    G = ed.build_translation_group(model.lattice)
+    with pytest.raises(ValueError, match="64-bit"):
+        ed.build_symmetry_orbit_catalog(second_quantized_model=model, n_filled=32, symmetry_group=G,
+                                        particle_statistics=model.particle_statistics)
