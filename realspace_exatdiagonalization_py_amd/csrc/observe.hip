@@ -70,7 +70,7 @@ __global__ void __launch_bounds__(kBlock) sumsq(int64_t n, const double2* amp, d
   red[threadIdx.x] = s;
   __syncthreads();
   for (int w = kBlock / 2; w >= 1; w >>= 1) {
-    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
     __syncthreads();
   }
   if (threadIdx.x == 0) partial[blockIdx.x] = red[0];

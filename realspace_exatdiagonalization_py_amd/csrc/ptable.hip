@@ -846,8 +846,6 @@ int edx_hamiltonian_from_csr(edx_ctx* ctx, int64_t dim, int64_t nnz, const int64
     edx::DeviceGuard dg(ctx->device);
     hipStream_t st = ctx->stream;
     auto ham = new edx_hamiltonian();
-    edx::SellPlan plan;  // sliced copy written by ptab_values (not for twisted tables)
-    bool sell_written = false;
     try {
       ham->device = ctx->device;
       ham->dim = dim;
