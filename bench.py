@@ -336,7 +336,7 @@ def serve_bench(args) -> int:
         "n_gpus": 1,
         "higher_is_better": True,
         "data": "synthetic, seeded request stream",
-        "dtype": "complex64 sweeps + complex128 refinement",
+        "dtype": "c64+c128",
         "config": {"workload": "Poisson stream over the Heisenberg, Hubbard and checkerboard families: random "
                                "size, momentum sector, twist angle in [0, 2pi) and 1-10 eigenvalues",
                    "requests": len(reqs), "offered_rate_per_s": args.serve_rate, "workers": args.workers,
@@ -490,7 +490,9 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "complex64 sweeps + complex128 refinement" if prec == ed.PREC_MIXED else "complex128",
+        "dtype": "c64+c128" if prec == ed.PREC_MIXED else "c128",
+        "precision_detail": ("complex64 sweeps (FP64 Gram accumulation, coefficients and Ritz problems), then "
+                             "complex128 refinement" if prec == ed.PREC_MIXED else "complex128 throughout"),
         "data": "synthetic (model fully determined by its parameters)",
         "config": {
             "workload": "spinful Hubbard 3x4 PBC, t=1, U=4, 12 fermions on 24 orbitals, C(24,12)=2704156, "
