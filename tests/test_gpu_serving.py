@@ -35,9 +35,10 @@ def _requests():
     return [reqs[i] for i in order]
 
 
-def test_serving_mix_matches_alone():
+@pytest.mark.parametrize("policy", ["fifo", "sjf"])
+def test_serving_mix_matches_alone(policy):
     reqs = _requests()
-    with SectorSolveServer(workers=4) as srv:
+    with SectorSolveServer(workers=4, policy=policy) as srv:
         print(f"[serving] {len(reqs)} requests, budget {srv.stats()}", flush=True)
         mixed = serve_stream(srv, reqs, poisson_arrivals(len(reqs), 200.0, seed=11))
         st = srv.stats()
