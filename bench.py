@@ -472,6 +472,13 @@ def main() -> int:
                      "frac": round(iso_bytes / iso_ms / 1e6 / HBM_PEAK_GBS, 4),
                      "note": "same kernel, sector k=(0,0), one stream, 20 launches after the timed region"},
         "kernel_time_share": {k: round(v[0] / total_k, 3) for k, v in ks.items() if total_k > 0},
+        # every timed class against the same peak: algorithmic bytes (SpMM:
+        # table, gathered and written vectors; Gram: the vectors read; update:
+        # the vectors read and written) over the event-timed duration, which
+        # for the Gram includes its partial-sum reduction
+        "by_class": {k: {"launches": v[1], "avg_launch_ms": round(v[0] / v[1], 5),
+                         "achieved": round(v[2] / v[0] / 1e6, 1), "frac": round(v[2] / v[0] / 1e6 / HBM_PEAK_GBS, 4)}
+                     for k, v in ks.items() if v[1] > 0 and v[0] > 0 and v[2] > 0},
     }
     cpu = None
     if not args.no_cpu_baseline:
