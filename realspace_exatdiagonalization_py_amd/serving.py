@@ -144,7 +144,8 @@ class SectorSolveServer:
         self._catalog_budget = int(catalog_cache_bytes if catalog_cache_bytes is not None else self._budget // 4)
         self._catalogs: collections.OrderedDict = collections.OrderedDict()
         self._cat_lock = threading.Lock()
-        self._terms: dict = {}
+        self._terms: collections.OrderedDict = collections.OrderedDict()  # twisted term lists, LRU
+        self._max_term_sets = 256
         self._done: list[SolveResult] = []
         self._peak_in_use = 0
         self._t0 = time.perf_counter()
@@ -216,7 +217,7 @@ class SectorSolveServer:
         limit = 1 if head[4] >= self._max_skip else len(self._pending)
         for idx in range(min(limit, len(self._pending))):
             ent = self._pending[idx]
-            if self._in_use + ent[3] <= self._budget or self._running == 0:
+            if self._fits(ent):
                 del self._pending[idx]
                 if idx > 0:
                     head[4] += 1
@@ -292,15 +293,24 @@ class SectorSolveServer:
     def _bilinear_terms(self, req: SolveRequest):
         if req.twist is None or not any(req.twist):
             return req.model.bilinear_terms
+        # the entry holds the model itself, so its id cannot be reused by
+        # another model while the entry exists
         key = (id(req.model), tuple(float(x) for x in req.twist))
-        terms = self._terms.get(key)
-        if terms is None:
-            from .lattice import generate_bilinear_terms
+        with self._cat_lock:
+            hit = self._terms.get(key)
+            if hit is not None and hit[0] is req.model:
+                self._terms.move_to_end(key)
+                return hit[1]
+        from .lattice import generate_bilinear_terms
 
-            if req.model.tb_model is None:
-                raise ValueError("a twist needs the model's tight-binding description (tb_model)")
-            terms = generate_bilinear_terms(req.model.tb_model, twisted_phases_over_2π=list(req.twist))
-            self._terms[key] = terms
+        if req.model.tb_model is None:
+            raise ValueError("a twist needs the model's tight-binding description (tb_model)")
+        terms = generate_bilinear_terms(req.model.tb_model, twisted_phases_over_2π=list(req.twist))
+        with self._cat_lock:
+            self._terms[key] = (req.model, terms)
+            self._terms.move_to_end(key)
+            while len(self._terms) > self._max_term_sets:
+                self._terms.popitem(last=False)
         return terms
 
     def _catalog(self, req: SolveRequest) -> _CatalogEntry:

@@ -135,3 +135,24 @@ def test_sjf_aging_bounds_the_wait():
 def test_unknown_policy_is_rejected():
     with pytest.raises(ValueError):
         SectorSolveServer(1, hbm_budget=10, policy="lifo", solve_fn=lambda *a: None, context_factory=lambda: None)
+
+
+def test_twisted_term_lists_are_cached_per_model_object():
+    from realspace_exatdiagonalization_py_amd import models
+
+    m1 = models.build_spinful_hubbard_second_quantized_model([2, 2], 1.0, 4.0)
+    m2 = models.build_spinful_hubbard_second_quantized_model([2, 2], 1.0, 4.0)
+    srv = _server(_Probe(), workers=1)
+    try:
+        r = lambda m, tw: SolveRequest(model=m, n_filled=2, sector=(0, 0), twist=tw)  # noqa: E731
+        a = srv._bilinear_terms(r(m1, (0.25, 0.0)))
+        assert srv._bilinear_terms(r(m1, (0.25, 0.0))) is a
+        b = srv._bilinear_terms(r(m2, (0.25, 0.0)))
+        assert b is not a and b == a  # another model object gets its own entry
+        assert srv._bilinear_terms(r(m1, (0.0, 0.0))) is m1.bilinear_terms  # no twist: the model's own terms
+        srv._max_term_sets = 2
+        c = srv._bilinear_terms(r(m1, (0.5, 0.0)))
+        assert len(srv._terms) == 2 and c != a
+        assert srv._bilinear_terms(r(m1, (0.25, 0.0))) is not a  # the oldest entry was dropped
+    finally:
+        srv.shutdown()
