@@ -30,6 +30,7 @@ from __future__ import annotations
 import math
 import os
 import pickle
+import threading
 import time
 from dataclasses import dataclass, field
 from fractions import Fraction
@@ -41,8 +42,6 @@ import scipy.sparse as sp
 from . import _edx
 from ._edx import Context, lib, ptr
 
-_WORKER_LOCK = __import__("threading").Lock()
-_WORKER_CONTEXTS: dict[int, list[Context]] = {}
 from .bitwise_operations import COMPLEX_ONE, MAX_N_SITE_UINT64, Mask
 from .groups import (
     Finite_Symmetry_Group,
@@ -59,6 +58,10 @@ from .groups import (
     group_order,
 )
 from .second_quantized_model import Particle_Statistics, Real_Space_Second_Quantized_Model
+
+#: Worker contexts of ``run_concurrently``, per device, reused across calls.
+_WORKER_LOCK = threading.Lock()
+_WORKER_CONTEXTS: dict[int, list[Context]] = {}
 
 #: Split-table rank gate of the reference (``SRE:163``); the device ranks
 #: with the O(k) colex formula at every size, the tables are built on request.
@@ -794,8 +797,6 @@ def run_concurrently(items, fn, workers: int) -> None:
         for it in items:
             fn(it)
         return
-    import threading
-
     ctxs = _worker_contexts(min(workers, len(items)))
     lock = threading.Lock()
     pending = list(reversed(items))
@@ -873,7 +874,9 @@ def save_checkpoint(ed_data: Symmetry_Resolved_ED_Data, path: str) -> None:
 
 
 def load_checkpoint(path: str) -> Symmetry_Resolved_ED_Data:
-    """Loads a checkpoint this package wrote (SRE:2552-2563)."""
+    """Loads a checkpoint this package wrote (SRE:2552-2563).  The format is a
+    pickle, as in the reference, so loading runs code from the file: load only
+    checkpoints from a trusted source."""
     with open(path, "rb") as f:
         return pickle.load(f)
 
