@@ -79,6 +79,10 @@ def make_case(name: str):
         m = M.build_checkerboard_second_quantized_model([2, 3], dict(M.my_optimal_param))
         m, G = _twisted(m, (0.29, 0.0))
         return m, 12, G
+    if name == "hub23_disorder":  # seeded on-site disorder, identity group: D = C(12, 6) = 924
+        eps = np.random.default_rng(11).uniform(-0.1, 0.1, 6)
+        m = M.build_spinful_hubbard_second_quantized_model([2, 3], 1.0, 4.0, onsite_disorder=eps)
+        return m, 6, build_identity_group(m.lattice.n_site)
     raise KeyError(name)
 
 

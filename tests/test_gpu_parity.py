@@ -17,6 +17,7 @@ from cases import is_fermionic, make_case, random_vector
 from oracle import ed_oracle as O
 import realspace_exatdiagonalization_py_amd as ed
 from realspace_exatdiagonalization_py_amd.groups import build_irrep_list, group_arrays
+from realspace_exatdiagonalization_py_amd.symmetry_resolved_ed import DENSE_CUTOFF
 
 pytestmark = pytest.mark.gpu
 
@@ -78,7 +79,8 @@ def _oracle_block(model, oc, irrep, fermionic):
     return reps, r, c, v, hd, k
 
 
-@pytest.mark.parametrize("name", ["heis16", "hub23", "hal23", "hal23_twist", "cb23_twist", "hub34"])
+@pytest.mark.parametrize("name", ["heis16", "hub23", "hal23", "hal23_twist", "cb23_twist", "hub34",
+                                  "hub23_disorder"])
 def test_projection_table(name):
     print(f"[table] {name}", flush=True)
     model, ne, G, oc, gc = _both(name)
@@ -177,11 +179,27 @@ def _solve_both(name, irrep_index, nev, precision=ed.PREC_FP64):
 
 
 @pytest.mark.parametrize("name,irrep,nev", [("heis16", 0, 4), ("hal23", 0, 3), ("hal23_twist", 1, 3),
-                                            ("hub24", 1, 6), ("cb23_twist", 0, 4)])
+                                            ("hub24", 1, 6), ("cb23_twist", 0, 4), ("hub23_disorder", 0, 6)])
 def test_eigs_vs_oracle(name, irrep, nev):
     print(f"[eigs] {name} irrep={irrep} nev={nev}", flush=True)
     w, vecs, w_ref, Href = _solve_both(name, irrep, nev)
-    assert np.max(np.abs(w - w_ref)) <= 1e-10, (w, w_ref)
+    # exact levels.  Up to the dense cutoff the reference's own call is dense
+    # (eigh of the upper triangle) and exact.  Above it, the dense spectrum of
+    # the block's Hermitian part (the block itself unless twisted; a twisted
+    # block's eigenvalues have the same real parts to second order in its
+    # complex64 rounding): the reference's ARPACK call can return too few
+    # copies of a degenerate level (a spin triplet of the disordered case
+    # comes back twice).  Where it returns all of them the device values
+    # match it too.
+    if Href.shape[0] <= DENSE_CUTOFF:
+        w_exact = w_ref
+    else:
+        w_exact = np.linalg.eigvalsh(((Href + Href.getH()) * 0.5).toarray())[:nev]
+    assert np.max(np.abs(w - w_exact)) <= 1e-10, (w, w_exact)
+    if np.max(np.abs(w_ref - w_exact)) <= 1e-10:
+        assert np.max(np.abs(w - w_ref)) <= 1e-10, (w, w_ref)
+    else:
+        assert name == "hub23_disorder", (w_ref, w_exact)
     # eigenvector residuals
     R = Href @ vecs - vecs * w
     assert np.max(np.linalg.norm(R, axis=0)) <= 1e-6
