@@ -19,8 +19,11 @@ Other modes: ``--serve`` sends a seeded Poisson stream of mixed requests
 (Heisenberg chains, Hubbard and checkerboard lattices at several sizes, random
 momentum sector, twist angle and number of eigenvalues) through the serving
 loop and prints its own JSON line (latency percentiles, solves/s, |dE| of a
-sample against FP64); ``--profile-spmm`` runs only the SpMM kernels of one
-sector; ``--copy-bytes N`` runs a plain device copy of N bytes.
+sample against FP64); ``--flux-sweep`` solves the 18 momentum sectors of the
+Haldane 3x6 boson workload (D ~ 5.23 M) at successive twist angles, the
+catalog's phases refreshed in place per twist; ``--profile-spmm`` runs only
+the SpMM kernels of one sector; ``--copy-bytes N`` runs a plain device copy
+of N bytes.
 """
 
 from __future__ import annotations
@@ -367,6 +370,88 @@ def _traffic(kernel: str):
         return None
 
 
+def flux_sweep(args) -> int:
+    """Twisted-boundary sweep of the Haldane 3x6 boson workload: 36 sites, 9
+    hard-core bosons, the 18 momentum sectors (D ~ 5.23 M each, ~436 M stored
+    entries) at theta_x = 2 pi j / 32, theta_y = 0, 5 lowest levels each, mixed
+    precision.  The catalog is built once; per twist its phases are refreshed
+    in place (update_orbit_stabilizer_phases), the terms are regenerated and
+    the 18 sector solves go to the worker pool.  One warmup twist (j = 0),
+    then --flux-twists timed twists (j = 1, 2, ...).  Prints one JSON line."""
+    os.environ.setdefault("EDX_QUIET", "1")
+    import realspace_exatdiagonalization_py_amd as ed
+    from realspace_exatdiagonalization_py_amd import _edx
+    from realspace_exatdiagonalization_py_amd import models as M
+    from realspace_exatdiagonalization_py_amd import second_quantized_model as SQ
+    from realspace_exatdiagonalization_py_amd.groups import build_irrep_list, build_translation_group
+
+    nev, ne = 5, 9
+    _edx.set_autotune(not args.no_autotune, ed.PREC_MIXED, nev)
+    model = M.build_zero_flux_bosonic_fci_second_quantized_model(sample_size=[3, 6])
+    stats = model.particle_statistics
+    t0 = time.perf_counter()
+    cat = ed.build_symmetry_orbit_catalog(second_quantized_model=model, n_filled=ne,
+                                          symmetry_group=build_translation_group(model.lattice),
+                                          particle_statistics=stats)
+    t_cat = time.perf_counter() - t0
+    log(f"catalog: {cat.n_orbits} orbits in {t_cat:.2f}s")
+    dims, refresh, last = [], [], {}
+
+    def one_twist(j):
+        theta = [j / 32, 0.0]
+        SQ.update_second_quantized_model_with_twisted_phases(model, twisted_phases_over_2π=theta)
+        G = build_translation_group(model.lattice, theta)
+        t0 = time.perf_counter()
+        ed.update_orbit_stabilizer_phases(cat, G, stats)
+        refresh.append(time.perf_counter() - t0)
+        irreps = build_irrep_list(G, model.lattice)
+
+        def one(i):
+            basis = ed.build_symmetry_sector_basis(cat, irreps[i])
+            dims.append(basis.dim)
+            H = ed.MatrixFreeHamiltonian(basis, model.bilinear_terms, model.density_density_terms, None)
+            w, _ = ed.diagonalize_block_matrixfree(H, H.n, nev=nev, precision=ed.PREC_MIXED, eig_tol=args.eig_tol,
+                                                   want_vectors=False)
+            last[i] = (np.asarray(w), H)
+            if i != 0:
+                del last[i]
+
+        ed.run_concurrently(range(len(irreps)), one, args.flux_workers)
+        return len(irreps)
+
+    one_twist(0)
+    log("warmup twist done")
+    dims.clear()
+    refresh.clear()
+    t0 = time.perf_counter()
+    n_solves = sum(one_twist(j) for j in range(1, args.flux_twists + 1))
+    elapsed = time.perf_counter() - t0
+    log(f"{args.flux_twists} twists ({n_solves} sector solves) in {elapsed:.2f}s")
+    # accuracy: sector 0 of the last twist again in FP64
+    w32, H = last.pop(0)
+    w64, _ = ed.diagonalize_block_matrixfree(H, H.n, nev=nev, precision=ed.PREC_FP64, eig_tol=args.eig_tol,
+                                             want_vectors=False)
+    del H
+    line = {
+        "mode": "flux_sweep",
+        "workload": "Haldane honeycomb 3x6 (36 sites), 9 hard-core bosons, t''=-0.58, 18 momentum sectors, "
+                    "theta_x = 2 pi j/32, 5 lowest levels each, catalog phases refreshed per twist",
+        "value": round(n_solves / elapsed, 3),
+        "unit": "sector solves/s",
+        "twists": args.flux_twists,
+        "ms_per_twist": round(elapsed / args.flux_twists * 1e3, 1),
+        "refresh_ms": round(float(np.mean(refresh)) * 1e3, 2),
+        "catalog_s": round(t_cat, 3),
+        "sector_dims": [min(dims), max(dims)],
+        "workers": args.flux_workers,
+        "precision": "mixed",
+        "max_abs_dE_vs_fp64_one_sector": float(np.max(np.abs(w32 - w64))),
+        "full_sweep_estimate_s": round(576 / (n_solves / elapsed), 1),
+    }
+    print(json.dumps(line))
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -384,6 +469,11 @@ def main() -> int:
                     help="no live per-kernel HIP-event timing (for runs under a profiler, which times kernels itself)")
     ap.add_argument("--profile-spmm", action="store_true", help="run only the SpMM kernels of one sector")
     ap.add_argument("--copy-bytes", type=float, default=0, help="run only a plain device copy of this many bytes")
+    ap.add_argument("--flux-sweep", action="store_true",
+                    help="twisted-boundary sweep of the Haldane 3x6 boson workload (18 sectors per twist)")
+    ap.add_argument("--flux-twists", type=int, default=2, help="timed twists of --flux-sweep")
+    ap.add_argument("--flux-workers", type=int, default=2,
+                    help="concurrent sector solves of --flux-sweep (each holds ~33 GB of tables and bases)")
     ap.add_argument("--serve", action="store_true", help="serving workload: a Poisson stream of mixed requests")
     ap.add_argument("--serve-requests", type=int, default=48)
     ap.add_argument("--serve-rate", type=float, default=200.0, help="mean arrivals per second")
@@ -403,6 +493,8 @@ def main() -> int:
         return copy_kernel(args)
     if args.serve:
         return serve_bench(args)
+    if args.flux_sweep:
+        return flux_sweep(args)
 
     os.environ.setdefault("EDX_QUIET", "1")
     import realspace_exatdiagonalization_py_amd as ed
