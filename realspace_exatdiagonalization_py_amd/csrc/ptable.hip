@@ -466,16 +466,20 @@ __device__ double2 entry_sum(const int64_t* row_ptr, const int32_t* col, const d
 
 // herm = 1 stores the Hermitian part: every entry of the run (r, c) is halved
 // and the run's first entry also carries conj(sum of the run (c, r)) / 2, so the
-// run sums to (H_rc + conj(H_cr)) / 2.
+// run sums to (H_rc + conj(H_cr)) / 2.  kItemLanes lanes share a row, each
+// taking every kItemLanes-th entry: the transposed look-ups of a row are in
+// flight together, and with G = 4 the lanes of a slice store contiguous slots.
 __global__ void sell_fill(int64_t dim, int G, int herm, const int64_t* row_ptr, const int32_t* col,
                           const double2* val, const int64_t* off, int32_t* scol, double2* sval, float2* sval32) {
-  const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t r = t / kItemLanes;
+  const int sub = (int)(t % kItemLanes);
   if (r >= dim) return;
   const int R = 64 / G;
   const int64_t s = r / R;
   const int64_t base = off[s] + (r % R) * G;
   const int64_t e0 = row_ptr[r], e1 = row_ptr[r + 1];
-  for (int64_t e = e0; e < e1; ++e) {
+  for (int64_t e = e0 + sub; e < e1; e += kItemLanes) {
     const int64_t k = e - e0;
     const int64_t pos = base + (k / G) * 64 + (k % G);
     const int32_t c = col[e];
@@ -585,7 +589,7 @@ void build_sell(edx_hamiltonian& h, Ctx& ctx, int G) {
   EDX_HIP(hipMemsetAsync(h.sell_col.p, 0xff, h.sell_col.bytes(), st));
   EDX_HIP(hipMemsetAsync(h.sell_val.p, 0, h.sell_val.bytes(), st));
   EDX_HIP(hipMemsetAsync(h.sell_val32.p, 0, h.sell_val32.bytes(), st));
-  hipLaunchKernelGGL(sell_fill, dim3(grid_for(D, 256)), dim3(256), 0, st, D, G, h.herm_part,
+  hipLaunchKernelGGL(sell_fill, dim3(grid_for((size_t)D * kItemLanes, 256)), dim3(256), 0, st, D, G, h.herm_part,
                      (const int64_t*)h.row_ptr.p, (const int32_t*)h.col.p, (const double2*)h.val.p,
                      (const int64_t*)h.sell_off.p, h.sell_col.p, h.sell_val.p, h.sell_val32.p);
   EDX_HIP(hipGetLastError());
