@@ -5,7 +5,8 @@ sector dimensions, projection-table structure) must be bit-exact.  Floating
 point: complex64 amplitudes exact at zero flux and within one float32 ulp at
 nonzero flux (device sincos vs libm); complex128 matrix elements within 4e-16
 relative; H x within 1e-13 relative; eigenvalues within 1e-10 absolute of the
-oracle's ARPACK/LAPACK values (the tolerance the north star fixes).
+exact levels and of the oracle's ARPACK/LAPACK values (the accuracy the
+mixed-precision path is held to against FP64).
 """
 
 from __future__ import annotations
