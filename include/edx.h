@@ -135,6 +135,17 @@ int edx_matvec(edx_ctx* ctx, const edx_hamiltonian* ham, int32_t n_vec, const do
  * arrays. */
 int edx_matvec_device(edx_ctx* ctx, const edx_hamiltonian* ham, int32_t precision, int32_t n_vec,
                       const void* x_dev, void* y_dev);
+/* Allocation-free edx_matvec.  No counterpart in the reference: for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, hosts that forbid hipMalloc).  Same arguments, results and
+ * errors as edx_matvec, but its three work areas (3 x 16 x dim bytes per
+ * vector, vectors processed in batches of up to 16) come from one static
+ * 96 MiB device array per GPU, held from the first context on that GPU until
+ * the process ends; dim > 2 097 152 -> EDX_ERR_INVALID.  Calls on one GPU
+ * share that array: the library serialises them within a process and each
+ * call completes on the ctx stream before it returns; separate processes
+ * have separate arrays. */
+int edx_matvec_noalloc(edx_ctx* ctx, const edx_hamiltonian* ham, int32_t n_vec, const double* x, double* y);
 int edx_hamiltonian_destroy(edx_hamiltonian* ham);
 
 /* ---- eigensolver ----------------------------------------------------------

@@ -59,6 +59,7 @@ SIGNATURES = {
     "edx_hamiltonian_from_csr": (C.c_int, [P, i64, i64, P, P, P, P, PP]),
     "edx_matvec": (C.c_int, [P, P, i32, P, P]),
     "edx_matvec_device": (C.c_int, [P, P, i32, i32, P, P]),
+    "edx_matvec_noalloc": (C.c_int, [P, P, i32, P, P]),
     "edx_hamiltonian_destroy": (C.c_int, [P]),
     "edx_eigs": (C.c_int, [P, P, i32, i32, f64, u64, i32, P, P, P]),
     "edx_set_spmv_group": (C.c_int, [P, i32]),

@@ -246,6 +246,8 @@ int edx_ctx_create(int32_t device, edx_ctx** out) {
       throw edx::Error{EDX_ERR_HIP, std::string("context setup: ") + hipGetErrorString(e)};
     }
     edx::register_stream(device, ctx->stream);
+    // device address of the no-alloc entry point's scratch array (once per device)
+    edx::noalloc_init(device);
     *out = ctx;
   });
 }

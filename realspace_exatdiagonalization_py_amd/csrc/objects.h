@@ -95,6 +95,9 @@ void abandon_sell(const edx_hamiltonian& h, SellPlan& p);
 // Slice offsets and storage of a sliced copy with G lanes per row (contents
 // not written); returns the stored entry count.
 int64_t prepare_sell(edx_hamiltonian& h, Ctx& ctx, int G);
+// Looks up the device address of edx_matvec_noalloc's static scratch array
+// on `device` (current) once per device (noalloc.hip).
+void noalloc_init(int device);
 }  // namespace edx
 
 namespace edx {
