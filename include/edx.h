@@ -8,7 +8,8 @@
  * allocated stream-ordered on the ctx's stream and freed before the call
  * returns; arrays of the objects use ordinary device allocations.  Calls on one ctx are
  * serialised on that ctx's HIP stream; different ctxs may be used from
- * different threads.  "ref" citations are lines of the reference file
+ * different threads.  A ctx serves one thread at a time: a call made while
+ * another thread's call runs on the same ctx returns EDX_ERR_BUSY at once.  "ref" citations are lines of the reference file
  * src/realspace_exactdiagonalization_py/symmetry_resolved_ed.py (SRE) unless
  * another file is named.
  *
@@ -34,6 +35,8 @@ extern "C" {
 #define EDX_ERR_OOM (-4)       /* device allocation failure -> MemoryError     */
 #define EDX_ERR_NOCONV (-5)    /* eigensolver not converged -> RuntimeError    */
 #define EDX_ERR_INTERNAL (-6)  /* anything else             -> RuntimeError    */
+#define EDX_ERR_BUSY (-7)      /* ctx in use by another thread -> RuntimeError;
+                                  its text: edx_last_error(NULL) on the caller */
 
 #define EDX_PREC_FP64 0        /* complex128 vectors and matrix elements        */
 #define EDX_PREC_MIXED 1       /* complex64 sweeps, FP64 recurrence/refinement  */

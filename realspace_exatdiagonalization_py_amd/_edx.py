@@ -29,7 +29,7 @@ if not LIB_PATH.exists():
 lib = C.CDLL(str(LIB_PATH))
 
 OK = 0
-ERR_INVALID, ERR_ASSERT, ERR_HIP, ERR_OOM, ERR_NOCONV, ERR_INTERNAL = -1, -2, -3, -4, -5, -6
+ERR_INVALID, ERR_ASSERT, ERR_HIP, ERR_OOM, ERR_NOCONV, ERR_INTERNAL, ERR_BUSY = -1, -2, -3, -4, -5, -6, -7
 PREC_FP64, PREC_MIXED = 0, 1
 
 P = C.c_void_p
@@ -118,7 +118,8 @@ def ptr(a: np.ndarray | None):
 def raise_for(rc: int, ctx=None) -> None:
     if rc == OK:
         return
-    msg = lib.edx_last_error(ctx)
+    # a refused concurrent call reports through the calling thread's own text
+    msg = lib.edx_last_error(None if rc == ERR_BUSY else ctx)
     text = msg.decode() if msg else f"edx error {rc}"
     if rc == ERR_INVALID:
         raise ValueError(text)

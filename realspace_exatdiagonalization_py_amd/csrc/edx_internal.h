@@ -7,6 +7,8 @@
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+
+#include <atomic>
 #include <string>
 #include <vector>
 
@@ -190,6 +192,11 @@ struct Ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   std::string last_error;
+  // Thread whose API call is using the context (0: none) and its nesting
+  // depth: a context serves one thread at a time (its stream, staging
+  // buffers, kernel timer and error text are not shared between threads).
+  std::atomic<uintptr_t> owner{0};
+  int depth = 0;
   int autotune_group = 0;  // lanes per row in the SpMV (0 = pick by heuristics)
   KernelTimer timer;
   // pinned host staging owned by the context (allocated once, reused by

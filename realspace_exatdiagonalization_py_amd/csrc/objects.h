@@ -103,9 +103,54 @@ void noalloc_init(int device);
 namespace edx {
 
 inline void set_error(edx_ctx* ctx, const std::string& msg);
+// Thread-local fallback when no ctx is available.
+std::string& tls_error();
+
+// Token of the calling thread (address of a thread-local object; never 0).
+inline uintptr_t thread_token() {
+  thread_local char tag;
+  return reinterpret_cast<uintptr_t>(&tag);
+}
+
+// Claims ctx for the calling thread for the duration of an API call.  A
+// call from a second thread while another thread's call is running on the
+// same ctx is refused (ok = false) instead of racing on its stream, staging
+// buffers and kernel timer; nested calls on the owning thread are allowed.
+struct CtxClaim {
+  Ctx* ctx;
+  bool ok = true;
+  explicit CtxClaim(Ctx* c) : ctx(c) {
+    if (!ctx) return;
+    const uintptr_t me = thread_token();
+    if (ctx->owner.load(std::memory_order_acquire) == me) {
+      ++ctx->depth;
+      return;
+    }
+    uintptr_t expected = 0;
+    if (ctx->owner.compare_exchange_strong(expected, me, std::memory_order_acq_rel)) {
+      ctx->depth = 1;
+      return;
+    }
+    ok = false;
+  }
+  ~CtxClaim() {
+    if (!ctx || !ok) return;
+    if (--ctx->depth == 0) ctx->owner.store(0, std::memory_order_release);
+  }
+  CtxClaim(const CtxClaim&) = delete;
+  CtxClaim& operator=(const CtxClaim&) = delete;
+};
 
 template <class F>
 int guarded(edx_ctx* ctx, F&& f) {
+  CtxClaim claim(ctx);
+  if (!claim.ok) {
+    // the owner thread may be writing ctx->last_error: report through this
+    // thread's own error text (edx_last_error(NULL))
+    tls_error() = "edx context is in use by another thread: a context serves one thread at a time "
+                  "(create one context per thread)";
+    return EDX_ERR_BUSY;
+  }
   try {
     StreamScope scope(ctx ? ctx->stream : nullptr);
     f();
@@ -124,9 +169,6 @@ int guarded(edx_ctx* ctx, F&& f) {
     return EDX_ERR_INTERNAL;
   }
 }
-
-// Thread-local fallback when no ctx is available.
-std::string& tls_error();
 
 inline void set_error(edx_ctx* ctx, const std::string& msg) {
   if (ctx) ctx->last_error = msg;

@@ -149,6 +149,10 @@ class SectorSolveServer:
         self._done: list[SolveResult] = []
         self._peak_in_use = 0
         self._t0 = time.perf_counter()
+        # solve_alone runs on the caller's thread: its own context, so that it
+        # never shares a worker's stream, staging buffers or kernel timer
+        self._alone_context = factory()
+        self._alone_lock = threading.Lock()
         self._threads = [threading.Thread(target=self._worker, args=(c,), daemon=True) for c in self._contexts]
         for t in self._threads:
             t.start()
@@ -374,8 +378,9 @@ class SectorSolveServer:
     def solve_alone(self, req: SolveRequest) -> SolveResult:
         """The same solve on the calling thread, outside the queue (reference
         point for checking that results do not depend on the mix)."""
-        t = time.perf_counter()
-        vals, vecs, dim, info = self._solve_fn(self, req, self._contexts[0])
+        with self._alone_lock:
+            t = time.perf_counter()
+            vals, vecs, dim, info = self._solve_fn(self, req, self._alone_context)
         return SolveResult(req, vals, vecs, dim, t, t, time.perf_counter(), self._estimate(req), info)
 
 
