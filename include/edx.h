@@ -66,6 +66,23 @@ int edx_synchronize(edx_ctx* ctx);
 int edx_catalog_build(edx_ctx* ctx, int32_t n_site, int32_t n_filled, int32_t n_g,
                       const int32_t* perm, const double* perm_phases, const double* perm_angles,
                       int32_t fermionic, edx_catalog** out);
+/* A catalog from host copies of its tables (the layout edx_catalog_fetch
+ * writes), for catalogs restored from a checkpoint: the reference pickles the
+ * Symmetry_Orbit_Catalog dataclass with its tables (ref SRE:1151-1190,
+ * :2530-2563) and its consumers rebuild sector bases from the loaded catalog
+ * (ref observables/charge_pump.py:431-462).  The group arguments are those of
+ * edx_catalog_build; sizes, offsets, group indices, orbit indices and
+ * representatives are validated (EDX_ERR_INVALID). */
+int edx_catalog_upload(edx_ctx* ctx, int32_t n_site, int32_t n_filled, int32_t n_g, const int32_t* perm,
+                       const double* perm_phases, const double* perm_angles, int32_t fermionic, int64_t n_orbits,
+                       int64_t n_stab, const int32_t* rep_rank, const int32_t* rep_sym, const float* rep_amp,
+                       const uint64_t* reps, const int32_t* stab_order, const int64_t* stab_offsets,
+                       const int32_t* stab_g, const double* stab_phase, edx_catalog** out);
+/* Device copy of a catalog (all tables and the current phases).  The flux
+ * scan of ed_scan (ref SRE:2658-2703) refreshes the phases of a copy per twist
+ * in place, where the reference builds a fresh catalog per twist, so that the
+ * caller's catalog keeps its own phases. */
+int edx_catalog_clone(edx_ctx* ctx, const edx_catalog* src, edx_catalog** out);
 /* Sizes: C(n_site, n_filled), number of orbits, total stabilizer entries. */
 int edx_catalog_info(const edx_catalog* cat, int64_t* n_total, int64_t* n_orbits, int64_t* n_stab);
 /* Copies the catalog to host buffers (ref Symmetry_Orbit_Catalog fields,
@@ -82,6 +99,13 @@ int edx_catalog_fetch(edx_ctx* ctx, const edx_catalog* cat, int32_t* rep_rank, i
  * recomputed as running products.  Different permutations -> EDX_ERR_INVALID. */
 int edx_catalog_update_phases(edx_ctx* ctx, edx_catalog* cat, const int32_t* perm,
                               const double* perm_phases, const double* perm_angles);
+/* The same refresh with the formulas of a fresh build (alpha in exponent
+ * form, ref SRE:538-551): the tables then equal those edx_catalog_build gives
+ * at the new phases, bit for bit, without repeating the orbit scan.  Used by
+ * the flux scan of ed_scan, where the reference builds a fresh catalog per
+ * twist (ref SRE:2686-2691). */
+int edx_catalog_rephase(edx_ctx* ctx, edx_catalog* cat, const int32_t* perm,
+                        const double* perm_phases, const double* perm_angles);
 /* Batched O(1) canonical lookup (ref get_canonical / CanonicalMap,
  * SRE:1595-1632): for n masks of popcount n_filled, the representative mask,
  * the table's g (rep -> mask) and alpha (complex128 widened from the stored

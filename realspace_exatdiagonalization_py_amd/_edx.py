@@ -46,9 +46,12 @@ SIGNATURES = {
     "edx_device_count": (i32, []),
     "edx_synchronize": (C.c_int, [P]),
     "edx_catalog_build": (C.c_int, [P, i32, i32, i32, P, P, P, i32, PP]),
+    "edx_catalog_upload": (C.c_int, [P, i32, i32, i32, P, P, P, i32, i64, i64, P, P, P, P, P, P, P, P, PP]),
+    "edx_catalog_clone": (C.c_int, [P, P, PP]),
     "edx_catalog_info": (C.c_int, [P, pi64, pi64, pi64]),
     "edx_catalog_fetch": (C.c_int, [P, P, P, P, P, P, P, P, P, P]),
     "edx_catalog_update_phases": (C.c_int, [P, P, P, P, P]),
+    "edx_catalog_rephase": (C.c_int, [P, P, P, P, P]),
     "edx_catalog_lookup": (C.c_int, [P, P, i64, P, P, P, P]),
     "edx_catalog_destroy": (C.c_int, [P]),
     "edx_sector_build": (C.c_int, [P, P, P, f64, PP, pi64]),

@@ -148,33 +148,39 @@ catalog_stab(edx::GroupView gv, int64_t n_orbits, const uint64_t* reps, const ui
   }
 }
 
-// Phase refresh (update_orbit_stabilizer_phases): product-form alpha for the
-// stored (orbit, g) of every rank, and for every stabilizer element.
+// Phase refresh for the stored (orbit, g) of every rank and for every
+// stabilizer element.  fresh = 0: product-form alpha, as
+// update_orbit_stabilizer_phases computes it (SRE:1220-1269); fresh = 1: the
+// formulas of a fresh catalog build (alpha_of), so that the tables equal
+// those of edx_catalog_build at the new phases bit for bit.
+__device__ __forceinline__ double2 refresh_alpha(const edx::GroupView& gv, uint64_t m, int g, const uint8_t* perm_s,
+                                                 int fresh) {
+  if (fresh) return alpha_of(gv, m, g, perm_s);
+  return edxd::alpha_prod(m, gv.phase + (size_t)g * gv.n_site, perm_s + g * gv.n_site, gv.fermionic);
+}
+
 __global__ void __launch_bounds__(kBlock)
 catalog_update_table(edx::GroupView gv, int64_t n_total, const int32_t* rep_rank, const int32_t* rep_sym,
-                     const uint64_t* reps, float2* rep_amp) {
+                     const uint64_t* reps, float2* rep_amp, int fresh) {
   extern __shared__ unsigned char smem[];
   SharedGroup sg = load_group(gv, smem);
   const int64_t r = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (r >= n_total) return;
   const int g = rep_sym[r];
   const uint64_t m = reps[rep_rank[r]];
-  const double2 a = edxd::alpha_prod(m, gv.phase + (size_t)g * gv.n_site, sg.perm + g * gv.n_site, gv.fermionic);
+  const double2 a = refresh_alpha(gv, m, g, sg.perm, fresh);
   rep_amp[r] = make_float2((float)a.x, (float)a.y);
 }
 
 __global__ void __launch_bounds__(kBlock)
 catalog_update_stab(edx::GroupView gv, int64_t n_orbits, const uint64_t* reps, const uint32_t* stab_off,
-                    const uint8_t* stab_g, double2* stab_phase) {
+                    const uint8_t* stab_g, double2* stab_phase, int fresh) {
   extern __shared__ unsigned char smem[];
   SharedGroup sg = load_group(gv, smem);
   const int64_t o = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (o >= n_orbits) return;
   const uint64_t m = reps[o];
-  for (uint32_t j = stab_off[o]; j < stab_off[o + 1]; ++j) {
-    const int g = stab_g[j];
-    stab_phase[j] = edxd::alpha_prod(m, gv.phase + (size_t)g * gv.n_site, sg.perm + g * gv.n_site, gv.fermionic);
-  }
+  for (uint32_t j = stab_off[o]; j < stab_off[o + 1]; ++j) stab_phase[j] = refresh_alpha(gv, m, stab_g[j], sg.perm, fresh);
 }
 
 __global__ void __launch_bounds__(kBlock)
@@ -336,6 +342,130 @@ int edx_catalog_build(edx_ctx* ctx, int32_t n_site, int32_t n_filled, int32_t n_
   });
 }
 
+int edx_catalog_upload(edx_ctx* ctx, int32_t n_site, int32_t n_filled, int32_t n_g, const int32_t* perm,
+                       const double* perm_phases, const double* perm_angles, int32_t fermionic, int64_t n_orbits,
+                       int64_t n_stab, const int32_t* rep_rank, const int32_t* rep_sym, const float* rep_amp,
+                       const uint64_t* reps, const int32_t* stab_order, const int64_t* stab_offsets,
+                       const int32_t* stab_g, const double* stab_phase, edx_catalog** out) {
+  return edx::guarded(ctx, [&] {
+    EDX_REQUIRE(ctx && out && perm && perm_phases && perm_angles && rep_rank && rep_sym && rep_amp && reps &&
+                    stab_order && stab_offsets && (n_stab == 0 || (stab_g && stab_phase)),
+                EDX_ERR_INVALID, "null argument");
+    *out = nullptr;
+    EDX_REQUIRE(n_site >= 1 && n_site <= EDX_MAX_SITES, EDX_ERR_INVALID, "n_site must be in 1..63");
+    EDX_REQUIRE(n_filled >= 0 && n_filled <= n_site, EDX_ERR_ASSERT, "0 <= n_filled <= n_site violated");
+    EDX_REQUIRE(n_g >= 1 && n_g <= EDX_MAX_GROUP, EDX_ERR_INVALID, "group order must be in 1..64");
+    const uint64_t C = binom_u64(n_site, n_filled);
+    EDX_REQUIRE(C > 0 && C < (1ull << 31), EDX_ERR_INVALID,
+                "C(n_site, n_filled) must be below 2^31 (int32 representative tables)");
+    EDX_REQUIRE(n_orbits >= 1 && (uint64_t)n_orbits <= C && n_stab >= n_orbits, EDX_ERR_INVALID,
+                "catalog sizes out of range");
+    // the tables must describe a catalog of these sizes: offsets, group
+    // indices, orbit indices and representatives are checked before use
+    EDX_REQUIRE(stab_offsets[0] == 0 && stab_offsets[n_orbits] == n_stab, EDX_ERR_INVALID,
+                "stabilizer offsets must start at 0 and end at n_stab");
+    const uint64_t lim = n_site >= 64 ? ~0ull : (1ull << n_site);
+    for (int64_t o = 0; o < n_orbits; ++o) {
+      EDX_REQUIRE(stab_offsets[o + 1] - stab_offsets[o] == stab_order[o] && stab_order[o] >= 1, EDX_ERR_INVALID,
+                  "stabilizer offsets disagree with the stabilizer orders");
+      EDX_REQUIRE(reps[o] < lim && __builtin_popcountll(reps[o]) == n_filled && (o == 0 || reps[o - 1] < reps[o]),
+                  EDX_ERR_INVALID, "representatives must be ascending masks of the catalog's filling");
+    }
+    for (int64_t e = 0; e < n_stab; ++e)
+      EDX_REQUIRE(stab_g[e] >= 0 && stab_g[e] < n_g, EDX_ERR_INVALID, "stabilizer element out of range");
+    for (uint64_t r = 0; r < C; ++r)
+      EDX_REQUIRE(rep_rank[r] >= 0 && rep_rank[r] < n_orbits && rep_sym[r] >= 0 && rep_sym[r] < n_g,
+                  EDX_ERR_INVALID, "representative table entry out of range");
+    edx::DeviceGuard dg(ctx->device);
+    auto cat = new edx_catalog();
+    try {
+      cat->device = ctx->device;
+      cat->n_site = n_site;
+      cat->n_filled = n_filled;
+      cat->n_g = n_g;
+      cat->fermionic = fermionic ? 1 : 0;
+      cat->product_phase = (n_filled == 0 || n_filled == n_site) ? 1 : 0;
+      cat->n_total = (int64_t)C;
+      cat->n_orbits = n_orbits;
+      cat->n_stab = n_stab;
+      upload_group(*cat, *ctx, perm, perm_phases, perm_angles);
+      std::vector<uint32_t> off((size_t)n_orbits + 1);
+      for (int64_t o = 0; o <= n_orbits; ++o) off[o] = (uint32_t)stab_offsets[o];
+      std::vector<uint8_t> g8((size_t)n_stab);
+      for (int64_t e = 0; e < n_stab; ++e) g8[e] = (uint8_t)stab_g[e];
+      cat->rep_rank.alloc(C);
+      cat->rep_sym.alloc(C);
+      cat->rep_amp.alloc(C);
+      cat->reps.alloc(n_orbits);
+      cat->stab_order.alloc(n_orbits);
+      cat->stab_off.alloc(n_orbits + 1);
+      cat->stab_g.alloc(n_stab);
+      cat->stab_phase.alloc(n_stab);
+      hipStream_t st = ctx->stream;
+      EDX_HIP(hipMemcpyAsync(cat->rep_rank.p, rep_rank, C * 4, hipMemcpyHostToDevice, st));
+      EDX_HIP(hipMemcpyAsync(cat->rep_sym.p, rep_sym, C * 4, hipMemcpyHostToDevice, st));
+      EDX_HIP(hipMemcpyAsync(cat->rep_amp.p, rep_amp, C * 8, hipMemcpyHostToDevice, st));
+      EDX_HIP(hipMemcpyAsync(cat->reps.p, reps, n_orbits * 8, hipMemcpyHostToDevice, st));
+      EDX_HIP(hipMemcpyAsync(cat->stab_order.p, stab_order, n_orbits * 4, hipMemcpyHostToDevice, st));
+      EDX_HIP(hipMemcpyAsync(cat->stab_off.p, off.data(), off.size() * 4, hipMemcpyHostToDevice, st));
+      if (n_stab) {
+        EDX_HIP(hipMemcpyAsync(cat->stab_g.p, g8.data(), g8.size(), hipMemcpyHostToDevice, st));
+        EDX_HIP(hipMemcpyAsync(cat->stab_phase.p, stab_phase, n_stab * 16, hipMemcpyHostToDevice, st));
+      }
+      EDX_HIP(hipStreamSynchronize(st));
+    } catch (...) {
+      delete cat;
+      throw;
+    }
+    *out = cat;
+  });
+}
+
+int edx_catalog_clone(edx_ctx* ctx, const edx_catalog* src, edx_catalog** out) {
+  return edx::guarded(ctx, [&] {
+    EDX_REQUIRE(ctx && src && out, EDX_ERR_INVALID, "null argument");
+    *out = nullptr;
+    edx::DeviceGuard dg(ctx->device);
+    hipStream_t st = ctx->stream;
+    auto cat = new edx_catalog();
+    try {
+      cat->device = ctx->device;
+      cat->n_site = src->n_site;
+      cat->n_filled = src->n_filled;
+      cat->n_g = src->n_g;
+      cat->fermionic = src->fermionic;
+      cat->product_phase = src->product_phase;
+      cat->exact_alpha = src->exact_alpha;
+      cat->n_total = src->n_total;
+      cat->n_orbits = src->n_orbits;
+      cat->n_stab = src->n_stab;
+      cat->h_perm = src->h_perm;
+      auto copy = [&](auto& dst, const auto& from) {
+        dst.alloc(from.n);
+        if (from.n) EDX_HIP(hipMemcpyAsync(dst.p, from.p, from.bytes(), hipMemcpyDeviceToDevice, st));
+      };
+      copy(cat->perm, src->perm);
+      copy(cat->angle, src->angle);
+      copy(cat->phase, src->phase);
+      copy(cat->inv, src->inv);
+      copy(cat->binom, src->binom);
+      copy(cat->rep_rank, src->rep_rank);
+      copy(cat->rep_sym, src->rep_sym);
+      copy(cat->rep_amp, src->rep_amp);
+      copy(cat->reps, src->reps);
+      copy(cat->stab_order, src->stab_order);
+      copy(cat->stab_off, src->stab_off);
+      copy(cat->stab_g, src->stab_g);
+      copy(cat->stab_phase, src->stab_phase);
+      EDX_HIP(hipStreamSynchronize(st));
+    } catch (...) {
+      delete cat;
+      throw;
+    }
+    *out = cat;
+  });
+}
+
 int edx_catalog_info(const edx_catalog* cat, int64_t* n_total, int64_t* n_orbits, int64_t* n_stab) {
   if (!cat) return EDX_ERR_INVALID;
   if (n_total) *n_total = cat->n_total;
@@ -377,9 +507,11 @@ int edx_catalog_fetch(edx_ctx* ctx, const edx_catalog* cat, int32_t* rep_rank, i
   });
 }
 
-int edx_catalog_update_phases(edx_ctx* ctx, edx_catalog* cat, const int32_t* perm, const double* perm_phases,
-                              const double* perm_angles) {
-  return edx::guarded(ctx, [&] {
+}  // extern "C"
+
+namespace {
+void refresh_phases(edx_ctx* ctx, edx_catalog* cat, const int32_t* perm, const double* perm_phases,
+                    const double* perm_angles, int fresh) {
     EDX_REQUIRE(ctx && cat && perm && perm_phases && perm_angles, EDX_ERR_INVALID, "null argument");
     const size_t n = (size_t)cat->n_g * cat->n_site;
     EDX_REQUIRE(std::memcmp(perm, cat->h_perm.data(), n * 4) == 0, EDX_ERR_INVALID,
@@ -392,14 +524,26 @@ int edx_catalog_update_phases(edx_ctx* ctx, edx_catalog* cat, const int32_t* per
     const size_t smem = group_smem_bytes(cat->n_site, cat->n_filled, cat->n_g);
     hipLaunchKernelGGL(catalog_update_stab, dim3(edx::grid_for(cat->n_orbits, kBlock)), dim3(kBlock), smem,
                        ctx->stream, gv, cat->n_orbits, (const uint64_t*)cat->reps.p,
-                       (const uint32_t*)cat->stab_off.p, (const uint8_t*)cat->stab_g.p, cat->stab_phase.p);
+                       (const uint32_t*)cat->stab_off.p, (const uint8_t*)cat->stab_g.p, cat->stab_phase.p, fresh);
     EDX_HIP(hipGetLastError());
     hipLaunchKernelGGL(catalog_update_table, dim3(edx::grid_for(cat->n_total, kBlock)), dim3(kBlock), smem,
                        ctx->stream, gv, cat->n_total, (const int32_t*)cat->rep_rank.p,
-                       (const int32_t*)cat->rep_sym.p, (const uint64_t*)cat->reps.p, cat->rep_amp.p);
+                       (const int32_t*)cat->rep_sym.p, (const uint64_t*)cat->reps.p, cat->rep_amp.p, fresh);
     EDX_HIP(hipGetLastError());
     EDX_HIP(hipStreamSynchronize(ctx->stream));
-  });
+}
+}  // namespace
+
+extern "C" {
+
+int edx_catalog_update_phases(edx_ctx* ctx, edx_catalog* cat, const int32_t* perm, const double* perm_phases,
+                              const double* perm_angles) {
+  return edx::guarded(ctx, [&] { refresh_phases(ctx, cat, perm, perm_phases, perm_angles, 0); });
+}
+
+int edx_catalog_rephase(edx_ctx* ctx, edx_catalog* cat, const int32_t* perm, const double* perm_phases,
+                        const double* perm_angles) {
+  return edx::guarded(ctx, [&] { refresh_phases(ctx, cat, perm, perm_phases, perm_angles, 1); });
 }
 
 int edx_catalog_lookup(edx_ctx* ctx, const edx_catalog* cat, int64_t n, const uint64_t* masks, uint64_t* rep,

@@ -142,7 +142,7 @@ class Symmetry_Orbit_Catalog:
 
     def __init__(self, symmetry_group, dev: _DeviceCatalog, n_site: int, n_filled: int, fermionic: bool):
         self.symmetry_group = symmetry_group
-        self._dev = dev
+        self._dev_obj = dev
         self.n_site = n_site
         self.n_filled = n_filled
         self.fermionic = fermionic
@@ -152,6 +152,51 @@ class Symmetry_Orbit_Catalog:
         self._cache: dict[str, Any] = {}
         self.binom_table = _binom_table(n_site, n_filled)
         self._lin = None
+
+    # -- device handle, pickling ---------------------------------------------
+    _TABLE_KEYS = ("rep_rank", "rep_sym", "rep_amp", "reps", "stab_order", "stab_off", "stab_g", "stab_phase")
+
+    @property
+    def _dev(self) -> _DeviceCatalog:
+        """The device catalog; one restored from a pickle is uploaded to the
+        calling thread's context on first use."""
+        if self._dev_obj is None:
+            c = self._cache
+            perm, phases, angles = _group_inputs(self.symmetry_group)
+            ctx = _edx.current_context()
+            h = _edx.C.c_void_p()
+            ctx.check(lib.edx_catalog_upload(
+                ctx.handle, self.n_site, self.n_filled, group_order(self.symmetry_group), ptr(perm), ptr(phases),
+                ptr(angles), 1 if self.fermionic else 0, self.n_orbits, self.n_stab, ptr(c["rep_rank"]),
+                ptr(c["rep_sym"]), ptr(c["rep_amp"]), ptr(c["reps"]), ptr(c["stab_order"]), ptr(c["stab_off"]),
+                ptr(c["stab_g"]), ptr(c["stab_phase"]), _edx.C.byref(h)))
+            self._dev_obj = _DeviceCatalog(ctx, h)
+        return self._dev_obj
+
+    def __getstate__(self):
+        # the reference pickles the catalog dataclass with its tables
+        # (SRE:1151-1190); here they are fetched from the device into the
+        # state, and the device handle is left out
+        self._fetch_tables()
+        self._fetch_orbits()
+        state = {k: v for k, v in self.__dict__.items() if k not in ("_dev_obj", "_cache", "_lin")}
+        state["_tables"] = {k: self._cache[k] for k in self._TABLE_KEYS}
+        return state
+
+    def __setstate__(self, state):
+        tables = state.pop("_tables")
+        self.__dict__.update(state)
+        self._dev_obj = None
+        self._lin = None
+        self._cache = {k: np.ascontiguousarray(v) for k, v in tables.items()}
+
+    def clone(self) -> "Symmetry_Orbit_Catalog":
+        """A device copy of the catalog with its current phases."""
+        ctx = _edx.current_context(self._dev.ctx)
+        h = _edx.C.c_void_p()
+        ctx.check(lib.edx_catalog_clone(ctx.handle, self._dev.handle, _edx.C.byref(h)))
+        return Symmetry_Orbit_Catalog(self.symmetry_group, _DeviceCatalog(ctx, h), self.n_site, self.n_filled,
+                                      self.fermionic)
 
     # -- device fetches -------------------------------------------------
     def _fetch_tables(self):
@@ -296,6 +341,20 @@ def update_orbit_stabilizer_phases(
     ctx.check(lib.edx_catalog_update_phases(ctx.handle, catalog._dev.handle, ptr(perm), ptr(phases), ptr(angles)))
     catalog.symmetry_group = new_group
     catalog._invalidate("rep_amp", "rep_rank", "rep_sym", "stab_phase", "reps", "stab_order", "stab_off", "stab_g")
+    return catalog
+
+
+def _rephase_catalog(catalog: Symmetry_Orbit_Catalog, new_group: Finite_Symmetry_Group) -> Symmetry_Orbit_Catalog:
+    """In-place refresh to ``new_group``'s phases with the formulas of a fresh
+    build, so that the tables equal ``build_symmetry_orbit_catalog`` at
+    ``new_group`` bit for bit (the flux scan's per-twist catalog, SRE:2686-2691)."""
+    if catalog.symmetry_group is new_group:
+        return catalog
+    perm, phases, angles = _group_inputs(new_group)
+    ctx = _edx.current_context(catalog._dev.ctx)
+    ctx.check(lib.edx_catalog_rephase(ctx.handle, catalog._dev.handle, ptr(perm), ptr(phases), ptr(angles)))
+    catalog.symmetry_group = new_group
+    catalog._invalidate(*Symmetry_Orbit_Catalog._TABLE_KEYS)
     return catalog
 
 
@@ -717,10 +776,10 @@ class Symmetry_Resolved_ED_Data:
     ed_scan_res: dict[int, tuple[np.ndarray, np.ndarray]] = field(default_factory=dict)
 
     def __getstate__(self):
-        # device handles do not pickle: checkpoints carry the results
+        # the catalog pickles its tables (fetched from the device); snapshots
+        # of the results, so that a checkpoint written while other sectors
+        # finish is consistent
         state = dict(self.__dict__)
-        state["orbit_catalog"] = None
-        # snapshots, so that a checkpoint written while other sectors finish is consistent
         state["ed_scan_res"] = dict(state["ed_scan_res"])
         state["sector_dims"] = list(state["sector_dims"])
         return state
@@ -861,7 +920,7 @@ def _ed_scan_sectors(ed_data, *, nev=5, mode="matrix", checkpoint_path=None, sca
 
 
 def save_checkpoint(ed_data: Symmetry_Resolved_ED_Data, path: str) -> None:
-    """Atomic pickle of the ED data without its device catalog (SRE:2530-2549)."""
+    """Atomic pickle of the ED data, its catalog tables included (SRE:2530-2549)."""
     os.makedirs(os.path.dirname(os.path.abspath(path)) or ".", exist_ok=True)
     tmp = path + ".tmp"
     try:
@@ -874,9 +933,10 @@ def save_checkpoint(ed_data: Symmetry_Resolved_ED_Data, path: str) -> None:
 
 
 def load_checkpoint(path: str) -> Symmetry_Resolved_ED_Data:
-    """Loads a checkpoint this package wrote (SRE:2552-2563).  The format is a
-    pickle, as in the reference, so loading runs code from the file: load only
-    checkpoints from a trusted source."""
+    """Loads a checkpoint this package wrote (SRE:2552-2563): the ED data with
+    its results and its orbit catalog, whose tables are uploaded to the device
+    when first used.  The format is a pickle, as in the reference, so loading
+    runs code from the file: load only checkpoints from a trusted source."""
     with open(path, "rb") as f:
         return pickle.load(f)
 
@@ -894,8 +954,12 @@ def ed_scan(ed_data: Symmetry_Resolved_ED_Data, *, nev: int = 5, mode: str = "ma
             flux_direction: int = 1, twisted_phases_over_2π_list=None, concurrent_sectors: int = 1):
     """Unified scan with flux points and checkpoint resume (SRE:2608-2703).
 
-    Flux points reuse the catalog: only the stabilizer phases and amplitudes
-    are refreshed for each twist (``update_orbit_stabilizer_phases``).
+    Flux points reuse the catalog: where the reference builds a fresh catalog
+    per twist, one device copy of ``ed_data.orbit_catalog`` has its stabilizer
+    phases and amplitudes rewritten in place for each twist with the formulas
+    of a fresh build (the same tables, bit for bit, without the orbit scan);
+    the caller's catalog keeps its phases.  Each per-twist checkpoint holds
+    that twist's catalog tables.
     ``concurrent_sectors`` (not in the reference) solves that many sectors of
     a flux point at a time on separate HIP streams.
     """
@@ -911,6 +975,7 @@ def ed_scan(ed_data: Symmetry_Resolved_ED_Data, *, nev: int = 5, mode: str = "ma
     model = ed_data.second_quantized_model
     os.makedirs(checkpoint_dir, exist_ok=True)
     paths = []
+    catalog = None  # the scan's copy of the catalog, refreshed per twist
     for θ_val in twisted_phases_over_2π_list:
         flux = [0.0] * model.lattice.dim
         flux[flux_direction - 1] = float(θ_val)
@@ -920,9 +985,11 @@ def ed_scan(ed_data: Symmetry_Resolved_ED_Data, *, nev: int = 5, mode: str = "ma
             continue
         update_second_quantized_model_with_twisted_phases(model, twisted_phases_over_2π=flux)
         group = build_translation_group(model.lattice, flux)
-        update_orbit_stabilizer_phases(ed_data.orbit_catalog, group, model.particle_statistics)
+        if catalog is None:
+            catalog = ed_data.orbit_catalog.clone()
+        _rephase_catalog(catalog, group)
         θ_data = Symmetry_Resolved_ED_Data(model, ed_data.n_filled, ed_data.filling_fraction, group,
-                                           build_irrep_list(group, model.lattice), ed_data.orbit_catalog,
+                                           build_irrep_list(group, model.lattice), catalog,
                                            [0] * len(ed_data.irrep_list))
         _ed_scan_sectors(θ_data, nev=nev, mode=mode, scanned_sectors=scanned_sectors, overwrite=overwrite,
                          concurrent_sectors=concurrent_sectors)

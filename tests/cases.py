@@ -52,6 +52,20 @@ def make_case(name: str):
         m = M.build_heisenberg_second_quantized_model(12)
         m, G = _twisted(m, (0.27, 0.0))
         return m, 6, G
+    if name == "heis18_twist":  # sectors of ~2.7k states
+        m = M.build_heisenberg_second_quantized_model(18)
+        m, G = _twisted(m, (0.27, 0.0))
+        return m, 9, G
+    if name == "hal34_twist":  # sectors of ~11.2k states
+        m = M.build_zero_flux_bosonic_fci_second_quantized_model(sample_size=[3, 4])
+        m, G = _twisted(m, (0.31, 0.0))
+        return m, 6, G
+    if name == "cb34_twist":
+        p = dict(M.my_optimal_param)
+        p["t''"] = -0.2
+        m = M.build_checkerboard_second_quantized_model([3, 4], p)
+        m, G = _twisted(m, (0.37, 0.0))
+        return m, 4, G
     if name == "hal34":
         m = M.build_zero_flux_bosonic_fci_second_quantized_model(sample_size=[3, 4])
         return m, 6, build_translation_group(m.lattice)
@@ -101,3 +115,44 @@ def oracle_catalog(name: str):
 def random_vector(n: int, seed: int = 7) -> np.ndarray:
     rng = np.random.default_rng(seed)
     return rng.standard_normal(n) + 1j * rng.standard_normal(n)
+
+
+def oracle_sector_levels(model, n_filled, group, irrep, nev, extra=4):
+    """The oracle's solve of one sector in the reference's way (CSR table,
+    ``eigs(SR)`` with the seeded start vector, real parts ascending; dense
+    ``eigh`` for dims <= 500) and the exact lowest ``nev + extra`` levels of
+    the block's Hermitian part (dense for dims <= 3000, else ``eigsh`` at
+    tolerance 1e-13).  Returns ``(dim, w_reference_method, w_hermitian_part)``."""
+    import scipy.sparse.linalg as spla
+
+    from oracle import ed_oracle as O
+
+    perm, ph = group_arrays(group)
+    ferm = is_fermionic(model)
+    oc = O.build_catalog(model.lattice.n_site, n_filled, perm, ph, ferm)
+    reps, stab, o2b = O.sector_basis(oc, irrep.values)
+    r, c, v, hd = O.projection_table(oc, reps, stab, o2b, model.bilinear_terms, model.density_density_terms,
+                                     irrep.values, ferm)
+    D = len(reps)
+    H = O.csr_hamiltonian(r, c, v, hd, D)
+    w_ref, _ = O.diagonalize(H, nev)
+    Hh = ((H + H.getH()) * 0.5).tocsr()
+    k = min(nev + extra, D - 2)
+    if D <= 3000:
+        herm = np.linalg.eigvalsh(Hh.toarray())[:k]
+    else:
+        herm = np.sort(spla.eigsh(Hh, k=k, which="SA", tol=1e-13, v0=O.reference_start_vector(D))[0])
+    return D, np.asarray(w_ref), np.asarray(herm)
+
+
+def isolated_levels(herm, nev, gap=1e-6):
+    """Indices i < nev whose level is farther than ``gap`` from its neighbours
+    in the exact spectrum ``herm`` (ascending; the last one is not judged
+    above if it is the end of the list)."""
+    out = []
+    for i in range(min(nev, len(herm))):
+        lo = herm[i] - herm[i - 1] if i > 0 else np.inf
+        hi = herm[i + 1] - herm[i] if i + 1 < len(herm) else 0.0
+        if lo > gap and hi > gap:
+            out.append(i)
+    return out

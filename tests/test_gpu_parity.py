@@ -53,8 +53,11 @@ def test_catalog_bit_exact(name):
     np.testing.assert_array_equal(gc.rep_sym_table, oc.rep_sym)
     assert gc.stabilizer_order_list == [int(x) for x in oc.stab_order]
     assert gc.stabilizer_g_indices_list == [[int(g) for g in s] for s in oc.stab_g]
-    zero_flux = np.allclose(np.asarray(G.operations[1 % len(G.operations)].perm_phases), 1.0)
-    assert _amp_close(gc.rep_amp_table, oc.rep_amp, exact=zero_flux)
+    # alpha is a sign, exact in complex64, when every site phase of every
+    # operation is +1 or -1; otherwise within one float32 ulp per component
+    ph = np.asarray(group_arrays(G)[1])
+    exact_alpha = bool(np.all(ph.imag == 0.0) and np.all(np.abs(ph.real) == 1.0))
+    assert _amp_close(gc.rep_amp_table, oc.rep_amp, exact=exact_alpha)
     ph_dev = np.concatenate([np.asarray(p, complex) for p in gc.stabilizer_phases_list])
     ph_ref = np.concatenate([np.asarray(p, complex) for p in oc.stab_phase])
     assert np.abs(ph_dev - ph_ref).max(initial=0.0) <= 4e-16
@@ -252,8 +255,10 @@ def test_twisted_block_solves_its_hermitian_part():
     it), so the block is Hermitian only to that rounding.  H x applies the
     block itself; the eigensolver's SpMM applies (H + H^H) / 2, whose lowest
     eigenvalues both precisions return within 1e-10.  The reference's eigs
-    returns the real parts of the block's eigenvalues: the same numbers except
-    inside multiplets that the rounding splits, hence 1e-7 there."""
+    returns the real parts of the block's eigenvalues: the same numbers to
+    second order in the rounding (the multiplets the rounding splits here are
+    split by ~1e-10; tests/test_gpu_twisted.py lists them), so within 1e-10
+    too."""
     import scipy.sparse as sp
 
     model, ne, G, oc, gc = _both("hub24_twist")
@@ -280,7 +285,7 @@ def test_twisted_block_solves_its_hermitian_part():
     # the reference's convention on the oracle's own block
     reps, r, c, v, hd, k = _oracle_block(model, oc, irrep, is_fermionic(model))
     w_ref, _ = O.diagonalize(O.csr_hamiltonian(r, c, v, hd, len(reps)), 6)
-    assert np.max(np.abs(w64 - w_ref)) <= 1e-7, (w64, w_ref)
+    assert np.max(np.abs(w64 - w_ref)) <= 1e-10, (w64, w_ref)
 
 
 def test_twisted_matrix_mode_matches_matrixfree():
