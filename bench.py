@@ -1,29 +1,31 @@
 #!/usr/bin/env python
 """Sector-solve benchmark of the MI355X symmetry-resolved ED path.
 
-Workload (one GPU): spinful Fermi-Hubbard model on a 3x4 periodic square
-lattice, 24 spin-orbitals, 12 fermions (C(24,12) = 2 704 156 masks), t = 1,
-U = 4, translation group of order 12; one step solves all 12 momentum sectors
-for their 6 lowest eigenpairs.  The orbit catalog is built once outside the
-timed region (the reference's benchmark convention) and timed separately; a
-sector solve covers the irrep filter, the projection-table build, the
-eigensolve and the eigenvectors.  Inputs are synthetic (the model is fully
-determined by its parameters).
+Default workload (one GPU): the fermionic checkerboard flat-band model on
+4x6 unit cells x 2 sublattices (48 sites), 8 spinless fermions (C(48,8) =
+377 348 994 masks), t = 1, t' = +-0.3, t'' = -0.2, nearest-neighbour hopping
+phase pi/4, nearest-neighbour repulsion V = 1, translation group of order 24,
+24 momentum sectors of ~15.7 M states.  One step is one sector solve (the
+sectors in turn), covering the irrep filter, the projection-table build and
+the mixed-precision eigensolve of the 10 lowest levels (complex64 sweeps,
+FP64 refinement to an eigenvalue error bound of 1e-12); the eigenvectors stay
+in device memory.  The orbit catalog is built once outside the timed region
+(the reference's benchmark convention) and timed separately.  Inputs are
+synthetic (the model is fully determined by its parameters).
 
-Printed: one JSON line with solves/s (whole job), p50/p99 sector latency,
-|dE| against the FP64 solve, the live roofline of the dominant SpMM kernel
-(HIP events on the launch streams over the timed region) and the CPU oracle
-timed on a bounded sample (two sectors) on this host.
+Printed: one JSON line with sector solves/s, p50/p99 sector latency, |dE|
+against the FP64 solve of the same sector, the roofline of the dominant
+kernel (HIP events on the launch stream over the timed region), the
+residuals of returned eigenvectors, the Hubbard 3x4 workload as a secondary
+figure, and the CPU baseline (the reference method on a bounded sample,
+extrapolated to this sector as stated in its ``sample`` field).
 
-Other modes: ``--serve`` sends a seeded Poisson stream of mixed requests
-(Heisenberg chains, Hubbard and checkerboard lattices at several sizes, random
-momentum sector, twist angle and number of eigenvalues) through the serving
-loop and prints its own JSON line (latency percentiles, solves/s, |dE| of a
-sample against FP64); ``--flux-sweep`` solves the 18 momentum sectors of the
-Haldane 3x6 boson workload (D ~ 5.23 M) at successive twist angles, the
-catalog's phases refreshed in place per twist; ``--profile-spmm`` runs only
-the SpMM kernels of one sector; ``--copy-bytes N`` runs a plain device copy
-of N bytes.
+Other modes: ``--workload hubbard3x4`` (spinful Hubbard 3x4, all 12
+momentum sectors per step on 8 worker streams); ``--serve`` sends a seeded
+Poisson stream of mixed requests through the serving loop; ``--flux-sweep``
+solves the 18 momentum sectors of the Haldane 3x6 boson workload at
+successive twist angles; ``--profile-spmm`` runs only the SpMM kernels of one
+sector; ``--copy-bytes N`` runs a plain device copy of N bytes.
 """
 
 from __future__ import annotations
@@ -31,6 +33,7 @@ from __future__ import annotations
 import argparse
 import ctypes as C
 import json
+import math
 import os
 import sys
 import time
@@ -217,6 +220,31 @@ def profile_spmm(args) -> int:
     return 0
 
 
+def profile_solve(args) -> int:
+    """One mixed-precision solve of sector k=(0,0) of the 48-site checkerboard
+    workload (catalog, irrep filter, table, eigenvalues of the 10 lowest
+    levels), for counter runs."""
+    os.environ.setdefault("EDX_QUIET", "1")
+    import realspace_exatdiagonalization_py_amd as ed
+    from realspace_exatdiagonalization_py_amd import _edx
+    from realspace_exatdiagonalization_py_amd.groups import build_irrep_list, build_translation_group
+
+    _edx.set_autotune(False, ed.PREC_MIXED, 8)
+    model = checkerboard_model((4, 6))
+    G = build_translation_group(model.lattice)
+    cat = ed.build_symmetry_orbit_catalog(second_quantized_model=model, n_filled=8, symmetry_group=G,
+                                          particle_statistics=model.particle_statistics)
+    basis = ed.build_symmetry_sector_basis(cat, build_irrep_list(G, model.lattice)[0])
+    H = ed.MatrixFreeHamiltonian(basis, model.bilinear_terms, model.density_density_terms, None)
+    t0 = time.perf_counter()
+    w, _ = ed.diagonalize_block_matrixfree(H, H.n, nev=10, precision=ed.PREC_MIXED, eig_tol=1e-12,
+                                           want_vectors=False)
+    print(json.dumps({"dim": H.n, "entries": H._dev.n_entries, "lanes": H._dev.spmv_group,
+                      "solve_s": round(time.perf_counter() - t0, 3), "matvecs": int(H.last_info[0]),
+                      "levels": [float(x) for x in w]}))
+    return 0
+
+
 def copy_kernel(args) -> int:
     """A plain device copy of --copy-bytes bytes, five times (counter calibration)."""
     from realspace_exatdiagonalization_py_amd import _edx
@@ -228,6 +256,302 @@ def copy_kernel(args) -> int:
     for _ in range(5):
         ctx.check(_edx.lib.edx_device_copy(ctx.handle, b.ptr, a.ptr, n))
     print(json.dumps({"copy_bytes": n, "launches": 5}))
+    return 0
+
+
+# Headline workload: the 48-site checkerboard flat band (couplings of the
+# reference builder, t'_1 = -t'_2; nearest-neighbour phase 2 pi / 8).
+CHECKERBOARD_PARAMS = {"t": 1.0, "t'_1": 0.3, "t'_2": -0.3, "t''": -0.2, "phi_over_2pi": 1 / 8, "V1": 1.0,
+                       "V2": 0.0, "V3": 0.0}
+CHECKERBOARD_TRAFFIC_FILE = ROOT / "profiles" / "solver_checkerboard48_traffic.json"
+CHECKERBOARD_WORKLOAD = ("fermionic checkerboard flat band 4x6x2 (48 sites), 8 fermions, C(48,8)=377348994, t=1, "
+                         "t'=+-0.3, t''=-0.2, NN phase pi/4, NN V=1, translation group of order 24; one step = one "
+                         "momentum-sector solve (sectors in turn), 10 lowest levels, mixed precision")
+
+
+def checkerboard_model(size):
+    from realspace_exatdiagonalization_py_amd import models as M
+
+    return M.build_zero_flux_fermionic_fci_second_quantized_model(sample_size=list(size),
+                                                                  params=dict(CHECKERBOARD_PARAMS))
+
+
+def _traffic_entry(path: Path, kernel: str):
+    """Counter-measured HBM bytes per launch of `kernel` from a committed summary."""
+    try:
+        t = json.loads(path.read_text())
+        return t["kernels"][kernel]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
+def cpu_baseline_checkerboard(ed, H48, n_mv48, nev, row_fraction=16):
+    """The reference method on a bounded sample, extrapolated to one sector of
+    the 48-site workload (which the reference cannot build: its rank tables
+    stop at 42 sites, and the oracle's numpy catalog of 3.8e8 masks alone
+    takes about ten minutes):
+
+    1. the oracle's full solve of sector k=(0,0) of the 3x4 checkerboard
+       (same couplings, 8 fermions, D ~ 6.1e4): catalog, projection table,
+       CSR, eigs(SR, tol 1e-10, seeded start vector), which gives the catalog
+       time per mask, the table time per stored entry and ARPACK's time
+       outside the matvecs per matvec and row;
+    2. scipy CSR matvecs (the reference's MatrixFreeHamiltonian.__call__) on a
+       contiguous 1/16 of the rows of the 48-site sector's own table, built
+       on the device and fetched (columns over the whole sector);
+    3. the sector solve = table time (rate 1) + n_mv x (matvec time of 2 scaled
+       to all rows + ARPACK's other time of 1 scaled to D), with n_mv the
+       matvec count of the device solve of the same sector (complex64 and
+       FP64 block matvecs together)."""
+    import scipy.sparse.linalg as spla
+
+    from oracle import ed_oracle as O
+    from realspace_exatdiagonalization_py_amd.groups import build_irrep_list, build_translation_group, group_arrays
+
+    model = checkerboard_model((3, 4))
+    G = build_translation_group(model.lattice)
+    perm, ph = group_arrays(G)
+    t0 = time.perf_counter()
+    oc = O.build_catalog(model.lattice.n_site, 8, perm, ph, True)
+    t_cat = time.perf_counter() - t0
+    irrep = build_irrep_list(G, model.lattice)[0]
+    t0 = time.perf_counter()
+    reps, stab, o2b = O.sector_basis(oc, irrep.values)
+    r, c, v, hd = O.projection_table(oc, reps, stab, o2b, model.bilinear_terms, model.density_density_terms,
+                                     irrep.values, True)
+    Hs = O.csr_hamiltonian(r, c, v, hd, len(reps))
+    t_table = time.perf_counter() - t0
+    n_calls = [0]
+
+    def mv(x):
+        n_calls[0] += 1
+        return Hs @ np.ascontiguousarray(x, dtype=np.complex128)
+
+    x = O.reference_start_vector(Hs.shape[0])
+    t0 = time.perf_counter()
+    for _ in range(5):
+        Hs @ x
+    t_mv_small = (time.perf_counter() - t0) / 5
+    op = spla.LinearOperator(Hs.shape, matvec=mv, dtype=np.complex128)
+    t0 = time.perf_counter()
+    w_small, _ = spla.eigs(op, k=nev, which="SR", v0=x, tol=1e-10)
+    t_eigs = time.perf_counter() - t0
+    D_s, E_s, C_s = len(reps), len(r), len(oc.rep_rank)
+    other_per_mv_row = max(t_eigs - n_calls[0] * t_mv_small, 0.0) / (n_calls[0] * D_s)
+    del Hs, oc, r, c, v
+    # the 48-site sector's own CSR, a contiguous slice of its rows
+    D, E = H48._dev.dim, H48._dev.n_entries
+    n_rows = D // row_fraction
+    block, _ = H48._dev.fetch_csr(0, n_rows)
+    xs = np.random.default_rng(1).standard_normal(D) + 1j * np.random.default_rng(2).standard_normal(D)
+    block @ xs  # first touch
+    t0 = time.perf_counter()
+    reps_mv = 3
+    for _ in range(reps_mv):
+        block @ xs
+    t_slice = (time.perf_counter() - t0) / reps_mv
+    t_mv48 = t_slice * E / max(block.nnz, 1)
+    del block, xs
+    t_table48 = t_table / E_s * E
+    t_solve48 = t_table48 + n_mv48 * (t_mv48 + other_per_mv_row * D)
+    t_cat48 = t_cat / C_s * math.comb(48, 8)
+    return {
+        "value": round(1.0 / t_solve48, 8),
+        "unit": "sector solves/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": (f"extrapolated, not run: the oracle (numpy/scipy restatement of the reference method) solved "
+                   f"sector (0,0) of the 3x4 checkerboard (D={D_s}, {E_s} stored entries, nev={nev}: catalog "
+                   f"{t_cat:.2f} s, table+CSR {t_table:.2f} s, eigs {t_eigs:.2f} s with {n_calls[0]} matvecs), and "
+                   f"scipy CSR matvecs ran on rows [0, {n_rows}) of this 48-site sector's table ({t_slice:.3f} s per "
+                   f"matvec); sector solve = table time scaled by entries ({t_table48:.0f} s) + {n_mv48} matvecs "
+                   f"(the device solve's count) x ({t_mv48:.2f} s CSR matvec scaled to {E} entries + ARPACK's other "
+                   f"work scaled to D={D}); scipy CSR matvec and numpy on one thread"),
+        "sector_solve_s_extrapolated": round(t_solve48, 1),
+        "csr_matvec_s_extrapolated": round(t_mv48, 3),
+        "catalog_s_extrapolated": round(t_cat48, 1),
+        "sample_sector": {"dim": D_s, "entries": E_s, "eigs_s": round(t_eigs, 3), "matvecs": n_calls[0],
+                          "table_s": round(t_table, 3), "catalog_s": round(t_cat, 3),
+                          "lowest": [float(x) for x in np.sort(w_small.real)[:3]]},
+    }
+
+
+def hubbard_secondary(ed, _edx, workers=8, passes=2):
+    """The Hubbard 3x4 workload (all 12 momentum sectors per pass, 6 levels,
+    mixed precision) streamed through `workers` HIP streams: solves/s."""
+    from realspace_exatdiagonalization_py_amd.groups import build_translation_group
+
+    model = ed.build_spinful_hubbard_second_quantized_model([3, 4], 1.0, 4.0)
+    G = build_translation_group(model.lattice)
+    _edx.set_autotune(True, ed.PREC_MIXED, 6)
+    data = ed.build_ed_data(model, filling_fraction=Fraction(12, 24), symmetry_group=G)
+    solve_all(ed, data, 6, ed.PREC_MIXED, 1e-12, workers)
+    t0 = time.perf_counter()
+    lat, _ = solve_stream(ed, data, 6, ed.PREC_MIXED, 1e-12, workers, passes)
+    el = time.perf_counter() - t0
+    lat_ms = np.asarray(lat) * 1e3
+    return {"workload": "spinful Hubbard 3x4 PBC, t=1, U=4, 12 fermions on 24 orbitals, all 12 momentum sectors, "
+                        "6 lowest levels each, mixed precision",
+            "sector_solves": len(lat), "workers": workers, "solves_per_s": round(len(lat) / el, 3),
+            "p50_ms": round(float(np.percentile(lat_ms, 50)), 3), "p99_ms": round(float(np.percentile(lat_ms, 99)), 3)}
+
+
+def checkerboard_bench(args) -> int:
+    os.environ.setdefault("EDX_QUIET", "1")
+    import realspace_exatdiagonalization_py_amd as ed
+    from realspace_exatdiagonalization_py_amd import _edx
+    from realspace_exatdiagonalization_py_amd.groups import build_irrep_list, build_translation_group
+
+    nev = args.nev or 10
+    prec = ed.PREC_MIXED if args.precision == "mixed" else ed.PREC_FP64
+    ctx = _edx.Context.default()
+    _edx.set_autotune(not args.no_autotune, prec, min(nev, 8))
+    model = checkerboard_model((4, 6))
+    G = build_translation_group(model.lattice)
+    log("building catalog ...")
+    t0 = time.perf_counter()
+    cat = ed.build_symmetry_orbit_catalog(second_quantized_model=model, n_filled=8, symmetry_group=G,
+                                          particle_statistics=model.particle_statistics)
+    ctx.check(_edx.lib.edx_synchronize(ctx.handle))
+    t_cat = time.perf_counter() - t0
+    irreps = build_irrep_list(G, model.lattice)
+    n_sec = len(irreps)
+    log(f"catalog: {cat.n_orbits} orbits in {t_cat * 1e3:.1f} ms")
+
+    def sector(i):
+        basis = ed.build_symmetry_sector_basis(cat, irreps[i])
+        return basis, ed.MatrixFreeHamiltonian(basis, model.bilinear_terms, model.density_density_terms, None)
+
+    def solve(i, precision, eig_tol):
+        """One step: irrep filter, table and eigensolve of sector i."""
+        t0 = time.perf_counter()
+        basis, H = sector(i)
+        w, _ = ed.diagonalize_block_matrixfree(H, H.n, nev=nev, precision=precision, eig_tol=eig_tol,
+                                               want_vectors=False)
+        return time.perf_counter() - t0, np.asarray(w), basis.dim, np.array(H.last_info)
+
+    for j in range(args.warmup):
+        dt, _, dim, info = solve(j % n_sec, prec, args.eig_tol)
+        log(f"warmup step {j + 1}/{args.warmup}: sector {j % n_sec} D={dim} {dt:.2f}s ({info[0]:.0f} matvecs)")
+    _edx.kernel_stats_all(enable=0 if args.no_kernel_timer else 1, reset=True)
+    ctx.check(_edx.lib.edx_synchronize(ctx.handle))
+    lat, dims, infos, timed = [], [], [], []
+    t0 = time.perf_counter()
+    for j in range(args.steps):
+        i = (args.warmup + j) % n_sec
+        dt, w, dim, info = solve(i, prec, args.eig_tol)
+        lat.append(dt)
+        dims.append(dim)
+        infos.append(info)
+        timed.append(i)
+        last = (i, w)
+        log(f"step {j + 1}/{args.steps}: sector {i} D={dim} {dt:.2f}s ({info[0]:.0f} matvecs)")
+    elapsed = time.perf_counter() - t0
+    ks = _edx.kernel_stats_all(enable=0)
+    log(f"{args.steps} sector solves in {elapsed:.2f}s")
+
+    # the last timed sector again: FP64 with ARPACK's residual rule (the plain
+    # path and the accuracy reference), then with eigenvectors to the host
+    i_ref, w_mixed = last
+    basis, H = sector(i_ref)
+    t0 = time.perf_counter()
+    w64, _ = ed.diagonalize_block_matrixfree(H, H.n, nev=nev, precision=ed.PREC_FP64, eig_tol=0.0,
+                                             want_vectors=False)
+    t_plain = time.perf_counter() - t0
+    dE = float(np.max(np.abs(w_mixed - w64)))
+    log(f"plain FP64 solve of sector {i_ref}: {t_plain:.2f}s, max |dE| {dE:.2e}")
+    t0 = time.perf_counter()
+    wv, vecs = ed.diagonalize_block_matrixfree(H, H.n, nev=nev, precision=prec, eig_tol=args.eig_tol,
+                                               want_vectors=True)
+    t_vec = time.perf_counter() - t0
+    n_mv = int(H.last_info[0])
+    R = H(vecs) - vecs * wv
+    res_true = float(np.max(np.linalg.norm(R, axis=0) / np.maximum(1.0, np.abs(wv))))
+    del R, vecs
+    log(f"solve with eigenvectors to the host: {t_vec:.2f}s, true max relative residual {res_true:.1e}")
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("cpu baseline (oracle sample, extrapolated) ...")
+        cpu = cpu_baseline_checkerboard(ed, H, n_mv, nev)
+    del H, basis
+
+    # roofline: the class with the most kernel time in the timed region
+    total_k = sum(v[0] for v in ks.values())
+    cls = max(ks, key=lambda k: ks[k][0])
+    lanes = next((e["lanes"] for e in _edx.autotune_table() if e["block"] == min(nev, 8)), None)
+    nb = min(nev, 8)
+    names = {"spmm_c64": f"spmm_sell<float2,{nb},{lanes}>", "spmm_c128": f"spmm_sell<double2,{nb + 2},{lanes}>",
+             "gram": "gram_tiles + gram_reduce", "gemm": "gemm_tiles"}
+    k_ms, k_n, k_bytes = ks[cls]
+    avg_ms = k_ms / max(k_n, 1)
+    achieved = k_bytes / k_ms / 1e6 if k_ms > 0 else 0.0
+    tr = _traffic_entry(CHECKERBOARD_TRAFFIC_FILE, names.get(cls, cls))
+    traffic = tr["hbm_bytes_per_launch"] if tr else None
+    roofline = {
+        "bound": "hbm",
+        "achieved": round(achieved, 1),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": traffic,
+        "kernel": f"{names.get(cls, cls)} (the class with the most kernel time: projection-table H|psi> on a block "
+                  f"of {nb} complex64 vectors)" if cls == "spmm_c64" else names.get(cls, cls),
+        "avg_launch_ms": round(avg_ms, 5),
+        "launches": k_n,
+        "algorithmic_bytes_per_launch": round(k_bytes / max(k_n, 1)),
+        "hbm_achieved": round(traffic / avg_ms / 1e6, 1) if traffic and avg_ms > 0 else None,
+        "hbm_frac": round(traffic / avg_ms / 1e6 / HBM_PEAK_GBS, 4) if traffic and avg_ms > 0 else None,
+        "kernel_time_share": {k: round(v[0] / total_k, 3) for k, v in ks.items() if total_k > 0},
+        # every timed class against the same peak: algorithmic bytes (SpMM:
+        # table, gathered and written vectors; Gram: the vectors read; update:
+        # the vectors read and written) over the event-timed duration
+        "by_class": {k: {"launches": v[1], "avg_launch_ms": round(v[0] / v[1], 5),
+                         "achieved": round(v[2] / v[0] / 1e6, 1), "frac": round(v[2] / v[0] / 1e6 / HBM_PEAK_GBS, 4)}
+                     for k, v in ks.items() if v[1] > 0 and v[0] > 0 and v[2] > 0},
+    }
+    log("secondary workload (Hubbard 3x4) ...")
+    del cat
+    secondary = hubbard_secondary(ed, _edx, workers=args.workers)
+    lat_ms = np.asarray(lat) * 1e3
+    value = args.steps / elapsed
+    line = {
+        "metric": METRIC,
+        "value": round(value, 4),
+        "unit": "sector solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c64+c128" if prec == ed.PREC_MIXED else "c128",
+        "precision_detail": ("complex64 sweeps (FP64 Gram accumulation, coefficients and Ritz problems), then "
+                             "complex128 refinement" if prec == ed.PREC_MIXED else "complex128 throughout"),
+        "data": "synthetic (model fully determined by its parameters)",
+        "config": {"workload": CHECKERBOARD_WORKLOAD, "sectors_timed": timed,
+                   "sector_dims": [int(min(dims)), int(max(dims))], "nev": nev, "precision": args.precision,
+                   "concurrent_sectors": 1, "eigenvectors": "left in device memory (see pcie_inclusive_solve_s)"},
+        "p50_ms": round(float(np.percentile(lat_ms, 50)), 3),
+        "p99_ms": round(float(np.percentile(lat_ms, 99)), 3),
+        "max_abs_dE_vs_fp64": dE,
+        "plain_fp64_solves_per_s": round(1.0 / t_plain, 4),
+        "speedup_vs_plain_fp64": round(value * t_plain, 3),
+        "stop_rule": (f"eigenvalue error bound <= {args.eig_tol:g} (residual^2/gap, clusters) or relative residual "
+                      "<= 1e-10" if args.eig_tol > 0 else "relative residual <= 1e-10"),
+        "matvecs_per_solve": [int(x[0]) for x in infos],
+        "solver_max_rel_residual": float(max(x[2] for x in infos)),
+        "eigvec_true_max_rel_residual": res_true,
+        "pcie_inclusive_solve_s": round(t_vec, 3),
+        "catalog_ms": round(t_cat * 1e3, 3),
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "secondary": secondary,
+        "autotune": [{"dim_log2": e["dim_log2"], "block": e["block"], "lanes": e["lanes"],
+                      "ms_by_lanes": {str(k): round(v, 5) for k, v in e["ms"].items()}}
+                     for e in _edx.autotune_table()],
+    }
+    print(json.dumps(line))
     return 0
 
 
@@ -455,9 +779,11 @@ def flux_sweep(args) -> int:
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--nev", type=int, default=6)
+    ap.add_argument("--workload", choices=("checkerboard48", "hubbard3x4"), default="checkerboard48",
+                    help="checkerboard48: one 48-site sector solve per step; hubbard3x4: all 12 sectors per step")
+    ap.add_argument("--nev", type=int, default=None, help="levels per sector (default 10 / 6 per workload)")
     ap.add_argument("--precision", choices=("mixed", "fp64"), default="mixed")
     ap.add_argument("--eig-tol", type=float, default=1e-12,
                     help="eigenvalue error bound that ends a solve (0: ARPACK residual rule only)")
@@ -468,6 +794,8 @@ def main() -> int:
     ap.add_argument("--no-kernel-timer", action="store_true",
                     help="no live per-kernel HIP-event timing (for runs under a profiler, which times kernels itself)")
     ap.add_argument("--profile-spmm", action="store_true", help="run only the SpMM kernels of one sector")
+    ap.add_argument("--profile-solve", action="store_true",
+                    help="one solve of a 48-site checkerboard sector (for counter runs)")
     ap.add_argument("--copy-bytes", type=float, default=0, help="run only a plain device copy of this many bytes")
     ap.add_argument("--flux-sweep", action="store_true",
                     help="twisted-boundary sweep of the Haldane 3x6 boson workload (18 sectors per twist)")
@@ -489,17 +817,29 @@ def main() -> int:
         return 2
     if args.profile_spmm:
         return profile_spmm(args)
+    if args.profile_solve:
+        return profile_solve(args)
     if args.copy_bytes:
         return copy_kernel(args)
     if args.serve:
         return serve_bench(args)
     if args.flux_sweep:
         return flux_sweep(args)
+    if args.workload == "hubbard3x4":
+        return hubbard_bench(args)
+    return checkerboard_bench(args)
 
+
+def hubbard_bench(args) -> int:
+    """Spinful Hubbard 3x4, t = 1, U = 4, 12 fermions on 24 spin-orbitals
+    (C(24,12) = 2 704 156 masks); one step solves all 12 momentum sectors for
+    their 6 lowest eigenpairs, the K steps x 12 solves streamed through the
+    worker pool.  The CPU baseline is the oracle on two sectors."""
     os.environ.setdefault("EDX_QUIET", "1")
     import realspace_exatdiagonalization_py_amd as ed
     from realspace_exatdiagonalization_py_amd import _edx
 
+    args.nev = args.nev or 6
     prec = ed.PREC_MIXED if args.precision == "mixed" else ed.PREC_FP64
     ctx = _edx.Context.default()
     # SpMM lanes per row timed once per sector shape (the warmup step pays)

@@ -143,6 +143,13 @@ int edx_hamiltonian_build(edx_ctx* ctx, const edx_catalog* cat, const edx_sector
  * grouped by row, sorted by (column, term) within a row.  NULL skips. */
 int edx_hamiltonian_fetch_table(edx_ctx* ctx, const edx_hamiltonian* ham, int64_t* row, int64_t* col,
                                 int32_t* term, double* vals, double* h_diag);
+/* Rows [row_begin, row_end) of the same table in CSR form (the off-diagonal
+ * part of the CSR that MatrixFreeHamiltonian builds, ref SRE:1958-1963):
+ * row_ptr int64[n+1] starting at 0, col int32 and vals complex128 of the
+ * rows' entries (row_ptr[n] of them, sorted by (column, term) within a row),
+ * h_diag complex128[n].  col/vals/h_diag may be NULL. */
+int edx_hamiltonian_fetch_csr(edx_ctx* ctx, const edx_hamiltonian* ham, int64_t row_begin, int64_t row_end,
+                              int64_t* row_ptr, int32_t* col, double* vals, double* h_diag);
 /* Sector block given as a host CSR (off-diagonal part, any order within a
  * row) plus a diagonal: the matrix-mode block of build_ed_Hamiltonian_
  * symmetry_block (ref SRE:2102-2174) placed on the device for edx_eigs and
@@ -194,7 +201,7 @@ int edx_eigs(edx_ctx* ctx, const edx_hamiltonian* ham, int32_t nev, int32_t bloc
  *   eig_tol  > 0: a pair also counts as converged once the residual-squared-
  *            over-gap bound of its eigenvalue cluster is below eig_tol (the
  *            eigenvalue is then accurate to eig_tol; the vector to ~sqrt)
- *   mixed_tol residual at which EDX_PREC_MIXED hands over to FP64 (default 1e-6;
+ *   mixed_tol residual at which EDX_PREC_MIXED hands over to FP64 (default 3e-7;
  *            earlier when the complex64 sweeps stall below max(30 x mixed_tol,
  *            1e-4), as they do under a target below their rounding floor)
  *   basis    Ritz basis size (0: automatic), block as in edx_eigs */

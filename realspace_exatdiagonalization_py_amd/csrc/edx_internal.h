@@ -209,21 +209,26 @@ struct Ctx {
   hipEvent_t stage_ev[2] = {nullptr, nullptr};
 };
 
+// Largest Krylov basis and block width of the eigensolver.
+constexpr int kMaxBasis = 224;
+constexpr int kMaxBlock = 16;
 // Sizes of the context's pinned staging buffers (double2 elements).
-// pinned_in holds a Gram result ((128 + 16) x 16), the factor written by the
-// device Cholesky of a block (16 x 16) and its status word, then a ring of
-// kLogSlots block-step logs: projection coefficients ((128 + 16) x 16), the
-// block coupling R (16 x 16) and four words of status.
-constexpr size_t kPinnedGram = (128 + 16) * 16;
-constexpr size_t kPinnedFactor = 16 * 16 + 1;
+// pinned_in holds a Gram result ((kMaxBasis + kMaxBlock) x kMaxBlock), the
+// factor written by the device Cholesky of a block (16 x 16) and its status
+// word, then a ring of kLogSlots block-step logs: projection coefficients
+// ((kMaxBasis + kMaxBlock) x kMaxBlock), the block coupling R (16 x 16) and
+// four words of status.
+constexpr size_t kPinnedGram = (size_t)(kMaxBasis + kMaxBlock) * kMaxBlock;
+constexpr size_t kPinnedFactor = kMaxBlock * kMaxBlock + 1;
 constexpr int kLogSlots = 16;
-constexpr size_t kLogC = (128 + 16) * 16;
-constexpr size_t kLogSlot = kLogC + 16 * 16 + 4;
+constexpr size_t kLogC = (size_t)(kMaxBasis + kMaxBlock) * kMaxBlock;
+constexpr size_t kLogSlot = kLogC + kMaxBlock * kMaxBlock + 4;
 constexpr size_t kPinnedIn = kPinnedGram + kPinnedFactor + kLogSlots * kLogSlot;
-// pinned_out: a ring of four update-coefficient slots ((128 + 16) x 16
-// each), then the coefficients of a restart combination (up to 144 x 144).
-constexpr size_t kPinnedRing = 4 * (128 + 16) * 16;
-constexpr size_t kPinnedCombine = (128 + 16) * (128 + 16);
+// pinned_out: a ring of four update-coefficient slots ((kMaxBasis +
+// kMaxBlock) x kMaxBlock each), then the coefficients of a restart
+// combination (up to (kMaxBasis + kMaxBlock) squared).
+constexpr size_t kPinnedRing = 4 * (size_t)(kMaxBasis + kMaxBlock) * kMaxBlock;
+constexpr size_t kPinnedCombine = (size_t)(kMaxBasis + kMaxBlock) * (kMaxBasis + kMaxBlock);
 constexpr size_t kPinnedOut = kPinnedRing + kPinnedCombine;
 // Device -> host copies into pageable memory go through two pinned staging
 // buffers of this many bytes each (owned by the context, allocated on first

@@ -29,6 +29,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <type_traits>
 
 #include "dense.h"
 #include "objects.h"
@@ -45,8 +46,20 @@ void dense_matrix(const edx_hamiltonian& h, hipStream_t st, double2* out);
 
 namespace {
 
-constexpr int kMaxVec = 128;     // basis vectors the Gram/update kernels accept
-constexpr int kMaxW = 16;        // block width
+constexpr int kMaxVec = edx::kMaxBasis;  // basis vectors the Gram/update kernels accept
+constexpr int kMaxW = edx::kMaxBlock;    // block width
+// P = [Q, W]^H W of a block step is staged in LDS by step_coeffs: the basis
+// is capped so that (nbasis + b) * b <= kStepLds (48 KiB of complex128)
+constexpr int kStepLds = 3072;
+// Gram of complex64 vectors against at most 8 on the FP32 matrix cores from
+// this many rows (below, its grid is too small to fill the device); update on
+// the FP64 matrix cores (kMfmaUpdate) or the vector-ALU kernel with
+// kUpdateRows rows per lane.
+constexpr int64_t kMfmaGramRows = int64_t(1) << 20;
+constexpr bool kMfmaUpdate = false;
+constexpr int kUpdateRows = 2;
+// Largest LDS staging of an update's interleaved output copy (bytes).
+constexpr size_t kStageLdsBytes = 48 * 1024;
 constexpr size_t kLogC = edx::kLogC;
 
 __device__ __forceinline__ double2 to_d2(double2 v) { return v; }
@@ -262,10 +275,11 @@ constexpr int kCombineStage = 2;
 constexpr int kCombineTile = 12;
 
 // Out_k[r] = sum_i A_i[r] S[i][k] for k < nk (S row-major, leading dimension
-// ldS), FP64 accumulation.  One workgroup per (256 rows, tile of NK outputs);
-// the 1-D grid is mapped so that the tiles of a row block run on one XCD,
-// whose L2 then serves the block's inputs to all of them.  Each lane owns one
-// row.  The tile's coefficients are staged in LDS (every lane reads the same
+// ldS), FP64 accumulation.  One workgroup per (256 RPT rows, tile of NK
+// outputs); the 1-D grid is mapped so that the tiles of a row block run on one
+// XCD, whose L2 then serves the block's inputs to all of them.  Each lane owns
+// RPT rows (r, r + 256, ...), so each coefficient read from LDS serves RPT
+// rows.  The tile's coefficients are staged in LDS (every lane reads the same
 // entry, a broadcast), with zero rows up to a multiple of two stages, so the
 // input loop needs no bounds test: it loads vector min(i, nv - 1) and the zero
 // coefficient cancels it.  Loads of the next U vectors are in flight while the
@@ -274,10 +288,10 @@ constexpr int kCombineTile = 12;
 // input of row r is read by the lane of row r before it writes any output of
 // row r, so Out may be one of the A arrays (same leading dimension); Xi
 // (optional, one tile only) receives the interleaved copy [r][k].
-template <typename V, int NK, int U>
+template <typename V, int NK, int U, int RPT = 1>
 __global__ void __launch_bounds__(256)
 gemm_tiles(int64_t D, VecPair<V> A, int nk, V* Ob, int64_t ldo, const double2* __restrict__ S, int ldS, V* Xi,
-           int n_kt, int n_tiles, StepPred pred) {
+           int n_kt, int n_tiles, int stage_xi, StepPred pred) {
   extern __shared__ double2 s_tile[];  // [nvp][NK]
   if (!pred.run()) return;
   if (!pred.final_update()) Xi = nullptr;
@@ -293,28 +307,39 @@ gemm_tiles(int64_t D, VecPair<V> A, int nk, V* Ob, int64_t ldo, const double2* _
     s_tile[e] = (i < nv && k0 + k < nk) ? S[(int64_t)i * ldS + k0 + k] : make_double2(0.0, 0.0);
   }
   __syncthreads();
-  const int64_t r = (int64_t)rb * 256 + threadIdx.x;
-  const bool live = r < D;
-  const int64_t rc = live ? r : D;
-  double pr[NK], pi[NK];
+  int64_t rr[RPT], rc[RPT];
 #pragma unroll
-  for (int k = 0; k < NK; ++k) pr[k] = pi[k] = 0.0;
-  V b0[U], b1[U];
-#define EDX_GEMM_LOAD(B, I0)                                                          \
-  {                                                                                   \
-    _Pragma("unroll") for (int u = 0; u < U; ++u) B[u] = A.vec(min((I0) + u, nv - 1))[rc]; \
-    EDX_KEEP_PREFETCH();                                                              \
+  for (int q = 0; q < RPT; ++q) {
+    rr[q] = (int64_t)rb * 256 * RPT + 256 * q + threadIdx.x;
+    rc[q] = rr[q] < D ? rr[q] : D;
+  }
+  double pr[RPT][NK], pi[RPT][NK];
+#pragma unroll
+  for (int q = 0; q < RPT; ++q)
+#pragma unroll
+    for (int k = 0; k < NK; ++k) pr[q][k] = pi[q][k] = 0.0;
+  V b0[U][RPT], b1[U][RPT];
+#define EDX_GEMM_LOAD(B, I0)                                                  \
+  {                                                                           \
+    _Pragma("unroll") for (int u = 0; u < U; ++u) {                           \
+      const V* vp = A.vec(min((I0) + u, nv - 1));                             \
+      _Pragma("unroll") for (int q = 0; q < RPT; ++q) B[u][q] = vp[rc[q]];    \
+    }                                                                         \
+    EDX_KEEP_PREFETCH();                                                      \
   }
 #define EDX_GEMM_FMA(B, I0)                                \
   {                                                        \
     _Pragma("unroll") for (int u = 0; u < U; ++u) {        \
-      const double2 x = to_d2(B[u]);                       \
+      double2 x[RPT];                                      \
+      _Pragma("unroll") for (int q = 0; q < RPT; ++q) x[q] = to_d2(B[u][q]); \
       _Pragma("unroll") for (int k = 0; k < NK; ++k) {     \
         const double2 sv = s_tile[((I0) + u) * NK + k];    \
-        pr[k] = fma(x.x, sv.x, pr[k]);                     \
-        pr[k] = fma(-x.y, sv.y, pr[k]);                    \
-        pi[k] = fma(x.x, sv.y, pi[k]);                     \
-        pi[k] = fma(x.y, sv.x, pi[k]);                     \
+        _Pragma("unroll") for (int q = 0; q < RPT; ++q) {  \
+          pr[q][k] = fma(x[q].x, sv.x, pr[q][k]);          \
+          pr[q][k] = fma(-x[q].y, sv.y, pr[q][k]);         \
+          pi[q][k] = fma(x[q].x, sv.y, pi[q][k]);          \
+          pi[q][k] = fma(x[q].y, sv.x, pi[q][k]);          \
+        }                                                  \
       }                                                    \
     }                                                      \
   }
@@ -327,15 +352,263 @@ gemm_tiles(int64_t D, VecPair<V> A, int nk, V* Ob, int64_t ldo, const double2* _
   }
 #undef EDX_GEMM_LOAD
 #undef EDX_GEMM_FMA
-  if (!live) return;
 #pragma unroll
-  for (int k = 0; k < NK; ++k) {
-    if (k0 + k < nk) {
-      const V o = from_d2<V>(pr[k], pi[k]);
-      Ob[(int64_t)(k0 + k) * ldo + r] = o;
-      if (Xi) Xi[r * nk + k0 + k] = o;  // interleaved copy for the next SpMM
+  for (int q = 0; q < RPT; ++q) {
+    if (rr[q] >= D) continue;
+#pragma unroll
+    for (int k = 0; k < NK; ++k)
+      if (k0 + k < nk) Ob[(int64_t)(k0 + k) * ldo + rr[q]] = from_d2<V>(pr[q][k], pi[q][k]);
+  }
+  if (!Xi) return;
+  // interleaved copy [r][k] for the next SpMM (one tile, k0 = 0): staged in
+  // LDS (when the launch gave room for it) so that the rows of the workgroup
+  // are written as one contiguous run
+  if (!stage_xi) {
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) {
+      if (rr[q] >= D) continue;
+#pragma unroll
+      for (int k = 0; k < NK; ++k)
+        if (k < nk) Xi[rr[q] * nk + k] = from_d2<V>(pr[q][k], pi[q][k]);
+    }
+    return;
+  }
+  __syncthreads();  // the coefficients are no longer read
+  V* stage = reinterpret_cast<V*>(s_tile);
+  const int64_t row0 = (int64_t)rb * 256 * RPT;
+#pragma unroll
+  for (int q = 0; q < RPT; ++q)
+#pragma unroll
+    for (int k = 0; k < NK; ++k)
+      if (k < nk) stage[(q * 256 + (int)threadIdx.x) * nk + k] = from_d2<V>(pr[q][k], pi[q][k]);
+  __syncthreads();
+  const int64_t n = (min(D, row0 + 256 * RPT) - row0) * nk;
+  for (int64_t e = threadIdx.x; e < n; e += 256) Xi[row0 * nk + e] = stage[e];
+}
+
+// ---------------------------------------------------------------------------
+// Matrix-core versions of the Gram and of the update.  Operand layouts of the
+// 16x16x4 MFMAs on gfx950 (checked with a one-wave probe): lane l holds
+// A[l % 16][l / 16] and B[l / 16][l % 16]; the f32 result tile holds
+// D[4 (l / 16) + v][l % 16] and the f64 one D[l / 16 + 4 v][l % 16],
+// v = 0..3.  The K index of a product may be mapped to any row (Gram) or input
+// (update) as long as both operands use the same map.
+// ---------------------------------------------------------------------------
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+typedef double f64x4_t __attribute__((ext_vector_type(4)));
+
+// Gram of complex64 vectors on the FP32 matrix cores: partial[rb][i*nw + j] =
+// sum over row block rb of conj(A_i[r]) W_j[r], nw <= 8.  A wave computes, for
+// MT tiles of 16 vectors of A, the real products C1 = Ar^T [Wr | Wi] and
+// C2 = Ai^T [Wr | Wi] (column j < 8: Wr_j, j >= 8: Wi_(j-8)), so that
+// Re G = C1[., j] + C2[., j + 8] and Im G = C1[., j + 8] - C2[., j].  K runs
+// over rows: in a chunk of 16 rows, lane group g takes rows 4g .. 4g + 3,
+// one per MFMA step, and loads them with two 16-byte loads per vector (the
+// four lane groups together read 128 contiguous bytes of each vector).  The
+// FP32 tiles are added into FP64 accumulators every 32 rows; the products of
+// complex64 values and these short FP32 sums err by about 2^-24 of their
+// size, so a Gram entry of unit vectors errs by ~1e-9 at D = 1.6e7, far below
+// the complex64 rounding of the vectors themselves.  Each wave takes every
+// fourth chunk of the row block; the four waves' sums are added in a fixed
+// order, so results do not depend on scheduling.  Chunks never read past
+// row ld - 1 (row blocks and chunks start at multiples of 16, ld is a
+// multiple of 64 above D), and rows [D, ld) are zero.  The 1-D grid maps all
+// tiles of a row block onto one XCD, whose L2 then serves W to all of them.
+template <int MT>
+__global__ void __launch_bounds__(256)
+gram_mfma32(int64_t D, VecPair<float2> A, const float2* __restrict__ Wb, int64_t ldw, int nw,
+            int64_t rows_per_block, int n_itiles, int n_tiles, double2* __restrict__ partial, StepPred pred) {
+  __shared__ double2 red[4][MT * 16][8];
+  if (!pred.run()) return;
+  const int per_xcd = (n_tiles + 7) / 8;
+  const int logical = (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);
+  if (logical >= n_tiles) return;
+  const int itile = logical % n_itiles, rblk = logical / n_itiles;
+  const int nv = A.n0 + A.n1;
+  const int i0 = itile * MT * 16;
+  const int64_t rb = (int64_t)rblk * rows_per_block;
+  const int64_t re = min(D, rb + rows_per_block);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int col = lane & 15, g = lane >> 4;
+  const float2* ap[MT];
+#pragma unroll
+  for (int t = 0; t < MT; ++t) ap[t] = A.vec(min(i0 + t * 16 + col, nv - 1)) + 4 * g;
+  const int wm = col & 7;
+  const bool wlive = wm < nw, wim = col >= 8;
+  const float2* wp = Wb + (int64_t)(wlive ? wm : 0) * ldw + 4 * g;
+  f32x4_t c1[MT], c2[MT];
+  double s1[MT][4], s2[MT][4];
+#pragma unroll
+  for (int t = 0; t < MT; ++t) {
+    c1[t] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+    c2[t] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int v = 0; v < 4; ++v) s1[t][v] = s2[t][v] = 0.0;
+  }
+  // chunk starts past the block's end are replaced by the block's first rows
+  // (always inside the array); their loads are discarded
+  auto safe = [&](int64_t r) { return r < re ? r : rb; };
+  float4 a0[MT][2], a1[MT][2], w0[2], w1[2];
+#define EDX_MG_LOAD(AB, WB, R)                                                        \
+  {                                                                                    \
+    const int64_t r_ = (R);                                                            \
+    _Pragma("unroll") for (int t = 0; t < MT; ++t) {                                   \
+      const float4* q = reinterpret_cast<const float4*>(ap[t] + r_);                   \
+      AB[t][0] = q[0];                                                                 \
+      AB[t][1] = q[1];                                                                 \
+    }                                                                                  \
+    const float4* qw = reinterpret_cast<const float4*>(wp + r_);                       \
+    WB[0] = qw[0];                                                                     \
+    WB[1] = qw[1];                                                                     \
+    EDX_KEEP_PREFETCH();                                                               \
+  }
+#define EDX_MG_STEP(AB, WB)                                                           \
+  {                                                                                    \
+    const float wv[4] = {wim ? WB[0].y : WB[0].x, wim ? WB[0].w : WB[0].z,            \
+                         wim ? WB[1].y : WB[1].x, wim ? WB[1].w : WB[1].z};           \
+    _Pragma("unroll") for (int t = 0; t < MT; ++t) {                                   \
+      const float ar[4] = {AB[t][0].x, AB[t][0].z, AB[t][1].x, AB[t][1].z};           \
+      const float ai[4] = {AB[t][0].y, AB[t][0].w, AB[t][1].y, AB[t][1].w};           \
+      _Pragma("unroll") for (int k = 0; k < 4; ++k) {                                  \
+        const float b = wlive ? wv[k] : 0.f;                                           \
+        c1[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar[k], b, c1[t], 0, 0, 0);         \
+        c2[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(ai[k], b, c2[t], 0, 0, 0);         \
+      }                                                                                \
+    }                                                                                  \
+  }
+  int64_t r = rb + wave * 16;
+  EDX_MG_LOAD(a0, w0, safe(r))
+  for (; r < re; r += 128) {
+    EDX_MG_LOAD(a1, w1, safe(r + 64))
+    EDX_MG_STEP(a0, w0)
+    EDX_MG_LOAD(a0, w0, safe(r + 128))
+    if (r + 64 < re) EDX_MG_STEP(a1, w1)
+#pragma unroll
+    for (int t = 0; t < MT; ++t) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        s1[t][v] += (double)c1[t][v];
+        s2[t][v] += (double)c2[t][v];
+      }
+      c1[t] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+      c2[t] = f32x4_t{0.f, 0.f, 0.f, 0.f};
     }
   }
+#undef EDX_MG_LOAD
+#undef EDX_MG_STEP
+  // lane (g, col) holds C1[4g + v][col] and C2[4g + v][col]; columns j and
+  // j + 8 sit in lanes col and col ^ 8
+#pragma unroll
+  for (int t = 0; t < MT; ++t)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const double o1 = __shfl_xor(s1[t][v], 8, 64);
+      const double o2 = __shfl_xor(s2[t][v], 8, 64);
+      if (col < 8) red[wave][t * 16 + 4 * g + v][col] = make_double2(s1[t][v] + o2, o1 - s2[t][v]);
+    }
+  __syncthreads();
+  const int nout = nv * nw;
+  for (int o = threadIdx.x; o < MT * 16 * 8; o += 256) {
+    const int i = o / 8, j = o % 8;
+    if (i0 + i < nv && j < nw) {
+      double xr = 0.0, xi = 0.0;
+      for (int wv = 0; wv < 4; ++wv) {
+        xr += red[wv][i][j].x;
+        xi += red[wv][i][j].y;
+      }
+      partial[(int64_t)rblk * nout + (int64_t)(i0 + i) * nw + j] = make_double2(xr, xi);
+    }
+  }
+}
+
+// Out_k[r] = sum_i A_i[r] S[i][k] (S row-major, leading dimension ldS) on the
+// FP64 matrix cores, FP64 accumulation as in gemm_tiles.  The complex product
+// is the real one [Xr Xi] [[Sr, Si], [-Si, Sr]]: K index (i, p) takes the real
+// (p = 0) or imaginary (p = 1) part of input i, result column (m, q) the real
+// (q = 0) or imaginary part of output m.  A workgroup owns 256 rows and a tile
+// of 8 outputs (k0 .. k0 + 7); each of its waves 4 tiles of 16 rows (four
+// independent accumulators keep the matrix pipe busy).  The tile's
+// coefficients are staged in LDS (zero past nv and nk), one 16-byte read per
+// lane and MFMA step serves the four row tiles.  Inputs past nv read input
+// nv - 1 (zero coefficient), rows past D read row D (zero padding, never
+// written) and store nothing.  Every input of a row is read before any output
+// of it is written and each row belongs to one wave, so with one output tile
+// (nk <= 8) Out may be one of the inputs; Xi (optional, one tile only) gets
+// the interleaved copy [r][k].  The grid maps the output tiles of a row block
+// onto one XCD, whose L2 then serves the block's inputs to all of them.
+template <typename V>
+__global__ void __launch_bounds__(256)
+update_mfma64(int64_t D, VecPair<V> A, int nk, V* Ob, int64_t ldo, const double2* __restrict__ S, int ldS, V* Xi,
+              int n_kt, int n_tiles, StepPred pred) {
+  extern __shared__ double2 s_coef[];  // [nvp][8]
+  constexpr int RT = 4;
+  if (!pred.run()) return;
+  if (!pred.final_update()) Xi = nullptr;
+  const int per_xcd = (n_tiles + 7) / 8;
+  const int logical = (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);
+  if (logical >= n_tiles) return;  // whole workgroup
+  const int kt = logical % n_kt, rblk = logical / n_kt;
+  const int k0 = kt * 8;
+  const int nv = A.n0 + A.n1;
+  const int nvp = (nv + 1) / 2 * 2;
+  for (int e = threadIdx.x; e < nvp * 8; e += 256) {
+    const int i = e / 8, m = e % 8;
+    s_coef[e] = (i < nv && k0 + m < nk) ? S[(int64_t)i * ldS + k0 + m] : make_double2(0.0, 0.0);
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int col = lane & 15, g = lane >> 4;
+  const int p = g & 1, m = col & 7, q = col >> 3;
+  const int64_t r0 = (int64_t)rblk * 256 + wave * 64;
+  int64_t rc[RT];
+#pragma unroll
+  for (int t = 0; t < RT; ++t) {
+    const int64_t rr = r0 + 16 * t + col;
+    rc[t] = rr < D ? rr : D;
+  }
+  f64x4_t acc[RT];
+#pragma unroll
+  for (int t = 0; t < RT; ++t) acc[t] = f64x4_t{0.0, 0.0, 0.0, 0.0};
+  const int nsteps = nvp / 2;
+  V x0[RT], x1[RT];
+#define EDX_MU_LOAD(XB, S_)                                                         \
+  {                                                                                  \
+    const V* xp = A.vec(min(2 * (S_) + (g >> 1), nv - 1));                           \
+    _Pragma("unroll") for (int t = 0; t < RT; ++t) XB[t] = xp[rc[t]];                \
+    EDX_KEEP_PREFETCH();                                                             \
+  }
+#define EDX_MU_STEP(XB, S_)                                                         \
+  {                                                                                  \
+    const double2 sc = s_coef[(2 * (S_) + (g >> 1)) * 8 + m];                        \
+    const double b = p == 0 ? (q == 0 ? sc.x : sc.y) : (q == 0 ? -sc.y : sc.x);      \
+    _Pragma("unroll") for (int t = 0; t < RT; ++t) {                                 \
+      const double2 x = to_d2(XB[t]);                                                \
+      acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(p ? x.y : x.x, b, acc[t], 0, 0, 0); \
+    }                                                                                \
+  }
+  EDX_MU_LOAD(x0, 0)
+  for (int st = 0; st < nsteps; st += 2) {
+    EDX_MU_LOAD(x1, min(st + 1, nsteps - 1))
+    EDX_MU_STEP(x0, st)
+    EDX_MU_LOAD(x0, min(st + 2, nsteps - 1))
+    if (st + 1 < nsteps) EDX_MU_STEP(x1, st + 1)
+  }
+#undef EDX_MU_LOAD
+#undef EDX_MU_STEP
+  // lane (g, col) holds result rows 16t + g + 4v, column col: the real part
+  // of output m (col < 8) or its imaginary part (col >= 8, lane col ^ 8)
+#pragma unroll
+  for (int t = 0; t < RT; ++t)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const double other = __shfl_xor(acc[t][v], 8, 64);
+      const int64_t row = r0 + 16 * t + g + 4 * v;
+      if (col < 8 && row < D && k0 + m < nk) {
+        const V o = from_d2<V>(acc[t][v], other);
+        Ob[(int64_t)(k0 + m) * ldo + row] = o;
+        if (Xi) Xi[row * nk + k0 + m] = o;
+      }
+    }
 }
 
 // Second Cholesky QR pass of a block, on the device: G (b x b, row-major,
@@ -431,7 +704,7 @@ step_coeffs(const double2* __restrict__ P, int nbasis, int b, double kappa2, dou
             double single_tol, int second_pass, double2* __restrict__ S, double2* __restrict__ R1out,
             double2* __restrict__ log, int* __restrict__ ctl) {
   __shared__ double2 G[kMaxW][kMaxW], R[kMaxW][kMaxW], X[kMaxW][kMaxW];
-  __shared__ double2 Ps[(kMaxVec + kMaxW) * kMaxW];  // P, read once with all lanes
+  __shared__ double2 Ps[kStepLds];  // P, read once with all lanes
   __shared__ int status_s;
   if (ctl[1]) return;  // halted: the host finishes an earlier step first
   const int tid = threadIdx.x;
@@ -602,6 +875,24 @@ struct Options {
   uint64_t seed = 1;
 };
 
+// Automatic Krylov basis size: 5 (nev + b) vectors; for complex64 sweeps
+// after many levels (nev >= 8) up to ratio / 4.5 vectors, with ratio = SpMM
+// bytes / bytes of one vector.  A block step's Gram and update read the basis
+// about twice, so that size keeps the per-step vector work below about half
+// of the SpMM's, and a larger basis restarts less often.  Measured (mixed,
+// eigenvalue bound 1e-12): the 48-site checkerboard sector (nev = 10, block
+// 8, ratio ~790) takes 6.7 s with 176 vectors against 7.9 s with 96 and 7.4 s
+// with 208; the 36-site Haldane sector (nev = 5, ratio ~560) is faster with
+// 50 vectors (0.75 s) than with 125 (0.85 s), and so is the FP64 phase of the
+// checkerboard solve with 100 rather than 200, so both keep 5 (nev + b).
+int auto_basis(int64_t D, int64_t E, int prec, int b, int nev) {
+  const int base = std::max({5 * (nev + b), 6 * b, 24});
+  if (prec == EDX_PREC_FP64 || nev < 8 || D <= 0) return base;
+  const double esz = 8.0;
+  const double spmm = (double)D * (esz + 2.0 * b * esz) + (double)E * (4.0 + esz + (double)b * esz);
+  return std::max(base, (int)(spmm / ((double)D * esz) / 4.5));
+}
+
 // Convergence of the nev lowest Ritz pairs.  A pair passes only when its
 // cluster (the Ritz values within 1e-4 relative of it, which may reach past
 // the nev-th) is resolved as a whole: until then the error of each of its Ritz
@@ -690,12 +981,13 @@ class BlockLanczos {
     // kernels store rows < D only), which the Gram kernel's unconditional
     // loads rely on
     ld_ = (D_ + 64) / 64 * 64;
-    int base = opt.basis > 0 ? opt.basis : std::max({5 * (opt.nev + b_), 6 * b_, 24});
+    int base = opt.basis > 0 ? opt.basis : auto_basis(h.dim, h.n_entries, prec, b_, opt.nev);
     base = (base + b_ - 1) / b_ * b_;
-    nq_max_ = std::min(base, kMaxVec - 2 * b_);
+    nq_max_ = std::min({base, kMaxVec - 2 * b_, kStepLds / b_ - b_});
     nq_max_ = std::max(nq_max_ / b_ * b_, opt.nev + b_);
     cap_ = nq_max_ + b_;
-    EDX_REQUIRE(cap_ + b_ <= kMaxVec, EDX_ERR_INVALID, "Krylov basis too large for nev/block");
+    EDX_REQUIRE(cap_ + b_ <= kMaxVec && (nq_max_ + b_) * b_ <= kStepLds, EDX_ERR_INVALID,
+                "Krylov basis too large for nev/block");
     // a single projection pass is kept while no column loses more than this
     // factor (squared) of its norm to it: the orthogonality it leaves is
     // about the unit roundoff times that factor
@@ -903,8 +1195,30 @@ class BlockLanczos {
                        rows_per_block_, n_itiles, n_tiles, partial_.p, pred);
   }
 
+  // Gram of complex64 vectors against at most 8 on the FP32 matrix cores.
+  // Tiles of 32 vectors per wave, or 16 for Grams of at most 16 vectors (the
+  // second Cholesky QR pass), whose second tile would only repeat loads.
+  void launch_gram_mfma(const void* A0, int n0, const void* A1, int n1, const void* W, int nw, StepPred pred) {
+    const int nv = n0 + n1;
+    const int MT = nv <= 16 ? 1 : 2;
+    const int n_itiles = (nv + 16 * MT - 1) / (16 * MT);
+    const int n_tiles = n_itiles * n_rowblocks_;
+    const dim3 grid((unsigned)((n_tiles + 7) / 8 * 8));
+    VecPair<float2> A{(const float2*)A0, ld_, n0, (const float2*)A1, ld_, n1};
+    if (MT == 1)
+      hipLaunchKernelGGL((gram_mfma32<1>), grid, dim3(256), 0, ctx_.stream, D_, A, (const float2*)W, ld_, nw,
+                         rows_per_block_, n_itiles, n_tiles, partial_.p, pred);
+    else
+      hipLaunchKernelGGL((gram_mfma32<2>), grid, dim3(256), 0, ctx_.stream, D_, A, (const float2*)W, ld_, nw,
+                         rows_per_block_, n_itiles, n_tiles, partial_.p, pred);
+  }
+
   template <typename VT>
   void launch_tiles_nw(const void* A0, int n0, const void* A1, int n1, const void* W, int nw, StepPred pred) {
+    if (std::is_same<VT, float2>::value && nw <= 8 && D_ >= kMfmaGramRows) {
+      launch_gram_mfma(A0, n0, A1, n1, W, nw, pred);
+      return;
+    }
     switch (nw) {
 #define EDX_GRAM_CASE(NWV, TIV) \
   case NWV:                     \
@@ -1022,30 +1336,57 @@ class BlockLanczos {
     launch_gemm(A0, n0, A1, n1, Out, nk, ds, nk, Xi);
   }
 
-  template <typename VT, int NK, int U = kGemmStage>
+  template <typename VT, int NK, int U = kGemmStage, int RPT = 1>
   void launch_gemm_t(const void* A0, int n0, const void* A1, int n1, void* Out, int nk, const double2* S, int ldS,
                      void* Xi, StepPred pred) {
     VecPair<VT> A{(const VT*)A0, ld_, n0, (const VT*)A1, ld_, n1};
     const int nv = n0 + n1;
     const int n_kt = (nk + NK - 1) / NK;
-    const int64_t n_tiles = (int64_t)n_kt * ((D_ + 255) / 256);
+    const int64_t n_tiles = (int64_t)n_kt * ((D_ + 256 * RPT - 1) / (256 * RPT));
     EDX_REQUIRE(n_tiles < (1ll << 30), EDX_ERR_INTERNAL, "update grid");
     const int nvp = (nv + 2 * U - 1) / (2 * U) * (2 * U);
-    hipLaunchKernelGGL((gemm_tiles<VT, NK, U>), dim3((unsigned)((n_tiles + 7) / 8 * 8)), dim3(256),
-                       (size_t)nvp * NK * sizeof(double2), ctx_.stream, D_, A, nk, (VT*)Out, ld_, S, ldS, (VT*)Xi,
-                       n_kt, (int)n_tiles, pred);
+    // LDS: the coefficients, and (with an interleaved copy, room permitting)
+    // the staged outputs
+    size_t lds = (size_t)nvp * NK * sizeof(double2);
+    const size_t stage = (size_t)256 * RPT * nk * sizeof(VT);
+    const int stage_xi = Xi && stage <= kStageLdsBytes ? 1 : 0;
+    if (stage_xi) lds = std::max(lds, stage);
+    hipLaunchKernelGGL((gemm_tiles<VT, NK, U, RPT>), dim3((unsigned)((n_tiles + 7) / 8 * 8)), dim3(256), lds,
+                       ctx_.stream, D_, A, nk, (VT*)Out, ld_, S, ldS, (VT*)Xi, n_kt, (int)n_tiles, stage_xi, pred);
   }
 
   // One tile of the smallest width that holds nk (nk <= 16), otherwise tiles
   // of up to 12 outputs (restart combinations: 97 us against 129 us with
   // tiles of 6 at 60 -> 33 vectors, complex64, D = 225k).
+  // Out = [A0, A1] S on the FP64 matrix cores, in tiles of 8 outputs.
+  template <typename VT>
+  void launch_update_mfma(const void* A0, int n0, const void* A1, int n1, void* Out, int nk, const double2* S,
+                          int ldS, void* Xi, StepPred pred) {
+    VecPair<VT> A{(const VT*)A0, ld_, n0, (const VT*)A1, ld_, n1};
+    const int nv = n0 + n1;
+    const int n_kt = (nk + 7) / 8;
+    const int64_t n_tiles = (int64_t)n_kt * ((D_ + 255) / 256);
+    EDX_REQUIRE(n_tiles < (1ll << 30), EDX_ERR_INTERNAL, "update grid");
+    const int nvp = (nv + 1) / 2 * 2;
+    hipLaunchKernelGGL((update_mfma64<VT>), dim3((unsigned)((n_tiles + 7) / 8 * 8)), dim3(256),
+                       (size_t)nvp * 8 * sizeof(double2), ctx_.stream, D_, A, nk, (VT*)Out, ld_, S, ldS, (VT*)Xi,
+                       n_kt, (int)n_tiles, pred);
+  }
+
   template <typename VT>
   void launch_gemm_v(const void* A0, int n0, const void* A1, int n1, void* Out, int nk, const double2* S, int ldS,
                      void* Xi, StepPred pd) {
-    if (nk <= 2) launch_gemm_t<VT, 2>(A0, n0, A1, n1, Out, nk, S, ldS, Xi, pd);
-    else if (nk <= 4) launch_gemm_t<VT, 4>(A0, n0, A1, n1, Out, nk, S, ldS, Xi, pd);
-    else if (nk <= 6) launch_gemm_t<VT, 6>(A0, n0, A1, n1, Out, nk, S, ldS, Xi, pd);
-    else if (nk <= 8) launch_gemm_t<VT, 8>(A0, n0, A1, n1, Out, nk, S, ldS, Xi, pd);
+    // one tile of at most 8 outputs (block updates, in place or not), or the
+    // many-tile restart combinations (outputs never alias the inputs)
+    if (kMfmaUpdate && (nk <= 8 || nk > 16)) {
+      launch_update_mfma<VT>(A0, n0, A1, n1, Out, nk, S, ldS, Xi, pd);
+      return;
+    }
+    constexpr int R = kUpdateRows;
+    if (nk <= 2) launch_gemm_t<VT, 2, kGemmStage, R>(A0, n0, A1, n1, Out, nk, S, ldS, Xi, pd);
+    else if (nk <= 4) launch_gemm_t<VT, 4, kGemmStage, R>(A0, n0, A1, n1, Out, nk, S, ldS, Xi, pd);
+    else if (nk <= 6) launch_gemm_t<VT, 6, kGemmStage, R>(A0, n0, A1, n1, Out, nk, S, ldS, Xi, pd);
+    else if (nk <= 8) launch_gemm_t<VT, 8, kGemmStage, R>(A0, n0, A1, n1, Out, nk, S, ldS, Xi, pd);
     else if (nk <= 12) launch_gemm_t<VT, 12>(A0, n0, A1, n1, Out, nk, S, ldS, Xi, pd);
     else if (nk <= 16) launch_gemm_t<VT, 16>(A0, n0, A1, n1, Out, nk, S, ldS, Xi, pd);
     else {
@@ -1055,10 +1396,10 @@ class BlockLanczos {
       // against 91.4 us in tiles of 12, complex64, D = 225k)
       const int n_kt = (nk + kCombineTile - 1) / kCombineTile;
       switch ((nk + n_kt - 1) / n_kt) {
-        case 9: launch_gemm_t<VT, 9, kCombineStage>(A0, n0, A1, n1, Out, nk, S, ldS, Xi, pd); break;
-        case 10: launch_gemm_t<VT, 10, kCombineStage>(A0, n0, A1, n1, Out, nk, S, ldS, Xi, pd); break;
-        case 11: launch_gemm_t<VT, 11, kCombineStage>(A0, n0, A1, n1, Out, nk, S, ldS, Xi, pd); break;
-        default: launch_gemm_t<VT, kCombineTile, kCombineStage>(A0, n0, A1, n1, Out, nk, S, ldS, Xi, pd);
+        case 9: launch_gemm_t<VT, 9, kCombineStage, R>(A0, n0, A1, n1, Out, nk, S, ldS, Xi, pd); break;
+        case 10: launch_gemm_t<VT, 10, kCombineStage, R>(A0, n0, A1, n1, Out, nk, S, ldS, Xi, pd); break;
+        case 11: launch_gemm_t<VT, 11, kCombineStage, R>(A0, n0, A1, n1, Out, nk, S, ldS, Xi, pd); break;
+        default: launch_gemm_t<VT, kCombineTile, kCombineStage, R>(A0, n0, A1, n1, Out, nk, S, ldS, Xi, pd);
       }
     }
   }
@@ -1481,7 +1822,7 @@ void edx_eigs_default_options(edx_eigs_options* o) {
   o->max_restarts = 5000;
   o->tol = 1e-10;
   o->eig_tol = 0.0;
-  o->mixed_tol = 1e-6;
+  o->mixed_tol = 3e-7;
   o->seed = 12345;
 }
 
@@ -1545,7 +1886,7 @@ void eigs_impl(edx_ctx* ctx, const edx_hamiltonian* ham, const edx_eigs_options&
       const int n32 = (int)std::min<int64_t>(std::min(nev, kMaxW), D - 2);
       Options o32 = opt;
       o32.nev = n32;
-      o32.tol = std::max(tol, o.mixed_tol > 0 ? o.mixed_tol : 1e-6);
+      o32.tol = std::max(tol, o.mixed_tol > 0 ? o.mixed_tol : 3e-7);
       o32.eig_tol = 0.0;
       o32.out_cluster = true;
       // a target below the complex64 rounding floor still hands over: the

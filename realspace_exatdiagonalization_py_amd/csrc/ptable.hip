@@ -829,6 +829,28 @@ int edx_hamiltonian_fetch_table(edx_ctx* ctx, const edx_hamiltonian* ham, int64_
   });
 }
 
+int edx_hamiltonian_fetch_csr(edx_ctx* ctx, const edx_hamiltonian* ham, int64_t row_begin, int64_t row_end,
+                              int64_t* row_ptr, int32_t* col, double* vals, double* h_diag) {
+  return edx::guarded(ctx, [&] {
+    EDX_REQUIRE(ctx && ham && row_ptr, EDX_ERR_INVALID, "null argument");
+    EDX_REQUIRE(0 <= row_begin && row_begin <= row_end && row_end <= ham->dim, EDX_ERR_INVALID,
+                "row range outside the sector");
+    edx::DeviceGuard dg(ctx->device);
+    const int64_t n = row_end - row_begin;
+    EDX_HIP(hipMemcpyAsync(row_ptr, ham->row_ptr.p + row_begin, (n + 1) * 8, hipMemcpyDeviceToHost, ctx->stream));
+    EDX_HIP(hipStreamSynchronize(ctx->stream));
+    const int64_t e0 = row_ptr[0], e1 = row_ptr[n];
+    for (int64_t r = 0; r <= n; ++r) row_ptr[r] -= e0;
+    if (col && e1 > e0)
+      EDX_HIP(hipMemcpyAsync(col, ham->col.p + e0, (e1 - e0) * 4, hipMemcpyDeviceToHost, ctx->stream));
+    if (vals && e1 > e0)
+      EDX_HIP(hipMemcpyAsync(vals, ham->val.p + e0, (e1 - e0) * 16, hipMemcpyDeviceToHost, ctx->stream));
+    if (h_diag && n)
+      EDX_HIP(hipMemcpyAsync(h_diag, ham->hdiag.p + row_begin, n * 16, hipMemcpyDeviceToHost, ctx->stream));
+    EDX_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
 namespace {
 __global__ void to_c64(int64_t n, const double2* a, float2* b) {
   const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;

@@ -571,6 +571,21 @@ class _DeviceHamiltonian:
         self.ctx.check(lib.edx_matvec(self.ctx.handle, self.handle, k, ptr(xs), ptr(ys)))
         return ys[0] if single else ys.T.copy()
 
+    def fetch_csr(self, row_begin: int = 0, row_end: int | None = None) -> tuple[sp.csr_matrix, np.ndarray]:
+        """Rows ``[row_begin, row_end)`` of the off-diagonal table as a scipy
+        CSR block (columns over the whole sector) and their diagonal."""
+        row_end = self.dim if row_end is None else int(row_end)
+        n = row_end - int(row_begin)
+        rp = np.empty(n + 1, np.int64)
+        self.ctx.check(lib.edx_hamiltonian_fetch_csr(self.ctx.handle, self.handle, int(row_begin), row_end, ptr(rp),
+                                                     None, None, None))
+        col = np.empty(int(rp[-1]), np.int32)
+        vals = np.empty(int(rp[-1]), np.complex128)
+        hd = np.empty(n, np.complex128)
+        self.ctx.check(lib.edx_hamiltonian_fetch_csr(self.ctx.handle, self.handle, int(row_begin), row_end, ptr(rp),
+                                                     ptr(col), ptr(vals), ptr(hd)))
+        return sp.csr_matrix((vals, col, rp), shape=(n, self.dim)), hd
+
     def fetch_table(self):
         E, D = self.n_entries, self.dim
         row = np.empty(E, np.int64)
