@@ -52,11 +52,9 @@ constexpr int kMaxW = edx::kMaxBlock;    // block width
 // is capped so that (nbasis + b) * b <= kStepLds (48 KiB of complex128)
 constexpr int kStepLds = 3072;
 // Gram of complex64 vectors against at most 8 on the FP32 matrix cores from
-// this many rows (below, its grid is too small to fill the device); update on
-// the FP64 matrix cores (kMfmaUpdate) or the vector-ALU kernel with
-// kUpdateRows rows per lane.
+// this many rows (below, its grid is too small to fill the device).
 constexpr int64_t kMfmaGramRows = int64_t(1) << 20;
-constexpr bool kMfmaUpdate = false;
+// Rows per lane of the update kernel (each coefficient read serves them all).
 constexpr int kUpdateRows = 2;
 // Largest LDS staging of an update's interleaved output copy (bytes).
 constexpr size_t kStageLdsBytes = 48 * 1024;
@@ -387,15 +385,13 @@ gemm_tiles(int64_t D, VecPair<V> A, int nk, V* Ob, int64_t ldo, const double2* _
 }
 
 // ---------------------------------------------------------------------------
-// Matrix-core versions of the Gram and of the update.  Operand layouts of the
-// 16x16x4 MFMAs on gfx950 (checked with a one-wave probe): lane l holds
-// A[l % 16][l / 16] and B[l / 16][l % 16]; the f32 result tile holds
-// D[4 (l / 16) + v][l % 16] and the f64 one D[l / 16 + 4 v][l % 16],
-// v = 0..3.  The K index of a product may be mapped to any row (Gram) or input
-// (update) as long as both operands use the same map.
+// Matrix-core Gram.  Operand layout of v_mfma_f32_16x16x4f32 on gfx950
+// (checked with a one-wave probe): lane l holds A[l % 16][l / 16] and
+// B[l / 16][l % 16]; the result tile holds D[4 (l / 16) + v][l % 16],
+// v = 0..3.  The K index of a product may be mapped to any row as long as
+// both operands use the same map.
 // ---------------------------------------------------------------------------
 typedef float f32x4_t __attribute__((ext_vector_type(4)));
-typedef double f64x4_t __attribute__((ext_vector_type(4)));
 
 // Gram of complex64 vectors on the FP32 matrix cores: partial[rb][i*nw + j] =
 // sum over row block rb of conj(A_i[r]) W_j[r], nw <= 8.  A wave computes, for
@@ -519,96 +515,6 @@ gram_mfma32(int64_t D, VecPair<float2> A, const float2* __restrict__ Wb, int64_t
       partial[(int64_t)rblk * nout + (int64_t)(i0 + i) * nw + j] = make_double2(xr, xi);
     }
   }
-}
-
-// Out_k[r] = sum_i A_i[r] S[i][k] (S row-major, leading dimension ldS) on the
-// FP64 matrix cores, FP64 accumulation as in gemm_tiles.  The complex product
-// is the real one [Xr Xi] [[Sr, Si], [-Si, Sr]]: K index (i, p) takes the real
-// (p = 0) or imaginary (p = 1) part of input i, result column (m, q) the real
-// (q = 0) or imaginary part of output m.  A workgroup owns 256 rows and a tile
-// of 8 outputs (k0 .. k0 + 7); each of its waves 4 tiles of 16 rows (four
-// independent accumulators keep the matrix pipe busy).  The tile's
-// coefficients are staged in LDS (zero past nv and nk), one 16-byte read per
-// lane and MFMA step serves the four row tiles.  Inputs past nv read input
-// nv - 1 (zero coefficient), rows past D read row D (zero padding, never
-// written) and store nothing.  Every input of a row is read before any output
-// of it is written and each row belongs to one wave, so with one output tile
-// (nk <= 8) Out may be one of the inputs; Xi (optional, one tile only) gets
-// the interleaved copy [r][k].  The grid maps the output tiles of a row block
-// onto one XCD, whose L2 then serves the block's inputs to all of them.
-template <typename V>
-__global__ void __launch_bounds__(256)
-update_mfma64(int64_t D, VecPair<V> A, int nk, V* Ob, int64_t ldo, const double2* __restrict__ S, int ldS, V* Xi,
-              int n_kt, int n_tiles, StepPred pred) {
-  extern __shared__ double2 s_coef[];  // [nvp][8]
-  constexpr int RT = 4;
-  if (!pred.run()) return;
-  if (!pred.final_update()) Xi = nullptr;
-  const int per_xcd = (n_tiles + 7) / 8;
-  const int logical = (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);
-  if (logical >= n_tiles) return;  // whole workgroup
-  const int kt = logical % n_kt, rblk = logical / n_kt;
-  const int k0 = kt * 8;
-  const int nv = A.n0 + A.n1;
-  const int nvp = (nv + 1) / 2 * 2;
-  for (int e = threadIdx.x; e < nvp * 8; e += 256) {
-    const int i = e / 8, m = e % 8;
-    s_coef[e] = (i < nv && k0 + m < nk) ? S[(int64_t)i * ldS + k0 + m] : make_double2(0.0, 0.0);
-  }
-  __syncthreads();
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int col = lane & 15, g = lane >> 4;
-  const int p = g & 1, m = col & 7, q = col >> 3;
-  const int64_t r0 = (int64_t)rblk * 256 + wave * 64;
-  int64_t rc[RT];
-#pragma unroll
-  for (int t = 0; t < RT; ++t) {
-    const int64_t rr = r0 + 16 * t + col;
-    rc[t] = rr < D ? rr : D;
-  }
-  f64x4_t acc[RT];
-#pragma unroll
-  for (int t = 0; t < RT; ++t) acc[t] = f64x4_t{0.0, 0.0, 0.0, 0.0};
-  const int nsteps = nvp / 2;
-  V x0[RT], x1[RT];
-#define EDX_MU_LOAD(XB, S_)                                                         \
-  {                                                                                  \
-    const V* xp = A.vec(min(2 * (S_) + (g >> 1), nv - 1));                           \
-    _Pragma("unroll") for (int t = 0; t < RT; ++t) XB[t] = xp[rc[t]];                \
-    EDX_KEEP_PREFETCH();                                                             \
-  }
-#define EDX_MU_STEP(XB, S_)                                                         \
-  {                                                                                  \
-    const double2 sc = s_coef[(2 * (S_) + (g >> 1)) * 8 + m];                        \
-    const double b = p == 0 ? (q == 0 ? sc.x : sc.y) : (q == 0 ? -sc.y : sc.x);      \
-    _Pragma("unroll") for (int t = 0; t < RT; ++t) {                                 \
-      const double2 x = to_d2(XB[t]);                                                \
-      acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(p ? x.y : x.x, b, acc[t], 0, 0, 0); \
-    }                                                                                \
-  }
-  EDX_MU_LOAD(x0, 0)
-  for (int st = 0; st < nsteps; st += 2) {
-    EDX_MU_LOAD(x1, min(st + 1, nsteps - 1))
-    EDX_MU_STEP(x0, st)
-    EDX_MU_LOAD(x0, min(st + 2, nsteps - 1))
-    if (st + 1 < nsteps) EDX_MU_STEP(x1, st + 1)
-  }
-#undef EDX_MU_LOAD
-#undef EDX_MU_STEP
-  // lane (g, col) holds result rows 16t + g + 4v, column col: the real part
-  // of output m (col < 8) or its imaginary part (col >= 8, lane col ^ 8)
-#pragma unroll
-  for (int t = 0; t < RT; ++t)
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const double other = __shfl_xor(acc[t][v], 8, 64);
-      const int64_t row = r0 + 16 * t + g + 4 * v;
-      if (col < 8 && row < D && k0 + m < nk) {
-        const V o = from_d2<V>(acc[t][v], other);
-        Ob[(int64_t)(k0 + m) * ldo + row] = o;
-        if (Xi) Xi[row * nk + k0 + m] = o;
-      }
-    }
 }
 
 // Second Cholesky QR pass of a block, on the device: G (b x b, row-major,
@@ -1358,30 +1264,9 @@ class BlockLanczos {
   // One tile of the smallest width that holds nk (nk <= 16), otherwise tiles
   // of up to 12 outputs (restart combinations: 97 us against 129 us with
   // tiles of 6 at 60 -> 33 vectors, complex64, D = 225k).
-  // Out = [A0, A1] S on the FP64 matrix cores, in tiles of 8 outputs.
-  template <typename VT>
-  void launch_update_mfma(const void* A0, int n0, const void* A1, int n1, void* Out, int nk, const double2* S,
-                          int ldS, void* Xi, StepPred pred) {
-    VecPair<VT> A{(const VT*)A0, ld_, n0, (const VT*)A1, ld_, n1};
-    const int nv = n0 + n1;
-    const int n_kt = (nk + 7) / 8;
-    const int64_t n_tiles = (int64_t)n_kt * ((D_ + 255) / 256);
-    EDX_REQUIRE(n_tiles < (1ll << 30), EDX_ERR_INTERNAL, "update grid");
-    const int nvp = (nv + 1) / 2 * 2;
-    hipLaunchKernelGGL((update_mfma64<VT>), dim3((unsigned)((n_tiles + 7) / 8 * 8)), dim3(256),
-                       (size_t)nvp * 8 * sizeof(double2), ctx_.stream, D_, A, nk, (VT*)Out, ld_, S, ldS, (VT*)Xi,
-                       n_kt, (int)n_tiles, pred);
-  }
-
   template <typename VT>
   void launch_gemm_v(const void* A0, int n0, const void* A1, int n1, void* Out, int nk, const double2* S, int ldS,
                      void* Xi, StepPred pd) {
-    // one tile of at most 8 outputs (block updates, in place or not), or the
-    // many-tile restart combinations (outputs never alias the inputs)
-    if (kMfmaUpdate && (nk <= 8 || nk > 16)) {
-      launch_update_mfma<VT>(A0, n0, A1, n1, Out, nk, S, ldS, Xi, pd);
-      return;
-    }
     constexpr int R = kUpdateRows;
     if (nk <= 2) launch_gemm_t<VT, 2, kGemmStage, R>(A0, n0, A1, n1, Out, nk, S, ldS, Xi, pd);
     else if (nk <= 4) launch_gemm_t<VT, 4, kGemmStage, R>(A0, n0, A1, n1, Out, nk, S, ldS, Xi, pd);
