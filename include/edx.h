@@ -138,6 +138,23 @@ int edx_hamiltonian_build(edx_ctx* ctx, const edx_catalog* cat, const edx_sector
                           const int32_t* i_to, const double* t, int32_t n_density,
                           const int32_t* v_i, const int32_t* v_j, const double* v,
                           edx_hamiltonian** out, int64_t* n_entries);
+/* The same operator with no stored table (replaces the reference's
+ * matrix-free product _apply_hamiltonian_kernel, ref SRE:826-923, which
+ * recomputes the elements at every call).  Keeps the terms grouped by source
+ * site, the irrep characters and the density diagonal; every SpMM walks the
+ * valid hops of each row's representative, gathers the target's orbit, group
+ * element and alpha from the catalog and forms the element as the stored
+ * build does, in FP64.  n_entries receives the entry count the stored table
+ * would hold.  Requires a catalog whose site phases are all +1 or -1
+ * (EDX_ERR_INVALID otherwise): the product uses H[r][x] = conj(H[x][r]).
+ * edx_hamiltonian_fetch_table / _fetch_csr / _set_spmv_group / _autotune
+ * return EDX_ERR_INVALID for such a Hamiltonian; edx_matvec, edx_eigs* and
+ * edx_matvec_noalloc accept it. */
+int edx_hamiltonian_build_on_the_fly(edx_ctx* ctx, const edx_catalog* cat, const edx_sector* sec,
+                                     const double* chi, int32_t n_bilinear, const int32_t* i_from,
+                                     const int32_t* i_to, const double* t, int32_t n_density,
+                                     const int32_t* v_i, const int32_t* v_j, const double* v,
+                                     edx_hamiltonian** out, int64_t* n_entries);
 /* Projection table in the reference's triplet form: row/col int64[n_entries],
  * vals complex128[n_entries] (re, im), h_diag complex128[dim].  Triplets come
  * grouped by row, sorted by (column, term) within a row.  NULL skips. */

@@ -58,6 +58,7 @@ SIGNATURES = {
     "edx_sector_fetch": (C.c_int, [P, P, P, P, P]),
     "edx_sector_destroy": (C.c_int, [P]),
     "edx_hamiltonian_build": (C.c_int, [P, P, P, P, i32, P, P, P, i32, P, P, P, PP, pi64]),
+    "edx_hamiltonian_build_on_the_fly": (C.c_int, [P, P, P, P, i32, P, P, P, i32, P, P, P, PP, pi64]),
     "edx_hamiltonian_fetch_table": (C.c_int, [P, P, P, P, P, P, P]),
     "edx_hamiltonian_fetch_csr": (C.c_int, [P, P, i64, i64, P, P, P, P]),
     "edx_hamiltonian_from_csr": (C.c_int, [P, i64, i64, P, P, P, P, PP]),

@@ -299,6 +299,7 @@ int edx_hamiltonian_set_spmv_group(edx_ctx* ctx, edx_hamiltonian* ham, int32_t l
   return edx::guarded(ctx, [&] {
     EDX_REQUIRE(ctx && ham, EDX_ERR_INVALID, "null argument");
     EDX_REQUIRE(lanes == 1 || lanes == 2 || lanes == 4, EDX_ERR_INVALID, "lanes per row must be 1, 2 or 4");
+    EDX_REQUIRE(!ham->on_the_fly, EDX_ERR_INVALID, "an on-the-fly Hamiltonian has no sliced table");
     edx::DeviceGuard dg(ctx->device);
     if (lanes != ham->sell_g) edx::build_sell(*ham, *ctx, lanes);
   });

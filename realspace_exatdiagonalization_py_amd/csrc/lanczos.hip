@@ -41,7 +41,7 @@ using edx::dense::Mat;
 namespace edx {
 void spmm(const edx_hamiltonian& h, Ctx& ctx, int precision, int nb, const void* X, void* Y, int64_t ldy);
 void interleave(hipStream_t st, int precision, int64_t dim, int nb, const void* src, int64_t ld, void* dst);
-void dense_matrix(const edx_hamiltonian& h, hipStream_t st, double2* out);
+void dense_matrix(const edx_hamiltonian& h, Ctx& ctx, double2* out);
 }  // namespace edx
 
 namespace {
@@ -1658,7 +1658,7 @@ class BlockLanczos {
 void dense_solve(edx_ctx& ctx, const edx_hamiltonian& h, int nev, double* vals, double* vecs) {
   const int64_t D = h.dim;
   DevBuf<double2> dmat((size_t)D * D);
-  edx::dense_matrix(h, ctx.stream, dmat.p);
+  edx::dense_matrix(h, ctx, dmat.p);
   std::vector<double2> hm((size_t)D * D);
   EDX_HIP(hipMemcpyAsync(hm.data(), dmat.p, hm.size() * sizeof(double2), hipMemcpyDeviceToHost, ctx.stream));
   EDX_HIP(hipStreamSynchronize(ctx.stream));

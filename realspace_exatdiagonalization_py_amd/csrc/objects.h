@@ -69,6 +69,14 @@ struct edx_hamiltonian {
   edx::ObjBuf<int32_t> sell_col;      // [sell_entries], -1 = padding
   edx::ObjBuf<double2> sell_val;      // [sell_entries]
   edx::ObjBuf<float2> sell_val32;     // [sell_entries] complex64 copy
+  // 1: no stored table; every product recomputes the elements from the terms
+  // and the catalog (the arrays below), CSR and sliced arrays are empty
+  int on_the_fly = 0;
+  edx::ObjBuf<int32_t> fly_toff;      // [n_site + 1] terms grouped by source site
+  edx::ObjBuf<uint8_t> fly_to;        // [n_bilinear] target bit, grouped order
+  edx::ObjBuf<double2> fly_amp;       // [n_bilinear] amplitude, grouped order
+  edx::ObjBuf<uint64_t> fly_between;  // [n_bilinear] Jordan-Wigner string, grouped order
+  edx::ObjBuf<double2> fly_chi;       // [EDX_MAX_GROUP] characters of the sector's irrep
 };
 
 namespace edx {

@@ -421,13 +421,192 @@ double spmm_bytes(const edx_hamiltonian& h, int precision, int nb) {
   return (double)h.dim * (s + 2.0 * nb * s) + (double)h.n_entries * (4.0 + s + nb * s);
 }
 
+// ---------------------------------------------------------------------------
+// On-the-fly product (no stored table).  Reference: _apply_hamiltonian_kernel
+// (SRE:826-923) recomputes H x from the terms and the representative tables
+// at every call.  Here lane group of row r (G lanes, each taking every G-th
+// occupied site) walks the valid hops of r's representative m, as the table
+// build does for column r: target mask nm = m ^ hop, O(N_e) colex rank of
+// nm, (orbit, g, alpha) gathered from the catalog, row x = orbit_to_basis,
+// element v(x, r) = t s_JW(m) alpha conj(chi(g)) sqrt(stab_x / stab_r) in the
+// reference's multiplication order.  That is entry H[x, r]; for catalogs with
+// site phases +-1 (alpha = +-1, the block Hermitian to FP64 rounding)
+// H[r, x] = conj(H[x, r]) summed over the hops, so y[r] = sum conj(v) X[x]
+// without atomics.  Bytes per entry: the three catalog arrays at rank(nm),
+// the row index and stabilizer order of x, and the gather of X[x].
+// ---------------------------------------------------------------------------
+struct FlyView {
+  int64_t dim;
+  int n_site, n_filled, n_terms, fermionic;
+  const uint64_t* reps;
+  const int32_t* stab;
+  const int32_t* o2b;
+  const int32_t* rep_rank;
+  const int32_t* rep_sym;
+  const float2* rep_amp;
+  const uint32_t* binom;
+  const int32_t* toff;
+  const uint8_t* to;
+  const double2* amp;
+  const uint64_t* between;
+  const double2* chi;
+};
+
+size_t fly_smem_bytes(int n_site, int n_filled, int n_terms) {
+  return (size_t)n_terms * 16 + (size_t)n_terms * 8 + (size_t)EDX_MAX_GROUP * 16 + (size_t)(n_site + 1) * 4 +
+         (size_t)(n_site + 1) * (n_filled + 1) * 4 + (size_t)n_terms + 16;
+}
+
+template <typename V, int NB, int G>
+__global__ void __launch_bounds__(256)
+spmm_fly(FlyView fv, const V* __restrict__ hdiag, const V* __restrict__ X, V* __restrict__ Y, int64_t ldy) {
+  using T = typename Real<V>::T;
+  extern __shared__ unsigned char fly_smem[];
+  double2* chi = reinterpret_cast<double2*>(fly_smem);
+  double2* amp = chi + EDX_MAX_GROUP;
+  uint64_t* betw = reinterpret_cast<uint64_t*>(amp + fv.n_terms);
+  int32_t* toff = reinterpret_cast<int32_t*>(betw + fv.n_terms);
+  const int nb = (fv.n_site + 1) * (fv.n_filled + 1);
+  uint32_t* binom = reinterpret_cast<uint32_t*>(toff + fv.n_site + 1);
+  uint8_t* to = reinterpret_cast<uint8_t*>(binom + nb);
+  for (int i = threadIdx.x; i < fv.n_terms; i += 256) {
+    amp[i] = fv.amp[i];
+    betw[i] = fv.between[i];
+    to[i] = fv.to[i];
+  }
+  for (int i = threadIdx.x; i < EDX_MAX_GROUP; i += 256) chi[i] = fv.chi[i];
+  for (int i = threadIdx.x; i <= fv.n_site; i += 256) toff[i] = fv.toff[i];
+  for (int i = threadIdx.x; i < nb; i += 256) binom[i] = fv.binom[i];
+  __syncthreads();
+  const int64_t r = ((int64_t)blockIdx.x * 256 + threadIdx.x) / G;
+  const int sub = (int)(threadIdx.x % G);
+  T ar[NB], ai[NB];
+#pragma unroll
+  for (int q = 0; q < NB; ++q) ar[q] = ai[q] = (T)0;
+  if (r < fv.dim) {
+    const int nf1 = fv.n_filled + 1;
+    const uint64_t m = fv.reps[r];
+    const double stab_r = (double)fv.stab[r];
+    uint64_t occ = m;
+    int site = 0;
+    while (occ) {
+      const int s = edxd::ctz64(occ);
+      occ &= occ - 1;
+      if ((site++ % G) != sub) continue;
+      const int q1 = toff[s + 1];
+      for (int q = toff[s]; q < q1; ++q) {
+        const int tb = to[q];
+        if ((m >> tb) & 1ull) continue;
+        const uint64_t nm = m ^ (1ull << s) ^ (1ull << tb);
+        const uint64_t idx = edxd::rank_colex(nm, binom, nf1);
+        const int32_t x = fv.o2b[fv.rep_rank[idx]];
+        if (x < 0) continue;
+        const int g = fv.rep_sym[idx];
+        const float2 a32 = fv.rep_amp[idx];
+        const double2 a = make_double2((double)a32.x, (double)a32.y);
+        const double sj = (fv.fermionic && (edxd::popc64(m & betw[q]) & 1)) ? -1.0 : 1.0;
+        const double2 t = amp[q];
+        double2 h = make_double2(t.x * sj - t.y * 0.0, t.x * 0.0 + t.y * sj);
+        h = cmul(h, a);
+        h = cmul(h, make_double2(chi[g].x, -chi[g].y));
+        const double qs = sqrt((double)fv.stab[x] / stab_r);
+        h = make_double2(h.x * qs - h.y * 0.0, h.x * 0.0 + h.y * qs);
+        const T vr = (T)h.x, vi = (T)-h.y;  // conj(v)
+        const V* xp = X + (int64_t)x * NB;
+#pragma unroll
+        for (int q2 = 0; q2 < NB; ++q2) {
+          const V xv = xp[q2];
+          ar[q2] = fma(vr, xv.x, ar[q2]);
+          ar[q2] = fma(-vi, xv.y, ar[q2]);
+          ai[q2] = fma(vr, xv.y, ai[q2]);
+          ai[q2] = fma(vi, xv.x, ai[q2]);
+        }
+      }
+    }
+  }
+  if (G > 1) {
+#pragma unroll
+    for (int sh = G / 2; sh >= 1; sh >>= 1)
+#pragma unroll
+      for (int q = 0; q < NB; ++q) {
+        ar[q] += __shfl_xor(ar[q], sh, G);
+        ai[q] += __shfl_xor(ai[q], sh, G);
+      }
+  }
+  if (sub == 0 && r < fv.dim) {
+    const V d = hdiag[r];
+    const V* xr = X + r * NB;
+#pragma unroll
+    for (int q = 0; q < NB; ++q) {
+      const V xv = xr[q];
+      V out;
+      out.x = fma(-d.y, xv.y, fma(d.x, xv.x, ar[q]));
+      out.y = fma(d.y, xv.x, fma(d.x, xv.y, ai[q]));
+      Y[(int64_t)q * ldy + r] = out;
+    }
+  }
+}
+
+constexpr int kFlyLanes = 4;
+
+FlyView fly_view(const edx_hamiltonian& h) {
+  const edx_sector& sec = *h.sec;
+  const edx_catalog& cat = *sec.cat;
+  return FlyView{h.dim, cat.n_site, cat.n_filled, h.n_bilinear, cat.fermionic, sec.reps.p, sec.stab_order.p,
+                 sec.orbit_to_basis.p, cat.rep_rank.p, cat.rep_sym.p, cat.rep_amp.p, cat.binom.p, h.fly_toff.p,
+                 h.fly_to.p, h.fly_amp.p, h.fly_between.p, h.fly_chi.p};
+}
+
+template <typename V, int NB>
+void launch_fly_nb(const edx_hamiltonian& h, hipStream_t st, const V* hd, const V* X, V* Y, int64_t ldy) {
+  const FlyView fv = fly_view(h);
+  const size_t smem = fly_smem_bytes(fv.n_site, fv.n_filled, fv.n_terms);
+  hipLaunchKernelGGL((spmm_fly<V, NB, kFlyLanes>), dim3(grid_for((size_t)h.dim * kFlyLanes, 256)), dim3(256), smem,
+                     st, fv, hd, X, Y, ldy);
+  EDX_HIP(hipGetLastError());
+}
+
+template <typename V>
+void launch_fly(const edx_hamiltonian& h, hipStream_t st, int nb, const V* hd, const V* X, V* Y, int64_t ldy) {
+  switch (nb) {
+#define EDX_FLY_CASE(N) \
+  case N:               \
+    launch_fly_nb<V, N>(h, st, hd, X, Y, ldy); \
+    break;
+    EDX_FLY_CASE(1) EDX_FLY_CASE(2) EDX_FLY_CASE(3) EDX_FLY_CASE(4) EDX_FLY_CASE(5) EDX_FLY_CASE(6)
+    EDX_FLY_CASE(7) EDX_FLY_CASE(8) EDX_FLY_CASE(9) EDX_FLY_CASE(10) EDX_FLY_CASE(11) EDX_FLY_CASE(12)
+    EDX_FLY_CASE(13) EDX_FLY_CASE(14) EDX_FLY_CASE(15) EDX_FLY_CASE(16)
+#undef EDX_FLY_CASE
+    default:
+      throw Error{EDX_ERR_INTERNAL, "SpMM block width out of range"};
+  }
+}
+
+// Algorithmic bytes of one on-the-fly launch: per row its mask, stabilizer
+// order, diagonal, own x entries and y writes; per entry the three catalog
+// arrays at the target's rank (16 B), its orbit's sector row and stabilizer
+// order (8 B) and the gather of nb x values.
+double fly_bytes(const edx_hamiltonian& h, int precision, int nb) {
+  const double s = precision == EDX_PREC_FP64 ? 16.0 : 8.0;
+  return (double)h.dim * (8.0 + 4.0 + s + 2.0 * nb * s) + (double)h.n_entries * (16.0 + 8.0 + nb * s);
+}
+
 // Applies H to nb <= 16 interleaved vectors.  precision selects the
-// complex128 or complex64 element copy.
+// complex128 or complex64 element copy (on-the-fly tables: the vectors'
+// precision, elements formed in FP64).
 void spmm(const edx_hamiltonian& h, Ctx& ctx, int precision, int nb, const void* X, void* Y, int64_t ldy) {
   if (h.dim == 0) return;
   EDX_REQUIRE(nb >= 1 && nb <= 16, EDX_ERR_INVALID, "at most 16 vectors per SpMM");
   hipStream_t st = ctx.stream;
   hipEvent_t t0 = ctx.timer.start(st);
+  if (h.on_the_fly) {
+    if (precision == EDX_PREC_FP64)
+      launch_fly<double2>(h, st, nb, h.hdiag.p, (const double2*)X, (double2*)Y, ldy);
+    else
+      launch_fly<float2>(h, st, nb, h.hdiag32.p, (const float2*)X, (float2*)Y, ldy);
+    ctx.timer.stop(st, t0, precision == EDX_PREC_FP64 ? K_SPMM : K_SPMM32, fly_bytes(h, precision, nb));
+    return;
+  }
   if (precision == EDX_PREC_FP64)
     launch_sell<double2>(h, st, nb, h.sell_val.p, h.hdiag.p, (const double2*)X, (double2*)Y, ldy);
   else
@@ -632,7 +811,40 @@ __global__ void dense_kernel(int64_t dim, const int64_t* row_ptr, const int32_t*
   o[r].y = o[r].y + hdiag[r].y;
 }
 
-void dense_matrix(const edx_hamiltonian& h, hipStream_t st, double2* out) {
+__global__ void unit_block(int64_t dim, int nb, int64_t j0, double2* X) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= dim * nb) return;
+  const int64_t r = i / nb, q = i % nb;
+  X[i] = make_double2(r == j0 + q ? 1.0 : 0.0, 0.0);
+}
+
+__global__ void conj_inplace(int64_t n, double2* a) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) a[i].y = -a[i].y;
+}
+
+// Dense block of an on-the-fly Hamiltonian: H applied to 16 unit vectors at a
+// time gives columns c of H, written as rows c of out; conjugated, out[c][r]
+// = conj(H[r][c]) = H[c][r] (the block is Hermitian).
+void dense_matrix_fly(const edx_hamiltonian& h, Ctx& ctx, double2* out) {
+  const int64_t D = h.dim;
+  DevBuf<double2> X((size_t)D * 16);
+  for (int64_t j0 = 0; j0 < D; j0 += 16) {
+    const int nb = (int)std::min<int64_t>(16, D - j0);
+    hipLaunchKernelGGL(unit_block, dim3(grid_for((size_t)D * nb, 256)), dim3(256), 0, ctx.stream, D, nb, j0, X.p);
+    EDX_HIP(hipGetLastError());
+    spmm(h, ctx, EDX_PREC_FP64, nb, X.p, out + j0 * D, D);
+  }
+  hipLaunchKernelGGL(conj_inplace, dim3(grid_for((size_t)D * D, 256)), dim3(256), 0, ctx.stream, D * D, out);
+  EDX_HIP(hipGetLastError());
+}
+
+void dense_matrix(const edx_hamiltonian& h, Ctx& ctx, double2* out) {
+  if (h.on_the_fly) {
+    dense_matrix_fly(h, ctx, out);
+    return;
+  }
+  hipStream_t st = ctx.stream;
   EDX_HIP(hipMemsetAsync(out, 0, (size_t)h.dim * h.dim * sizeof(double2), st));
   hipLaunchKernelGGL(dense_kernel, dim3(grid_for(h.dim, 256)), dim3(256), 0, st, h.dim,
                      (const int64_t*)h.row_ptr.p, (const int32_t*)h.col.p, (const double2*)h.val.p,
@@ -642,14 +854,20 @@ void dense_matrix(const edx_hamiltonian& h, hipStream_t st, double2* out) {
 
 }  // namespace edx
 
-extern "C" {
+namespace {
 
-int edx_hamiltonian_build(edx_ctx* ctx, const edx_catalog* cat, const edx_sector* sec, const double* chi,
-                          int32_t n_bilinear, const int32_t* i_from, const int32_t* i_to, const double* t,
-                          int32_t n_density, const int32_t* v_i, const int32_t* v_j, const double* v,
-                          edx_hamiltonian** out, int64_t* n_entries) {
+// Shared body of the stored-table and on-the-fly builds.  fly = 1 keeps the
+// grouped term arrays, the characters and the diagonal, counts the entries
+// and stores no table.
+int build_hamiltonian(edx_ctx* ctx, const edx_catalog* cat, const edx_sector* sec, const double* chi,
+                      int32_t n_bilinear, const int32_t* i_from, const int32_t* i_to, const double* t,
+                      int32_t n_density, const int32_t* v_i, const int32_t* v_j, const double* v,
+                      edx_hamiltonian** out, int64_t* n_entries, int fly) {
   return edx::guarded(ctx, [&] {
     EDX_REQUIRE(ctx && cat && sec && chi && out, EDX_ERR_INVALID, "null argument");
+    EDX_REQUIRE(!fly || cat->exact_alpha, EDX_ERR_INVALID,
+                "on-the-fly products need a catalog whose site phases are all +1 or -1 (the block is then "
+                "Hermitian); build a stored table for twisted catalogs");
     EDX_REQUIRE(sec->cat == cat, EDX_ERR_INVALID, "sector was built from a different catalog");
     EDX_REQUIRE(n_bilinear >= 0 && n_bilinear <= EDX_MAX_TERMS, EDX_ERR_INVALID, "too many bilinear terms");
     EDX_REQUIRE(n_density >= 0, EDX_ERR_INVALID, "negative density term count");
@@ -733,11 +951,50 @@ int edx_hamiltonian_build(edx_ctx* ctx, const edx_catalog* cat, const edx_sector
       ham->dim = D;
       ham->n_bilinear = T;
       ham->herm_part = cat->exact_alpha ? 0 : 1;
-      ham->row_ptr.alloc(D + 1);
       ham->hdiag.alloc(D);
       ham->hdiag32.alloc(D);
       const size_t smem = terms_smem_bytes(N, cat->n_filled, T);
       int64_t E = 0;
+      if (fly) {
+        ham->on_the_fly = 1;
+        ham->sell_g = 0;
+        EDX_REQUIRE(edx::fly_smem_bytes(N, cat->n_filled, T) <= 65536, EDX_ERR_INVALID,
+                    "too many bilinear terms for the on-the-fly product's shared-memory term list");
+        std::vector<double2> amp_g(std::max(T, 1));
+        std::vector<uint64_t> betw_g(std::max(T, 1));
+        for (size_t q = 0; q < tk.size(); ++q) {
+          amp_g[q] = make_double2(t[2 * tk[q]], t[2 * tk[q] + 1]);
+          betw_g[q] = betw[tk[q]];
+        }
+        ham->fly_toff.alloc(N + 1);
+        ham->fly_to.alloc(std::max(T, 1));
+        ham->fly_amp.alloc(std::max(T, 1));
+        ham->fly_between.alloc(std::max(T, 1));
+        ham->fly_chi.alloc(EDX_MAX_GROUP);
+        h2d(ham->fly_toff.p, toff.data(), toff.size() * 4);
+        h2d(ham->fly_to.p, tto.data(), tto.size());
+        h2d(ham->fly_amp.p, amp_g.data(), (size_t)T * 16);
+        h2d(ham->fly_between.p, betw_g.data(), (size_t)T * 8);
+        h2d(ham->fly_chi.p, chi_pad.data(), chi_pad.size() * 8);
+        if (D > 0) {
+          DevBuf<uint32_t> cnt(D);
+          DevBuf<int64_t> rp(D + 1);
+          EDX_HIP(hipMemsetAsync(cnt.p, 0, D * 4, st));
+          hipLaunchKernelGGL(ptab_count, dim3(edx::grid_for(D, kBlock)), dim3(kBlock), smem, st, tv, sv, cnt.p);
+          EDX_HIP(hipGetLastError());
+          E = (int64_t)edx::exclusive_scan_u32_to_i64(*ctx, cnt.p, rp.p, (size_t)D);
+          hipLaunchKernelGGL(ptab_diag, dim3(edx::grid_for(D, kBlock)), dim3(kBlock), 0, st, D,
+                             (const uint64_t*)sec->reps.p, n_density, (const uint8_t*)d_vi.p,
+                             (const uint8_t*)d_vj.p, (const double2*)d_v.p, ham->hdiag.p, ham->hdiag32.p);
+          EDX_HIP(hipGetLastError());
+        }
+        EDX_HIP(hipStreamSynchronize(st));
+        ham->n_entries = E;
+        *out = ham;
+        if (n_entries) *n_entries = E;
+        return;
+      }
+      ham->row_ptr.alloc(D + 1);
       if (D > 0) {
         DevBuf<uint32_t> cnt(D), rmax(1);
         EDX_HIP(hipMemsetAsync(cnt.p, 0, D * 4, st));
@@ -801,10 +1058,32 @@ int edx_hamiltonian_build(edx_ctx* ctx, const edx_catalog* cat, const edx_sector
   });
 }
 
+}  // namespace
+
+extern "C" {
+
+int edx_hamiltonian_build(edx_ctx* ctx, const edx_catalog* cat, const edx_sector* sec, const double* chi,
+                          int32_t n_bilinear, const int32_t* i_from, const int32_t* i_to, const double* t,
+                          int32_t n_density, const int32_t* v_i, const int32_t* v_j, const double* v,
+                          edx_hamiltonian** out, int64_t* n_entries) {
+  return build_hamiltonian(ctx, cat, sec, chi, n_bilinear, i_from, i_to, t, n_density, v_i, v_j, v, out,
+                           n_entries, 0);
+}
+
+int edx_hamiltonian_build_on_the_fly(edx_ctx* ctx, const edx_catalog* cat, const edx_sector* sec,
+                                     const double* chi, int32_t n_bilinear, const int32_t* i_from,
+                                     const int32_t* i_to, const double* t, int32_t n_density, const int32_t* v_i,
+                                     const int32_t* v_j, const double* v, edx_hamiltonian** out,
+                                     int64_t* n_entries) {
+  return build_hamiltonian(ctx, cat, sec, chi, n_bilinear, i_from, i_to, t, n_density, v_i, v_j, v, out,
+                           n_entries, 1);
+}
+
 int edx_hamiltonian_fetch_table(edx_ctx* ctx, const edx_hamiltonian* ham, int64_t* row, int64_t* col,
                                 int32_t* term, double* vals, double* h_diag) {
   return edx::guarded(ctx, [&] {
     EDX_REQUIRE(ctx && ham, EDX_ERR_INVALID, "null argument");
+    EDX_REQUIRE(!ham->on_the_fly, EDX_ERR_INVALID, "an on-the-fly Hamiltonian stores no table");
     edx::DeviceGuard dg(ctx->device);
     const int64_t E = ham->n_entries, D = ham->dim;
     std::vector<int64_t> rp(D + 1);
@@ -833,6 +1112,7 @@ int edx_hamiltonian_fetch_csr(edx_ctx* ctx, const edx_hamiltonian* ham, int64_t 
                               int64_t* row_ptr, int32_t* col, double* vals, double* h_diag) {
   return edx::guarded(ctx, [&] {
     EDX_REQUIRE(ctx && ham && row_ptr, EDX_ERR_INVALID, "null argument");
+    EDX_REQUIRE(!ham->on_the_fly, EDX_ERR_INVALID, "an on-the-fly Hamiltonian stores no table");
     EDX_REQUIRE(0 <= row_begin && row_begin <= row_end && row_end <= ham->dim, EDX_ERR_INVALID,
                 "row range outside the sector");
     edx::DeviceGuard dg(ctx->device);

@@ -239,6 +239,7 @@ int edx_hamiltonian_autotune(edx_ctx* ctx, edx_hamiltonian* ham, int32_t precisi
     EDX_REQUIRE(precision == EDX_PREC_FP64 || precision == EDX_PREC_MIXED, EDX_ERR_INVALID, "unknown precision");
     EDX_REQUIRE(nb >= 1 && nb <= 16, EDX_ERR_INVALID, "block width must be 1..16");
     EDX_REQUIRE(ham->dim > 0, EDX_ERR_INVALID, "empty Hamiltonian");
+    EDX_REQUIRE(!ham->on_the_fly, EDX_ERR_INVALID, "an on-the-fly Hamiltonian has no sliced table to tune");
     edx::DeviceGuard dg(ctx->device);
     const edx::TuneEntry e = edx::tune_spmm(*ham, *ctx, precision, nb);
     if (lanes) *lanes = e.lanes;

@@ -536,10 +536,11 @@ def _term_arrays(terms):
 
 
 class _DeviceHamiltonian:
-    def __init__(self, ctx: Context, handle, keep: tuple):
+    def __init__(self, ctx: Context, handle, keep: tuple, on_the_fly: bool = False):
         self.ctx = ctx
         self.handle = handle
         self.keep = keep  # sector / catalog handles the table refers to
+        self.on_the_fly = on_the_fly  # no stored table: every product recomputes the elements
         d, e, g = _edx.C.c_int64(), _edx.C.c_int64(), _edx.C.c_int32()
         lib.edx_hamiltonian_info(handle, _edx.C.byref(d), _edx.C.byref(e), _edx.C.byref(g))
         self.dim, self.n_entries, self.spmv_group = d.value, e.value, g.value
@@ -598,19 +599,70 @@ class _DeviceHamiltonian:
         return row, col, term, vals, hd
 
 
+TABLE_MODES = ("stored", "on_the_fly", "auto")
+
+# Device bytes per stored entry of the table and its sliced copy (column,
+# term index, complex128 value; sliced column, complex128 and complex64
+# values with padding), plus the build's sort keys.
+_STORED_BYTES_PER_ENTRY = 4 + 2 + 16 + 1.15 * (4 + 16 + 8) + 8
+# Share of free device memory a stored table may take under table="auto".
+_AUTO_TABLE_SHARE = 0.6
+
+
+def _phases_exact(symmetry_group) -> bool:
+    """True when every site phase is exactly +1 or -1: alpha is then +-1 and the
+    sector block Hermitian to FP64 rounding (the on-the-fly product's condition)."""
+    _perm, phases, _angles = _group_inputs(symmetry_group)
+    p = np.asarray(phases, dtype=np.complex128)
+    return bool(np.all(p.imag == 0.0) and np.all(np.abs(p.real) == 1.0))
+
+
 def _build_device_hamiltonian(basis: Symmetry_Sector_Basis, bilinear_terms, density_terms,
-                              cmap: CanonicalMap | None = None) -> _DeviceHamiltonian:
+                              cmap: CanonicalMap | None = None, table: str = "stored") -> _DeviceHamiltonian:
+    """Sector Hamiltonian on the device.
+
+    ``table``: ``"stored"`` keeps the projection table (SRE:1719-1769) in device
+    memory; ``"on_the_fly"`` stores none and recomputes every element at each
+    product, as the reference's matrix-free kernel does (SRE:826-923; catalogs
+    whose site phases are all +1 or -1 only); ``"auto"`` stores the table when
+    it fits in 60 % of the free device memory and goes on the fly otherwise
+    (stored for twisted catalogs)."""
+    if table not in TABLE_MODES:
+        raise ValueError(f"table must be one of {TABLE_MODES}, got {table!r}")
     cat = basis.catalog
     ctx = _edx.current_context(cat._dev.ctx)
     fi, ti, t = _term_arrays(bilinear_terms)
     vi, vj, v = _term_arrays(density_terms)
     chi = np.ascontiguousarray(np.asarray(basis.irrep.values, dtype=np.complex128))
-    h = _edx.C.c_void_p()
-    e = _edx.C.c_int64()
-    ctx.check(lib.edx_hamiltonian_build(ctx.handle, cat._dev.handle, basis._dev.handle, ptr(chi), fi.shape[0],
-                                        ptr(fi), ptr(ti), ptr(t), vi.shape[0], ptr(vi), ptr(vj), ptr(v),
-                                        _edx.C.byref(h), _edx.C.byref(e)))
-    return _DeviceHamiltonian(ctx, h, (basis._dev, cat._dev))
+    keep = (basis._dev, cat._dev)
+
+    def build(fn, fly):
+        h = _edx.C.c_void_p()
+        e = _edx.C.c_int64()
+        ctx.check(fn(ctx.handle, cat._dev.handle, basis._dev.handle, ptr(chi), fi.shape[0], ptr(fi), ptr(ti),
+                     ptr(t), vi.shape[0], ptr(vi), ptr(vj), ptr(v), _edx.C.byref(h), _edx.C.byref(e)))
+        return _DeviceHamiltonian(ctx, h, keep, on_the_fly=fly)
+
+    if table == "auto":
+        if not _phases_exact(cat.symmetry_group):
+            table = "stored"
+        else:
+            free, _total = _edx.mem_info(ctx)
+            budget = _AUTO_TABLE_SHARE * free
+            # upper bound on the entries: every occupied site hops along each of its terms
+            per_site = np.bincount(fi - 1, minlength=cat.n_site) if fi.size else np.zeros(1, np.int64)
+            bound = basis.dim * min(cat.n_filled, cat.n_site) * int(per_site.max(initial=0))
+            if bound * _STORED_BYTES_PER_ENTRY <= budget:
+                table = "stored"
+            else:
+                fly = build(lib.edx_hamiltonian_build_on_the_fly, True)
+                if fly.n_entries * _STORED_BYTES_PER_ENTRY > budget:
+                    return fly
+                del fly
+                table = "stored"
+    if table == "on_the_fly":
+        return build(lib.edx_hamiltonian_build_on_the_fly, True)
+    return build(lib.edx_hamiltonian_build, False)
 
 
 def build_matrixfree_projection_table(basis: Symmetry_Sector_Basis, bilinear_terms, density_terms,
@@ -624,14 +676,20 @@ def build_matrixfree_projection_table(basis: Symmetry_Sector_Basis, bilinear_ter
 
 
 class MatrixFreeHamiltonian:
-    """Callable ``H(x) -> y`` over the device-resident projection table (SRE:1884-1976)."""
+    """Callable ``H(x) -> y`` over the device-resident sector Hamiltonian (SRE:1884-1976).
 
-    def __init__(self, basis: Symmetry_Sector_Basis, bilinear_terms, density_terms, cmap: CanonicalMap):
+    ``table`` chooses between the stored projection table and on-the-fly
+    elements (see ``_build_device_hamiltonian``); ``self.on_the_fly`` tells
+    which one was built."""
+
+    def __init__(self, basis: Symmetry_Sector_Basis, bilinear_terms, density_terms, cmap: CanonicalMap,
+                 table: str = "stored"):
         self.basis = basis
         self.bilinear_terms = [(i, j, complex(t)) for (i, j, t) in bilinear_terms]
         self.density_terms = [(i, j, complex(v)) for (i, j, v) in density_terms]
         self.cmap = cmap
-        self._dev = _build_device_hamiltonian(basis, self.bilinear_terms, self.density_terms, cmap)
+        self._dev = _build_device_hamiltonian(basis, self.bilinear_terms, self.density_terms, cmap, table=table)
+        self.on_the_fly = self._dev.on_the_fly
         self.n = self._dev.dim
 
     @property
@@ -659,9 +717,9 @@ def apply_hamiltonian(y: np.ndarray, x: np.ndarray, basis, bilinear_terms, densi
                                           np.zeros((1, len(x)), dtype=np.complex128))
 
 
-def hamiltonian_linear_operator(basis, bilinear_terms, density_terms, cmap):
+def hamiltonian_linear_operator(basis, bilinear_terms, density_terms, cmap, table: str = "stored"):
     """``(H_op, n)`` (SRE:2072-2094)."""
-    H = MatrixFreeHamiltonian(basis, bilinear_terms, density_terms, cmap)
+    H = MatrixFreeHamiltonian(basis, bilinear_terms, density_terms, cmap, table=table)
     return H, H.n
 
 
@@ -823,8 +881,9 @@ def ed_scan_at_irrep_matrix(irrep_label, ed_data: Symmetry_Resolved_ED_Data, *, 
 
 
 def ed_scan_at_irrep_matrixfree(irrep_label, ed_data: Symmetry_Resolved_ED_Data, *, nev: int = 5,
-                                precision: int = _edx.PREC_FP64):
-    """Matrix-free sector solve (SRE:2402-2459) on the device."""
+                                precision: int = _edx.PREC_FP64, table: str = "auto"):
+    """Matrix-free sector solve (SRE:2402-2459) on the device; ``table`` as for
+    ``MatrixFreeHamiltonian`` (default: stored when it fits, else on the fly)."""
     irrep_idx = _find_irrep_idx(ed_data, irrep_label)
     irrep = ed_data.irrep_list[irrep_idx]
     stats = ed_data.second_quantized_model.particle_statistics
@@ -840,7 +899,8 @@ def ed_scan_at_irrep_matrixfree(irrep_label, ed_data: Symmetry_Resolved_ED_Data,
     t0 = time.perf_counter()
     model = ed_data.second_quantized_model
     cmap = CanonicalMap(ed_data.symmetry_group, stats, ed_data.orbit_catalog)
-    H_op, n = hamiltonian_linear_operator(basis, model.bilinear_terms, model.density_density_terms, cmap)
+    H_op, n = hamiltonian_linear_operator(basis, model.bilinear_terms, model.density_density_terms, cmap,
+                                          table=table)
     vals, vecs = diagonalize_block_matrixfree(H_op, n, nev=nev, precision=precision)
     if _verbose():
         print(f"Done. t={round(time.perf_counter() - t0, 3)}s")

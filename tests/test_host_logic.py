@@ -162,3 +162,16 @@ def test_more_than_63_sites_is_refused_before_any_device_work():
     with pytest.raises(ValueError, match="64-bit"):
         ed.build_symmetry_orbit_catalog(second_quantized_model=model, n_filled=32, symmetry_group=G,
                                         particle_statistics=model.particle_statistics)
+
+
+def test_table_mode_host_rules():
+    """The on-the-fly product's catalog condition (every site phase exactly +1
+    or -1, as the library checks) and the table-mode argument, on the host."""
+    from cases import make_case
+
+    for name, exact in (("hub24", True), ("heis16", True), ("hal34", True), ("hub24_twist", False),
+                        ("cb34_twist", False)):
+        _m, _ne, G = make_case(name)
+        assert sre._phases_exact(G) is exact, name
+    with pytest.raises(ValueError, match="table must be one of"):
+        sre._build_device_hamiltonian(None, [], [], None, table="dense")
