@@ -285,20 +285,29 @@ def _traffic_entry(path: Path, kernel: str):
         return None
 
 
-def cpu_baseline_checkerboard(ed, H48, n_mv48, nev, row_fraction=16):
-    """The reference method on a bounded sample, extrapolated to one sector of
-    the 48-site workload (which the reference cannot build: its rank tables
-    stop at 42 sites, and the oracle's numpy catalog of 3.8e8 masks alone
-    takes about ten minutes):
+def _nearest_level_dE(w_dev, w_ref):
+    """Largest distance from a device level to the nearest oracle level (ARPACK
+    may return a degenerate level with fewer copies, then levels beyond)."""
+    w_dev, w_ref = np.asarray(w_dev, dtype=float), np.asarray(w_ref, dtype=float)
+    return float(max(np.min(np.abs(w_ref - x)) for x in w_dev)) if len(w_dev) and len(w_ref) else 0.0
 
-    1. the oracle's full solve of sector k=(0,0) of the 3x4 checkerboard
-       (same couplings, 8 fermions, D ~ 6.1e4): catalog, projection table,
-       CSR, eigs(SR, tol 1e-10, seeded start vector), which gives the catalog
-       time per mask, the table time per stored entry and ARPACK's time
-       outside the matvecs per matvec and row;
+
+def cpu_baseline_extrapolated(ed, small, H_big, n_mv_big, nev, n_masks_big, label, eig_tol, row_fraction=16):
+    """The reference method on a bounded sample, extrapolated to one sector of
+    a workload too large for it (the reference's rank tables stop at 42
+    sites, and the oracle's numpy catalog and table of 1e8-1e9 entries take
+    far longer than a sample should):
+
+    1. the oracle's full solve of one sector of the same family at a smaller
+       size (`small` = (model, n_filled, group, irrep), same couplings and
+       twist): catalog, projection table, CSR, eigs(SR, tol 1e-10, seeded
+       start vector), which gives the catalog time per mask, the table time
+       per stored entry and ARPACK's time outside the matvecs per matvec and
+       row; the device solves the same small sector (mixed precision, the
+       run's stopping rule) and its levels are compared with the oracle's;
     2. scipy CSR matvecs (the reference's MatrixFreeHamiltonian.__call__) on a
-       contiguous 1/16 of the rows of the 48-site sector's own table, built
-       on the device and fetched (columns over the whole sector);
+       contiguous 1/16 of the rows of the large sector's own table, built on
+       the device and fetched (columns over the whole sector);
     3. the sector solve = table time (rate 1) + n_mv x (matvec time of 2 scaled
        to all rows + ARPACK's other time of 1 scaled to D), with n_mv the
        matvec count of the device solve of the same sector (complex64 and
@@ -306,19 +315,18 @@ def cpu_baseline_checkerboard(ed, H48, n_mv48, nev, row_fraction=16):
     import scipy.sparse.linalg as spla
 
     from oracle import ed_oracle as O
-    from realspace_exatdiagonalization_py_amd.groups import build_irrep_list, build_translation_group, group_arrays
+    from realspace_exatdiagonalization_py_amd.groups import group_arrays
 
-    model = checkerboard_model((3, 4))
-    G = build_translation_group(model.lattice)
+    model, ne, G, irrep = small
+    ferm = model.particle_statistics.name == "FERMIONIC"
     perm, ph = group_arrays(G)
     t0 = time.perf_counter()
-    oc = O.build_catalog(model.lattice.n_site, 8, perm, ph, True)
+    oc = O.build_catalog(model.lattice.n_site, ne, perm, ph, ferm)
     t_cat = time.perf_counter() - t0
-    irrep = build_irrep_list(G, model.lattice)[0]
     t0 = time.perf_counter()
     reps, stab, o2b = O.sector_basis(oc, irrep.values)
     r, c, v, hd = O.projection_table(oc, reps, stab, o2b, model.bilinear_terms, model.density_density_terms,
-                                     irrep.values, True)
+                                     irrep.values, ferm)
     Hs = O.csr_hamiltonian(r, c, v, hd, len(reps))
     t_table = time.perf_counter() - t0
     n_calls = [0]
@@ -336,13 +344,23 @@ def cpu_baseline_checkerboard(ed, H48, n_mv48, nev, row_fraction=16):
     t0 = time.perf_counter()
     w_small, _ = spla.eigs(op, k=nev, which="SR", v0=x, tol=1e-10)
     t_eigs = time.perf_counter() - t0
+    w_small = np.sort(w_small.real)
     D_s, E_s, C_s = len(reps), len(r), len(oc.rep_rank)
     other_per_mv_row = max(t_eigs - n_calls[0] * t_mv_small, 0.0) / (n_calls[0] * D_s)
     del Hs, oc, r, c, v
-    # the 48-site sector's own CSR, a contiguous slice of its rows
-    D, E = H48._dev.dim, H48._dev.n_entries
+    # the same small sector on the device, against the oracle
+    cat_d = ed.build_symmetry_orbit_catalog(second_quantized_model=model, n_filled=ne, symmetry_group=G,
+                                            particle_statistics=model.particle_statistics)
+    basis_d = ed.build_symmetry_sector_basis(cat_d, irrep)
+    Hd = ed.MatrixFreeHamiltonian(basis_d, model.bilinear_terms, model.density_density_terms, None)
+    w_dev, _ = ed.diagonalize_block_matrixfree(Hd, Hd.n, nev=nev, precision=ed.PREC_MIXED, eig_tol=eig_tol,
+                                               want_vectors=False)
+    dE_small = _nearest_level_dE(w_dev, w_small)
+    del Hd, basis_d, cat_d
+    # the large sector's own CSR, a contiguous slice of its rows
+    D, E = H_big._dev.dim, H_big._dev.n_entries
     n_rows = D // row_fraction
-    block, _ = H48._dev.fetch_csr(0, n_rows)
+    block, _ = H_big._dev.fetch_csr(0, n_rows)
     xs = np.random.default_rng(1).standard_normal(D) + 1j * np.random.default_rng(2).standard_normal(D)
     block @ xs  # first touch
     t0 = time.perf_counter()
@@ -350,30 +368,46 @@ def cpu_baseline_checkerboard(ed, H48, n_mv48, nev, row_fraction=16):
     for _ in range(reps_mv):
         block @ xs
     t_slice = (time.perf_counter() - t0) / reps_mv
-    t_mv48 = t_slice * E / max(block.nnz, 1)
+    t_mv_big = t_slice * E / max(block.nnz, 1)
     del block, xs
-    t_table48 = t_table / E_s * E
-    t_solve48 = t_table48 + n_mv48 * (t_mv48 + other_per_mv_row * D)
-    t_cat48 = t_cat / C_s * math.comb(48, 8)
+    t_table_big = t_table / E_s * E
+    t_solve_big = t_table_big + n_mv_big * (t_mv_big + other_per_mv_row * D)
+    t_cat_big = t_cat / C_s * n_masks_big
     return {
-        "value": round(1.0 / t_solve48, 8),
+        "value": round(1.0 / t_solve_big, 8),
         "unit": "sector solves/s",
         "cores": 1,
         "kind": "port",
         "sample": (f"extrapolated, not run: the oracle (numpy/scipy restatement of the reference method) solved "
-                   f"sector (0,0) of the 3x4 checkerboard (D={D_s}, {E_s} stored entries, nev={nev}: catalog "
+                   f"sector {tuple(irrep.label)} of {label} (D={D_s}, {E_s} stored entries, nev={nev}: catalog "
                    f"{t_cat:.2f} s, table+CSR {t_table:.2f} s, eigs {t_eigs:.2f} s with {n_calls[0]} matvecs), and "
-                   f"scipy CSR matvecs ran on rows [0, {n_rows}) of this 48-site sector's table ({t_slice:.3f} s per "
-                   f"matvec); sector solve = table time scaled by entries ({t_table48:.0f} s) + {n_mv48} matvecs "
-                   f"(the device solve's count) x ({t_mv48:.2f} s CSR matvec scaled to {E} entries + ARPACK's other "
-                   f"work scaled to D={D}); scipy CSR matvec and numpy on one thread"),
-        "sector_solve_s_extrapolated": round(t_solve48, 1),
-        "csr_matvec_s_extrapolated": round(t_mv48, 3),
-        "catalog_s_extrapolated": round(t_cat48, 1),
-        "sample_sector": {"dim": D_s, "entries": E_s, "eigs_s": round(t_eigs, 3), "matvecs": n_calls[0],
-                          "table_s": round(t_table, 3), "catalog_s": round(t_cat, 3),
-                          "lowest": [float(x) for x in np.sort(w_small.real)[:3]]},
+                   f"scipy CSR matvecs ran on rows [0, {n_rows}) of the timed workload's sector table "
+                   f"({t_slice:.3f} s per matvec); sector solve = table time scaled by entries "
+                   f"({t_table_big:.0f} s) + {n_mv_big} matvecs (the device solve's count) x ({t_mv_big:.2f} s CSR "
+                   f"matvec scaled to {E} entries + ARPACK's other work scaled to D={D}); scipy CSR matvec and "
+                   f"numpy on one thread"),
+        "sector_solve_s_extrapolated": round(t_solve_big, 1),
+        "csr_matvec_s_extrapolated": round(t_mv_big, 3),
+        "catalog_s_extrapolated": round(t_cat_big, 1),
+        "sample_sector": {"workload": label, "sector": list(irrep.label), "dim": D_s, "entries": E_s,
+                          "eigs_s": round(t_eigs, 3), "matvecs": n_calls[0], "table_s": round(t_table, 3),
+                          "catalog_s": round(t_cat, 3), "oracle_levels": [float(x) for x in w_small],
+                          "device_levels": [float(x) for x in w_dev],
+                          "max_abs_dE_device_vs_oracle": dE_small},
     }
+
+
+def cpu_baseline_checkerboard(ed, H48, n_mv48, nev, eig_tol, row_fraction=16):
+    """cpu_baseline_extrapolated for the 48-site checkerboard: the sample is
+    sector k=(0,0) of the 3x4 checkerboard (same couplings, 8 fermions,
+    D ~ 6.1e4)."""
+    from realspace_exatdiagonalization_py_amd.groups import build_irrep_list, build_translation_group
+
+    model = checkerboard_model((3, 4))
+    G = build_translation_group(model.lattice)
+    small = (model, 8, G, build_irrep_list(G, model.lattice)[0])
+    return cpu_baseline_extrapolated(ed, small, H48, n_mv48, nev, math.comb(48, 8), "the 3x4 checkerboard", eig_tol,
+                                     row_fraction)
 
 
 def hubbard_secondary(ed, _edx, workers=8, passes=2):
@@ -472,7 +506,7 @@ def checkerboard_bench(args) -> int:
     cpu = None
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle sample, extrapolated) ...")
-        cpu = cpu_baseline_checkerboard(ed, H, n_mv, nev)
+        cpu = cpu_baseline_checkerboard(ed, H, n_mv, nev, args.eig_tol)
     del H, basis
 
     # roofline: the class with the most kernel time in the timed region
@@ -616,6 +650,60 @@ def serve_requests(ed, n_req, max_dim, seed, eig_tol):
     return reqs, meta
 
 
+def cpu_baseline_serve(ed, reqs, meta, results, budget_s=30.0, max_masks=3e6, max_dim=2.5e5):
+    """The reference method on a bounded sample of the served requests: the
+    oracle (numpy/scipy restatement: catalog of the request's twisted group,
+    projection table, CSR, eigs(SR, tol 1e-10, seeded start vector)) solves
+    requests of at most `max_masks` masks and `max_dim` states in stream
+    order until `budget_s` of oracle time is spent; each device result is
+    compared level by level with the oracle's values."""
+    from oracle import ed_oracle as O
+    from realspace_exatdiagonalization_py_amd import second_quantized_model as SQ
+    from realspace_exatdiagonalization_py_amd.groups import build_irrep_list, build_translation_group, group_arrays
+
+    spent, checks, dE = 0.0, [], 0.0
+    for j, (req, (fam, size), res) in enumerate(zip(reqs, meta, results)):
+        if spent >= budget_s:
+            break
+        n_site = req.model.lattice.n_site
+        if math.comb(n_site, req.n_filled) > max_masks or res.dim > max_dim:
+            continue
+        model, ne = _serve_model(ed, fam, size)  # a fresh copy, twisted for the oracle
+        SQ.update_second_quantized_model_with_twisted_phases(model, twisted_phases_over_2π=list(req.twist))
+        G = build_translation_group(model.lattice, list(req.twist))
+        irrep = next(ir for ir in build_irrep_list(G, model.lattice) if ir.label == tuple(req.sector))
+        ferm = model.particle_statistics.name == "FERMIONIC"
+        perm, ph = group_arrays(G)
+        t0 = time.perf_counter()
+        oc = O.build_catalog(n_site, ne, perm, ph, ferm)
+        reps, stab, o2b = O.sector_basis(oc, irrep.values)
+        r, c, v, hd = O.projection_table(oc, reps, stab, o2b, model.bilinear_terms, model.density_density_terms,
+                                         irrep.values, ferm)
+        Hs = O.csr_hamiltonian(r, c, v, hd, len(reps))
+        w, _ = O.diagonalize(Hs, req.nev)
+        dt = time.perf_counter() - t0
+        spent += dt
+        d = _nearest_level_dE(res.eigenvalues, w)
+        dE = max(dE, d)
+        checks.append({"request": j, "family": fam, "size": size, "sector": list(req.sector), "twist": req.twist[0],
+                       "nev": req.nev, "dim": len(reps), "oracle_s": round(dt, 3), "dE": d})
+        del oc, Hs, r, c, v
+    if not checks:
+        return None
+    return {
+        "value": round(len(checks) / spent, 6),
+        "unit": "sector solves/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": (f"the oracle (numpy/scipy restatement of the reference method: catalog, CSR projection table, "
+                   f"eigs SR tol 1e-10 with the reference start vector) on the first {len(checks)} served requests "
+                   f"of at most {max_masks:.0e} masks and {max_dim:.1e} states ({spent:.1f} s in all); each device "
+                   f"level against the nearest oracle level; scipy CSR matvec single-threaded"),
+        "max_abs_dE_device_vs_oracle": dE,
+        "oracle_checks": checks,
+    }
+
+
 def serve_bench(args) -> int:
     """A seeded Poisson stream of sector-solve requests drawn uniformly from
     the three model families (random size, momentum sector, twist angle and
@@ -678,6 +766,7 @@ def serve_bench(args) -> int:
         "families": by_fam,
         "max_abs_dE_vs_fp64_sample": dE,
         "fp64_checks": checks,
+        "cpu_baseline": None if args.no_cpu_baseline else cpu_baseline_serve(ed, reqs, meta, results),
         "peak_admitted_bytes": st.get("peak_admitted_bytes"),
         "hbm_budget_bytes": st.get("hbm_budget_bytes"),
     }
@@ -753,8 +842,19 @@ def flux_sweep(args) -> int:
     log(f"{args.flux_twists} twists ({n_solves} sector solves) in {elapsed:.2f}s")
     # accuracy: sector 0 of the last twist again in FP64
     w32, H = last.pop(0)
+    n_mv = int(H.last_info[0])
     w64, _ = ed.diagonalize_block_matrixfree(H, H.n, nev=nev, precision=ed.PREC_FP64, eig_tol=args.eig_tol,
                                              want_vectors=False)
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("cpu baseline (oracle sample at the last twist, extrapolated) ...")
+        theta = [args.flux_twists / 32, 0.0]
+        small_model = M.build_zero_flux_bosonic_fci_second_quantized_model(sample_size=[3, 4])
+        SQ.update_second_quantized_model_with_twisted_phases(small_model, twisted_phases_over_2π=theta)
+        Gs = build_translation_group(small_model.lattice, theta)
+        small = (small_model, 6, Gs, build_irrep_list(Gs, small_model.lattice)[0])
+        cpu = cpu_baseline_extrapolated(ed, small, H, n_mv, nev, math.comb(36, 9),
+                                        f"the Haldane 3x4 boson model at the same twist (6 bosons)", args.eig_tol)
     del H
     line = {
         "mode": "flux_sweep",
@@ -771,6 +871,7 @@ def flux_sweep(args) -> int:
         "precision": "mixed",
         "max_abs_dE_vs_fp64_one_sector": float(np.max(np.abs(w32 - w64))),
         "full_sweep_estimate_s": round(576 / (n_solves / elapsed), 1),
+        "cpu_baseline": cpu,
     }
     print(json.dumps(line))
     return 0
