@@ -584,6 +584,12 @@ def checkerboard_bench(args) -> int:
         "autotune": [{"dim_log2": e["dim_log2"], "block": e["block"], "lanes": e["lanes"],
                       "ms_by_lanes": {str(k): round(v, 5) for k, v in e["ms"].items()}}
                      for e in _edx.autotune_table()],
+        "schedules": [{"precision": "fp64" if e["precision"] == 0 else "c64", "dim_log2": e["dim_log2"],
+                       "block": e["block"], "basis": e["basis"], "gram": _edx.GRAM_SCHEDULES[e["gram"]],
+                       "update_stage_rows": _edx.UPDATE_VARIANTS[e["update"]],
+                       "gram_ms": {_edx.GRAM_SCHEDULES[k]: round(v, 4) for k, v in e["gram_ms"].items() if v > 0},
+                       "update_ms": {"%d/%d" % _edx.UPDATE_VARIANTS[k]: round(v, 4) for k, v in e["update_ms"].items()}}
+                      for e in _edx.schedule_table()],
     }
     print(json.dumps(line))
     return 0
@@ -1055,6 +1061,12 @@ def hubbard_bench(args) -> int:
         "autotune": [{"dim_log2": e["dim_log2"], "block": e["block"], "lanes": e["lanes"],
                       "ms_by_lanes": {str(k): round(v, 5) for k, v in e["ms"].items()}}
                      for e in _edx.autotune_table()],
+        "schedules": [{"precision": "fp64" if e["precision"] == 0 else "c64", "dim_log2": e["dim_log2"],
+                       "block": e["block"], "basis": e["basis"], "gram": _edx.GRAM_SCHEDULES[e["gram"]],
+                       "update_stage_rows": _edx.UPDATE_VARIANTS[e["update"]],
+                       "gram_ms": {_edx.GRAM_SCHEDULES[k]: round(v, 4) for k, v in e["gram_ms"].items() if v > 0},
+                       "update_ms": {"%d/%d" % _edx.UPDATE_VARIANTS[k]: round(v, 4) for k, v in e["update_ms"].items()}}
+                      for e in _edx.schedule_table()],
     }
     print(json.dumps(line))
     return 0

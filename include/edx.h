@@ -281,9 +281,9 @@ int edx_hamiltonian_info(const edx_hamiltonian* ham, int64_t* dim, int64_t* n_en
  * edx_hamiltonian_from_csr) picks the lanes per row of its sliced copy from a
  * process-wide table keyed by (precision, block width nb, floor(log2 dim),
  * mean row length in quarter octaves); on a miss it times the 1-, 2- and
- * 4-lane layouts (warm-up plus five launches of the SpMM at that precision
- * and width, on the build's stream), records the three times and keeps the
- * fastest.  A fixed edx_set_spmv_group choice overrides tuning; sectors with
+ * 4-lane layouts (warm-up plus seven individually timed launches of the
+ * SpMM at that precision and width, on the build's stream; the fastest
+ * launch counts), records the three times and keeps the fastest.  A fixed edx_set_spmv_group choice overrides tuning; sectors with
  * dim <= 500 (dense path) are not tuned.  edx_autotune_table returns, per
  * entry, keys[5*i..] = (precision, nb, dim_log2, row_quarter_octaves, lanes)
  * and ms[3*i..] = average launch times of the 1/2/4-lane layouts; n is the
@@ -296,6 +296,23 @@ int edx_autotune_table(int32_t capacity, int32_t* keys, float* ms, int32_t* n);
 int edx_autotune_clear(void);
 int edx_hamiltonian_autotune(edx_ctx* ctx, edx_hamiltonian* ham, int32_t precision, int32_t nb, int32_t* lanes,
                              float* ms);
+/* Eigensolver schedules.  With tuning on, each eigensolve of dim >= 65536
+ * and block width <= 8 looks up (precision, block, floor(log2 dim), basis
+ * size) in a second process-wide table; on a miss it times, on its own
+ * Krylov banks at the shape of a block step three quarters into the basis,
+ * the Gram of [Q, W] against W on vector FMAs and (complex64) on the FP32
+ * matrix cores, and the three block-update variants (vectors per prefetch
+ * stage, rows per lane: 2/2, 2/1, 4/1), and keeps the fastest of each.  The
+ * update variants sum every output in the same order, so they give
+ * identical results.  edx_schedule_table returns keys[6*i..] = (precision,
+ * block, dim_log2, basis, gram, update) with gram 1 = vector FMAs, 2 =
+ * matrix cores and update 1..3, and ms[5*i..] = fastest launch times of the
+ * two Gram schedules (0 when not timed) and the three updates.
+ * edx_set_solver_schedule fixes a context's choices (0 = tuned or default),
+ * overriding the table.  edx_autotune_clear clears both tables.
+ * No reference counterpart. */
+int edx_schedule_table(int32_t capacity, int32_t* keys, float* ms, int32_t* n);
+int edx_set_solver_schedule(edx_ctx* ctx, int32_t gram, int32_t update);
 
 /* ---- host-only helper (no device needed) ------------------------------------
  * Eigen-decomposition of a Hermitian n x n matrix (row-major (re, im) pairs)

@@ -73,6 +73,8 @@ SIGNATURES = {
     "edx_set_autotune": (C.c_int, [i32, i32, i32]),
     "edx_autotune_table": (C.c_int, [i32, P, P, pi32]),
     "edx_autotune_clear": (C.c_int, []),
+    "edx_schedule_table": (C.c_int, [i32, P, P, pi32]),
+    "edx_set_solver_schedule": (C.c_int, [P, i32, i32]),
     "edx_hamiltonian_autotune": (C.c_int, [P, P, i32, i32, pi32, P]),
     "edx_host_herm_eig": (C.c_int, [i32, P, i32, P, P]),
     "edx_device_alloc": (C.c_int, [P, u64, PP]),
@@ -243,6 +245,35 @@ def autotune_table() -> list[dict]:
 
 def autotune_clear() -> None:
     raise_for(lib.edx_autotune_clear())
+
+
+GRAM_SCHEDULES = {1: "vector", 2: "matrix"}
+UPDATE_VARIANTS = {1: (2, 2), 2: (2, 1), 3: (4, 1)}  # (vectors per prefetch stage, rows per lane)
+
+
+def schedule_table() -> list[dict]:
+    """Entries of the eigensolver schedule table: key, kept Gram schedule
+    (1 vector FMAs, 2 FP32 matrix cores) and update variant (1..3), and the
+    fastest timed launch (ms) of each candidate (0.0: not timed)."""
+    n = C.c_int32()
+    raise_for(lib.edx_schedule_table(0, None, None, C.byref(n)))
+    cap = n.value
+    keys = np.zeros(6 * max(cap, 1), dtype=np.int32)
+    ms = np.zeros(5 * max(cap, 1), dtype=np.float32)
+    raise_for(lib.edx_schedule_table(cap, ptr(keys), ptr(ms), C.byref(n)))
+    out = []
+    for i in range(min(cap, n.value)):
+        k, m = keys[6 * i:6 * i + 6], ms[5 * i:5 * i + 5]
+        out.append({"precision": int(k[0]), "block": int(k[1]), "dim_log2": int(k[2]), "basis": int(k[3]),
+                    "gram": int(k[4]), "update": int(k[5]),
+                    "gram_ms": {1: float(m[0]), 2: float(m[1])},
+                    "update_ms": {1: float(m[2]), 2: float(m[3]), 3: float(m[4])}})
+    return out
+
+
+def set_solver_schedule(ctx: "Context", gram: int = 0, update: int = 0) -> None:
+    """Fixes the eigensolver schedule of solves on ``ctx`` (0: tuned or default)."""
+    ctx.check(lib.edx_set_solver_schedule(ctx.handle, int(gram), int(update)))
 
 
 def mem_info(ctx: "Context") -> tuple[int, int]:

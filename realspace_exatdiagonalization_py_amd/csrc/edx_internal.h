@@ -198,6 +198,11 @@ struct Ctx {
   std::atomic<uintptr_t> owner{0};
   int depth = 0;
   int autotune_group = 0;  // lanes per row in the SpMV (0 = pick by heuristics)
+  // orthogonalisation schedule of the eigensolver (0 = tuned or default):
+  // Gram 1 = vector FMAs, 2 = FP32 matrix cores (complex64, blocks <= 8);
+  // block update 1..3 = stage/rows variants (kUpdateVariants)
+  int sched_gram = 0;
+  int sched_update = 0;
   KernelTimer timer;
   // pinned host staging owned by the context (allocated once, reused by
   // every solve on it): projection coefficients device -> host, and the
@@ -208,6 +213,33 @@ struct Ctx {
   char* stage[2] = {nullptr, nullptr};  // pinned staging for device -> host copies (lazy)
   hipEvent_t stage_ev[2] = {nullptr, nullptr};
 };
+
+// Orthogonalisation schedules of the eigensolver picked by timing (see
+// tune.hip): a process-wide table keyed by (precision, block width,
+// floor(log2 dim), basis size).
+struct SchedKey {
+  int prec, b, dim_log2, basis;
+  bool operator<(const SchedKey& o) const {
+    if (prec != o.prec) return prec < o.prec;
+    if (b != o.b) return b < o.b;
+    if (dim_log2 != o.dim_log2) return dim_log2 < o.dim_log2;
+    return basis < o.basis;
+  }
+};
+struct SchedEntry {
+  int gram = 0, update = 0;            // kept choices (1-based)
+  float gram_ms[2] = {0.f, 0.f};       // vector FMAs, matrix cores (0: not timed)
+  float update_ms[3] = {0.f, 0.f, 0.f};
+};
+enum class SchedLookup { found, claimed, unavailable };
+// true when the autotuner is on (edx_set_autotune)
+bool tuning_enabled();
+// found: *out holds the entry; claimed: the caller times the key and calls
+// schedule_store (or schedule_release on failure); unavailable: another
+// caller is still timing it, use the defaults
+SchedLookup schedule_lookup(const SchedKey& k, SchedEntry* out);
+void schedule_store(const SchedKey& k, const SchedEntry& e);
+void schedule_release(const SchedKey& k);
 
 // Largest Krylov basis and block width of the eigensolver.
 constexpr int kMaxBasis = 224;

@@ -56,6 +56,20 @@ constexpr int kStepLds = 3072;
 constexpr int64_t kMfmaGramRows = int64_t(1) << 20;
 // Rows per lane of the update kernel (each coefficient read serves them all).
 constexpr int kUpdateRows = 2;
+// Block-update schedules (blocks of at most 8 vectors): vectors per prefetch
+// stage and rows per lane.  Every variant sums each output in the same order
+// (input vectors ascending, one lane per row), so they give identical
+// results; the first is the default, the tuner may pick another.
+struct UpdateVariant {
+  int stage, rows;
+};
+constexpr UpdateVariant kUpdateVariants[3] = {{2, 2}, {2, 1}, {4, 1}};
+// Gram schedules: vector FMAs, or the FP32 matrix cores (complex64 Grams
+// against at most 8 vectors).
+constexpr int kGramVector = 1, kGramMatrix = 2;
+// Sectors from this many rows have their schedule tuned (smaller ones are
+// bound by launch latency, where timings say little).
+constexpr int64_t kScheduleTuneRows = 65536;
 // Largest LDS staging of an update's interleaved output copy (bytes).
 constexpr size_t kStageLdsBytes = 48 * 1024;
 constexpr size_t kLogC = edx::kLogC;
@@ -936,6 +950,7 @@ class BlockLanczos {
     s2buf_.alloc((size_t)kMaxW * kMaxW);
     ctl_.alloc(4);
     EDX_HIP(hipMemsetAsync(ctl_.p, 0, 4 * sizeof(int), ctx_.stream));
+    choose_schedule();
   }
   BlockLanczos(const BlockLanczos&) = delete;
   BlockLanczos& operator=(const BlockLanczos&) = delete;
@@ -1121,7 +1136,7 @@ class BlockLanczos {
 
   template <typename VT>
   void launch_tiles_nw(const void* A0, int n0, const void* A1, int n1, const void* W, int nw, StepPred pred) {
-    if (std::is_same<VT, float2>::value && nw <= 8 && D_ >= kMfmaGramRows) {
+    if (std::is_same<VT, float2>::value && nw <= 8 && gram_mode_ == kGramMatrix) {
       launch_gram_mfma(A0, n0, A1, n1, W, nw, pred);
       return;
     }
@@ -1261,6 +1276,97 @@ class BlockLanczos {
                        ctx_.stream, D_, A, nk, (VT*)Out, ld_, S, ldS, (VT*)Xi, n_kt, (int)n_tiles, stage_xi, pred);
   }
 
+  // A block update of at most 8 outputs with the schedule of this solve.
+  template <typename VT, int NK>
+  void launch_update(const void* A0, int n0, const void* A1, int n1, void* Out, int nk, const double2* S, int ldS,
+                     void* Xi, StepPred pd) {
+    static_assert(kUpdateVariants[0].stage == 2 && kUpdateVariants[0].rows == 2 && kUpdateVariants[1].stage == 2 &&
+                      kUpdateVariants[1].rows == 1 && kUpdateVariants[2].stage == 4 && kUpdateVariants[2].rows == 1,
+                  "update variants and their instantiations below must agree");
+    switch (upd_var_) {
+      case 1: launch_gemm_t<VT, NK, 2, 1>(A0, n0, A1, n1, Out, nk, S, ldS, Xi, pd); break;
+      case 2: launch_gemm_t<VT, NK, 4, 1>(A0, n0, A1, n1, Out, nk, S, ldS, Xi, pd); break;
+      default: launch_gemm_t<VT, NK, 2, 2>(A0, n0, A1, n1, Out, nk, S, ldS, Xi, pd);
+    }
+  }
+
+  // Fastest of three individually timed runs of `f` (after one warm-up) on
+  // the solve's stream, in milliseconds.
+  template <class F>
+  float time_fastest(F&& f) {
+    f();
+    hipEvent_t ev[4];
+    for (auto& e : ev) EDX_HIP(hipEventCreate(&e));
+    EDX_HIP(hipEventRecord(ev[0], ctx_.stream));
+    for (int r = 0; r < 3; ++r) {
+      f();
+      EDX_HIP(hipEventRecord(ev[r + 1], ctx_.stream));
+    }
+    EDX_HIP(hipEventSynchronize(ev[3]));
+    float best = INFINITY;
+    for (int r = 0; r < 3; ++r) {
+      float ms = 0.0f;
+      EDX_HIP(hipEventElapsedTime(&ms, ev[r], ev[r + 1]));
+      best = std::min(best, ms);
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    return best;
+  }
+
+  // Orthogonalisation schedule of this solve: the context's fixed choice,
+  // else (tuner on, D >= kScheduleTuneRows, b <= 8) the process-wide table
+  // entry of (precision, b, floor(log2 D), basis size), timed on a miss with
+  // this solve's own banks at the shape of a block step three quarters into
+  // the basis (a Gram of [Q, W] against W and the update [Q, W] S into the
+  // other bank and X); else the defaults (matrix-core Gram for complex64
+  // from kMfmaGramRows rows, first update variant).  The banks hold no
+  // vectors yet: the timed kernels read them and write only areas that the
+  // run overwrites before use.
+  void choose_schedule() {
+    gram_mode_ = (esz_ == 8 && D_ >= kMfmaGramRows) ? kGramMatrix : kGramVector;
+    upd_var_ = 0;
+    const bool gram_fixed = ctx_.sched_gram != 0, upd_fixed = ctx_.sched_update != 0;
+    if (gram_fixed) gram_mode_ = ctx_.sched_gram;
+    if (upd_fixed) upd_var_ = ctx_.sched_update - 1;
+    if ((gram_fixed && upd_fixed) || b_ > 8 || D_ < kScheduleTuneRows || !edx::tuning_enabled()) return;
+    const edx::SchedKey key{prec_, b_, (int)std::floor(std::log2((double)D_)), nq_max_};
+    edx::SchedEntry e;
+    const edx::SchedLookup look = edx::schedule_lookup(key, &e);
+    if (look == edx::SchedLookup::unavailable) return;
+    if (look == edx::SchedLookup::claimed) {
+      const bool timer_on = ctx_.timer.on;
+      ctx_.timer.on = false;
+      try {
+        const int nq = std::max(b_, nq_max_ * 3 / 4 / b_ * b_);
+        void* Q = vec(0, 0);
+        void* Out = vec(1, 0);
+        EDX_HIP(hipMemsetAsync(sbuf_.p, 0, (size_t)(nq + b_) * b_ * sizeof(double2), ctx_.stream));
+        const int gram_modes = esz_ == 8 ? 2 : 1;
+        for (int g = 0; g < gram_modes; ++g) {
+          gram_mode_ = g + 1;
+          e.gram_ms[g] = time_fastest([&] { gram_device(Q, nq, W_.p, b_, W_.p, b_, pbuf_.p); });
+        }
+        e.gram = (gram_modes == 2 && e.gram_ms[1] < e.gram_ms[0]) ? kGramMatrix : kGramVector;
+        for (int v = 0; v < 3; ++v) {
+          upd_var_ = v;
+          e.update_ms[v] = time_fastest([&] { launch_gemm(Q, nq, W_.p, b_, Out, b_, sbuf_.p, b_, X_.p); });
+        }
+        int best = 0;
+        for (int v = 1; v < 3; ++v)
+          if (e.update_ms[v] < e.update_ms[best]) best = v;
+        e.update = best + 1;
+      } catch (...) {
+        ctx_.timer.on = timer_on;
+        edx::schedule_release(key);
+        throw;
+      }
+      ctx_.timer.on = timer_on;
+      edx::schedule_store(key, e);
+    }
+    if (!gram_fixed) gram_mode_ = e.gram;
+    if (!upd_fixed) upd_var_ = e.update - 1;
+  }
+
   // One tile of the smallest width that holds nk (nk <= 16), otherwise tiles
   // of up to 12 outputs (restart combinations: 97 us against 129 us with
   // tiles of 6 at 60 -> 33 vectors, complex64, D = 225k).
@@ -1268,10 +1374,10 @@ class BlockLanczos {
   void launch_gemm_v(const void* A0, int n0, const void* A1, int n1, void* Out, int nk, const double2* S, int ldS,
                      void* Xi, StepPred pd) {
     constexpr int R = kUpdateRows;
-    if (nk <= 2) launch_gemm_t<VT, 2, kGemmStage, R>(A0, n0, A1, n1, Out, nk, S, ldS, Xi, pd);
-    else if (nk <= 4) launch_gemm_t<VT, 4, kGemmStage, R>(A0, n0, A1, n1, Out, nk, S, ldS, Xi, pd);
-    else if (nk <= 6) launch_gemm_t<VT, 6, kGemmStage, R>(A0, n0, A1, n1, Out, nk, S, ldS, Xi, pd);
-    else if (nk <= 8) launch_gemm_t<VT, 8, kGemmStage, R>(A0, n0, A1, n1, Out, nk, S, ldS, Xi, pd);
+    if (nk <= 2) launch_update<VT, 2>(A0, n0, A1, n1, Out, nk, S, ldS, Xi, pd);
+    else if (nk <= 4) launch_update<VT, 4>(A0, n0, A1, n1, Out, nk, S, ldS, Xi, pd);
+    else if (nk <= 6) launch_update<VT, 6>(A0, n0, A1, n1, Out, nk, S, ldS, Xi, pd);
+    else if (nk <= 8) launch_update<VT, 8>(A0, n0, A1, n1, Out, nk, S, ldS, Xi, pd);
     else if (nk <= 12) launch_gemm_t<VT, 12>(A0, n0, A1, n1, Out, nk, S, ldS, Xi, pd);
     else if (nk <= 16) launch_gemm_t<VT, 16>(A0, n0, A1, n1, Out, nk, S, ldS, Xi, pd);
     else {
@@ -1622,6 +1728,8 @@ class BlockLanczos {
   Options opt_;
   int64_t D_ = 0, ld_ = 0;
   int b_ = 1, esz_ = 16, nq_max_ = 0, cap_ = 0, cur_ = 0, restarts_ = 0;
+  int gram_mode_ = kGramVector;  // Gram schedule (kGramVector / kGramMatrix)
+  int upd_var_ = 0;              // block-update schedule (index into kUpdateVariants)
   double kappa2_ = 1e4;
   double unit_roundoff_ = 1.1e-16;
   double single_qr_tol_ = 0.0;

@@ -178,6 +178,54 @@ void abandon_sell(const edx_hamiltonian& h, SellPlan& p) {
   p.tune = false;
 }
 
+bool tuning_enabled() {
+  TuneState& s = state();
+  std::lock_guard<std::mutex> lk(s.mu);
+  return s.enabled;
+}
+
+namespace {
+struct SchedState {
+  std::mutex mu;
+  std::condition_variable cv;
+  std::map<SchedKey, SchedEntry> table;
+  std::set<SchedKey> busy;
+};
+SchedState& sched_state() {
+  static SchedState s;
+  return s;
+}
+}  // namespace
+
+SchedLookup schedule_lookup(const SchedKey& k, SchedEntry* out) {
+  SchedState& s = sched_state();
+  std::unique_lock<std::mutex> lk(s.mu);
+  s.cv.wait_for(lk, std::chrono::seconds(30), [&] { return !s.busy.count(k); });
+  auto it = s.table.find(k);
+  if (it != s.table.end()) {
+    *out = it->second;
+    return SchedLookup::found;
+  }
+  if (s.busy.count(k)) return SchedLookup::unavailable;
+  s.busy.insert(k);
+  return SchedLookup::claimed;
+}
+
+void schedule_store(const SchedKey& k, const SchedEntry& e) {
+  SchedState& s = sched_state();
+  std::lock_guard<std::mutex> lk(s.mu);
+  s.table.emplace(k, e);
+  s.busy.erase(k);
+  s.cv.notify_all();
+}
+
+void schedule_release(const SchedKey& k) {
+  SchedState& s = sched_state();
+  std::lock_guard<std::mutex> lk(s.mu);
+  s.busy.erase(k);
+  s.cv.notify_all();
+}
+
 void build_sell_for(edx_hamiltonian& h, Ctx& ctx) {
   SellPlan p = plan_sell(h, ctx);
   try {
@@ -226,10 +274,50 @@ int edx_autotune_table(int32_t capacity, int32_t* keys, float* ms, int32_t* n) {
 }
 
 int edx_autotune_clear(void) {
-  auto& s = edx::state();
+  {
+    auto& s = edx::state();
+    std::lock_guard<std::mutex> lk(s.mu);
+    s.table.clear();
+  }
+  auto& s = edx::sched_state();
   std::lock_guard<std::mutex> lk(s.mu);
   s.table.clear();
   return EDX_OK;
+}
+
+int edx_schedule_table(int32_t capacity, int32_t* keys, float* ms, int32_t* n) {
+  if (!n || (capacity > 0 && (!keys || !ms))) return EDX_ERR_INVALID;
+  auto& s = edx::sched_state();
+  std::lock_guard<std::mutex> lk(s.mu);
+  int i = 0;
+  for (const auto& kv : s.table) {
+    if (i < capacity) {
+      int32_t* r = keys + 6 * i;
+      r[0] = kv.first.prec;
+      r[1] = kv.first.b;
+      r[2] = kv.first.dim_log2;
+      r[3] = kv.first.basis;
+      r[4] = kv.second.gram;
+      r[5] = kv.second.update;
+      float* m = ms + 5 * i;
+      m[0] = kv.second.gram_ms[0];
+      m[1] = kv.second.gram_ms[1];
+      for (int c = 0; c < 3; ++c) m[2 + c] = kv.second.update_ms[c];
+    }
+    ++i;
+  }
+  *n = i;
+  return EDX_OK;
+}
+
+int edx_set_solver_schedule(edx_ctx* ctx, int32_t gram, int32_t update) {
+  return edx::guarded(ctx, [&] {
+    EDX_REQUIRE(ctx, EDX_ERR_INVALID, "null argument");
+    EDX_REQUIRE(gram >= 0 && gram <= 2, EDX_ERR_INVALID, "Gram schedule must be 0 (automatic), 1 or 2");
+    EDX_REQUIRE(update >= 0 && update <= 3, EDX_ERR_INVALID, "update schedule must be 0 (automatic) or 1..3");
+    ctx->sched_gram = gram;
+    ctx->sched_update = update;
+  });
 }
 
 int edx_hamiltonian_autotune(edx_ctx* ctx, edx_hamiltonian* ham, int32_t precision, int32_t nb, int32_t* lanes,

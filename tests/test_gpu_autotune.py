@@ -1,7 +1,14 @@
-"""GPU: the SpMM autotuner (lanes per row of the sliced table picked by
-timing).  The tuned layout must be the fastest of the three it timed, be
-reused for builds of the same shape, and leave the eigenvalues unchanged:
-every layout stores the same entries, so eigenvalues agree to 1e-10."""
+"""GPU: the autotuner.
+
+* SpMM layout (lanes per row of the sliced table): the tuned layout must be
+  the fastest of the three it timed, be reused for builds of the same shape,
+  and leave the eigenvalues unchanged (every layout stores the same entries:
+  1e-10).
+* Eigensolver schedules (Gram on vector FMAs or FP32 matrix cores; three
+  block-update variants): on a 3x4 Hubbard sector (D = 2.3e5) each kept
+  choice is the fastest one timed; the update variants give bit-identical
+  eigenvalues and eigenvectors; the two Gram schedules agree within 1e-10.
+"""
 
 from __future__ import annotations
 
@@ -23,6 +30,7 @@ def tuning():
     yield
     _edx.set_autotune(False)
     _edx.autotune_clear()
+    _edx.set_solver_schedule(_edx.Context.default(), 0, 0)
 
 
 def _data():
@@ -73,3 +81,62 @@ def test_direct_tuning_of_one_hamiltonian(tuning):
     assert g.value == r["lanes"]
     np.testing.assert_allclose(H._dev.matvec(x), y0, rtol=0, atol=1e-12)
     assert len(_edx.autotune_table()) == 0  # direct tuning does not touch the table
+
+
+def _hubbard34_sector(idx=1):
+    model = ed.build_spinful_hubbard_second_quantized_model([3, 4], 1.0, 4.0)
+    G = ed.build_translation_group(model.lattice)
+    cat = ed.build_symmetry_orbit_catalog(second_quantized_model=model, n_filled=12, symmetry_group=G,
+                                          particle_statistics=model.particle_statistics)
+    from realspace_exatdiagonalization_py_amd.groups import build_irrep_list
+
+    basis = ed.build_symmetry_sector_basis(cat, build_irrep_list(G, model.lattice)[idx])
+    return ed.MatrixFreeHamiltonian(basis, model.bilinear_terms, model.density_density_terms, None)
+
+
+def test_solver_schedule_tuned_fastest_and_identical(tuning):
+    H = _hubbard34_sector()
+    assert H.n >= 65536
+    ctx = H._dev.ctx
+    w_t, _ = ed.diagonalize_block_matrixfree(H, H.n, nev=6, precision=ed.PREC_MIXED, want_vectors=False)
+    table = _edx.schedule_table()
+    print("[schedule]", table, flush=True)
+    c64 = [e for e in table if e["precision"] == _edx.PREC_MIXED]
+    assert len(c64) == 1 and c64[0]["block"] == 6
+    for e in table:
+        timed_g = {k: v for k, v in e["gram_ms"].items() if v > 0}
+        assert e["gram"] == min(timed_g, key=timed_g.get)
+        assert all(v > 0 for v in e["update_ms"].values())
+        assert e["update"] == min(e["update_ms"], key=e["update_ms"].get)
+    assert all(v > 0 for v in c64[0]["gram_ms"].values())  # complex64: both Gram schedules timed
+    n_entries = len(table)
+    # a second solve of the same shape reuses the entries
+    ed.diagonalize_block_matrixfree(H, H.n, nev=6, precision=ed.PREC_MIXED, want_vectors=False)
+    assert len(_edx.schedule_table()) == n_entries
+    # every update variant: identical eigenpairs (same summation order)
+    for prec in (ed.PREC_FP64, ed.PREC_MIXED):
+        ref = None
+        for upd in (1, 2, 3):
+            _edx.set_solver_schedule(ctx, 1, upd)
+            w, v = ed.diagonalize_block_matrixfree(H, H.n, nev=6, precision=prec)
+            if ref is None:
+                ref = (w, v)
+                continue
+            np.testing.assert_array_equal(w, ref[0])
+            np.testing.assert_array_equal(v, ref[1])
+    # the two Gram schedules (complex64 sweeps) agree to the tolerance
+    ws = []
+    for gram in (1, 2):
+        _edx.set_solver_schedule(ctx, gram, 1)
+        w, _ = ed.diagonalize_block_matrixfree(H, H.n, nev=6, precision=ed.PREC_MIXED, want_vectors=False)
+        ws.append(w)
+    _edx.set_solver_schedule(ctx, 0, 0)
+    assert np.max(np.abs(ws[0] - ws[1])) <= 1e-10 and np.max(np.abs(ws[0] - w_t)) <= 1e-10
+
+
+def test_solver_schedule_arguments_checked():
+    ctx = _edx.Context.default()
+    with pytest.raises(ValueError, match="Gram schedule"):
+        _edx.set_solver_schedule(ctx, 3, 0)
+    with pytest.raises(ValueError, match="update schedule"):
+        _edx.set_solver_schedule(ctx, 0, 4)
