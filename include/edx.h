@@ -236,6 +236,20 @@ typedef struct edx_eigs_options {
 void edx_eigs_default_options(edx_eigs_options* opts);
 int edx_eigs_ex(edx_ctx* ctx, const edx_hamiltonian* ham, const edx_eigs_options* opts, double* vals,
                 double* vecs, double* info);
+/* Batched sector solves: n Hamiltonians (any dimensions; shared options)
+ * solved together from the calling thread, as edx_eigs_ex solves each.  Solve
+ * i runs on its own context ctxs[i] (distinct, not ctx, on ctx's device; each
+ * is claimed for the call, EDX_ERR_BUSY if another thread holds one).  Every
+ * round, the block-step SpMMs of the solves that are stepping go to grouped
+ * launches on ctx's stream (sliced tables of equal block width and lanes per
+ * row in one kernel), and the rest of each block step runs on the solve's own
+ * stream.  vals: [n][nev]; vecs (optional): n pointers, each NULL or
+ * complex128 [nev][dim_i]; info (optional): [n][4] as edx_eigs_ex.  Results
+ * are the same, bit for bit, as edx_eigs_ex on the same Hamiltonian with the
+ * same schedule.  No reference counterpart (the reference solves sectors one
+ * after another with ARPACK). */
+int edx_eigs_batch(edx_ctx* ctx, int32_t n, edx_ctx* const* ctxs, const edx_hamiltonian* const* hams,
+                   const edx_eigs_options* opts, double* vals, double* const* vecs, double* info);
 
 /* ---- eigenvector consumers -----------------------------------------------------
  * Replaces _expand_sector_state_to_fock_amplitudes (ref

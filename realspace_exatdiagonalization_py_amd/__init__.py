@@ -29,6 +29,7 @@ from .symmetry_resolved_ed import (
     build_identity_group, build_identity_irrep_list, build_irrep_list, build_matrixfree_projection_table,
     build_symmetry_orbit_catalog, build_symmetry_sector_basis, build_translation_group,
     build_translation_irrep_list, diagonalize_block_arpack, diagonalize_block_dense, diagonalize_block_matrixfree,
+    diagonalize_blocks_batched, solve_sectors_batched,
     ed_scan, ed_scan_at_irrep_matrix, ed_scan_at_irrep_matrixfree, ed_scan_checkpoint_filename, full_ed,
     get_canonical, get_canonical_representative, group_order, hamiltonian_linear_operator, hopping_phase_for_stats,
     load_checkpoint, populate_canonical_map, print_spectrum, project_to_sector, project_to_unnormalized_sector,

@@ -74,6 +74,7 @@ SIGNATURES = {
     "edx_autotune_table": (C.c_int, [i32, P, P, pi32]),
     "edx_autotune_clear": (C.c_int, []),
     "edx_schedule_table": (C.c_int, [i32, P, P, pi32]),
+    "edx_eigs_batch": (C.c_int, [P, i32, P, P, P, P, P, P]),
     "edx_set_solver_schedule": (C.c_int, [P, i32, i32]),
     "edx_hamiltonian_autotune": (C.c_int, [P, P, i32, i32, pi32, P]),
     "edx_host_herm_eig": (C.c_int, [i32, P, i32, P, P]),

@@ -28,7 +28,9 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <exception>
 #include <memory>
+#include <thread>
 #include <type_traits>
 
 #include "dense.h"
@@ -959,128 +961,161 @@ class BlockLanczos {
   // column-major with leading dimension start_ld) seeds the first block with
   // n_start vectors (rest random).  Ritz vectors are left in vec(0..nev).
   Result run(const void* start, int n_start, int64_t start_ld) {
-    const double t_start = now_ms();
-    Result res;
+    begin(start, n_start, start_ld);
+    for (;;)
+      if (step(false) && sync_point()) return res_;
+  }
+
+  // The same run in pieces, for a driver that advances several solves
+  // together: begin(), then step() until it returns true, then
+  // sync_point(), which returns true once the run has converged (result()).
+  void begin(const void* start, int n_start, int64_t start_ld) {
+    t_start_ = now_ms();
+    res_ = Result();
     cur_ = 0;
-    int nq = 0;
+    nq_ = 0;
     random_block(vec(cur_, 0), b_, opt_.seed);
     if (start && n_start > 0) {
       const int ns = std::min(n_start, b_);
       copy_block(vec(cur_, 0), ld_, start, start_ld, ns);
     }
     start_block();
-    Mat M(cap_, cap_), B(b_, cap_);
-    std::vector<double> theta;
-    int since_check = 0;
-    double best_worst = INFINITY;  // stall detection
-    int since_best = 0;
-    std::vector<int> pending;  // basis size before each block step enqueued since the last sync
-    for (;;) {
-      // one block step on the device: W = H Vr (Vr at vec(nq), interleaved
-      // copy in X_), projection, coefficients, update; the host does not wait
-      edx::spmm(h_, ctx_, prec_, b_, X_.p, W_.p, ld_);
-      enqueue_step(nq, (int)pending.size());
-      pending.push_back(nq);
-      nq += b_;
-      bool full = nq + b_ > nq_max_;
-      bool due = nq >= opt_.nev && (full || ++since_check >= opt_.check_every);
-      if (!due && (int)pending.size() < edx::kLogSlots) continue;
-      wait();
-      settle(true);
-      const int n_enq = (int)pending.size();
-      int n_acc = 0;
-      nq = absorb(pending, M, res, n_acc);
-      pending.clear();
-      if (n_acc < n_enq) {  // a step was finished on the host; the later ones were skipped
-        since_check = std::max(0, since_check - (n_enq - n_acc));
-        full = nq + b_ > nq_max_;
-        due = nq >= opt_.nev && (full || since_check >= opt_.check_every);
+    M_ = Mat(cap_, cap_);
+    B_ = Mat(b_, cap_);
+    theta_.clear();
+    since_check_ = 0;
+    best_worst_ = INFINITY;  // stall detection
+    since_best_ = 0;
+    pending_.clear();
+  }
+
+  // One block step on the device: W = H Vr (Vr at vec(nq), interleaved copy
+  // in X_; enqueued here unless the caller has enqueued it, spmm_enqueued),
+  // projection, coefficients, update; the host does not wait.  Returns true
+  // when the host must now wait for the stream and call sync_point().
+  bool step(bool spmm_enqueued) {
+    if (!spmm_enqueued) edx::spmm(h_, ctx_, prec_, b_, X_.p, W_.p, ld_);
+    enqueue_step(nq_, (int)pending_.size());
+    pending_.push_back(nq_);
+    nq_ += b_;
+    full_ = nq_ + b_ > nq_max_;
+    due_ = nq_ >= opt_.nev && (full_ || ++since_check_ >= opt_.check_every);
+    return due_ || (int)pending_.size() >= edx::kLogSlots;
+  }
+
+  // Waits for the stream, folds the enqueued steps into the projected matrix
+  // and, when a Ritz check is due, checks convergence, then finishes (returns
+  // true) or restarts when the basis is full.
+  bool sync_point() {
+    wait();
+    settle(true);
+    const int n_enq = (int)pending_.size();
+    int n_acc = 0;
+    nq_ = absorb(pending_, M_, res_, n_acc);
+    pending_.clear();
+    if (n_acc < n_enq) {  // a step was finished on the host; the later ones were skipped
+      since_check_ = std::max(0, since_check_ - (n_enq - n_acc));
+      full_ = nq_ + b_ > nq_max_;
+      due_ = nq_ >= opt_.nev && (full_ || since_check_ >= opt_.check_every);
+    }
+    if (!due_) return false;
+    since_check_ = 0;
+    const int nq = nq_;
+    const bool full = full_;
+    // coupling of the newest block: H Q = Q M + Vr B with B = [0, R]
+    settle(false);
+    B_ = Mat(b_, cap_);
+    for (int i = 0; i < b_; ++i)
+      for (int j = 0; j < b_; ++j) B_(i, nq - b_ + j) = Rstep_(i, j);
+    const int nev = opt_.nev;
+    // vectors kept by a thick restart
+    int k = nev + (nq - nev) / 2;
+    k = std::max(k, nev + std::min(b_, nq - nev));
+    k = std::min(k, nq - b_);
+    k = std::max(k, nev);
+    // Ritz values, rows nq-b .. nq-1 of every Ritz vector (the residual
+    // norms need only those) and, when restarting, the k leading vectors
+    Mat Yl, Yk;
+    ritz(M_, nq, theta_, full ? k : 0, full ? &Yk : nullptr, &Yl);
+    std::vector<double> rn(nq);
+    for (int c = 0; c < nq; ++c) {
+      double rn2 = 0.0;
+      for (int i = 0; i < b_; ++i) {
+        cplx sum(0.0, 0.0);
+        for (int j = 0; j < b_; ++j) sum += B_(i, nq - b_ + j) * Yl(j, c);
+        rn2 += std::norm(sum);
       }
-      if (!due) continue;
-      since_check = 0;
-      // coupling of the newest block: H Q = Q M + Vr B with B = [0, R]
-      settle(false);
-      B = Mat(b_, cap_);
-      for (int i = 0; i < b_; ++i)
-        for (int j = 0; j < b_; ++j) B(i, nq - b_ + j) = Rstep_(i, j);
-      const int nev = opt_.nev;
-      // vectors kept by a thick restart
-      int k = nev + (nq - nev) / 2;
-      k = std::max(k, nev + std::min(b_, nq - nev));
-      k = std::min(k, nq - b_);
-      k = std::max(k, nev);
-      // Ritz values, rows nq-b .. nq-1 of every Ritz vector (the residual
-      // norms need only those) and, when restarting, the k leading vectors
-      Mat Yl, Yk;
-      ritz(M, nq, theta, full ? k : 0, full ? &Yk : nullptr, &Yl);
-      std::vector<double> rn(nq);
-      for (int c = 0; c < nq; ++c) {
-        double rn2 = 0.0;
-        for (int i = 0; i < b_; ++i) {
-          cplx sum(0.0, 0.0);
-          for (int j = 0; j < b_; ++j) sum += B(i, nq - b_ + j) * Yl(j, c);
-          rn2 += std::norm(sum);
-        }
-        rn[c] = std::sqrt(rn2);
-      }
-      double worst = 0.0;
-      bool conv = converged(theta, rn, nev, opt_.tol, opt_.eig_tol, opt_.gap_ref, opt_.gap_ref_res, &worst);
-      res.max_rel_res = worst;
-      if (!conv && opt_.stall_checks > 0) {
-        if (worst < 0.9 * best_worst) {
-          best_worst = worst;
-          since_best = 0;
-        } else if (++since_best >= opt_.stall_checks && worst <= opt_.stall_ceiling) {
-          conv = true;
-        }
-      }
-      if (conv) {
-        int nout = nev;
-        if (opt_.out_cluster) {
-          const double sc = std::max(1.0, std::fabs(theta[nev - 1]));
-          while (nout < nq && nout < kMaxW && std::fabs(theta[nout] - theta[nev - 1]) <= 1e-4 * sc) ++nout;
-        }
-        Mat Yn;
-        if (full && nout <= k) {
-          Yn = Mat(nq, nout);
-          for (int i = 0; i < nq; ++i)
-            for (int c = 0; c < nout; ++c) Yn(i, c) = Yk(i, c);
-        } else {
-          std::vector<double> th2;
-          ritz(M, nq, th2, nout, &Yn, nullptr);
-        }
-        combine(vec(cur_, 0), nq, vec(1 - cur_, 0), nout, Yn);
-        cur_ = 1 - cur_;
-        res.vals.assign(theta.begin(), theta.begin() + nev);
-        res.nvec = nout;
-        res.ritz = theta;
-        res.ritz_res = rn;
-        res.restarts = restarts_;
-        finish(res, t_start);
-        return res;
-      }
-      if (full) {
-        EDX_REQUIRE(restarts_ < opt_.max_restarts, EDX_ERR_NOCONV,
-                    "block Lanczos did not converge within " + std::to_string(opt_.max_restarts) +
-                        " restarts (max relative residual " + std::to_string(worst) + ")");
-        combine(vec(cur_, 0), nq, vec(1 - cur_, 0), k, Yk);
-        copy_block(vec(1 - cur_, k), ld_, vec(cur_, nq), ld_, b_);
-        Mat Bn(b_, cap_);
-        for (int i = 0; i < b_; ++i)
-          for (int c = 0; c < k; ++c) {
-            cplx sum(0.0, 0.0);
-            for (int j = 0; j < b_; ++j) sum += B(i, nq - b_ + j) * Yl(j, c);
-            Bn(i, c) = sum;
-          }
-        M = Mat(cap_, cap_);
-        for (int c = 0; c < k; ++c) M(c, c) = theta[c];
-        B = Bn;
-        nq = k;
-        cur_ = 1 - cur_;
-        ++restarts_;
+      rn[c] = std::sqrt(rn2);
+    }
+    double worst = 0.0;
+    bool conv = converged(theta_, rn, nev, opt_.tol, opt_.eig_tol, opt_.gap_ref, opt_.gap_ref_res, &worst);
+    res_.max_rel_res = worst;
+    if (!conv && opt_.stall_checks > 0) {
+      if (worst < 0.9 * best_worst_) {
+        best_worst_ = worst;
+        since_best_ = 0;
+      } else if (++since_best_ >= opt_.stall_checks && worst <= opt_.stall_ceiling) {
+        conv = true;
       }
     }
+    if (conv) {
+      int nout = nev;
+      if (opt_.out_cluster) {
+        const double sc = std::max(1.0, std::fabs(theta_[nev - 1]));
+        while (nout < nq && nout < kMaxW && std::fabs(theta_[nout] - theta_[nev - 1]) <= 1e-4 * sc) ++nout;
+      }
+      Mat Yn;
+      if (full && nout <= k) {
+        Yn = Mat(nq, nout);
+        for (int i = 0; i < nq; ++i)
+          for (int c = 0; c < nout; ++c) Yn(i, c) = Yk(i, c);
+      } else {
+        std::vector<double> th2;
+        ritz(M_, nq, th2, nout, &Yn, nullptr);
+      }
+      combine(vec(cur_, 0), nq, vec(1 - cur_, 0), nout, Yn);
+      cur_ = 1 - cur_;
+      res_.vals.assign(theta_.begin(), theta_.begin() + nev);
+      res_.nvec = nout;
+      res_.ritz = theta_;
+      res_.ritz_res = rn;
+      res_.restarts = restarts_;
+      finish(res_, t_start_);
+      return true;
+    }
+    if (full) {
+      EDX_REQUIRE(restarts_ < opt_.max_restarts, EDX_ERR_NOCONV,
+                  "block Lanczos did not converge within " + std::to_string(opt_.max_restarts) +
+                      " restarts (max relative residual " + std::to_string(worst) + ")");
+      combine(vec(cur_, 0), nq, vec(1 - cur_, 0), k, Yk);
+      copy_block(vec(1 - cur_, k), ld_, vec(cur_, nq), ld_, b_);
+      Mat Bn(b_, cap_);
+      for (int i = 0; i < b_; ++i)
+        for (int c = 0; c < k; ++c) {
+          cplx sum(0.0, 0.0);
+          for (int j = 0; j < b_; ++j) sum += B_(i, nq - b_ + j) * Yl(j, c);
+          Bn(i, c) = sum;
+        }
+      M_ = Mat(cap_, cap_);
+      for (int c = 0; c < k; ++c) M_(c, c) = theta_[c];
+      B_ = Bn;
+      nq_ = k;
+      cur_ = 1 - cur_;
+      ++restarts_;
+    }
+    return false;
   }
+
+  const Result& result() const { return res_; }
+  // What a grouped SpMM needs of this solve: its operator, precision, block
+  // width, input (interleaved) and output (column-major, leading dimension
+  // ld) blocks, and its stream.
+  const edx_hamiltonian& op() const { return h_; }
+  int precision() const { return prec_; }
+  int block() const { return b_; }
+  const void* spmm_in() const { return X_.p; }
+  void* spmm_out() { return W_.p; }
+  hipStream_t stream() const { return ctx_.stream; }
 
   // n vectors (rows < D) from src to dst, both of this run's precision.
   void copy_block(void* dst, int64_t ldd, const void* src, int64_t lds, int n) {
@@ -1728,6 +1763,14 @@ class BlockLanczos {
   Options opt_;
   int64_t D_ = 0, ld_ = 0;
   int b_ = 1, esz_ = 16, nq_max_ = 0, cap_ = 0, cur_ = 0, restarts_ = 0;
+  // state of a run between begin() and the end of sync_point()
+  Result res_;
+  Mat M_, B_;
+  std::vector<double> theta_;
+  std::vector<int> pending_;  // basis size before each block step enqueued since the last sync
+  int nq_ = 0, since_check_ = 0, since_best_ = 0;
+  double best_worst_ = INFINITY, t_start_ = 0.0;
+  bool full_ = false, due_ = false;
   int gram_mode_ = kGramVector;  // Gram schedule (kGramVector / kGramMatrix)
   int upd_var_ = 0;              // block-update schedule (index into kUpdateVariants)
   double kappa2_ = 1e4;
@@ -1789,10 +1832,30 @@ void dense_solve(edx_ctx& ctx, const edx_hamiltonian& h, int nev, double* vals, 
 
 void eigs_impl(edx_ctx* ctx, const edx_hamiltonian* ham, const edx_eigs_options& o, double* vals, double* vecs,
                double* info);
+void eigs_batch_impl(edx_ctx* bctx, int n, edx_ctx* const* ctxs, const edx_hamiltonian* const* hams,
+                     const edx_eigs_options& o, double* vals, double* const* vecs, double* info);
 
 }  // namespace
 
 extern "C" {
+
+int edx_eigs_batch(edx_ctx* ctx, int32_t n, edx_ctx* const* ctxs, const edx_hamiltonian* const* hams,
+                   const edx_eigs_options* o, double* vals, double* const* vecs, double* info) {
+  return edx::guarded(ctx, [&] {
+    EDX_REQUIRE(ctx && o && vals && n >= 1 && ctxs && hams, EDX_ERR_INVALID, "null argument or n < 1");
+    // every solve drives its own context from this thread
+    std::vector<std::unique_ptr<edx::CtxClaim>> claims;
+    for (int i = 0; i < n; ++i) {
+      EDX_REQUIRE(ctxs[i] && hams[i], EDX_ERR_INVALID, "null context or Hamiltonian");
+      EDX_REQUIRE(ctxs[i] != ctx && ctxs[i]->device == ctx->device, EDX_ERR_INVALID,
+                  "each solve needs its own context on the batch context's device");
+      for (int j = 0; j < i; ++j) EDX_REQUIRE(ctxs[j] != ctxs[i], EDX_ERR_INVALID, "contexts must be distinct");
+      claims.emplace_back(new edx::CtxClaim(ctxs[i]));
+      if (!claims.back()->ok) throw edx::Error{EDX_ERR_BUSY, "a solve's context is in use by another thread"};
+    }
+    eigs_batch_impl(ctx, n, ctxs, hams, *o, vals, vecs, info);
+  });
+}
 
 int edx_eigs(edx_ctx* ctx, const edx_hamiltonian* ham, int32_t nev, int32_t block, double tol, uint64_t seed,
              int32_t precision, double* vals, double* vecs, double* info) {
@@ -1935,6 +1998,211 @@ void eigs_impl(edx_ctx* ctx, const edx_hamiltonian* ham, const edx_eigs_options&
       info[1] = (double)res.restarts;
       info[2] = res.max_rel_res;
       info[3] = (double)b;
+    }
+  }
+}
+
+// Host threads that run the checks of a batch's solves.
+constexpr int kBatchHostThreads = 8;
+
+// One solve of a batch.
+struct BatchSolve {
+  int idx;
+  edx_ctx* ctx;
+  const edx_hamiltonian* h;
+  std::unique_ptr<BlockLanczos> lz;
+  hipEvent_t ev = nullptr;
+  int64_t matvecs = 0;
+  int restarts = 0;
+};
+
+// Runs the BlockLanczos of every solve to convergence together.  Each round,
+// the solves that are stepping hand their SpMM to one grouped launch on the
+// batch context's stream (which first waits for each of their streams), then
+// enqueue the rest of their block step on their own streams (which wait for
+// the grouped launch); a solve whose step asks for the host leaves the round,
+// and once none is stepping, each waits for its own stream and runs its
+// check (on a few host threads).  Solves that converge leave the batch.
+// `starts[i]` (optional) seeds solve i as BlockLanczos::run does.
+void run_batch_phase(edx_ctx& bctx, std::vector<BatchSolve*>& solves, int precision,
+                     const std::vector<std::pair<const void*, int>>& starts) {
+  for (size_t i = 0; i < solves.size(); ++i) {
+    BatchSolve* s = solves[i];
+    edx::StreamScope scope(s->ctx->stream);
+    s->lz->begin(starts[i].first, starts[i].second, s->lz->ld());
+  }
+  hipEvent_t done_ev;
+  EDX_HIP(hipEventCreateWithFlags(&done_ev, hipEventDisableTiming));
+  try {
+    std::vector<BatchSolve*> active = solves;
+    std::vector<edx::SpmmJob> jobs;
+    while (!active.empty()) {
+      std::vector<BatchSolve*> stepping = active;
+      while (!stepping.empty()) {
+        jobs.clear();
+        for (BatchSolve* s : stepping) {
+          EDX_HIP(hipEventRecord(s->ev, s->ctx->stream));
+          EDX_HIP(hipStreamWaitEvent(bctx.stream, s->ev, 0));
+          jobs.push_back(edx::SpmmJob{&s->lz->op(), s->lz->block(), s->lz->spmm_in(), s->lz->spmm_out(), s->lz->ld()});
+        }
+        edx::spmm_group(jobs.data(), (int)jobs.size(), bctx, precision);
+        EDX_HIP(hipEventRecord(done_ev, bctx.stream));
+        std::vector<BatchSolve*> next;
+        for (BatchSolve* s : stepping) {
+          edx::StreamScope scope(s->ctx->stream);
+          EDX_HIP(hipStreamWaitEvent(s->ctx->stream, done_ev, 0));
+          if (!s->lz->step(true)) next.push_back(s);
+        }
+        stepping.swap(next);
+      }
+      // the checks (stream wait, Ritz problem, restart) of the solves run on
+      // up to kBatchHostThreads threads; each solve is touched by one thread
+      const int n_act = (int)active.size();
+      std::vector<char> cont(n_act, 0);
+      std::vector<std::exception_ptr> err(n_act);
+      auto check = [&](int t, int n_thr) {
+        for (int i = t; i < n_act; i += n_thr) {
+          BatchSolve* s = active[i];
+          try {
+            edx::DeviceGuard dg(s->ctx->device);
+            edx::StreamScope scope(s->ctx->stream);
+            cont[i] = s->lz->sync_point() ? 0 : 1;
+          } catch (...) {
+            err[i] = std::current_exception();
+          }
+        }
+      };
+      const int n_thr = std::min(n_act, kBatchHostThreads);
+      std::vector<std::thread> pool;
+      for (int t = 1; t < n_thr; ++t) pool.emplace_back(check, t, n_thr);
+      check(0, n_thr);
+      for (auto& th : pool) th.join();
+      for (const auto& e : err)
+        if (e) std::rethrow_exception(e);
+      std::vector<BatchSolve*> still;
+      for (int i = 0; i < n_act; ++i)
+        if (cont[i]) still.push_back(active[i]);
+      active.swap(still);
+    }
+  } catch (...) {
+    (void)hipStreamSynchronize(bctx.stream);
+    (void)hipEventDestroy(done_ev);
+    throw;
+  }
+  EDX_HIP(hipStreamSynchronize(bctx.stream));
+  (void)hipEventDestroy(done_ev);
+}
+
+void eigs_batch_impl(edx_ctx* bctx, int n, edx_ctx* const* ctxs, const edx_hamiltonian* const* hams,
+                     const edx_eigs_options& o, double* vals, double* const* vecs, double* info) {
+  const int nev = o.nev;
+  EDX_REQUIRE(nev >= 1, EDX_ERR_INVALID, "nev must be >= 1");
+  EDX_REQUIRE(o.precision == EDX_PREC_FP64 || o.precision == EDX_PREC_MIXED, EDX_ERR_INVALID, "bad precision");
+  EDX_REQUIRE(o.tol > 0.0, EDX_ERR_INVALID, "tol must be positive");
+  EDX_REQUIRE(o.max_restarts >= 1, EDX_ERR_INVALID, "max_restarts must be >= 1");
+  edx::DeviceGuard dg(bctx->device);
+  if (info) std::fill(info, info + 4 * (size_t)n, 0.0);
+  // sectors of the dense path (and empty ones) are solved one by one
+  std::vector<std::unique_ptr<BatchSolve>> own;
+  std::vector<BatchSolve*> lanczos;
+  for (int i = 0; i < n; ++i) {
+    const int64_t D = hams[i]->dim;
+    if (D == 0) continue;
+    if (D <= 500) {
+      edx::StreamScope scope(ctxs[i]->stream);
+      dense_solve(*ctxs[i], *hams[i], nev, vals + (size_t)i * nev, vecs ? vecs[i] : nullptr);
+      continue;
+    }
+    EDX_REQUIRE(nev < D - 1, EDX_ERR_INVALID, "nev must be < sector dimension - 1");
+    own.emplace_back(new BatchSolve{i, ctxs[i], hams[i], nullptr});
+    lanczos.push_back(own.back().get());
+  }
+  for (BatchSolve* s : lanczos) EDX_HIP(hipEventCreateWithFlags(&s->ev, hipEventDisableTiming));
+  struct EventGuard {
+    std::vector<BatchSolve*>& v;
+    ~EventGuard() {
+      for (BatchSolve* s : v)
+        if (s->ev) (void)hipEventDestroy(s->ev);
+    }
+  } eg{lanczos};
+  if (lanczos.empty()) return;
+  int b = o.block > 0 ? o.block : std::min(nev, 8);
+  b = std::max(1, std::min(b, kMaxW));
+  Options opt;
+  opt.nev = nev;
+  opt.block = b;
+  opt.basis = o.basis;
+  opt.tol = o.tol;
+  opt.eig_tol = o.eig_tol;
+  opt.max_restarts = o.max_restarts;
+  opt.seed = o.seed;
+  std::vector<std::pair<const void*, int>> no_start(lanczos.size(), {nullptr, 0});
+  if (o.precision == EDX_PREC_MIXED) {
+    // complex64 sweeps of every solve together, then the FP64 phase of every
+    // solve together, each started as in a single solve (eigs_impl)
+    for (BatchSolve* s : lanczos) {
+      const int n32 = (int)std::min<int64_t>(std::min(nev, kMaxW), s->h->dim - 2);
+      Options o32 = opt;
+      o32.nev = n32;
+      o32.tol = std::max(o.tol, o.mixed_tol > 0 ? o.mixed_tol : 3e-7);
+      o32.eig_tol = 0.0;
+      o32.out_cluster = true;
+      o32.stall_checks = 8;
+      o32.stall_ceiling = std::max(30.0 * o32.tol, 1e-4);
+      edx::StreamScope scope(s->ctx->stream);
+      s->lz.reset(new BlockLanczos(*s->ctx, *s->h, EDX_PREC_MIXED, o32));
+    }
+    run_batch_phase(*bctx, lanczos, EDX_PREC_MIXED, no_start);
+    std::vector<DevBuf<double2>> starts(lanczos.size());
+    std::vector<std::pair<const void*, int>> st(lanczos.size());
+    for (size_t i = 0; i < lanczos.size(); ++i) {
+      BatchSolve* s = lanczos[i];
+      edx::StreamScope scope(s->ctx->stream);
+      const Result& r32 = s->lz->result();
+      const int n32 = (int)std::min<int64_t>(std::min(nev, kMaxW), s->h->dim - 2);
+      const int n_start = std::max(n32, std::min(r32.nvec, kMaxW));
+      Options o64 = opt;
+      o64.block = std::min(kMaxW, std::max(b, n_start));
+      o64.basis = 0;
+      o64.check_every = 1;
+      o64.gap_ref = r32.ritz;
+      o64.gap_ref_res = r32.ritz_res;
+      s->matvecs = r32.matvecs;
+      s->restarts = r32.restarts;
+      const int64_t D = s->h->dim, ld = s->lz->ld();
+      starts[i].alloc((size_t)n_start * ld);
+      hipLaunchKernelGGL((convert_vec<float2, double2>), dim3(edx::grid_for(D, 256)), dim3(256), 0, s->ctx->stream, D,
+                         n_start, (const float2*)s->lz->vec(s->lz->current(), 0), ld, starts[i].p, ld);
+      EDX_HIP(hipGetLastError());
+      st[i] = {starts[i].p, n_start};
+      s->lz.reset(new BlockLanczos(*s->ctx, *s->h, EDX_PREC_FP64, o64));
+    }
+    run_batch_phase(*bctx, lanczos, EDX_PREC_FP64, st);
+  } else {
+    opt.check_every = 2;
+    for (BatchSolve* s : lanczos) {
+      edx::StreamScope scope(s->ctx->stream);
+      s->lz.reset(new BlockLanczos(*s->ctx, *s->h, EDX_PREC_FP64, opt));
+    }
+    run_batch_phase(*bctx, lanczos, EDX_PREC_FP64, no_start);
+  }
+  for (size_t i = 0; i < lanczos.size(); ++i) {
+    BatchSolve* s = lanczos[i];
+    edx::StreamScope scope(s->ctx->stream);
+    const Result& res = s->lz->result();
+    double* v = vals + (size_t)s->idx * nev;
+    for (int c = 0; c < nev; ++c) v[c] = res.vals[c];
+    const int64_t D = s->h->dim;
+    if (vecs && vecs[s->idx])
+      edx::copy_to_host(*s->ctx, vecs[s->idx], D * 16, s->lz->vec(s->lz->current(), 0), s->lz->ld() * 16, D * 16,
+                        nev);
+    EDX_HIP(hipStreamSynchronize(s->ctx->stream));
+    if (info) {
+      double* f = info + 4 * (size_t)s->idx;
+      f[0] = (double)(s->matvecs + res.matvecs);
+      f[1] = (double)(s->restarts + res.restarts);
+      f[2] = res.max_rel_res;
+      f[3] = (double)b;
     }
   }
 }

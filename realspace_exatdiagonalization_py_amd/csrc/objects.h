@@ -98,6 +98,18 @@ struct SellPlan {
   int prec = 0, nb = 0;
 };
 SellPlan plan_sell(const edx_hamiltonian& h, Ctx& ctx);
+// One product of a grouped SpMM: Y = H X for nb interleaved vectors X, Y
+// column-major with leading dimension ldy.
+struct SpmmJob {
+  const edx_hamiltonian* h;
+  int nb;
+  const void* X;
+  void* Y;
+  int64_t ldy;
+};
+// The jobs (one element precision) on ctx's stream; sliced tables of equal
+// block width and lanes per row share launches, with spmm()'s arithmetic.
+void spmm_group(const SpmmJob* jobs, int n, Ctx& ctx, int precision);
 void finish_sell(edx_hamiltonian& h, Ctx& ctx, SellPlan& p);
 void abandon_sell(const edx_hamiltonian& h, SellPlan& p);
 // Slice offsets and storage of a sliced copy with G lanes per row (contents

@@ -23,6 +23,7 @@
 // Hops are enumerated per occupied site over terms grouped by source site,
 // so every lane walks the same number of candidates.
 #include <algorithm>
+#include <type_traits>
 #include <cmath>
 
 #include "objects.h"
@@ -319,15 +320,14 @@ template <> struct Real<float2> { using T = float; };
 // column -1.  X is interleaved (nb values per row, one contiguous gather per
 // entry); Y is column-major with leading dimension ldy.
 // ---------------------------------------------------------------------------
+// Slice s of one table (wavefront `lane` lanes); shared by the single-table
+// and the grouped kernels, so both give identical results.
 template <typename V, int NB, int G>
-__global__ void __launch_bounds__(256)
-spmm_sell(int64_t dim, int64_t n_slices, const int64_t* __restrict__ off, const int32_t* __restrict__ col,
-          const V* __restrict__ val, const V* __restrict__ hdiag, const V* __restrict__ X, V* __restrict__ Y,
-          int64_t ldy) {
+__device__ __forceinline__ void sell_slice(int64_t s, int lane, int64_t dim, const int64_t* __restrict__ off,
+                                           const int32_t* __restrict__ col, const V* __restrict__ val,
+                                           const V* __restrict__ hdiag, const V* __restrict__ X, V* __restrict__ Y,
+                                           int64_t ldy) {
   using T = typename Real<V>::T;
-  const int lane = threadIdx.x & 63;
-  const int64_t s = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (s >= n_slices) return;
   const int64_t o0 = off[s];
   const int64_t L = (off[s + 1] - o0) >> 6;
   const int64_t row = s * (64 / G) + lane / G;
@@ -372,6 +372,48 @@ spmm_sell(int64_t dim, int64_t n_slices, const int64_t* __restrict__ off, const 
       Y[(int64_t)q * ldy + row] = out;
     }
   }
+}
+
+template <typename V, int NB, int G>
+__global__ void __launch_bounds__(256)
+spmm_sell(int64_t dim, int64_t n_slices, const int64_t* __restrict__ off, const int32_t* __restrict__ col,
+          const V* __restrict__ val, const V* __restrict__ hdiag, const V* __restrict__ X, V* __restrict__ Y,
+          int64_t ldy) {
+  const int64_t s = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (s >= n_slices) return;
+  sell_slice<V, NB, G>(s, threadIdx.x & 63, dim, off, col, val, hdiag, X, Y, ldy);
+}
+
+// Grouped SpMM: the sliced products of up to kGroupMax tables (same element
+// precision, block width and lanes per row) in one launch.  Workgroups
+// [first[i], first[i + 1]) serve table i, four slices each, as in spmm_sell.
+constexpr int kGroupMax = 32;
+template <typename V>
+struct SellTable {
+  int64_t dim, n_slices, ldy;
+  const int64_t* off;
+  const int32_t* col;
+  const V* val;
+  const V* hdiag;
+  const V* X;
+  V* Y;
+};
+template <typename V>
+struct SellGroup {
+  int n;
+  int64_t first[kGroupMax + 1];
+  SellTable<V> t[kGroupMax];
+};
+
+template <typename V, int NB, int G>
+__global__ void __launch_bounds__(256) spmm_sell_group(SellGroup<V> g) {
+  const int64_t blk = blockIdx.x;
+  int i = 0;
+  while (i + 1 < g.n && blk >= g.first[i + 1]) ++i;
+  const SellTable<V>& t = g.t[i];
+  const int64_t s = (blk - g.first[i]) * 4 + (threadIdx.x >> 6);
+  if (s >= t.n_slices) return;
+  sell_slice<V, NB, G>(s, threadIdx.x & 63, t.dim, t.off, t.col, t.val, t.hdiag, t.X, t.Y, t.ldy);
 }
 
 template <typename V, int NB>
@@ -612,6 +654,93 @@ void spmm(const edx_hamiltonian& h, Ctx& ctx, int precision, int nb, const void*
   else
     launch_sell<float2>(h, st, nb, h.sell_val32.p, h.hdiag32.p, (const float2*)X, (float2*)Y, ldy);
   ctx.timer.stop(st, t0, precision == EDX_PREC_FP64 ? K_SPMM : K_SPMM32, spmm_bytes(h, precision, nb));
+}
+
+template <typename V, int NB>
+void launch_group_nb(const SellGroup<V>& g, int lanes, unsigned blocks, hipStream_t st) {
+  switch (lanes) {
+    case 1: hipLaunchKernelGGL((spmm_sell_group<V, NB, 1>), dim3(blocks), dim3(256), 0, st, g); break;
+    case 2: hipLaunchKernelGGL((spmm_sell_group<V, NB, 2>), dim3(blocks), dim3(256), 0, st, g); break;
+    default: hipLaunchKernelGGL((spmm_sell_group<V, NB, 4>), dim3(blocks), dim3(256), 0, st, g);
+  }
+  EDX_HIP(hipGetLastError());
+}
+
+template <typename V>
+void launch_group(const SpmmJob* jobs, const std::vector<int>& idx, int precision, hipStream_t st) {
+  SellGroup<V> g{};
+  g.n = (int)idx.size();
+  g.first[0] = 0;
+  for (int i = 0; i < g.n; ++i) {
+    const SpmmJob& j = jobs[idx[i]];
+    const edx_hamiltonian& h = *j.h;
+    SellTable<V>& t = g.t[i];
+    t.dim = h.dim;
+    t.n_slices = h.sell_slices;
+    t.ldy = j.ldy;
+    t.off = h.sell_off.p;
+    t.col = h.sell_col.p;
+    if constexpr (std::is_same<V, double2>::value) {
+      t.val = h.sell_val.p;
+      t.hdiag = h.hdiag.p;
+    } else {
+      t.val = h.sell_val32.p;
+      t.hdiag = h.hdiag32.p;
+    }
+    t.X = (const V*)j.X;
+    t.Y = (V*)j.Y;
+    g.first[i + 1] = g.first[i] + (h.sell_slices + 3) / 4;
+  }
+  EDX_REQUIRE(g.first[g.n] < (int64_t(1) << 31), EDX_ERR_INTERNAL, "grouped SpMM grid");
+  const unsigned blocks = (unsigned)g.first[g.n];
+  const int lanes = jobs[idx[0]].h->sell_g;
+  switch (jobs[idx[0]].nb) {
+#define EDX_GROUP_CASE(N) \
+  case N:                 \
+    launch_group_nb<V, N>(g, lanes, blocks, st); \
+    break;
+    EDX_GROUP_CASE(1) EDX_GROUP_CASE(2) EDX_GROUP_CASE(3) EDX_GROUP_CASE(4) EDX_GROUP_CASE(5) EDX_GROUP_CASE(6)
+    EDX_GROUP_CASE(7) EDX_GROUP_CASE(8) EDX_GROUP_CASE(9) EDX_GROUP_CASE(10) EDX_GROUP_CASE(11) EDX_GROUP_CASE(12)
+    EDX_GROUP_CASE(13) EDX_GROUP_CASE(14) EDX_GROUP_CASE(15) EDX_GROUP_CASE(16)
+#undef EDX_GROUP_CASE
+    default:
+      throw Error{EDX_ERR_INTERNAL, "SpMM block width out of range"};
+  }
+  (void)precision;
+}
+
+// Applies each job's H to its block (all of one precision) on ctx's stream.
+// Sliced tables of equal block width and lanes per row share one launch (up
+// to kGroupMax tables each); on-the-fly tables run one by one.  Every table
+// gets exactly the arithmetic of spmm().
+void spmm_group(const SpmmJob* jobs, int n, Ctx& ctx, int precision) {
+  hipStream_t st = ctx.stream;
+  std::vector<char> left(n, 1);
+  for (int i = 0; i < n; ++i) {
+    const edx_hamiltonian& h = *jobs[i].h;
+    EDX_REQUIRE(jobs[i].nb >= 1 && jobs[i].nb <= 16, EDX_ERR_INVALID, "at most 16 vectors per SpMM");
+    if (h.dim == 0) {
+      left[i] = 0;
+    } else if (h.on_the_fly) {
+      spmm(h, ctx, precision, jobs[i].nb, jobs[i].X, jobs[i].Y, jobs[i].ldy);
+      left[i] = 0;
+    }
+  }
+  for (int i0 = 0; i0 < n; ++i0) {
+    if (!left[i0]) continue;
+    std::vector<int> idx;
+    double bytes = 0.0;
+    for (int i = i0; i < n && (int)idx.size() < kGroupMax; ++i)
+      if (left[i] && jobs[i].nb == jobs[i0].nb && jobs[i].h->sell_g == jobs[i0].h->sell_g) {
+        idx.push_back(i);
+        left[i] = 0;
+        bytes += spmm_bytes(*jobs[i].h, precision, jobs[i].nb);
+      }
+    hipEvent_t t0 = ctx.timer.start(st);
+    if (precision == EDX_PREC_FP64) launch_group<double2>(jobs, idx, precision, st);
+    else launch_group<float2>(jobs, idx, precision, st);
+    ctx.timer.stop(st, t0, precision == EDX_PREC_FP64 ? K_SPMM : K_SPMM32, bytes);
+  }
 }
 
 // ---- construction of the sliced copy from the CSR table

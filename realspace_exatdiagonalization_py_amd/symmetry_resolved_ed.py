@@ -59,9 +59,11 @@ from .groups import (
 )
 from .second_quantized_model import Particle_Statistics, Real_Space_Second_Quantized_Model
 
-#: Worker contexts of ``run_concurrently``, per device, reused across calls.
+#: Worker contexts of ``run_concurrently`` and solve contexts of batched
+#: solves, per device, reused across calls.
 _WORKER_LOCK = threading.Lock()
 _WORKER_CONTEXTS: dict[int, list[Context]] = {}
+_BATCH_CONTEXTS: dict[int, list[Context]] = {}
 
 #: Split-table rank gate of the reference (``SRE:163``); the device ranks
 #: with the O(k) colex formula at every size, the tables are built on request.
@@ -833,6 +835,79 @@ def diagonalize_block_matrixfree(H_op: MatrixFreeHamiltonian, n: int, *, nev: in
     return vals, vecs
 
 
+def diagonalize_blocks_batched(H_ops, *, nev: int = 5, precision: int = _edx.PREC_FP64, eig_tol: float = 0.0,
+                               want_vectors: bool = True, tol: float = 1e-10, seed: int = 12345):
+    """Lowest ``nev`` eigenpairs of several device operators solved together
+    (``edx_eigs_batch``): every solve on its own HIP stream, the block-step
+    SpMMs of all solves grouped into shared launches, one host thread.  Each
+    result equals ``diagonalize_block_matrixfree`` on the same operator bit for
+    bit (same schedules).  Returns a list of ``(vals, vecs)``; each operator's
+    ``last_info`` is set.  Sectors of the dense path (n <= 500) and empty ones
+    are solved one by one."""
+    ops = list(H_ops)
+    out: list = [None] * len(ops)
+    batch = [i for i, H in enumerate(ops) if H.n > DENSE_CUTOFF]
+    for i, H in enumerate(ops):
+        if i not in batch:
+            out[i] = diagonalize_block_matrixfree(H, H.n, nev=nev, precision=precision, eig_tol=eig_tol,
+                                                  want_vectors=want_vectors)
+    if not batch:
+        return out
+    bctx = _edx.current_context()
+    ctxs = _batch_contexts(len(batch))
+    n = len(batch)
+    vals = np.zeros((n, nev), dtype=np.float64)
+    info = np.zeros((n, 4), dtype=np.float64)
+    vecs = [np.empty((nev, ops[i].n), dtype=np.complex128) for i in batch] if want_vectors else None
+    ctx_arr = (_edx.C.c_void_p * n)(*[c.handle for c in ctxs])
+    ham_arr = (_edx.C.c_void_p * n)(*[ops[i]._dev.handle for i in batch])
+    vec_arr = (_edx.C.c_void_p * n)(*[v.ctypes.data for v in vecs]) if want_vectors else None
+    opts = _edx.eigs_options(nev=nev, precision=precision, tol=tol, eig_tol=eig_tol, seed=seed)
+    bctx.check(lib.edx_eigs_batch(bctx.handle, n, ctx_arr, ham_arr, _edx.C.byref(opts), ptr(vals), vec_arr,
+                                  ptr(info)))
+    for j, i in enumerate(batch):
+        ops[i].last_info = info[j].copy()
+        out[i] = (vals[j].copy(), vecs[j].T if want_vectors else None)
+    return out
+
+
+def solve_sectors_batched(ed_data: "Symmetry_Resolved_ED_Data", *, nev: int = 5, precision: int = _edx.PREC_MIXED,
+                          scanned_sectors=None, eig_tol: float = 0.0, want_vectors: bool = True,
+                          table: str = "auto"):
+    """The matrix-free solves of several sectors (all, or ``scanned_sectors``)
+    as one batch (``diagonalize_blocks_batched``); results go to
+    ``ed_data.ed_scan_res`` and ``sector_dims`` as ``ed_scan`` stores them."""
+    model = ed_data.second_quantized_model
+    stats = model.particle_statistics
+    cmap = CanonicalMap(ed_data.symmetry_group, stats, ed_data.orbit_catalog)
+    todo = [i for i, irrep in enumerate(ed_data.irrep_list)
+            if scanned_sectors is None or irrep.label in scanned_sectors]
+    built: dict = {}
+
+    def build(irrep_idx):  # irrep filter and table, on the worker's stream
+        basis = build_symmetry_sector_basis(ed_data.orbit_catalog, ed_data.irrep_list[irrep_idx])
+        H = None
+        if basis.dim:
+            H, _ = hamiltonian_linear_operator(basis, model.bilinear_terms, model.density_density_terms, cmap,
+                                               table=table)
+        built[irrep_idx] = (basis.dim, H)
+
+    run_concurrently(todo, build, max(1, min(len(todo), 8)))
+    idx, ops = [], []
+    for irrep_idx in todo:
+        dim, H = built[irrep_idx]
+        ed_data.sector_dims[irrep_idx] = dim
+        if H is None:
+            ed_data.ed_scan_res[irrep_idx] = (np.zeros(0, dtype=np.float64), np.zeros((0, 0), dtype=np.complex128))
+            continue
+        idx.append(irrep_idx)
+        ops.append(H)
+    for irrep_idx, r in zip(idx, diagonalize_blocks_batched(ops, nev=nev, precision=precision, eig_tol=eig_tol,
+                                                            want_vectors=want_vectors)):
+        ed_data.ed_scan_res[irrep_idx] = r
+    return [ed_data.ed_scan_res[i] for i in idx]
+
+
 # ---------------------------------------------------------------------------
 # ED data and scans
 # ---------------------------------------------------------------------------
@@ -916,6 +991,17 @@ def _worker_contexts(n: int) -> list[Context]:
     with _WORKER_LOCK:
         dev = Context.default().device
         pool = _WORKER_CONTEXTS.setdefault(dev, [])
+        while len(pool) < n:
+            pool.append(Context(dev))
+        return pool[:n]
+
+
+def _batch_contexts(n: int) -> list[Context]:
+    """``n`` contexts for the solves of a batch (a pool apart from the worker
+    pool, so a batch never claims a worker's context), kept for reuse."""
+    with _WORKER_LOCK:
+        dev = Context.default().device
+        pool = _BATCH_CONTEXTS.setdefault(dev, [])
         while len(pool) < n:
             pool.append(Context(dev))
         return pool[:n]
@@ -1141,6 +1227,6 @@ __all__ = [
     "build_ed_Hamiltonian_symmetry_block", "hamiltonian_linear_operator", "apply_hamiltonian",
     "apply_hamiltonian_with_threads", "build_matrixfree_projection_table", "rank_combination", "group_order",
     "ed_scan", "ed_scan_at_irrep_matrix", "ed_scan_at_irrep_matrixfree", "diagonalize_block_dense",
-    "diagonalize_block_arpack", "diagonalize_block_matrixfree", "full_ed", "save_checkpoint", "load_checkpoint",
-    "ed_scan_checkpoint_filename", "print_spectrum",
+    "diagonalize_block_arpack", "diagonalize_block_matrixfree", "diagonalize_blocks_batched", "solve_sectors_batched",
+    "full_ed", "save_checkpoint", "load_checkpoint", "ed_scan_checkpoint_filename", "print_spectrum",
 ]
