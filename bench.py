@@ -437,7 +437,50 @@ def hubbard_secondary(ed, _edx, workers=8, passes=2):
             "sector_solves": len(lat), "workers": workers, "solves_per_s": round(len(lat) / el, 3),
             "p50_ms": round(float(np.percentile(lat_ms, 50)), 3), "p99_ms": round(float(np.percentile(lat_ms, 99)), 3),
             "batched_solves_per_s": round(passes * len(data.irrep_list) / el_b, 3),
-            "batched_max_abs_dE_vs_streams": d_b}
+            "batched_max_abs_dE_vs_streams": d_b,
+            "grouped_products": grouped_products(ed, _edx, data)}
+
+
+def grouped_products(ed, _edx, data, nb=6, reps=50):
+    """H X for all 12 sectors (complex64, nb vectors each, device buffers):
+    12 single-sector calls, one grouped launch, and a replay of the captured
+    graph of that launch; wall time per sweep of the 12 products."""
+    model = data.second_quantized_model
+    ctx = _edx.current_context()
+    ops = []
+    for irrep in data.irrep_list:
+        basis = ed.build_symmetry_sector_basis(data.orbit_catalog, irrep)
+        ops.append(ed.MatrixFreeHamiltonian(basis, model.bilinear_terms, model.density_density_terms, None))
+    hams = [H._dev for H in ops]
+    rng = np.random.default_rng(4)
+    xs, ys = [], []
+    for H in ops:
+        x = (rng.standard_normal((H.n, nb)) + 1j * rng.standard_normal((H.n, nb))).astype(np.complex64)
+        xs.append(_edx.DeviceArray(ctx, x.nbytes))
+        xs[-1].upload(x)
+        ys.append(_edx.DeviceArray(ctx, x.nbytes))
+
+    def wall(f):
+        f()
+        ctx.check(_edx.lib.edx_synchronize(ctx.handle))
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            f()
+        ctx.check(_edx.lib.edx_synchronize(ctx.handle))
+        return (time.perf_counter() - t0) / reps * 1e6
+
+    def singles():
+        for h, x, y in zip(hams, xs, ys):
+            ctx.check(_edx.lib.edx_matvec_device(ctx.handle, h.handle, ed.PREC_MIXED, nb, x.ptr, y.ptr))
+
+    g = _edx.MatvecGroupGraph(ctx, hams, ed.PREC_MIXED, nb, xs, ys)
+    out = {"sectors": len(ops), "vectors_each": nb, "precision": "c64",
+           "single_calls_us": round(wall(singles), 1),
+           "grouped_launch_us": round(wall(lambda: _edx.matvec_group(ctx, hams, ed.PREC_MIXED, nb, xs, ys)), 1),
+           "graph_replay_us": round(wall(g.launch), 1)}
+    bytes_ = sum(H._dev.dim * (8 + 16 * nb) + H._dev.n_entries * (12 + 8 * nb) for H in ops)
+    out["graph_replay_algorithmic_GBps"] = round(bytes_ / out["graph_replay_us"] / 1e3, 1)
+    return out
 
 
 def checkerboard_bench(args) -> int:

@@ -45,6 +45,7 @@ typedef struct edx_ctx edx_ctx;
 typedef struct edx_catalog edx_catalog;
 typedef struct edx_sector edx_sector;
 typedef struct edx_hamiltonian edx_hamiltonian;
+typedef struct edx_graph edx_graph;
 
 /* ---- context ------------------------------------------------------------
  * Device context owning one HIP stream.  No reference counterpart (the
@@ -197,6 +198,24 @@ int edx_matvec_device(edx_ctx* ctx, const edx_hamiltonian* ham, int32_t precisio
  * call completes on the ctx stream before it returns; separate processes
  * have separate arrays. */
 int edx_matvec_noalloc(edx_ctx* ctx, const edx_hamiltonian* ham, int32_t n_vec, const double* x, double* y);
+/* Grouped products on device buffers: y_dev[i] = H_i x_dev[i] for n sector
+ * Hamiltonians, n_vec <= 16 vectors each (layouts as edx_matvec_device,
+ * leading dimension dim_i), in one launch per group of sliced tables of equal
+ * lanes per row (on-the-fly tables one by one).  Applies the eigensolver's
+ * operator (the Hermitian part for twisted tables, see edx_matvec_device).
+ * edx_matvec_group runs it and waits; edx_matvec_group_graph captures the
+ * same launches into a HIP graph (no memory operations in it), which
+ * edx_graph_launch enqueues on a ctx stream without waiting: every launch
+ * reads the buffers' current contents and overwrites y (same results, bit for
+ * bit, as edx_matvec_group).  The Hamiltonians and buffers must outlive the
+ * graph.  No reference counterpart (the reference applies one sector at a
+ * time, SRE:1884-1976). */
+int edx_matvec_group(edx_ctx* ctx, int32_t n, const edx_hamiltonian* const* hams, int32_t precision,
+                     int32_t n_vec, const void* const* x_dev, void* const* y_dev);
+int edx_matvec_group_graph(edx_ctx* ctx, int32_t n, const edx_hamiltonian* const* hams, int32_t precision,
+                           int32_t n_vec, const void* const* x_dev, void* const* y_dev, edx_graph** out);
+int edx_graph_launch(edx_ctx* ctx, const edx_graph* graph);
+int edx_graph_destroy(edx_graph* graph);
 int edx_hamiltonian_destroy(edx_hamiltonian* ham);
 
 /* ---- eigensolver ----------------------------------------------------------

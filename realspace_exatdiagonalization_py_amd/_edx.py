@@ -75,6 +75,10 @@ SIGNATURES = {
     "edx_autotune_clear": (C.c_int, []),
     "edx_schedule_table": (C.c_int, [i32, P, P, pi32]),
     "edx_eigs_batch": (C.c_int, [P, i32, P, P, P, P, P, P]),
+    "edx_matvec_group": (C.c_int, [P, i32, P, i32, i32, P, P]),
+    "edx_matvec_group_graph": (C.c_int, [P, i32, P, i32, i32, P, P, PP]),
+    "edx_graph_launch": (C.c_int, [P, P]),
+    "edx_graph_destroy": (C.c_int, [P]),
     "edx_set_solver_schedule": (C.c_int, [P, i32, i32]),
     "edx_hamiltonian_autotune": (C.c_int, [P, P, i32, i32, pi32, P]),
     "edx_host_herm_eig": (C.c_int, [i32, P, i32, P, P]),
@@ -321,6 +325,45 @@ class DeviceArray:
         if p:
             lib.edx_device_free(self.ctx.handle, p)
             self.ptr = None
+
+
+def _group_args(hams, xs, ys):
+    n = len(hams)
+    assert n == len(xs) == len(ys) and n >= 1
+    return (n, (C.c_void_p * n)(*[h.handle for h in hams]), (C.c_void_p * n)(*[x.ptr.value for x in xs]),
+            (C.c_void_p * n)(*[y.ptr.value for y in ys]))
+
+
+def matvec_group(ctx: "Context", hams, precision: int, n_vec: int, xs, ys) -> None:
+    """``ys[i] = H_i xs[i]`` for device Hamiltonians ``hams`` (``_DeviceHamiltonian``)
+    and ``DeviceArray`` buffers, grouped launches; waits."""
+    n, h, x, y = _group_args(hams, xs, ys)
+    ctx.check(lib.edx_matvec_group(ctx.handle, n, h, int(precision), int(n_vec), x, y))
+
+
+class MatvecGroupGraph:
+    """The grouped products of ``matvec_group`` captured once into a HIP graph;
+    ``launch()`` enqueues a replay on ``ctx``'s stream (``synchronize()``
+    waits).  Hamiltonians and buffers are kept alive with the graph."""
+
+    def __init__(self, ctx: "Context", hams, precision: int, n_vec: int, xs, ys):
+        self.ctx = ctx
+        self._keep = (list(hams), list(xs), list(ys))
+        n, h, x, y = _group_args(hams, xs, ys)
+        g = C.c_void_p()
+        ctx.check(lib.edx_matvec_group_graph(ctx.handle, n, h, int(precision), int(n_vec), x, y, C.byref(g)))
+        self.handle = g
+
+    def launch(self) -> None:
+        self.ctx.check(lib.edx_graph_launch(self.ctx.handle, self.handle))
+
+    def synchronize(self) -> None:
+        self.ctx.check(lib.edx_synchronize(self.ctx.handle))
+
+    def __del__(self):
+        if getattr(self, "handle", None):
+            lib.edx_graph_destroy(self.handle)
+            self.handle = None
 
 
 def host_herm_eig(a: np.ndarray, upper: bool = True):

@@ -79,6 +79,13 @@ struct edx_hamiltonian {
   edx::ObjBuf<double2> fly_chi;       // [EDX_MAX_GROUP] characters of the sector's irrep
 };
 
+// A captured HIP graph of grouped sector products and its executable form.
+struct edx_graph {
+  int device = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
 namespace edx {
 // (Re)builds the sliced copy of the table with G lanes per row (1, 2 or 4).
 void build_sell(edx_hamiltonian& h, Ctx& ctx, int G);

@@ -1344,6 +1344,101 @@ int edx_matvec_device(edx_ctx* ctx, const edx_hamiltonian* ham, int32_t precisio
   });
 }
 
+}  // extern "C"
+
+namespace {
+
+// Checked job list of a grouped product: Y_i = H_i X_i on device buffers
+// (interleaved X [dim][n_vec], column-major Y [n_vec][dim]).
+std::vector<edx::SpmmJob> group_jobs(edx_ctx* ctx, int32_t n, const edx_hamiltonian* const* hams, int32_t precision,
+                                     int32_t n_vec, const void* const* x_dev, void* const* y_dev) {
+  EDX_REQUIRE(ctx && hams && x_dev && y_dev && n >= 1, EDX_ERR_INVALID, "null argument or n < 1");
+  EDX_REQUIRE(precision == EDX_PREC_FP64 || precision == EDX_PREC_MIXED, EDX_ERR_INVALID, "bad precision");
+  EDX_REQUIRE(n_vec >= 1 && n_vec <= 16, EDX_ERR_INVALID, "1 to 16 vectors per product");
+  std::vector<edx::SpmmJob> jobs(n);
+  for (int i = 0; i < n; ++i) {
+    EDX_REQUIRE(hams[i] && x_dev[i] && y_dev[i], EDX_ERR_INVALID, "null Hamiltonian or buffer");
+    EDX_REQUIRE(hams[i]->device == ctx->device, EDX_ERR_INVALID, "Hamiltonian on another device");
+    EDX_REQUIRE(hams[i]->on_the_fly || hams[i]->dim == 0 || hams[i]->sell_slices > 0, EDX_ERR_INVALID,
+                "Hamiltonian without a sliced table");
+    EDX_REQUIRE((int64_t)(hams[i]->sell_slices + 3) / 4 < (int64_t(1) << 31), EDX_ERR_INVALID, "table too large");
+    jobs[i] = edx::SpmmJob{hams[i], n_vec, x_dev[i], y_dev[i], hams[i]->dim};
+  }
+  return jobs;
+}
+
+}  // namespace
+
+extern "C" {
+
+int edx_matvec_group(edx_ctx* ctx, int32_t n, const edx_hamiltonian* const* hams, int32_t precision, int32_t n_vec,
+                     const void* const* x_dev, void* const* y_dev) {
+  return edx::guarded(ctx, [&] {
+    const std::vector<edx::SpmmJob> jobs = group_jobs(ctx, n, hams, precision, n_vec, x_dev, y_dev);
+    edx::DeviceGuard dg(ctx->device);
+    edx::spmm_group(jobs.data(), n, *ctx, precision);
+    EDX_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+int edx_matvec_group_graph(edx_ctx* ctx, int32_t n, const edx_hamiltonian* const* hams, int32_t precision,
+                           int32_t n_vec, const void* const* x_dev, void* const* y_dev, edx_graph** out) {
+  return edx::guarded(ctx, [&] {
+    EDX_REQUIRE(out, EDX_ERR_INVALID, "null argument");
+    *out = nullptr;
+    const std::vector<edx::SpmmJob> jobs = group_jobs(ctx, n, hams, precision, n_vec, x_dev, y_dev);
+    edx::DeviceGuard dg(ctx->device);
+    EDX_HIP(hipStreamSynchronize(ctx->stream));
+    auto g = new edx_graph();
+    g->device = ctx->device;
+    // the launches only: the kernel timer's events stay out of the graph
+    const bool timer_on = ctx->timer.on;
+    ctx->timer.on = false;
+    hipError_t cap = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
+    try {
+      EDX_HIP(cap);
+      edx::spmm_group(jobs.data(), n, *ctx, precision);
+      EDX_HIP(hipStreamEndCapture(ctx->stream, &g->graph));
+      cap = hipErrorUnknown;  // capture ended
+      EDX_HIP(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
+    } catch (...) {
+      ctx->timer.on = timer_on;
+      if (cap == hipSuccess) {  // still capturing: end it so the stream is usable again
+        hipGraph_t partial = nullptr;
+        (void)hipStreamEndCapture(ctx->stream, &partial);
+        if (partial) (void)hipGraphDestroy(partial);
+      }
+      if (g->exec) (void)hipGraphExecDestroy(g->exec);
+      if (g->graph) (void)hipGraphDestroy(g->graph);
+      delete g;
+      throw;
+    }
+    ctx->timer.on = timer_on;
+    *out = g;
+  });
+}
+
+int edx_graph_launch(edx_ctx* ctx, const edx_graph* g) {
+  return edx::guarded(ctx, [&] {
+    EDX_REQUIRE(ctx && g && g->exec, EDX_ERR_INVALID, "null argument");
+    EDX_REQUIRE(g->device == ctx->device, EDX_ERR_INVALID, "graph captured on another device");
+    edx::DeviceGuard dg(ctx->device);
+    EDX_HIP(hipGraphLaunch(g->exec, ctx->stream));
+  });
+}
+
+int edx_graph_destroy(edx_graph* g) {
+  if (!g) return EDX_OK;
+  int prev = 0;
+  (void)hipGetDevice(&prev);
+  (void)hipSetDevice(g->device);
+  if (g->exec) (void)hipGraphExecDestroy(g->exec);
+  if (g->graph) (void)hipGraphDestroy(g->graph);
+  (void)hipSetDevice(prev);
+  delete g;
+  return EDX_OK;
+}
+
 int edx_time_matvec(edx_ctx* ctx, const edx_hamiltonian* ham, int32_t precision, int32_t n_vec, int32_t n_rep,
                     const void* x_dev, void* y_dev, double* avg_ms) {
   return edx::guarded(ctx, [&] {
