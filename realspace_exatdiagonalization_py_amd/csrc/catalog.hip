@@ -510,6 +510,34 @@ int edx_catalog_fetch(edx_ctx* ctx, const edx_catalog* cat, int32_t* rep_rank, i
 }  // extern "C"
 
 namespace {
+__global__ void rank_records(int64_t n, const int32_t* rep_rank, const int32_t* rep_sym, const float2* rep_amp,
+                             edx::RankRec* out) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const float2 a = rep_amp[i];
+  out[i] = edx::RankRec{rep_rank[i], rep_sym[i], a.x, a.y};
+}
+
+void fill_rank_records(edx_catalog& cat, hipStream_t st) {
+  if (cat.n_total == 0) return;
+  hipLaunchKernelGGL(rank_records, dim3(edx::grid_for(cat.n_total, kBlock)), dim3(kBlock), 0, st, cat.n_total,
+                     (const int32_t*)cat.rep_rank.p, (const int32_t*)cat.rep_sym.p, (const float2*)cat.rep_amp.p,
+                     cat.rank_rec.p);
+  EDX_HIP(hipGetLastError());
+}
+}  // namespace
+
+namespace edx {
+void ensure_rank_records(edx_catalog& cat, Ctx& ctx) {
+  std::lock_guard<std::mutex> lk(cat.rank_rec_mu);
+  if (cat.rank_rec.p || cat.n_total == 0) return;
+  cat.rank_rec.alloc((size_t)cat.n_total);
+  fill_rank_records(cat, ctx.stream);
+  EDX_HIP(hipStreamSynchronize(ctx.stream));
+}
+}  // namespace edx
+
+namespace {
 void refresh_phases(edx_ctx* ctx, edx_catalog* cat, const int32_t* perm, const double* perm_phases,
                     const double* perm_angles, int fresh) {
     EDX_REQUIRE(ctx && cat && perm && perm_phases && perm_angles, EDX_ERR_INVALID, "null argument");
@@ -530,6 +558,10 @@ void refresh_phases(edx_ctx* ctx, edx_catalog* cat, const int32_t* perm, const d
                        ctx->stream, gv, cat->n_total, (const int32_t*)cat->rep_rank.p,
                        (const int32_t*)cat->rep_sym.p, (const uint64_t*)cat->reps.p, cat->rep_amp.p, fresh);
     EDX_HIP(hipGetLastError());
+    {
+      std::lock_guard<std::mutex> lk(cat->rank_rec_mu);
+      if (cat->rank_rec.p) fill_rank_records(*cat, ctx->stream);
+    }
     EDX_HIP(hipStreamSynchronize(ctx->stream));
 }
 }  // namespace

@@ -280,6 +280,14 @@ struct DeviceGuard {
 // ---------------------------------------------------------------------------
 // Group description shared by the catalog / phase / table kernels.
 // ---------------------------------------------------------------------------
+// One rank's catalog entry packed for a single 16-byte load (the on-the-fly
+// product's gather): orbit index, group element, alpha (complex64).
+struct __attribute__((aligned(16))) RankRec {
+  int32_t orbit;
+  int32_t g;
+  float ax, ay;
+};
+
 struct GroupView {
   int n_site;
   int n_filled;

@@ -2,6 +2,7 @@
 #pragma once
 
 #include <exception>
+#include <mutex>
 #include <new>
 
 #include "edx_internal.h"
@@ -29,6 +30,10 @@ struct edx_catalog {
   edx::ObjBuf<uint32_t> stab_off;     // [n_orbits + 1]
   edx::ObjBuf<uint8_t> stab_g;        // [n_stab]
   edx::ObjBuf<double2> stab_phase;    // [n_stab]
+  // [C] rep_rank, rep_sym and rep_amp packed per rank, built on first use by
+  // an on-the-fly Hamiltonian and rebuilt by every phase refresh
+  edx::ObjBuf<edx::RankRec> rank_rec;
+  std::mutex rank_rec_mu;
 
   edx::GroupView view() const {
     return edx::GroupView{n_site, n_filled, n_g, fermionic, product_phase, perm.p, angle.p,
@@ -87,6 +92,8 @@ struct edx_graph {
 };
 
 namespace edx {
+// Builds cat.rank_rec if it is not there (thread-safe).
+void ensure_rank_records(edx_catalog& cat, Ctx& ctx);
 // (Re)builds the sliced copy of the table with G lanes per row (1, 2 or 4).
 void build_sell(edx_hamiltonian& h, Ctx& ctx, int G);
 // Lanes per row for a table of this shape.

@@ -483,9 +483,7 @@ struct FlyView {
   const uint64_t* reps;
   const int32_t* stab;
   const int32_t* o2b;
-  const int32_t* rep_rank;
-  const int32_t* rep_sym;
-  const float2* rep_amp;
+  const RankRec* recs;
   const uint32_t* binom;
   const int32_t* toff;
   const uint8_t* to;
@@ -541,11 +539,11 @@ spmm_fly(FlyView fv, const V* __restrict__ hdiag, const V* __restrict__ X, V* __
         if ((m >> tb) & 1ull) continue;
         const uint64_t nm = m ^ (1ull << s) ^ (1ull << tb);
         const uint64_t idx = edxd::rank_colex(nm, binom, nf1);
-        const int32_t x = fv.o2b[fv.rep_rank[idx]];
+        const RankRec rec = fv.recs[idx];  // one 16-byte load
+        const int32_t x = fv.o2b[rec.orbit];
         if (x < 0) continue;
-        const int g = fv.rep_sym[idx];
-        const float2 a32 = fv.rep_amp[idx];
-        const double2 a = make_double2((double)a32.x, (double)a32.y);
+        const int g = rec.g;
+        const double2 a = make_double2((double)rec.ax, (double)rec.ay);
         const double sj = (fv.fermionic && (edxd::popc64(m & betw[q]) & 1)) ? -1.0 : 1.0;
         const double2 t = amp[q];
         double2 h = make_double2(t.x * sj - t.y * 0.0, t.x * 0.0 + t.y * sj);
@@ -594,9 +592,13 @@ constexpr int kFlyLanes = 4;
 FlyView fly_view(const edx_hamiltonian& h) {
   const edx_sector& sec = *h.sec;
   const edx_catalog& cat = *sec.cat;
+  EDX_REQUIRE(cat.exact_alpha, EDX_ERR_INVALID,
+              "the catalog's phases were refreshed to a twist after this on-the-fly Hamiltonian was built: its "
+              "block is no longer Hermitian; build a stored table");
+  EDX_REQUIRE(cat.rank_rec.p, EDX_ERR_INTERNAL, "catalog rank records missing");
   return FlyView{h.dim, cat.n_site, cat.n_filled, h.n_bilinear, cat.fermionic, sec.reps.p, sec.stab_order.p,
-                 sec.orbit_to_basis.p, cat.rep_rank.p, cat.rep_sym.p, cat.rep_amp.p, cat.binom.p, h.fly_toff.p,
-                 h.fly_to.p, h.fly_amp.p, h.fly_between.p, h.fly_chi.p};
+                 sec.orbit_to_basis.p, cat.rank_rec.p, cat.binom.p, h.fly_toff.p, h.fly_to.p, h.fly_amp.p,
+                 h.fly_between.p, h.fly_chi.p};
 }
 
 template <typename V, int NB>
@@ -1089,6 +1091,7 @@ int build_hamiltonian(edx_ctx* ctx, const edx_catalog* cat, const edx_sector* se
       if (fly) {
         ham->on_the_fly = 1;
         ham->sell_g = 0;
+        edx::ensure_rank_records(*const_cast<edx_catalog*>(cat), *ctx);
         EDX_REQUIRE(edx::fly_smem_bytes(N, cat->n_filled, T) <= 65536, EDX_ERR_INVALID,
                     "too many bilinear terms for the on-the-fly product's shared-memory term list");
         std::vector<double2> amp_g(std::max(T, 1));

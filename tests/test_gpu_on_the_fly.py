@@ -17,7 +17,10 @@ Haldane-type bosons, one sector past the dense cutoff and one below it):
   within 1e-10;
 * ``edx_matvec_noalloc`` on the on-the-fly Hamiltonian equals ``edx_matvec``;
 * the table accessors refuse it, a twisted catalog is refused, and
-  ``table="auto"`` keeps the stored table when it fits.
+  ``table="auto"`` keeps the stored table when it fits;
+* after its catalog's phases are refreshed to a twist the product is refused,
+  and after a refresh back to zero twist it equals the first product bit for
+  bit (the packed rank records are rebuilt with the phases).
 """
 
 from __future__ import annotations
@@ -160,3 +163,25 @@ def test_on_the_fly_refusals_and_auto(monkeypatch):
         ed.MatrixFreeHamiltonian(basis, model.bilinear_terms, model.density_density_terms, None, table="on_the_fly")
     Ht = ed.MatrixFreeHamiltonian(basis, model.bilinear_terms, model.density_density_terms, None, table="auto")
     assert not Ht.on_the_fly
+
+
+def test_on_the_fly_follows_catalog_phase_refresh():
+    from realspace_exatdiagonalization_py_amd.groups import build_translation_group
+    from realspace_exatdiagonalization_py_amd import second_quantized_model as SQ
+
+    model, ne, G = make_case("hal34")
+    cat = ed.build_symmetry_orbit_catalog(second_quantized_model=model, n_filled=ne, symmetry_group=G,
+                                          particle_statistics=model.particle_statistics)
+    basis = ed.build_symmetry_sector_basis(cat, build_irrep_list(G, model.lattice)[1])
+    Hf = ed.MatrixFreeHamiltonian(basis, model.bilinear_terms, model.density_density_terms, None,
+                                  table="on_the_fly")
+    x = random_vector(Hf.n, 21)
+    y0 = Hf(x)
+    m2, _ne, _G = make_case("hal34")
+    SQ.update_second_quantized_model_with_twisted_phases(m2, twisted_phases_over_2π=[0.3, 0.0])
+    ed.update_orbit_stabilizer_phases(cat, build_translation_group(m2.lattice, [0.3, 0.0]),
+                                      model.particle_statistics)
+    with pytest.raises(ValueError, match="refreshed to a twist"):
+        Hf(x)
+    ed.update_orbit_stabilizer_phases(cat, G, model.particle_statistics)
+    np.testing.assert_array_equal(Hf(x), y0)
