@@ -145,7 +145,9 @@ int edx_hamiltonian_build(edx_ctx* ctx, const edx_catalog* cat, const edx_sector
  * site, the irrep characters and the density diagonal; every SpMM walks the
  * valid hops of each row's representative, gathers the target's orbit, group
  * element and alpha from the catalog and forms the element as the stored
- * build does, in FP64.  n_entries receives the entry count the stored table
+ * build does, in FP64.  The first such build on a catalog adds to it a packed
+ * copy of its rank tables (16 bytes per mask, kept and refreshed with the
+ * catalog's phases); a product after a refresh to a twist is refused.  n_entries receives the entry count the stored table
  * would hold.  Requires a catalog whose site phases are all +1 or -1
  * (EDX_ERR_INVALID otherwise): the product uses H[r][x] = conj(H[x][r]).
  * edx_hamiltonian_fetch_table / _fetch_csr / _set_spmv_group / _autotune
