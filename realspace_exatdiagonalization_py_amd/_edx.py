@@ -289,7 +289,10 @@ def mem_info(ctx: "Context") -> tuple[int, int]:
 
 
 def kernel_stats_all(enable: int = -1, reset: bool = False) -> dict:
-    """``kernel_stats`` summed over every live context (all streams of the process)."""
+    """``kernel_stats`` summed over every live context (all streams of the
+    process).  Call it while no solve is running: a context in use by
+    another thread raises ``RuntimeError`` (``EDX_ERR_BUSY``) rather than
+    being read while its owner writes it."""
     with Context._live_lock:
         ctxs = list(Context._live)
     tot = {k: [0.0, 0, 0.0] for k in KERNEL_CLASSES}

@@ -64,6 +64,7 @@ from .second_quantized_model import Particle_Statistics, Real_Space_Second_Quant
 _WORKER_LOCK = threading.Lock()
 _WORKER_CONTEXTS: dict[int, list[Context]] = {}
 _BATCH_CONTEXTS: dict[int, list[Context]] = {}
+_BATCH_LOCK = threading.Lock()
 
 #: Split-table rank gate of the reference (``SRE:163``); the device ranks
 #: with the O(k) colex formula at every size, the tables are built on request.
@@ -854,17 +855,18 @@ def diagonalize_blocks_batched(H_ops, *, nev: int = 5, precision: int = _edx.PRE
     if not batch:
         return out
     bctx = _edx.current_context()
-    ctxs = _batch_contexts(len(batch))
     n = len(batch)
     vals = np.zeros((n, nev), dtype=np.float64)
     info = np.zeros((n, 4), dtype=np.float64)
     vecs = [np.empty((nev, ops[i].n), dtype=np.complex128) for i in batch] if want_vectors else None
-    ctx_arr = (_edx.C.c_void_p * n)(*[c.handle for c in ctxs])
     ham_arr = (_edx.C.c_void_p * n)(*[ops[i]._dev.handle for i in batch])
     vec_arr = (_edx.C.c_void_p * n)(*[v.ctypes.data for v in vecs]) if want_vectors else None
     opts = _edx.eigs_options(nev=nev, precision=precision, tol=tol, eig_tol=eig_tol, seed=seed)
-    bctx.check(lib.edx_eigs_batch(bctx.handle, n, ctx_arr, ham_arr, _edx.C.byref(opts), ptr(vals), vec_arr,
-                                  ptr(info)))
+    with _BATCH_LOCK:  # batches share the pool of solve contexts: one batch at a time
+        ctxs = _batch_contexts(n)
+        ctx_arr = (_edx.C.c_void_p * n)(*[c.handle for c in ctxs])
+        bctx.check(lib.edx_eigs_batch(bctx.handle, n, ctx_arr, ham_arr, _edx.C.byref(opts), ptr(vals), vec_arr,
+                                      ptr(info)))
     for j, i in enumerate(batch):
         ops[i].last_info = info[j].copy()
         out[i] = (vals[j].copy(), vecs[j].T if want_vectors else None)
@@ -941,16 +943,31 @@ def _find_irrep_idx(ed_data: Symmetry_Resolved_ED_Data, irrep_label) -> int:
 
 
 def ed_scan_at_irrep_matrix(irrep_label, ed_data: Symmetry_Resolved_ED_Data, *, nev: int = 5):
-    """Matrix-mode sector solve (SRE:2367-2399): device table -> CSR -> device eigensolver."""
+    """Matrix-mode sector solve (SRE:2367-2399): the block's triplets held on
+    the device as a CSR (the stored projection table, with its sliced copy)
+    and solved there with ARPACK's stopping rule, dense ``eigh`` of the upper
+    triangle for dims <= 500; the block does not travel through host memory
+    (``build_ed_Hamiltonian_symmetry_block`` still returns it as a scipy CSR).
+    A twisted block is solved through its Hermitian part, as
+    ``diagonalize_block_arpack`` does."""
     irrep_idx = _find_irrep_idx(ed_data, irrep_label)
     irrep = ed_data.irrep_list[irrep_idx]
     stats = ed_data.second_quantized_model.particle_statistics
     basis = build_symmetry_sector_basis(ed_data.orbit_catalog, irrep)
     ed_data.sector_dims[irrep_idx] = basis.dim
+    if basis.dim == 0:
+        empty = (np.zeros(0, dtype=np.float64), np.zeros((0, 0), dtype=np.complex128))
+        ed_data.ed_scan_res[irrep_idx] = empty
+        return empty
+    if _verbose():
+        print(f"\tMatrix mode @ irrep {irrep.label!r} (dim={basis.dim}, device CSR) ... ", end="", flush=True)
+    t0 = time.perf_counter()
     cmap = CanonicalMap(ed_data.symmetry_group, stats, ed_data.orbit_catalog)
-    H = build_ed_Hamiltonian_symmetry_block(basis, ed_data.second_quantized_model.bilinear_terms,
-                                            ed_data.second_quantized_model.density_density_terms, cmap)
-    vals, vecs = diagonalize_block_arpack(H, nev=nev)
+    model = ed_data.second_quantized_model
+    dh = _build_device_hamiltonian(basis, model.bilinear_terms, model.density_density_terms, cmap, table="stored")
+    vals, vecs, _info = _device_eigs(dh, nev)
+    if _verbose():
+        print(f"Done. nnz={dh.n_entries + basis.dim}, t={round(time.perf_counter() - t0, 3)}s")
     ed_data.ed_scan_res[irrep_idx] = (vals, vecs)
     return vals, vecs
 

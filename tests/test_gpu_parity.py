@@ -289,8 +289,10 @@ def test_twisted_block_solves_its_hermitian_part():
 
 
 def test_twisted_matrix_mode_matches_matrixfree():
-    """Matrix mode (scipy CSR block back to the device) solves the Hermitian
-    part of a twisted block too, so both modes agree within 1e-10."""
+    """Matrix mode (the block's triplets as a device CSR) solves the Hermitian
+    part of a twisted block too, so both modes agree within 1e-10; the scipy
+    block of build_ed_Hamiltonian_symmetry_block solved by
+    diagonalize_block_arpack gives the same levels."""
     from fractions import Fraction
 
     model, ne, G = make_case("hub24_twist")
@@ -300,6 +302,11 @@ def test_twisted_matrix_mode_matches_matrixfree():
     w_mf, _ = ed.ed_scan_at_irrep_matrixfree(label, data, nev=5)
     assert data.sector_dims[1] > 500
     assert np.max(np.abs(w_mat - w_mf)) <= 1e-10, (w_mat, w_mf)
+    basis = ed.build_symmetry_sector_basis(data.orbit_catalog, data.irrep_list[1])
+    Hs = ed.build_ed_Hamiltonian_symmetry_block(basis, model.bilinear_terms, model.density_density_terms,
+                                                ed.CanonicalMap(G, model.particle_statistics, data.orbit_catalog))
+    w_sp, _ = ed.diagonalize_block_arpack(Hs, nev=5)
+    assert np.max(np.abs(w_sp - w_mat)) <= 1e-10, (w_sp, w_mat)
 
 
 def test_mixed_handover_below_the_complex64_floor():
