@@ -95,3 +95,24 @@ def test_invalid_batches_are_refused(operators):
     with pytest.raises(ValueError, match="own context"):
         bctx.check(lib.edx_eigs_batch(bctx.handle, 1, own, ham_arr, _edx.C.byref(opts),
                                       vals.ctypes.data_as(_edx.C.c_void_p), None, None))
+
+
+def test_batch_edge_cases():
+    """A batch of one, a batch holding empty and one-state sectors (every
+    site filled: one sector of dimension 1, the others empty), and a batch
+    whose operators are on the fly only."""
+    H = _operator("hub24", 0)
+    [(w, v)] = ed.diagonalize_blocks_batched([H], nev=3)
+    w1, v1 = ed.diagonalize_block_matrixfree(H, H.n, nev=3)
+    np.testing.assert_array_equal(w, w1)
+    np.testing.assert_array_equal(v, v1)
+    model, ne, G = make_case("cb23_full_twist")
+    data = ed.build_ed_data(model, filling_fraction=Fraction(ne, model.lattice.n_site), symmetry_group=G)
+    res = ed.solve_sectors_batched(data, nev=2, precision=ed.PREC_FP64, table="stored")
+    assert sorted(data.sector_dims) == [0] * (len(data.sector_dims) - 1) + [1]
+    assert sum(len(r[0]) for r in res) == 1
+    fly = [_operator("hal34", s, table="on_the_fly") for s in (0, 1)]
+    got = ed.diagonalize_blocks_batched(fly, nev=3, precision=ed.PREC_MIXED, eig_tol=1e-12)
+    for Hf, (w, v) in zip(fly, got):
+        w1, v1 = ed.diagonalize_block_matrixfree(Hf, Hf.n, nev=3, precision=ed.PREC_MIXED, eig_tol=1e-12)
+        np.testing.assert_array_equal(w, w1)

@@ -102,3 +102,27 @@ def test_group_arguments_checked(operators):
         _edx.matvec_group(ctx, [operators[0]._dev], _edx.PREC_FP64, 17, [d], [d])
     with pytest.raises(ValueError, match="precision"):
         _edx.matvec_group(ctx, [operators[0]._dev], 7, 1, [d], [d])
+
+
+def test_group_with_an_empty_sector():
+    """A grouped product over an empty sector and a one-state sector."""
+    ctx = _edx.current_context()
+    m0, _ne, G0 = make_case("hal23_ne0")
+    cat0 = ed.build_symmetry_orbit_catalog(second_quantized_model=m0, n_filled=0, symmetry_group=G0,
+                                           particle_statistics=m0.particle_statistics)
+    irreps = build_irrep_list(G0, m0.lattice)
+    ops = [ed.MatrixFreeHamiltonian(ed.build_symmetry_sector_basis(cat0, ir), m0.bilinear_terms,
+                                    m0.density_density_terms, None) for ir in irreps[:2]]
+    assert sorted(H.n for H in ops) == [0, 1]
+    xs = [_edx.DeviceArray(ctx, max(H.n, 1) * 16 * NB) for H in ops]
+    ys = [_edx.DeviceArray(ctx, max(H.n, 1) * 16 * NB) for H in ops]
+    for x in xs:
+        x.upload(np.ones(NB, dtype=np.complex128))
+    _edx.matvec_group(ctx, [H._dev for H in ops], _edx.PREC_FP64, NB, xs, ys)
+    g = _edx.MatvecGroupGraph(ctx, [H._dev for H in ops], _edx.PREC_FP64, NB, xs, ys)
+    g.launch()
+    g.synchronize()
+    for H, y in zip(ops, ys):
+        if H.n:
+            got = y.download(np.empty((NB, 1), dtype=np.complex128))
+            np.testing.assert_array_equal(got, np.zeros((NB, 1)))  # N = 0: the empty state, H = 0

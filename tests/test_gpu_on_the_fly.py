@@ -185,3 +185,24 @@ def test_on_the_fly_follows_catalog_phase_refresh():
         Hf(x)
     ed.update_orbit_stabilizer_phases(cat, G, model.particle_statistics)
     np.testing.assert_array_equal(Hf(x), y0)
+
+
+def test_on_the_fly_diagonal_only_and_one_state():
+    """No bilinear terms (a diagonal block), and a one-state sector."""
+    model, ne, G = make_case("hub24")
+    cat = ed.build_symmetry_orbit_catalog(second_quantized_model=model, n_filled=ne, symmetry_group=G,
+                                          particle_statistics=model.particle_statistics)
+    basis = ed.build_symmetry_sector_basis(cat, build_irrep_list(G, model.lattice)[0])
+    Hs = ed.MatrixFreeHamiltonian(basis, [], model.density_density_terms, None)
+    Hf = ed.MatrixFreeHamiltonian(basis, [], model.density_density_terms, None, table="on_the_fly")
+    assert Hf._dev.n_entries == Hs._dev.n_entries == 0
+    x = random_vector(Hs.n, 5)
+    np.testing.assert_array_equal(Hf(x), Hs(x))
+    m0, _ne, G0 = make_case("hal23_ne0")  # no particles: one state, H = 0
+    cat0 = ed.build_symmetry_orbit_catalog(second_quantized_model=m0, n_filled=0, symmetry_group=G0,
+                                           particle_statistics=m0.particle_statistics)
+    b0 = ed.build_symmetry_sector_basis(cat0, build_irrep_list(G0, m0.lattice)[0])
+    assert b0.dim == 1
+    H1 = ed.MatrixFreeHamiltonian(b0, m0.bilinear_terms, m0.density_density_terms, None, table="on_the_fly")
+    w, _ = ed.diagonalize_block_matrixfree(H1, H1.n, nev=1)
+    assert w.shape == (1,) and abs(w[0]) <= 1e-15
