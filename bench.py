@@ -8,7 +8,9 @@ phase pi/4, nearest-neighbour repulsion V = 1, translation group of order 24,
 24 momentum sectors of ~15.7 M states.  One step is one sector solve (the
 sectors in turn), covering the irrep filter, the projection-table build and
 the mixed-precision eigensolve of the 10 lowest levels (complex64 sweeps,
-FP64 refinement to an eigenvalue error bound of 1e-12); the eigenvectors stay
+FP64 refinement to an eigenvalue error bound of 1e-11, ten times below the
+1e-10 accuracy target, which the line checks against an FP64 solve); the
+eigenvectors stay
 in device memory.  The orbit catalog is built once outside the timed region
 (the reference's benchmark convention) and timed separately.  Inputs are
 synthetic (the model is fully determined by its parameters).
@@ -237,7 +239,7 @@ def profile_solve(args) -> int:
     basis = ed.build_symmetry_sector_basis(cat, build_irrep_list(G, model.lattice)[0])
     H = ed.MatrixFreeHamiltonian(basis, model.bilinear_terms, model.density_density_terms, None)
     t0 = time.perf_counter()
-    w, _ = ed.diagonalize_block_matrixfree(H, H.n, nev=10, precision=ed.PREC_MIXED, eig_tol=1e-12,
+    w, _ = ed.diagonalize_block_matrixfree(H, H.n, nev=10, precision=ed.PREC_MIXED, eig_tol=1e-11,
                                            want_vectors=False)
     print(json.dumps({"dim": H.n, "entries": H._dev.n_entries, "lanes": H._dev.spmv_group,
                       "solve_s": round(time.perf_counter() - t0, 3), "matvecs": int(H.last_info[0]),
@@ -410,7 +412,7 @@ def cpu_baseline_checkerboard(ed, H48, n_mv48, nev, eig_tol, row_fraction=16):
                                      row_fraction)
 
 
-def hubbard_secondary(ed, _edx, workers=8, passes=2):
+def hubbard_secondary(ed, _edx, workers=8, passes=2, eig_tol=1e-11):
     """The Hubbard 3x4 workload (all 12 momentum sectors per pass, 6 levels,
     mixed precision) streamed through `workers` HIP streams: solves/s."""
     from realspace_exatdiagonalization_py_amd.groups import build_translation_group
@@ -419,17 +421,17 @@ def hubbard_secondary(ed, _edx, workers=8, passes=2):
     G = build_translation_group(model.lattice)
     _edx.set_autotune(True, ed.PREC_MIXED, 6)
     data = ed.build_ed_data(model, filling_fraction=Fraction(12, 24), symmetry_group=G)
-    solve_all(ed, data, 6, ed.PREC_MIXED, 1e-12, workers)
+    solve_all(ed, data, 6, ed.PREC_MIXED, eig_tol, workers)
     t0 = time.perf_counter()
-    lat, vals = solve_stream(ed, data, 6, ed.PREC_MIXED, 1e-12, workers, passes)
+    lat, vals = solve_stream(ed, data, 6, ed.PREC_MIXED, eig_tol, workers, passes)
     el = time.perf_counter() - t0
     lat_ms = np.asarray(lat) * 1e3
     # the same passes as batches: each pass's 12 sector solves advance together
     # from one thread, their SpMMs grouped into shared launches
-    ed.solve_sectors_batched(data, nev=6, precision=ed.PREC_MIXED, eig_tol=1e-12, table="stored")
+    ed.solve_sectors_batched(data, nev=6, precision=ed.PREC_MIXED, eig_tol=eig_tol, table="stored")
     t0 = time.perf_counter()
     for _ in range(passes):
-        res = ed.solve_sectors_batched(data, nev=6, precision=ed.PREC_MIXED, eig_tol=1e-12, table="stored")
+        res = ed.solve_sectors_batched(data, nev=6, precision=ed.PREC_MIXED, eig_tol=eig_tol, table="stored")
     el_b = time.perf_counter() - t0
     d_b = max(float(np.max(np.abs(np.asarray(r[0]) - np.asarray(v)))) for r, v in zip(res, vals))
     return {"workload": "spinful Hubbard 3x4 PBC, t=1, U=4, 12 fermions on 24 orbitals, all 12 momentum sectors, "
@@ -598,7 +600,7 @@ def checkerboard_bench(args) -> int:
     }
     log("secondary workload (Hubbard 3x4) ...")
     del cat
-    secondary = hubbard_secondary(ed, _edx, workers=args.workers)
+    secondary = hubbard_secondary(ed, _edx, workers=args.workers, eig_tol=args.eig_tol)
     lat_ms = np.asarray(lat) * 1e3
     value = args.steps / elapsed
     line = {
@@ -945,8 +947,9 @@ def main() -> int:
                     help="checkerboard48: one 48-site sector solve per step; hubbard3x4: all 12 sectors per step")
     ap.add_argument("--nev", type=int, default=None, help="levels per sector (default 10 / 6 per workload)")
     ap.add_argument("--precision", choices=("mixed", "fp64"), default="mixed")
-    ap.add_argument("--eig-tol", type=float, default=1e-12,
-                    help="eigenvalue error bound that ends a solve (0: ARPACK residual rule only)")
+    ap.add_argument("--eig-tol", type=float, default=1e-11,
+                    help="eigenvalue error bound that ends a solve, 10x below the 1e-10 accuracy target "
+                         "(0: ARPACK residual rule only)")
     ap.add_argument("--workers", type=int, default=8,
                     help="sectors solved concurrently, one HIP stream (context) per worker thread")
     ap.add_argument("--no-cpu-baseline", action="store_true")
