@@ -61,7 +61,9 @@ constexpr int kUpdateRows = 2;
 // Block-update schedules (blocks of at most 8 vectors): vectors per prefetch
 // stage and rows per lane.  Every variant sums each output in the same order
 // (input vectors ascending, one lane per row), so they give identical
-// results; the first is the default, the tuner may pick another.
+// results.  Defaults: 2/2 for complex64, 2/1 for complex128 (two rows per
+// lane hold too many FP64 sums: 2/1 was 1.0-1.6x faster at D = 2^17 and
+// 2^23, blocks 6-8); the tuner may pick another.
 struct UpdateVariant {
   int stage, rows;
 };
@@ -1354,12 +1356,13 @@ class BlockLanczos {
   // this solve's own banks at the shape of a block step three quarters into
   // the basis (a Gram of [Q, W] against W and the update [Q, W] S into the
   // other bank and X); else the defaults (matrix-core Gram for complex64
-  // from kMfmaGramRows rows, first update variant).  The banks hold no
+  // from kMfmaGramRows rows; update 2/2 for complex64, 2/1 for complex128).
+  // The banks hold no
   // vectors yet: the timed kernels read them and write only areas that the
   // run overwrites before use.
   void choose_schedule() {
     gram_mode_ = (esz_ == 8 && D_ >= kMfmaGramRows) ? kGramMatrix : kGramVector;
-    upd_var_ = 0;
+    upd_var_ = esz_ == 8 ? 0 : 1;
     const bool gram_fixed = ctx_.sched_gram != 0, upd_fixed = ctx_.sched_update != 0;
     if (gram_fixed) gram_mode_ = ctx_.sched_gram;
     if (upd_fixed) upd_var_ = ctx_.sched_update - 1;
