@@ -337,7 +337,9 @@ int edx_hamiltonian_autotune(edx_ctx* ctx, edx_hamiltonian* ham, int32_t precisi
  * Krylov banks at the shape of a block step three quarters into the basis,
  * the Gram of [Q, W] against W on vector FMAs and (complex64) on the FP32
  * matrix cores, and the three block-update variants (vectors per prefetch
- * stage, rows per lane: 2/2, 2/1, 4/1), and keeps the fastest of each.  The
+ * stage, rows per lane: 2/2, 2/1, 4/1), and keeps the fastest of each when
+ * it beats the default (matrix cores for complex64 from 2^20 rows; update
+ * 2/2 for complex64, 2/1 for complex128) by more than 3 %.  The
  * update variants sum every output in the same order, so they give
  * identical results.  edx_schedule_table returns keys[6*i..] = (precision,
  * block, dim_log2, basis, gram, update) with gram 1 = vector FMAs, 2 =

@@ -74,6 +74,8 @@ constexpr int kGramVector = 1, kGramMatrix = 2;
 // Sectors from this many rows have their schedule tuned (smaller ones are
 // bound by launch latency, where timings say little).
 constexpr int64_t kScheduleTuneRows = 65536;
+// A tuned schedule must beat the default by this fraction to replace it.
+constexpr float kScheduleMargin = 0.03f;
 // Largest LDS staging of an update's interleaved output copy (bytes).
 constexpr size_t kStageLdsBytes = 48 * 1024;
 constexpr size_t kLogC = edx::kLogC;
@@ -1384,14 +1386,23 @@ class BlockLanczos {
           gram_mode_ = g + 1;
           e.gram_ms[g] = time_fastest([&] { gram_device(Q, nq, W_.p, b_, W_.p, b_, pbuf_.p); });
         }
-        e.gram = (gram_modes == 2 && e.gram_ms[1] < e.gram_ms[0]) ? kGramMatrix : kGramVector;
+        // another schedule replaces the default only when it is faster by
+        // more than kScheduleMargin (differences below it are timing noise
+        // at this synthetic shape and did not carry over to solves)
+        const int gram_default = (esz_ == 8 && D_ >= kMfmaGramRows) ? kGramMatrix : kGramVector;
+        e.gram = gram_default;
+        if (gram_modes == 2) {
+          const int other = gram_default == kGramMatrix ? kGramVector : kGramMatrix;
+          if (e.gram_ms[other - 1] < (1.0f - kScheduleMargin) * e.gram_ms[gram_default - 1]) e.gram = other;
+        }
         for (int v = 0; v < 3; ++v) {
           upd_var_ = v;
           e.update_ms[v] = time_fastest([&] { launch_gemm(Q, nq, W_.p, b_, Out, b_, sbuf_.p, b_, X_.p); });
         }
-        int best = 0;
-        for (int v = 1; v < 3; ++v)
-          if (e.update_ms[v] < e.update_ms[best]) best = v;
+        const int upd_default = esz_ == 8 ? 0 : 1;
+        int best = upd_default;
+        for (int v = 0; v < 3; ++v)
+          if (e.update_ms[v] < (1.0f - kScheduleMargin) * e.update_ms[best]) best = v;
         e.update = best + 1;
       } catch (...) {
         ctx_.timer.on = timer_on;

@@ -6,7 +6,8 @@
   1e-10).
 * Eigensolver schedules (Gram on vector FMAs or FP32 matrix cores; three
   block-update variants): on a 3x4 Hubbard sector (D = 2.3e5) each kept
-  choice is the fastest one timed; the update variants give bit-identical
+  choice is the default unless another one timed is faster by more than 3 %,
+  and within 3 % of the fastest; the update variants give bit-identical
   eigenvalues and eigenvectors; the two Gram schedules agree within 1e-10.
 """
 
@@ -104,10 +105,15 @@ def test_solver_schedule_tuned_fastest_and_identical(tuning):
     c64 = [e for e in table if e["precision"] == _edx.PREC_MIXED]
     assert len(c64) == 1 and c64[0]["block"] == 6
     for e in table:
+        # the default is kept unless another schedule is faster by more than 3 %
         timed_g = {k: v for k, v in e["gram_ms"].items() if v > 0}
-        assert e["gram"] == min(timed_g, key=timed_g.get)
+        gram_default = 2 if (e["precision"] == _edx.PREC_MIXED and e["dim_log2"] >= 20) else 1
+        upd_default = 1 if e["precision"] == _edx.PREC_MIXED else 2
         assert all(v > 0 for v in e["update_ms"].values())
-        assert e["update"] == min(e["update_ms"], key=e["update_ms"].get)
+        for kept, ms, default in ((e["gram"], timed_g, gram_default), (e["update"], e["update_ms"], upd_default)):
+            fastest = min(ms, key=ms.get)
+            assert ms[kept] * 0.97 <= ms[fastest] + 1e-9, (kept, ms)
+            assert kept == default or ms[kept] < 0.97 * ms[default], (kept, ms)
     assert all(v > 0 for v in c64[0]["gram_ms"].values())  # complex64: both Gram schedules timed
     n_entries = len(table)
     # a second solve of the same shape reuses the entries
