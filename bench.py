@@ -231,7 +231,8 @@ def profile_solve(args) -> int:
     from realspace_exatdiagonalization_py_amd import _edx
     from realspace_exatdiagonalization_py_amd.groups import build_irrep_list, build_translation_group
 
-    _edx.set_autotune(False, ed.PREC_MIXED, 8)
+    nb = args.block if args.block is not None else CHECKERBOARD_BLOCK
+    _edx.set_autotune(False, ed.PREC_MIXED, nb if nb > 0 else 8)
     model = checkerboard_model((4, 6))
     G = build_translation_group(model.lattice)
     cat = ed.build_symmetry_orbit_catalog(second_quantized_model=model, n_filled=8, symmetry_group=G,
@@ -240,7 +241,7 @@ def profile_solve(args) -> int:
     H = ed.MatrixFreeHamiltonian(basis, model.bilinear_terms, model.density_density_terms, None)
     t0 = time.perf_counter()
     w, _ = ed.diagonalize_block_matrixfree(H, H.n, nev=10, precision=ed.PREC_MIXED, eig_tol=1e-11,
-                                           want_vectors=False)
+                                           want_vectors=False, block=nb)
     print(json.dumps({"dim": H.n, "entries": H._dev.n_entries, "lanes": H._dev.spmv_group,
                       "solve_s": round(time.perf_counter() - t0, 3), "matvecs": int(H.last_info[0]),
                       "levels": [float(x) for x in w]}))
@@ -266,6 +267,14 @@ def copy_kernel(args) -> int:
 CHECKERBOARD_PARAMS = {"t": 1.0, "t'_1": 0.3, "t'_2": -0.3, "t''": -0.2, "phi_over_2pi": 1 / 8, "V1": 1.0,
                        "V2": 0.0, "V3": 0.0}
 CHECKERBOARD_TRAFFIC_FILE = ROOT / "profiles" / "solver_checkerboard48_traffic.json"
+# Block width of the complex64 sweeps on this workload.  Measured on sectors
+# 0, 14 and 6 (mixed, 10 levels, eigenvalue bound 1e-11): width 4 (basis 104)
+# solves in 5.07 / 4.99 / 4.35 s against 5.97 / 6.37 / 4.43 s at the library
+# default 8 (basis 176); widths 2, 3, 5, 6, 10, 12, 16 and other bases for
+# width 4 were slower on average.  No level among the 10 lowest of any sector
+# has more than 2 copies, and all 24 sectors' levels equal the width-8
+# solve's within 2.9e-13 (111.7 s for the 24 sectors against 125.7 s).
+CHECKERBOARD_BLOCK = 4
 CHECKERBOARD_WORKLOAD = ("fermionic checkerboard flat band 4x6x2 (48 sites), 8 fermions, C(48,8)=377348994, t=1, "
                          "t'=+-0.3, t''=-0.2, NN phase pi/4, NN V=1, translation group of order 24; one step = one "
                          "momentum-sector solve (sectors in turn), 10 lowest levels, mixed precision")
@@ -493,8 +502,10 @@ def checkerboard_bench(args) -> int:
 
     nev = args.nev or 10
     prec = ed.PREC_MIXED if args.precision == "mixed" else ed.PREC_FP64
+    nb = args.block if args.block is not None else CHECKERBOARD_BLOCK
+    nb = nb if nb > 0 else min(nev, 8)
     ctx = _edx.Context.default()
-    _edx.set_autotune(not args.no_autotune, prec, min(nev, 8))
+    _edx.set_autotune(not args.no_autotune, prec, nb)
     model = checkerboard_model((4, 6))
     G = build_translation_group(model.lattice)
     log("building catalog ...")
@@ -516,7 +527,7 @@ def checkerboard_bench(args) -> int:
         t0 = time.perf_counter()
         basis, H = sector(i)
         w, _ = ed.diagonalize_block_matrixfree(H, H.n, nev=nev, precision=precision, eig_tol=eig_tol,
-                                               want_vectors=False)
+                                               want_vectors=False, block=nb)
         return time.perf_counter() - t0, np.asarray(w), basis.dim, np.array(H.last_info)
 
     for j in range(args.warmup):
@@ -551,7 +562,7 @@ def checkerboard_bench(args) -> int:
     log(f"plain FP64 solve of sector {i_ref}: {t_plain:.2f}s, max |dE| {dE:.2e}")
     t0 = time.perf_counter()
     wv, vecs = ed.diagonalize_block_matrixfree(H, H.n, nev=nev, precision=prec, eig_tol=args.eig_tol,
-                                               want_vectors=True)
+                                               want_vectors=True, block=nb)
     t_vec = time.perf_counter() - t0
     n_mv = int(H.last_info[0])
     R = H(vecs) - vecs * wv
@@ -567,9 +578,8 @@ def checkerboard_bench(args) -> int:
     # roofline: the class with the most kernel time in the timed region
     total_k = sum(v[0] for v in ks.values())
     cls = max(ks, key=lambda k: ks[k][0])
-    lanes = next((e["lanes"] for e in _edx.autotune_table() if e["block"] == min(nev, 8)), None)
-    nb = min(nev, 8)
-    names = {"spmm_c64": f"spmm_sell<float2,{nb},{lanes}>", "spmm_c128": f"spmm_sell<double2,{nb + 2},{lanes}>",
+    lanes = next((e["lanes"] for e in _edx.autotune_table() if e["block"] == nb), None)
+    names = {"spmm_c64": f"spmm_sell<float2,{nb},{lanes}>", "spmm_c128": f"spmm_sell<double2,{max(nb, nev)},{lanes}>",
              "gram": "gram_tiles + gram_reduce", "gemm": "gemm_tiles"}
     k_ms, k_n, k_bytes = ks[cls]
     avg_ms = k_ms / max(k_n, 1)
@@ -620,6 +630,7 @@ def checkerboard_bench(args) -> int:
         "data": "synthetic (model fully determined by its parameters)",
         "config": {"workload": CHECKERBOARD_WORKLOAD, "sectors_timed": timed,
                    "sector_dims": [int(min(dims)), int(max(dims))], "nev": nev, "precision": args.precision,
+                   "block": nb,
                    "concurrent_sectors": 1, "eigenvectors": "left in device memory (see pcie_inclusive_solve_s)"},
         "p50_ms": round(float(np.percentile(lat_ms, 50)), 3),
         "p99_ms": round(float(np.percentile(lat_ms, 99)), 3),
@@ -947,6 +958,8 @@ def main() -> int:
                     help="checkerboard48: one 48-site sector solve per step; hubbard3x4: all 12 sectors per step")
     ap.add_argument("--nev", type=int, default=None, help="levels per sector (default 10 / 6 per workload)")
     ap.add_argument("--precision", choices=("mixed", "fp64"), default="mixed")
+    ap.add_argument("--block", type=int, default=None,
+                    help=f"Lanczos block width of the 48-site workload (default {CHECKERBOARD_BLOCK}; 0: min(nev, 8))")
     ap.add_argument("--eig-tol", type=float, default=1e-11,
                     help="eigenvalue error bound that ends a solve, 10x below the 1e-10 accuracy target "
                          "(0: ARPACK residual rule only)")

@@ -822,15 +822,18 @@ def diagonalize_block_arpack(H, *, nev: int = 5):
 
 def diagonalize_block_matrixfree(H_op: MatrixFreeHamiltonian, n: int, *, nev: int = 5,
                                  precision: int = _edx.PREC_FP64, eig_tol: float = 0.0,
-                                 want_vectors: bool = True):
+                                 want_vectors: bool = True, block: int = 0):
     """Lowest ``nev`` eigenpairs of the device operator (SRE:2248-2299).
 
     Defaults reproduce ARPACK's stopping rule (relative residual 1e-10);
     ``precision=PREC_MIXED`` sweeps in complex64 and finishes in FP64;
-    ``eig_tol > 0`` stops once every eigenvalue is within ``eig_tol``."""
+    ``eig_tol > 0`` stops once every eigenvalue is within ``eig_tol``.
+    ``block`` is the block width of the Lanczos sweeps (0: min(nev, 8)); a
+    block finds up to that many copies of a degenerate level, so it must be
+    at least the largest multiplicity among the wanted levels."""
     if n == 0:
         return np.zeros(0, dtype=np.float64), np.zeros((0, 0), dtype=np.complex128)
-    vals, vecs, info = _device_eigs(H_op._dev, nev, precision=precision, eig_tol=eig_tol,
+    vals, vecs, info = _device_eigs(H_op._dev, nev, precision=precision, eig_tol=eig_tol, block=block,
                                     want_vectors=want_vectors)
     H_op.last_info = info
     return vals, vecs
