@@ -10,7 +10,8 @@ Checks: orbit counts and every sector dimension against the closed-form
 character count; table entries of sampled masks against the one-mask
 restatement of the catalog rules; the flux refresh against a fresh catalog
 at the same twist; one sector solved in mixed precision against FP64 with
-true residuals.
+true residuals (the 36-site one also at a twist); one full-size sector of
+the Hubbard 3x4 workload against the oracle's ARPACK solve.
 """
 
 from __future__ import annotations
@@ -24,6 +25,7 @@ import pytest
 import realspace_exatdiagonalization_py_amd as ed
 from oracle import ed_oracle as O
 from realspace_exatdiagonalization_py_amd import models as M
+from realspace_exatdiagonalization_py_amd import second_quantized_model as SQ
 from realspace_exatdiagonalization_py_amd.groups import (build_identity_group, build_identity_irrep_list,
                                                          build_irrep_list, build_translation_group, group_arrays)
 
@@ -85,6 +87,22 @@ def test_haldane36_catalog_flux_refresh_and_solve():
     ulp = np.spacing(np.abs(b.real)) + np.spacing(np.abs(b.imag))
     assert np.all(np.abs(a - b) <= 2 * ulp)
     del fresh
+    # one sector at that twist (twisted terms, refreshed catalog): complex64
+    # sweeps + FP64 refinement against FP64, on the block's Hermitian part
+    tmodel, _ = _haldane36()
+    SQ.update_second_quantized_model_with_twisted_phases(tmodel, twisted_phases_over_2π=theta)
+    tbasis = ed.build_symmetry_sector_basis(gc, build_irrep_list(Gt, tmodel.lattice)[2])
+    Ht = ed.MatrixFreeHamiltonian(tbasis, tmodel.bilinear_terms, tmodel.density_density_terms, None)
+    t0 = time.perf_counter()
+    w32, v32 = ed.diagonalize_block_matrixfree(Ht, Ht.n, nev=5, precision=ed.PREC_MIXED, eig_tol=1e-11)
+    t1 = time.perf_counter()
+    w64, _ = ed.diagonalize_block_matrixfree(Ht, Ht.n, nev=5, precision=ed.PREC_FP64, want_vectors=False)
+    print(f"[scale] twisted sector D={tbasis.dim}: mixed {t1 - t0:.2f}s fp64 {time.perf_counter() - t1:.2f}s "
+          f"|dE| {np.max(np.abs(w32 - w64)):.1e} E={w32.tolist()}", flush=True)
+    assert np.max(np.abs(w32 - w64)) <= 1e-10
+    R = Ht(v32) - v32 * w32
+    assert np.max(np.linalg.norm(R, axis=0)) <= 1e-5
+    del Ht, tbasis, v32, R
     # one sector at zero twist: complex64 sweeps + FP64 refinement against FP64
     ed.update_orbit_stabilizer_phases(gc, G, model.particle_statistics)
     irrep = build_irrep_list(G, model.lattice)[1]
@@ -241,3 +259,61 @@ def test_hubbard3x4_full_space_clean_and_disordered():
     print(f"[scale] Rayleigh quotient of the clean ground state {rq:.12f}, E0 + <V> {want:.12f}", flush=True)
     assert abs(rq - want) <= 1e-9
     assert rq >= wd[0] - 1e-10
+
+
+def _levels(w, tol=1e-8):
+    """Distinct levels of an ascending spectrum (values within tol merged) and
+    their multiplicities."""
+    out, mult = [], []
+    for x in w:
+        if out and abs(x - out[-1]) <= tol:
+            mult[-1] += 1
+        else:
+            out.append(float(x))
+            mult.append(1)
+    return np.array(out), mult
+
+
+def test_hubbard3x4_full_sector_against_oracle():
+    """One full-size momentum sector of the Hubbard 3x4 workload (U = 4, 12
+    fermions, D ~ 2.25e5, 5.6 M table entries): the device's 6 lowest levels,
+    FP64 and mixed precision, against the oracle's own catalog, table and
+    ARPACK call (eigs, which='SR', tol=1e-10, the reference's seeded start).
+    The sector holds spin triplets, of which ARPACK may return fewer copies
+    than there are, so levels are compared as sets: every ARPACK level lies
+    within 1e-10 of a device level, the device holds at least as many copies
+    of each (below the highest level, which the cut at 6 may split), and
+    every device level below ARPACK's highest is an ARPACK level."""
+    model = M.build_spinful_hubbard_second_quantized_model([3, 4], 1.0, 4.0)
+    G = build_translation_group(model.lattice)
+    irrep = build_irrep_list(G, model.lattice)[5]
+    perm, ph = group_arrays(G)
+    t0 = time.perf_counter()
+    oc = O.build_catalog(24, 12, perm, ph, True)
+    reps, stab, o2b = O.sector_basis(oc, irrep.values)
+    r, c, v, hd = O.projection_table(oc, reps, stab, o2b, model.bilinear_terms, model.density_density_terms,
+                                     irrep.values, True)
+    Href = O.csr_hamiltonian(r, c, v, hd, len(reps))
+    w_ref, _ = O.diagonalize(Href, 6)
+    print(f"[scale] hubbard3x4 sector 5 oracle D={len(reps)} {time.perf_counter() - t0:.1f}s E={w_ref.tolist()}",
+          flush=True)
+    gc = ed.build_symmetry_orbit_catalog(second_quantized_model=model, n_filled=12, symmetry_group=G,
+                                         particle_statistics=model.particle_statistics)
+    basis = ed.build_symmetry_sector_basis(gc, irrep)
+    np.testing.assert_array_equal(basis.representative_masks, reps)
+    H = ed.MatrixFreeHamiltonian(basis, model.bilinear_terms, model.density_density_terms, None)
+    lo, mo = _levels(w_ref)
+    for prec in (ed.PREC_FP64, ed.PREC_MIXED):
+        mixed = prec == ed.PREC_MIXED
+        w, vecs = ed.diagonalize_block_matrixfree(H, H.n, nev=6, precision=prec, eig_tol=1e-11 if mixed else 0.0)
+        ld, md = _levels(w)
+        print(f"[scale] device precision {prec}: E={w.tolist()}", flush=True)
+        for x, m in zip(lo, mo):
+            j = int(np.argmin(np.abs(ld - x)))
+            assert abs(ld[j] - x) <= 1e-10, (x, w, w_ref)
+            if x < lo[-1]:
+                assert md[j] >= m, (x, w, w_ref)
+        for x in ld[ld < lo[-1] - 1e-8]:
+            assert np.min(np.abs(lo - x)) <= 1e-10, (x, w, w_ref)
+        R = Href @ vecs - vecs * w
+        assert np.max(np.linalg.norm(R, axis=0)) <= (1e-5 if mixed else 1e-6)
