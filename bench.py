@@ -240,8 +240,9 @@ def profile_solve(args) -> int:
     basis = ed.build_symmetry_sector_basis(cat, build_irrep_list(G, model.lattice)[0])
     H = ed.MatrixFreeHamiltonian(basis, model.bilinear_terms, model.density_density_terms, None)
     t0 = time.perf_counter()
+    mtol = args.mixed_tol if args.mixed_tol is not None else CHECKERBOARD_MIXED_TOL
     w, _ = ed.diagonalize_block_matrixfree(H, H.n, nev=10, precision=ed.PREC_MIXED, eig_tol=1e-11,
-                                           want_vectors=False, block=nb)
+                                           want_vectors=False, block=nb, mixed_tol=mtol)
     print(json.dumps({"dim": H.n, "entries": H._dev.n_entries, "lanes": H._dev.spmv_group,
                       "solve_s": round(time.perf_counter() - t0, 3), "matvecs": int(H.last_info[0]),
                       "levels": [float(x) for x in w]}))
@@ -275,6 +276,15 @@ CHECKERBOARD_TRAFFIC_FILE = ROOT / "profiles" / "solver_checkerboard48_traffic.j
 # has more than 2 copies, and all 24 sectors' levels equal the width-8
 # solve's within 2.9e-13 (111.7 s for the 24 sectors against 125.7 s).
 CHECKERBOARD_BLOCK = 4
+# Relative residual at which this workload's mixed solves hand over from the
+# complex64 sweeps to FP64.  At width 4 the FP64 phase's block is 10 (it
+# starts from all 10 Ritz vectors) and costs 1.5x a complex64 step per
+# vector; at 1.5e-8 it takes 30-40 matvecs instead of 190-440 at the library
+# default 3e-7 (measured on sectors 0, 1, 6, 14, 20: 3.79-4.48 s against
+# 4.35-5.06 s; 1e-7, 5e-8, 3e-8, 1e-8 and 5e-9 were slower on average).  The
+# Hubbard, Haldane and Heisenberg workloads keep 3e-7 (lower was 0-8 %
+# slower there).
+CHECKERBOARD_MIXED_TOL = 1.5e-8
 CHECKERBOARD_WORKLOAD = ("fermionic checkerboard flat band 4x6x2 (48 sites), 8 fermions, C(48,8)=377348994, t=1, "
                          "t'=+-0.3, t''=-0.2, NN phase pi/4, NN V=1, translation group of order 24; one step = one "
                          "momentum-sector solve (sectors in turn), 10 lowest levels, mixed precision")
@@ -504,6 +514,7 @@ def checkerboard_bench(args) -> int:
     prec = ed.PREC_MIXED if args.precision == "mixed" else ed.PREC_FP64
     nb = args.block if args.block is not None else CHECKERBOARD_BLOCK
     nb = nb if nb > 0 else min(nev, 8)
+    mtol = args.mixed_tol if args.mixed_tol is not None else CHECKERBOARD_MIXED_TOL
     ctx = _edx.Context.default()
     _edx.set_autotune(not args.no_autotune, prec, nb)
     model = checkerboard_model((4, 6))
@@ -527,7 +538,7 @@ def checkerboard_bench(args) -> int:
         t0 = time.perf_counter()
         basis, H = sector(i)
         w, _ = ed.diagonalize_block_matrixfree(H, H.n, nev=nev, precision=precision, eig_tol=eig_tol,
-                                               want_vectors=False, block=nb)
+                                               want_vectors=False, block=nb, mixed_tol=mtol)
         return time.perf_counter() - t0, np.asarray(w), basis.dim, np.array(H.last_info)
 
     for j in range(args.warmup):
@@ -562,7 +573,7 @@ def checkerboard_bench(args) -> int:
     log(f"plain FP64 solve of sector {i_ref}: {t_plain:.2f}s, max |dE| {dE:.2e}")
     t0 = time.perf_counter()
     wv, vecs = ed.diagonalize_block_matrixfree(H, H.n, nev=nev, precision=prec, eig_tol=args.eig_tol,
-                                               want_vectors=True, block=nb)
+                                               want_vectors=True, block=nb, mixed_tol=mtol)
     t_vec = time.perf_counter() - t0
     n_mv = int(H.last_info[0])
     R = H(vecs) - vecs * wv
@@ -630,7 +641,7 @@ def checkerboard_bench(args) -> int:
         "data": "synthetic (model fully determined by its parameters)",
         "config": {"workload": CHECKERBOARD_WORKLOAD, "sectors_timed": timed,
                    "sector_dims": [int(min(dims)), int(max(dims))], "nev": nev, "precision": args.precision,
-                   "block": nb,
+                   "block": nb, "mixed_tol": mtol,
                    "concurrent_sectors": 1, "eigenvectors": "left in device memory (see pcie_inclusive_solve_s)"},
         "p50_ms": round(float(np.percentile(lat_ms, 50)), 3),
         "p99_ms": round(float(np.percentile(lat_ms, 99)), 3),
@@ -958,6 +969,8 @@ def main() -> int:
                     help="checkerboard48: one 48-site sector solve per step; hubbard3x4: all 12 sectors per step")
     ap.add_argument("--nev", type=int, default=None, help="levels per sector (default 10 / 6 per workload)")
     ap.add_argument("--precision", choices=("mixed", "fp64"), default="mixed")
+    ap.add_argument("--mixed-tol", type=float, default=None,
+                    help=f"complex64-to-FP64 handover residual of the 48-site workload (default {CHECKERBOARD_MIXED_TOL:g})")
     ap.add_argument("--block", type=int, default=None,
                     help=f"Lanczos block width of the 48-site workload (default {CHECKERBOARD_BLOCK}; 0: min(nev, 8))")
     ap.add_argument("--eig-tol", type=float, default=1e-11,

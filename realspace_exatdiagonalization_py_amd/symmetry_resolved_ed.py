@@ -749,7 +749,8 @@ def build_ed_Hamiltonian_symmetry_block(basis, bilinear_terms, density_terms, cm
 # ---------------------------------------------------------------------------
 
 def _device_eigs(dh: _DeviceHamiltonian, nev: int, *, precision: int = _edx.PREC_FP64, tol: float = 1e-10,
-                 eig_tol: float = 0.0, block: int = 0, seed: int = 12345, want_vectors: bool = True):
+                 eig_tol: float = 0.0, block: int = 0, seed: int = 12345, want_vectors: bool = True,
+                 mixed_tol: float = 0.0):
     """Device eigensolve.  ``tol``: relative residual (ARPACK's criterion);
     ``eig_tol > 0`` also accepts pairs whose eigenvalue error bound is below it."""
     n = dh.dim
@@ -762,6 +763,8 @@ def _device_eigs(dh: _DeviceHamiltonian, nev: int, *, precision: int = _edx.PREC
     vecs = np.empty((k, n), dtype=np.complex128) if want_vectors else None
     info = np.zeros(4, dtype=np.float64)
     opts = _edx.eigs_options(nev=k, block=block, precision=precision, tol=tol, eig_tol=eig_tol, seed=seed)
+    if mixed_tol > 0:
+        opts.mixed_tol = mixed_tol
     dh.ctx.check(lib.edx_eigs_ex(dh.ctx.handle, dh.handle, _edx.C.byref(opts), ptr(vals), ptr(vecs), ptr(info)))
     return vals, (vecs.T if want_vectors else None), info
 
@@ -822,7 +825,7 @@ def diagonalize_block_arpack(H, *, nev: int = 5):
 
 def diagonalize_block_matrixfree(H_op: MatrixFreeHamiltonian, n: int, *, nev: int = 5,
                                  precision: int = _edx.PREC_FP64, eig_tol: float = 0.0,
-                                 want_vectors: bool = True, block: int = 0):
+                                 want_vectors: bool = True, block: int = 0, mixed_tol: float = 0.0):
     """Lowest ``nev`` eigenpairs of the device operator (SRE:2248-2299).
 
     Defaults reproduce ARPACK's stopping rule (relative residual 1e-10);
@@ -830,11 +833,13 @@ def diagonalize_block_matrixfree(H_op: MatrixFreeHamiltonian, n: int, *, nev: in
     ``eig_tol > 0`` stops once every eigenvalue is within ``eig_tol``.
     ``block`` is the block width of the Lanczos sweeps (0: min(nev, 8)); a
     block finds up to that many copies of a degenerate level, so it must be
-    at least the largest multiplicity among the wanted levels."""
+    at least the largest multiplicity among the wanted levels.  ``mixed_tol``
+    is the relative residual at which mixed precision hands over from the
+    complex64 sweeps to FP64 (0: 3e-7)."""
     if n == 0:
         return np.zeros(0, dtype=np.float64), np.zeros((0, 0), dtype=np.complex128)
     vals, vecs, info = _device_eigs(H_op._dev, nev, precision=precision, eig_tol=eig_tol, block=block,
-                                    want_vectors=want_vectors)
+                                    want_vectors=want_vectors, mixed_tol=mixed_tol)
     H_op.last_info = info
     return vals, vecs
 
