@@ -336,8 +336,12 @@ __device__ __forceinline__ void sell_slice(int64_t s, int lane, int64_t dim, con
   for (int q = 0; q < NB; ++q) ar[q] = ai[q] = (T)0;
   const int32_t* cp = col + o0 + lane;
   const V* vp = val + o0 + lane;
+  // the next entry's column is loaded while this entry's element and x
+  // values are fetched, so each entry costs one memory round trip
+  int cn = L > 0 ? __builtin_nontemporal_load(cp) : -1;
   for (int64_t t = 0; t < L; ++t) {
-    const int c = __builtin_nontemporal_load(cp + t * 64);
+    const int c = cn;
+    if (t + 1 < L) cn = __builtin_nontemporal_load(cp + (t + 1) * 64);
     if (c >= 0) {
       const V v = vp[t * 64];
       const V* x = X + (int64_t)c * NB;
