@@ -48,6 +48,7 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
 METRIC = "sector solves/s + p99 latency at |ΔE|≤1e-10 vs FP64; H|ψ⟩ HBM GB/s"
 TRAFFIC_FILE = ROOT / "profiles" / "spmm_hubbard3x4_traffic.json"
 
@@ -295,6 +296,27 @@ def checkerboard_model(size):
 
     return M.build_zero_flux_fermionic_fci_second_quantized_model(sample_size=list(size),
                                                                   params=dict(CHECKERBOARD_PARAMS))
+
+
+def by_class(ks: dict) -> dict:
+    """Every timed class against the HBM peak by algorithmic bytes (SpMM:
+    table, gathered and written vectors; Gram: the vectors read; update and
+    combine: the vectors read and written) over the event-timed duration, and
+    the Gram, update and combine also against the FP64 FMA peak by their
+    complex multiply-adds (8 flops each).  The restart combine is bound by
+    FP64 FMAs, the others by HBM."""
+    out = {}
+    for k, (ms, n, nbytes, flops) in ks.items():
+        if n <= 0 or ms <= 0 or nbytes <= 0:
+            continue
+        e = {"launches": n, "avg_launch_ms": round(ms / n, 5), "achieved": round(nbytes / ms / 1e6, 1),
+             "frac": round(nbytes / ms / 1e6 / HBM_PEAK_GBS, 4),
+             "bound": "fp64" if k == "combine" else "hbm"}
+        if flops > 0:
+            e["fp64_tflops"] = round(flops / ms / 1e9, 2)
+            e["fp64_frac"] = round(flops / ms / 1e9 / FP64_PEAK_TFLOPS, 4)
+        out[k] = e
+    return out
 
 
 def _traffic_entry(path: Path, kernel: str):
@@ -591,8 +613,8 @@ def checkerboard_bench(args) -> int:
     cls = max(ks, key=lambda k: ks[k][0])
     lanes = next((e["lanes"] for e in _edx.autotune_table() if e["block"] == nb), None)
     names = {"spmm_c64": f"spmm_sell<float2,{nb},{lanes}>", "spmm_c128": f"spmm_sell<double2,{max(nb, nev)},{lanes}>",
-             "gram": "gram_tiles + gram_reduce", "gemm": "gemm_tiles"}
-    k_ms, k_n, k_bytes = ks[cls]
+             "gram": "gram_tiles + gram_reduce", "gemm": "gemm_tiles", "combine": "gemm_tiles (restart combine)"}
+    k_ms, k_n, k_bytes = ks[cls][:3]
     avg_ms = k_ms / max(k_n, 1)
     achieved = k_bytes / k_ms / 1e6 if k_ms > 0 else 0.0
     tr = _traffic_entry(CHECKERBOARD_TRAFFIC_FILE, names.get(cls, cls))
@@ -612,12 +634,7 @@ def checkerboard_bench(args) -> int:
         "hbm_achieved": round(traffic / avg_ms / 1e6, 1) if traffic and avg_ms > 0 else None,
         "hbm_frac": round(traffic / avg_ms / 1e6 / HBM_PEAK_GBS, 4) if traffic and avg_ms > 0 else None,
         "kernel_time_share": {k: round(v[0] / total_k, 3) for k, v in ks.items() if total_k > 0},
-        # every timed class against the same peak: algorithmic bytes (SpMM:
-        # table, gathered and written vectors; Gram: the vectors read; update:
-        # the vectors read and written) over the event-timed duration
-        "by_class": {k: {"launches": v[1], "avg_launch_ms": round(v[0] / v[1], 5),
-                         "achieved": round(v[2] / v[0] / 1e6, 1), "frac": round(v[2] / v[0] / 1e6 / HBM_PEAK_GBS, 4)}
-                     for k, v in ks.items() if v[1] > 0 and v[0] > 0 and v[2] > 0},
+        "by_class": by_class(ks),
     }
     log("secondary workload (Hubbard 3x4) ...")
     del cat
@@ -1067,7 +1084,7 @@ def hubbard_bench(args) -> int:
 
     # roofline of the dominant SpMM (the class with the most kernel time)
     cls = max(("spmm_c64", "spmm_c128"), key=lambda k: ks[k][0])
-    spmm_ms, spmm_n, spmm_bytes = ks[cls]
+    spmm_ms, spmm_n, spmm_bytes = ks[cls][:3]
     nb = min(args.nev, 8)
     H0 = _sector_hamiltonian(ed, data, 0)
     lanes = H0._dev.spmv_group
@@ -1093,13 +1110,8 @@ def hubbard_bench(args) -> int:
                      "frac": round(iso_bytes / iso_ms / 1e6 / HBM_PEAK_GBS, 4),
                      "note": "same kernel, sector k=(0,0), one stream, 20 launches after the timed region"},
         "kernel_time_share": {k: round(v[0] / total_k, 3) for k, v in ks.items() if total_k > 0},
-        # every timed class against the same peak: algorithmic bytes (SpMM:
-        # table, gathered and written vectors; Gram: the vectors read; update:
-        # the vectors read and written) over the event-timed duration, which
-        # for the Gram includes its partial-sum reduction
-        "by_class": {k: {"launches": v[1], "avg_launch_ms": round(v[0] / v[1], 5),
-                         "achieved": round(v[2] / v[0] / 1e6, 1), "frac": round(v[2] / v[0] / 1e6 / HBM_PEAK_GBS, 4)}
-                     for k, v in ks.items() if v[1] > 0 and v[0] > 0 and v[2] > 0},
+        # (the Gram's duration includes its partial-sum reduction)
+        "by_class": by_class(ks),
     }
     cpu = None
     if not args.no_cpu_baseline:

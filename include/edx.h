@@ -359,13 +359,21 @@ int edx_host_herm_eig(int32_t n, const double* a, int32_t upper, double* w, doub
 
 /* ---- live kernel timing -------------------------------------------------------
  * No reference counterpart.  With timing enabled every complex128 SpMM (class
- * 0), complex64 SpMM (1), Gram (2) and block-update/combine (3) launch on the
- * ctx stream is bracketed by HIP events; ms/launches/bytes (arrays of
+ * 0), complex64 SpMM (1), Gram (2), block update (3) and restart combination
+ * (4) launch on the ctx stream is bracketed by HIP events; ms/launches/bytes (arrays of
  * EDX_KERNEL_CLASSES, NULL skips) receive the totals (bytes = algorithmic
  * bytes of each launch).  enable: 1 on, 0 off, -1 keep; reset != 0 clears the
  * totals first. */
 #define EDX_KERNEL_CLASSES 5
 int edx_kernel_stats(edx_ctx* ctx, int32_t enable, int32_t reset, double* ms, int64_t* launches, double* bytes);
+
+/* Arithmetic of the same timed launches (flops[EDX_KERNEL_CLASSES], totals
+ * since the last reset): 8 flops per complex multiply-add of the Gram,
+ * update and combine classes (FP64 FMAs, or FP32 matrix-core products for
+ * the complex64 matrix-core Gram), 0 for the SpMM classes.  Class 4 is the
+ * restart combinations (updates of more than 16 outputs).  No reference
+ * counterpart. */
+int edx_kernel_flops(edx_ctx* ctx, double* flops);
 
 /* ---- device helpers for benchmarks and tests ------------------------------ */
 int edx_device_alloc(edx_ctx* ctx, uint64_t bytes, void** out);

@@ -90,6 +90,7 @@ SIGNATURES = {
     "edx_mem_info": (C.c_int, [P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     "edx_time_matvec": (C.c_int, [P, P, i32, i32, i32, P, P, C.POINTER(C.c_double)]),
     "edx_kernel_stats": (C.c_int, [P, i32, i32, P, P, P]),
+    "edx_kernel_flops": (C.c_int, [P, P]),
     "edx_eigs_ex": (C.c_int, [P, P, P, P, P, P]),
     "edx_eigs_default_options": (None, [P]),
     "edx_fock_expand": (C.c_int, [P, P, P, P, P, f64, i32, i64, P, P, pi64]),
@@ -210,17 +211,20 @@ def current_context(default: "Context | None" = None) -> "Context":
     return default if default is not None else Context.default()
 
 
-KERNEL_CLASSES = ("spmm_c128", "spmm_c64", "gram", "gemm", "other")
+KERNEL_CLASSES = ("spmm_c128", "spmm_c64", "gram", "gemm", "combine")
 
 
 def kernel_stats(ctx: "Context", enable: int = -1, reset: bool = False) -> dict:
-    """Totals of the live kernel timer: {class: (ms, launches, bytes)}."""
+    """Totals of the live kernel timer: {class: (ms, launches, bytes, flops)}
+    (algorithmic bytes and FP64 / matrix-core flops of the timed launches)."""
     k = len(KERNEL_CLASSES)
     ms = np.zeros(k)
     n = np.zeros(k, dtype=np.int64)
     b = np.zeros(k)
+    f = np.zeros(k)
+    ctx.check(lib.edx_kernel_flops(ctx.handle, ptr(f)))
     ctx.check(lib.edx_kernel_stats(ctx.handle, enable, 1 if reset else 0, ptr(ms), ptr(n), ptr(b)))
-    return {k: (float(ms[i]), int(n[i]), float(b[i])) for i, k in enumerate(KERNEL_CLASSES)}
+    return {k: (float(ms[i]), int(n[i]), float(b[i]), float(f[i])) for i, k in enumerate(KERNEL_CLASSES)}
 
 
 def set_autotune(enable: bool = True, precision: int = PREC_MIXED, block: int = 6) -> None:
@@ -295,10 +299,10 @@ def kernel_stats_all(enable: int = -1, reset: bool = False) -> dict:
     being read while its owner writes it."""
     with Context._live_lock:
         ctxs = list(Context._live)
-    tot = {k: [0.0, 0, 0.0] for k in KERNEL_CLASSES}
+    tot = {k: [0.0, 0, 0.0, 0.0] for k in KERNEL_CLASSES}
     for c in ctxs:
         for k, v in kernel_stats(c, enable, reset).items():
-            for m in range(3):
+            for m in range(4):
                 tot[k][m] += v[m]
     return {k: tuple(v) for k, v in tot.items()}
 

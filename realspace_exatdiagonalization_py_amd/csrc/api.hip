@@ -329,6 +329,16 @@ int edx_kernel_stats(edx_ctx* ctx, int32_t enable, int32_t reset, double* ms, in
   });
 }
 
+int edx_kernel_flops(edx_ctx* ctx, double* flops) {
+  return edx::guarded(ctx, [&] {
+    EDX_REQUIRE(ctx && flops, EDX_ERR_INVALID, "null argument");
+    edx::DeviceGuard dg(ctx->device);
+    EDX_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->timer.flush();
+    for (int i = 0; i < edx::K_NCLASS; ++i) flops[i] = ctx->timer.flops[i];
+  });
+}
+
 int edx_device_alloc(edx_ctx* ctx, uint64_t bytes, void** out) {
   return edx::guarded(ctx, [&] {
     EDX_REQUIRE(ctx && out, EDX_ERR_INVALID, "null argument");

@@ -121,20 +121,23 @@ using ObjBuf = DevBuf<T, false>;
 
 // Live per-class kernel timing with HIP events on the launch stream
 // (enabled on request; used by benchmarks to time kernels inside a solve).
-enum KernelClass { K_SPMM = 0, K_SPMM32 = 1, K_GRAM = 2, K_GEMM = 3, K_OTHER = 4, K_NCLASS = EDX_KERNEL_CLASSES };
+// K_GEMM: block updates (at most 16 outputs); K_COMBINE: restart
+// combinations (more outputs, bound by FP64 FMAs rather than HBM).
+enum KernelClass { K_SPMM = 0, K_SPMM32 = 1, K_GRAM = 2, K_GEMM = 3, K_COMBINE = 4, K_NCLASS = EDX_KERNEL_CLASSES };
 
 struct KernelTimer {
   bool on = false;
   struct Pending {
     hipEvent_t a, b;
     int cls;
-    double bytes;
+    double bytes, flops;
   };
   std::vector<Pending> pending;
   std::vector<hipEvent_t> pool;
   double ms[K_NCLASS] = {};
   int64_t launches[K_NCLASS] = {};
   double bytes[K_NCLASS] = {};
+  double flops[K_NCLASS] = {};  // FP64 (or FP32 matrix-core) arithmetic
 
   hipEvent_t take() {
     if (pool.empty()) {
@@ -153,12 +156,12 @@ struct KernelTimer {
     if (e) (void)hipEventRecord(e, st);
     return e;
   }
-  void stop(hipStream_t st, hipEvent_t a, int cls, double nbytes) {
+  void stop(hipStream_t st, hipEvent_t a, int cls, double nbytes, double nflops = 0.0) {
     if (!on || !a) return;
     hipEvent_t b = take();
     if (!b) return;
     (void)hipEventRecord(b, st);
-    pending.push_back(Pending{a, b, cls, nbytes});
+    pending.push_back(Pending{a, b, cls, nbytes, nflops});
     if (pending.size() > 4096) flush();
   }
   void flush() {
@@ -169,6 +172,7 @@ struct KernelTimer {
         ms[p.cls] += t;
         launches[p.cls] += 1;
         bytes[p.cls] += p.bytes;
+        flops[p.cls] += p.flops;
       }
       pool.push_back(p.a);
       pool.push_back(p.b);
@@ -177,7 +181,7 @@ struct KernelTimer {
   }
   void reset() {
     flush();
-    for (int i = 0; i < K_NCLASS; ++i) ms[i] = 0, launches[i] = 0, bytes[i] = 0;
+    for (int i = 0; i < K_NCLASS; ++i) ms[i] = 0, launches[i] = 0, bytes[i] = 0, flops[i] = 0;
   }
   ~KernelTimer() {
     for (auto& p : pending) {

@@ -1222,7 +1222,7 @@ class BlockLanczos {
     hipLaunchKernelGGL(gram_reduce, dim3((nout + 3) / 4), dim3(256), 0, ctx_.stream, n_rowblocks_, nout,
                        (const double2*)partial_.p, hc_dev_, StepPred{});
     EDX_HIP(hipGetLastError());
-    ctx_.timer.stop(ctx_.stream, t0, edx::K_GRAM, (double)(nv + nw) * D_ * esz_);
+    ctx_.timer.stop(ctx_.stream, t0, edx::K_GRAM, (double)(nv + nw) * D_ * esz_, 8.0 * nv * nw * D_);
     wait();
     settle(true);
     Mat C(nv, nw);
@@ -1244,7 +1244,7 @@ class BlockLanczos {
     hipLaunchKernelGGL(gram_reduce, dim3((nout + 3) / 4), dim3(256), 0, ctx_.stream, n_rowblocks_, nout,
                        (const double2*)partial_.p, out, pred);
     EDX_HIP(hipGetLastError());
-    ctx_.timer.stop(ctx_.stream, t0, edx::K_GRAM, (double)(nv + nw) * D_ * esz_);
+    ctx_.timer.stop(ctx_.stream, t0, edx::K_GRAM, (double)(nv + nw) * D_ * esz_, 8.0 * nv * nw * D_);
   }
 
   // Second Cholesky QR pass of the block at dst without a host round trip:
@@ -1465,7 +1465,8 @@ class BlockLanczos {
     if (prec_ == EDX_PREC_FP64) launch_gemm_v<double2>(A0, n0, A1, n1, Out, nk, S, ldS, Xi, pred);
     else launch_gemm_v<float2>(A0, n0, A1, n1, Out, nk, S, ldS, Xi, pred);
     EDX_HIP(hipGetLastError());
-    ctx_.timer.stop(ctx_.stream, t0, edx::K_GEMM, (double)(n0 + n1 + (Xi ? 2 : 1) * nk) * D_ * esz_);
+    ctx_.timer.stop(ctx_.stream, t0, nk > 16 ? edx::K_COMBINE : edx::K_GEMM,
+                    (double)(n0 + n1 + (Xi ? 2 : 1) * nk) * D_ * esz_, 8.0 * (n0 + n1) * nk * D_);
   }
 
   // Out[:, 0..nk) = V[:, 0..nv) Y (Out must not alias V), one launch over
