@@ -263,14 +263,21 @@ int edx_eigs_ex(edx_ctx* ctx, const edx_hamiltonian* ham, const edx_eigs_options
  * is claimed for the call, EDX_ERR_BUSY if another thread holds one).  Every
  * round, the block-step SpMMs of the solves that are stepping go to grouped
  * launches on ctx's stream (sliced tables of equal block width and lanes per
- * row in one kernel), and the rest of each block step runs on the solve's own
- * stream.  vals: [n][nev]; vecs (optional): n pointers, each NULL or
- * complex128 [nev][dim_i]; info (optional): [n][4] as edx_eigs_ex.  Results
+ * row in one kernel), replayed from a captured HIP graph of that
+ * set of solves (edx_batch_graph_stats), and the rest of each block step runs
+ * on the solve's own stream.  vals: [n][nev]; vecs (optional): n pointers,
+ * each NULL or complex128 [nev][dim_i]; info (optional): [n][4] as edx_eigs_ex.  Results
  * are the same, bit for bit, as edx_eigs_ex on the same Hamiltonian with the
  * same schedule.  No reference counterpart (the reference solves sectors one
  * after another with ARPACK). */
 int edx_eigs_batch(edx_ctx* ctx, int32_t n, edx_ctx* const* ctxs, const edx_hamiltonian* const* hams,
                    const edx_eigs_options* opts, double* vals, double* const* vecs, double* info);
+/* Round graphs of the batched solves driven from ctx so far: the grouped
+ * SpMM of a round (one launch over the sliced tables of every stepping solve)
+ * is captured into a HIP graph once per set of stepping
+ * solves in a phase and replayed by every round with that set.  captures /
+ * replays (optional) receive the totals.  No reference counterpart. */
+int edx_batch_graph_stats(edx_ctx* ctx, int64_t* captures, int64_t* replays);
 
 /* ---- eigenvector consumers -----------------------------------------------------
  * Replaces _expand_sector_state_to_fock_amplitudes (ref

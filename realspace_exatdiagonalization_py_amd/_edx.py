@@ -75,6 +75,7 @@ SIGNATURES = {
     "edx_autotune_clear": (C.c_int, []),
     "edx_schedule_table": (C.c_int, [i32, P, P, pi32]),
     "edx_eigs_batch": (C.c_int, [P, i32, P, P, P, P, P, P]),
+    "edx_batch_graph_stats": (C.c_int, [P, pi64, pi64]),
     "edx_matvec_group": (C.c_int, [P, i32, P, i32, i32, P, P]),
     "edx_matvec_group_graph": (C.c_int, [P, i32, P, i32, i32, P, P, PP]),
     "edx_graph_launch": (C.c_int, [P, P]),
@@ -225,6 +226,14 @@ def kernel_stats(ctx: "Context", enable: int = -1, reset: bool = False) -> dict:
     ctx.check(lib.edx_kernel_flops(ctx.handle, ptr(f)))
     ctx.check(lib.edx_kernel_stats(ctx.handle, enable, 1 if reset else 0, ptr(ms), ptr(n), ptr(b)))
     return {k: (float(ms[i]), int(n[i]), float(b[i]), float(f[i])) for i, k in enumerate(KERNEL_CLASSES)}
+
+
+def batch_graph_stats(ctx: "Context") -> tuple[int, int]:
+    """Round graphs of the batched solves driven from ``ctx`` so far:
+    (captured, replayed)."""
+    cap, rep = C.c_int64(), C.c_int64()
+    ctx.check(lib.edx_batch_graph_stats(ctx.handle, C.byref(cap), C.byref(rep)))
+    return int(cap.value), int(rep.value)
 
 
 def set_autotune(enable: bool = True, precision: int = PREC_MIXED, block: int = 6) -> None:

@@ -202,6 +202,8 @@ struct Ctx {
   std::atomic<uintptr_t> owner{0};
   int depth = 0;
   int autotune_group = 0;  // lanes per row in the SpMV (0 = pick by heuristics)
+  // batched solves driven from this context: round graphs captured, replayed
+  int64_t graph_captures = 0, graph_replays = 0;
   // orthogonalisation schedule of the eigensolver (0 = tuned or default):
   // Gram 1 = vector FMAs, 2 = FP32 matrix cores (complex64, blocks <= 8);
   // block update 1..3 = stage/rows variants (kUpdateVariants)

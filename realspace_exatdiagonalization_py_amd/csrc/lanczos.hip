@@ -29,6 +29,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <exception>
+#include <map>
 #include <memory>
 #include <thread>
 #include <type_traits>
@@ -1175,6 +1176,18 @@ class BlockLanczos {
 
   template <typename VT>
   void launch_tiles_nw(const void* A0, int n0, const void* A1, int n1, const void* W, int nw, StepPred pred) {
+    // the Gram of a block of at most 4 vectors with itself (second Cholesky
+    // QR pass): one tile of exactly that size on vector FMAs (on the matrix
+    // cores it would fill 16-wide tiles with 4 live columns: 274 us against
+    // the 63 us that its 0.5 GB take at the HBM peak, complex64, D = 1.6e7)
+    if (n0 + n1 == nw && nw <= 4) {
+      switch (nw) {
+        case 1: launch_tiles<VT, 1, 1>(A0, n0, A1, n1, W, pred); return;
+        case 2: launch_tiles<VT, 2, 2>(A0, n0, A1, n1, W, pred); return;
+        case 3: launch_tiles<VT, 3, 3>(A0, n0, A1, n1, W, pred); return;
+        default: launch_tiles<VT, 4, 4>(A0, n0, A1, n1, W, pred); return;
+      }
+    }
     if (std::is_same<VT, float2>::value && nw <= 8 && gram_mode_ == kGramMatrix) {
       launch_gram_mfma(A0, n0, A1, n1, W, nw, pred);
       return;
@@ -1872,6 +1885,14 @@ int edx_eigs_batch(edx_ctx* ctx, int32_t n, edx_ctx* const* ctxs, const edx_hami
   });
 }
 
+int edx_batch_graph_stats(edx_ctx* ctx, int64_t* captures, int64_t* replays) {
+  return edx::guarded(ctx, [&] {
+    EDX_REQUIRE(ctx, EDX_ERR_INVALID, "null argument");
+    if (captures) *captures = ctx->graph_captures;
+    if (replays) *replays = ctx->graph_replays;
+  });
+}
+
 int edx_eigs(edx_ctx* ctx, const edx_hamiltonian* ham, int32_t nev, int32_t block, double tol, uint64_t seed,
              int32_t precision, double* vals, double* vecs, double* info) {
   edx_eigs_options o;
@@ -2031,16 +2052,97 @@ struct BatchSolve {
   int restarts = 0;
 };
 
+// A captured HIP graph of one round's grouped SpMM launch(es).
+struct RoundGraph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  double bytes = 0.0;  // algorithmic bytes, for the kernel timer
+};
+
+// The round graphs of a batch phase, one per set of stepping solves.  Within
+// a phase the launch arguments of a set (tables, interleaved inputs, outputs,
+// leading dimensions) do not change, so a set's graph is captured once and
+// replayed by every later round with the same set.
+class RoundGraphs {
+ public:
+  explicit RoundGraphs(edx_ctx& bctx) : bctx_(bctx) {}
+  RoundGraphs(const RoundGraphs&) = delete;
+  RoundGraphs& operator=(const RoundGraphs&) = delete;
+  ~RoundGraphs() { clear(); }
+
+  // Enqueues the grouped SpMM of `jobs` (solve ids `key`) on the batch
+  // stream as a replay of the set's graph, capturing it first if needed.
+  void launch(const std::vector<int>& key, const std::vector<edx::SpmmJob>& jobs, int precision) {
+    auto it = graphs_.find(key);
+    if (it == graphs_.end()) {
+      if (graphs_.size() >= kMaxGraphs) clear();
+      it = graphs_.emplace(key, capture(jobs, precision)).first;
+      ++captures_;
+    }
+    hipEvent_t t0 = bctx_.timer.start(bctx_.stream);
+    EDX_HIP(hipGraphLaunch(it->second.exec, bctx_.stream));
+    bctx_.timer.stop(bctx_.stream, t0, precision == EDX_PREC_FP64 ? edx::K_SPMM : edx::K_SPMM32, it->second.bytes);
+    ++replays_;
+  }
+  int captures() const { return captures_; }
+  int replays() const { return replays_; }
+
+ private:
+  static constexpr size_t kMaxGraphs = 64;
+
+  RoundGraph capture(const std::vector<edx::SpmmJob>& jobs, int precision) {
+    RoundGraph r;
+    r.bytes = edx::spmm_group_bytes(jobs.data(), (int)jobs.size(), precision);
+    // the launches only: the kernel timer's events stay out of the graph
+    const bool timer_on = bctx_.timer.on;
+    bctx_.timer.on = false;
+    hipError_t cap = hipStreamBeginCapture(bctx_.stream, hipStreamCaptureModeThreadLocal);
+    try {
+      EDX_HIP(cap);
+      edx::spmm_group(jobs.data(), (int)jobs.size(), bctx_, precision);
+      EDX_HIP(hipStreamEndCapture(bctx_.stream, &r.graph));
+      cap = hipErrorUnknown;  // capture ended
+      EDX_HIP(hipGraphInstantiate(&r.exec, r.graph, nullptr, nullptr, 0));
+    } catch (...) {
+      bctx_.timer.on = timer_on;
+      if (cap == hipSuccess) {  // still capturing: end it so the stream is usable again
+        hipGraph_t partial = nullptr;
+        (void)hipStreamEndCapture(bctx_.stream, &partial);
+        if (partial) (void)hipGraphDestroy(partial);
+      }
+      if (r.exec) (void)hipGraphExecDestroy(r.exec);
+      if (r.graph) (void)hipGraphDestroy(r.graph);
+      throw;
+    }
+    bctx_.timer.on = timer_on;
+    return r;
+  }
+  // Replays already enqueued stay valid: HIP keeps a graph's resources until
+  // its launches have completed.
+  void clear() {
+    for (auto& kv : graphs_) {
+      if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
+      if (kv.second.graph) (void)hipGraphDestroy(kv.second.graph);
+    }
+    graphs_.clear();
+  }
+
+  edx_ctx& bctx_;
+  std::map<std::vector<int>, RoundGraph> graphs_;
+  int captures_ = 0, replays_ = 0;
+};
+
 // Runs the BlockLanczos of every solve to convergence together.  Each round,
 // the solves that are stepping hand their SpMM to one grouped launch on the
-// batch context's stream (which first waits for each of their streams), then
-// enqueue the rest of their block step on their own streams (which wait for
-// the grouped launch); a solve whose step asks for the host leaves the round,
+// batch context's stream (which first waits for each of their
+// streams), replayed from the round graph of that set of solves, then enqueue
+// the rest of their block step on their own streams (which wait for the
+// grouped launch); a solve whose step asks for the host leaves the round,
 // and once none is stepping, each waits for its own stream and runs its
 // check (on a few host threads).  Solves that converge leave the batch.
 // `starts[i]` (optional) seeds solve i as BlockLanczos::run does.
 void run_batch_phase(edx_ctx& bctx, std::vector<BatchSolve*>& solves, int precision,
-                     const std::vector<std::pair<const void*, int>>& starts) {
+                     const std::vector<std::pair<const void*, int>>& starts, int* graph_counts) {
   for (size_t i = 0; i < solves.size(); ++i) {
     BatchSolve* s = solves[i];
     edx::StreamScope scope(s->ctx->stream);
@@ -2048,19 +2150,23 @@ void run_batch_phase(edx_ctx& bctx, std::vector<BatchSolve*>& solves, int precis
   }
   hipEvent_t done_ev;
   EDX_HIP(hipEventCreateWithFlags(&done_ev, hipEventDisableTiming));
+  RoundGraphs graphs(bctx);
   try {
     std::vector<BatchSolve*> active = solves;
     std::vector<edx::SpmmJob> jobs;
+    std::vector<int> key;
     while (!active.empty()) {
       std::vector<BatchSolve*> stepping = active;
       while (!stepping.empty()) {
         jobs.clear();
+        key.clear();
         for (BatchSolve* s : stepping) {
           EDX_HIP(hipEventRecord(s->ev, s->ctx->stream));
           EDX_HIP(hipStreamWaitEvent(bctx.stream, s->ev, 0));
           jobs.push_back(edx::SpmmJob{&s->lz->op(), s->lz->block(), s->lz->spmm_in(), s->lz->spmm_out(), s->lz->ld()});
+          key.push_back(s->idx);
         }
-        edx::spmm_group(jobs.data(), (int)jobs.size(), bctx, precision);
+        graphs.launch(key, jobs, precision);
         EDX_HIP(hipEventRecord(done_ev, bctx.stream));
         std::vector<BatchSolve*> next;
         for (BatchSolve* s : stepping) {
@@ -2106,6 +2212,10 @@ void run_batch_phase(edx_ctx& bctx, std::vector<BatchSolve*>& solves, int precis
   }
   EDX_HIP(hipStreamSynchronize(bctx.stream));
   (void)hipEventDestroy(done_ev);
+  if (graph_counts) {
+    graph_counts[0] += graphs.captures();
+    graph_counts[1] += graphs.replays();
+  }
 }
 
 void eigs_batch_impl(edx_ctx* bctx, int n, edx_ctx* const* ctxs, const edx_hamiltonian* const* hams,
@@ -2117,6 +2227,15 @@ void eigs_batch_impl(edx_ctx* bctx, int n, edx_ctx* const* ctxs, const edx_hamil
   EDX_REQUIRE(o.max_restarts >= 1, EDX_ERR_INVALID, "max_restarts must be >= 1");
   edx::DeviceGuard dg(bctx->device);
   if (info) std::fill(info, info + 4 * (size_t)n, 0.0);
+  int graph_counts[2] = {0, 0};  // round graphs captured, replayed
+  struct CountGuard {
+    edx_ctx& c;
+    const int* g;
+    ~CountGuard() {
+      c.graph_captures += g[0];
+      c.graph_replays += g[1];
+    }
+  } count_guard{*bctx, graph_counts};
   // sectors of the dense path (and empty ones) are solved one by one
   std::vector<std::unique_ptr<BatchSolve>> own;
   std::vector<BatchSolve*> lanczos;
@@ -2167,7 +2286,7 @@ void eigs_batch_impl(edx_ctx* bctx, int n, edx_ctx* const* ctxs, const edx_hamil
       edx::StreamScope scope(s->ctx->stream);
       s->lz.reset(new BlockLanczos(*s->ctx, *s->h, EDX_PREC_MIXED, o32));
     }
-    run_batch_phase(*bctx, lanczos, EDX_PREC_MIXED, no_start);
+    run_batch_phase(*bctx, lanczos, EDX_PREC_MIXED, no_start, graph_counts);
     std::vector<DevBuf<double2>> starts(lanczos.size());
     std::vector<std::pair<const void*, int>> st(lanczos.size());
     for (size_t i = 0; i < lanczos.size(); ++i) {
@@ -2192,14 +2311,14 @@ void eigs_batch_impl(edx_ctx* bctx, int n, edx_ctx* const* ctxs, const edx_hamil
       st[i] = {starts[i].p, n_start};
       s->lz.reset(new BlockLanczos(*s->ctx, *s->h, EDX_PREC_FP64, o64));
     }
-    run_batch_phase(*bctx, lanczos, EDX_PREC_FP64, st);
+    run_batch_phase(*bctx, lanczos, EDX_PREC_FP64, st, graph_counts);
   } else {
     opt.check_every = 2;
     for (BatchSolve* s : lanczos) {
       edx::StreamScope scope(s->ctx->stream);
       s->lz.reset(new BlockLanczos(*s->ctx, *s->h, EDX_PREC_FP64, opt));
     }
-    run_batch_phase(*bctx, lanczos, EDX_PREC_FP64, no_start);
+    run_batch_phase(*bctx, lanczos, EDX_PREC_FP64, no_start, graph_counts);
   }
   for (size_t i = 0; i < lanczos.size(); ++i) {
     BatchSolve* s = lanczos[i];

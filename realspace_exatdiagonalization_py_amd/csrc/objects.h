@@ -124,6 +124,8 @@ struct SpmmJob {
 // The jobs (one element precision) on ctx's stream; sliced tables of equal
 // block width and lanes per row share launches, with spmm()'s arithmetic.
 void spmm_group(const SpmmJob* jobs, int n, Ctx& ctx, int precision);
+// Algorithmic bytes of those products (as the kernel timer counts them).
+double spmm_group_bytes(const SpmmJob* jobs, int n, int precision);
 void finish_sell(edx_hamiltonian& h, Ctx& ctx, SellPlan& p);
 void abandon_sell(const edx_hamiltonian& h, SellPlan& p);
 // Slice offsets and storage of a sliced copy with G lanes per row (contents

@@ -715,6 +715,15 @@ void launch_group(const SpmmJob* jobs, const std::vector<int>& idx, int precisio
   (void)precision;
 }
 
+double spmm_group_bytes(const SpmmJob* jobs, int n, int precision) {
+  double b = 0.0;
+  for (int i = 0; i < n; ++i)
+    if (jobs[i].h->dim > 0)
+      b += jobs[i].h->on_the_fly ? fly_bytes(*jobs[i].h, precision, jobs[i].nb)
+                                 : spmm_bytes(*jobs[i].h, precision, jobs[i].nb);
+  return b;
+}
+
 // Applies each job's H to its block (all of one precision) on ctx's stream.
 // Sliced tables of equal block width and lanes per row share one launch (up
 // to kGroupMax tables each); on-the-fly tables run one by one.  Every table

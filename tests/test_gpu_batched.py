@@ -5,9 +5,10 @@ Haldane bosons 3x4 (1.1e4, once with a stored table and once on the fly),
 Hubbard 3x4 (2.3e5), a twisted Hubbard 2x4 sector (the Hermitian-part
 table) and a sector of the dense path (Heisenberg 12, D <= 500) — is solved
 together and one by one, in FP64 and in mixed precision.  Every solve runs
-the same kernels on the same inputs in both cases (the grouped SpMM applies
-each table with the single-table kernel's arithmetic), so eigenvalues,
-eigenvectors and matvec counts must be identical bit for bit.  The sector
+the same kernels on the same inputs in both cases (the grouped SpMM,
+replayed from captured round graphs, applies each table with the
+single-table kernel's arithmetic), so eigenvalues, eigenvectors and
+matvec counts must be identical bit for bit; the round graphs are counted.  The sector
 loop ``solve_sectors_batched`` must equal ``ed_scan``'s results the same
 way, and invalid batches are refused.
 """
@@ -51,8 +52,15 @@ def operators():
 def test_batched_solves_equal_single_solves(operators, precision):
     nev = 5
     eig_tol = 1e-12 if precision == ed.PREC_MIXED else 0.0
+    cap0, rep0 = _edx.batch_graph_stats(_edx.current_context())
     got = ed.diagonalize_blocks_batched(operators, nev=nev, precision=precision, eig_tol=eig_tol)
     infos = [H.last_info.copy() for H in operators]
+    # every round's grouped SpMM was a replay of a captured graph, and rounds
+    # with an unchanged set of stepping solves reused their set's graph
+    cap, rep = _edx.batch_graph_stats(_edx.current_context())
+    rounds = sum(int(i[0] / i[3]) for i in infos if i[3] > 0)
+    print(f"[batch] round graphs: {cap - cap0} captured, {rep - rep0} replayed", flush=True)
+    assert 0 < cap - cap0 < rep - rep0 <= rounds
     for H, (w, v), inf in zip(operators, got, infos):
         w1, v1 = ed.diagonalize_block_matrixfree(H, H.n, nev=nev, precision=precision, eig_tol=eig_tol)
         print(f"[batch] D={H.n} prec={precision} matvecs={inf[0]:.0f} E0={w[0]:.12f}", flush=True)
