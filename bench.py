@@ -233,7 +233,8 @@ def profile_solve(args) -> int:
     from realspace_exatdiagonalization_py_amd.groups import build_irrep_list, build_translation_group
 
     nb = args.block if args.block is not None else CHECKERBOARD_BLOCK
-    _edx.set_autotune(False, ed.PREC_MIXED, nb if nb > 0 else 8)
+    # tuned SpMM layout and solver schedules, as in the benchmark's solves
+    _edx.set_autotune(not args.no_autotune, ed.PREC_MIXED, nb if nb > 0 else 8)
     model = checkerboard_model((4, 6))
     G = build_translation_group(model.lattice)
     cat = ed.build_symmetry_orbit_catalog(second_quantized_model=model, n_filled=8, symmetry_group=G,
