@@ -65,7 +65,7 @@ def build_workload():
     return model, G
 
 
-def solve_sector(ed, data, idx, nev, precision, eig_tol):
+def solve_sector(ed, data, idx, nev, precision, eig_tol, block=0):
     """One sector solve as the reference's ed_scan_at_irrep_matrixfree does it:
     irrep filter, projection table, eigenvalues and eigenvectors."""
     model = data.second_quantized_model
@@ -75,12 +75,12 @@ def solve_sector(ed, data, idx, nev, precision, eig_tol):
     data.sector_dims[idx] = basis.dim
     H = ed.MatrixFreeHamiltonian(basis, model.bilinear_terms, model.density_density_terms, cmap)
     w, v = ed.diagonalize_block_matrixfree(H, H.n, nev=nev, precision=precision, eig_tol=eig_tol,
-                                           want_vectors=True)
+                                           want_vectors=True, block=block)
     assert v.shape == (basis.dim, len(w))
     return time.perf_counter() - t0, np.asarray(w)
 
 
-def solve_stream(ed, data, nev, precision, eig_tol, workers, steps):
+def solve_stream(ed, data, nev, precision, eig_tol, workers, steps, block=0):
     """``steps`` passes over all sectors submitted as one stream of sector
     solves to the worker pool (the next pass starts as workers free up, so no
     worker idles at a pass boundary).  Returns the latencies and the values of
@@ -91,7 +91,7 @@ def solve_stream(ed, data, nev, precision, eig_tol, workers, steps):
 
     def one(j):
         k, i = items[j]
-        lat[j], w = solve_sector(ed, data, i, nev, precision, eig_tol)
+        lat[j], w = solve_sector(ed, data, i, nev, precision, eig_tol, block)
         if k == steps - 1:
             vals[i] = w
 
@@ -99,13 +99,13 @@ def solve_stream(ed, data, nev, precision, eig_tol, workers, steps):
     return lat, vals
 
 
-def solve_all(ed, data, nev, precision, eig_tol, workers=1):
+def solve_all(ed, data, nev, precision, eig_tol, workers=1, block=0):
     """All sectors, ``workers`` at a time (one HIP stream each)."""
     n = len(data.irrep_list)
     lat, vals = [0.0] * n, [None] * n
 
     def one(i):
-        lat[i], vals[i] = solve_sector(ed, data, i, nev, precision, eig_tol)
+        lat[i], vals[i] = solve_sector(ed, data, i, nev, precision, eig_tol, block)
 
     ed.run_concurrently(range(n), one, workers)
     return lat, vals
@@ -287,6 +287,16 @@ CHECKERBOARD_BLOCK = 4
 # Hubbard, Haldane and Heisenberg workloads keep 3e-7 (lower was 0-8 %
 # slower there).
 CHECKERBOARD_MIXED_TOL = 1.5e-8
+# Block widths of the other workloads, measured over all their sectors
+# (mixed, eigenvalue bound 1e-11).  The 36-site Haldane bosons at two twists
+# (36 solves) take 21.9 s at width 2 against 26.0 s at the default 5, levels
+# equal within 1.6e-13.  The 12 Hubbard 3x4 sectors, which hold spin triplets
+# among their 6 lowest levels, take 0.24 s one after another at width 3
+# against 0.28 s at 6 (levels equal within 1.1e-13), but on 8 streams width 3
+# gave 78.8-79.1 solves/s with p99 237-242 ms against 75.5 solves/s with p99
+# 171-174 ms at 6, so that workload keeps the library default (0).
+FLUX_BLOCK = 2
+HUBBARD_BLOCK = 0
 CHECKERBOARD_WORKLOAD = ("fermionic checkerboard flat band 4x6x2 (48 sites), 8 fermions, C(48,8)=377348994, t=1, "
                          "t'=+-0.3, t''=-0.2, NN phase pi/4, NN V=1, translation group of order 24; one step = one "
                          "momentum-sector solve (sectors in turn), 10 lowest levels, mixed precision")
@@ -461,23 +471,25 @@ def hubbard_secondary(ed, _edx, workers=8, passes=2, eig_tol=1e-11):
 
     model = ed.build_spinful_hubbard_second_quantized_model([3, 4], 1.0, 4.0)
     G = build_translation_group(model.lattice)
-    _edx.set_autotune(True, ed.PREC_MIXED, 6)
+    _edx.set_autotune(True, ed.PREC_MIXED, HUBBARD_BLOCK)
     data = ed.build_ed_data(model, filling_fraction=Fraction(12, 24), symmetry_group=G)
-    solve_all(ed, data, 6, ed.PREC_MIXED, eig_tol, workers)
+    solve_all(ed, data, 6, ed.PREC_MIXED, eig_tol, workers, HUBBARD_BLOCK)
     t0 = time.perf_counter()
-    lat, vals = solve_stream(ed, data, 6, ed.PREC_MIXED, eig_tol, workers, passes)
+    lat, vals = solve_stream(ed, data, 6, ed.PREC_MIXED, eig_tol, workers, passes, HUBBARD_BLOCK)
     el = time.perf_counter() - t0
     lat_ms = np.asarray(lat) * 1e3
     # the same passes as batches: each pass's 12 sector solves advance together
     # from one thread, their SpMMs grouped into shared launches
-    ed.solve_sectors_batched(data, nev=6, precision=ed.PREC_MIXED, eig_tol=eig_tol, table="stored")
+    ed.solve_sectors_batched(data, nev=6, precision=ed.PREC_MIXED, eig_tol=eig_tol, table="stored",
+                             block=HUBBARD_BLOCK)
     t0 = time.perf_counter()
     for _ in range(passes):
-        res = ed.solve_sectors_batched(data, nev=6, precision=ed.PREC_MIXED, eig_tol=eig_tol, table="stored")
+        res = ed.solve_sectors_batched(data, nev=6, precision=ed.PREC_MIXED, eig_tol=eig_tol, table="stored",
+                                       block=HUBBARD_BLOCK)
     el_b = time.perf_counter() - t0
     d_b = max(float(np.max(np.abs(np.asarray(r[0]) - np.asarray(v)))) for r, v in zip(res, vals))
     return {"workload": "spinful Hubbard 3x4 PBC, t=1, U=4, 12 fermions on 24 orbitals, all 12 momentum sectors, "
-                        "6 lowest levels each, mixed precision",
+                        f"6 lowest levels each, mixed precision, Lanczos block width {HUBBARD_BLOCK or 6}",
             "sector_solves": len(lat), "workers": workers, "solves_per_s": round(len(lat) / el, 3),
             "p50_ms": round(float(np.percentile(lat_ms, 50)), 3), "p99_ms": round(float(np.percentile(lat_ms, 99)), 3),
             "batched_solves_per_s": round(passes * len(data.irrep_list) / el_b, 3),
@@ -900,7 +912,8 @@ def flux_sweep(args) -> int:
     from realspace_exatdiagonalization_py_amd.groups import build_irrep_list, build_translation_group
 
     nev, ne = 5, 9
-    _edx.set_autotune(not args.no_autotune, ed.PREC_MIXED, nev)
+    nb = args.flux_block if args.flux_block is not None else FLUX_BLOCK
+    _edx.set_autotune(not args.no_autotune, ed.PREC_MIXED, nb if nb > 0 else nev)
     model = M.build_zero_flux_bosonic_fci_second_quantized_model(sample_size=[3, 6])
     stats = model.particle_statistics
     t0 = time.perf_counter()
@@ -925,7 +938,7 @@ def flux_sweep(args) -> int:
             dims.append(basis.dim)
             H = ed.MatrixFreeHamiltonian(basis, model.bilinear_terms, model.density_density_terms, None)
             w, _ = ed.diagonalize_block_matrixfree(H, H.n, nev=nev, precision=ed.PREC_MIXED, eig_tol=args.eig_tol,
-                                                   want_vectors=False)
+                                                   want_vectors=False, block=nb)
             last[i] = (np.asarray(w), H)
             if i != 0:
                 del last[i]
@@ -970,6 +983,7 @@ def flux_sweep(args) -> int:
         "sector_dims": [min(dims), max(dims)],
         "workers": args.flux_workers,
         "precision": "mixed",
+        "block": nb,
         "max_abs_dE_vs_fp64_one_sector": float(np.max(np.abs(w32 - w64))),
         "full_sweep_estimate_s": round(576 / (n_solves / elapsed), 1),
         "cpu_baseline": cpu,
@@ -1007,6 +1021,8 @@ def main() -> int:
     ap.add_argument("--flux-sweep", action="store_true",
                     help="twisted-boundary sweep of the Haldane 3x6 boson workload (18 sectors per twist)")
     ap.add_argument("--flux-twists", type=int, default=2, help="timed twists of --flux-sweep")
+    ap.add_argument("--flux-block", type=int, default=None,
+                    help=f"Lanczos block width of --flux-sweep (default {FLUX_BLOCK}; 0: min(nev, 8))")
     ap.add_argument("--flux-workers", type=int, default=2,
                     help="concurrent sector solves of --flux-sweep (each holds ~33 GB of tables and bases)")
     ap.add_argument("--serve", action="store_true", help="serving workload: a Poisson stream of mixed requests")
@@ -1048,9 +1064,11 @@ def hubbard_bench(args) -> int:
 
     args.nev = args.nev or 6
     prec = ed.PREC_MIXED if args.precision == "mixed" else ed.PREC_FP64
+    blk = args.block if args.block is not None else HUBBARD_BLOCK  # 0: min(nev, 8)
+    nb = blk if blk > 0 else min(args.nev, 8)
     ctx = _edx.Context.default()
     # SpMM lanes per row timed once per sector shape (the warmup step pays)
-    _edx.set_autotune(not args.no_autotune, prec, min(args.nev, 8))
+    _edx.set_autotune(not args.no_autotune, prec, nb)
     model, G = build_workload()
     log("building catalog ...")
     t0 = time.perf_counter()
@@ -1064,12 +1082,12 @@ def hubbard_bench(args) -> int:
     log(f"catalog: {data.orbit_catalog.n_orbits} orbits, first {t_cat*1e3:.1f} ms, again {t_cat2*1e3:.1f} ms")
 
     for i in range(args.warmup):
-        solve_all(ed, data, args.nev, prec, args.eig_tol, args.workers)
+        solve_all(ed, data, args.nev, prec, args.eig_tol, args.workers, blk)
         log(f"warmup step {i + 1}/{args.warmup} done")
     _edx.kernel_stats_all(enable=0 if args.no_kernel_timer else 1, reset=True)
     ctx.check(_edx.lib.edx_synchronize(ctx.handle))
     t0 = time.perf_counter()
-    lat_all, vals = solve_stream(ed, data, args.nev, prec, args.eig_tol, args.workers, args.steps)
+    lat_all, vals = solve_stream(ed, data, args.nev, prec, args.eig_tol, args.workers, args.steps, blk)
     elapsed = time.perf_counter() - t0
     log(f"{args.steps} steps ({len(lat_all)} sector solves) in {elapsed*1e3:.1f} ms")
     ks = _edx.kernel_stats_all(enable=0)
@@ -1086,7 +1104,6 @@ def hubbard_bench(args) -> int:
     # roofline of the dominant SpMM (the class with the most kernel time)
     cls = max(("spmm_c64", "spmm_c128"), key=lambda k: ks[k][0])
     spmm_ms, spmm_n, spmm_bytes = ks[cls][:3]
-    nb = min(args.nev, 8)
     H0 = _sector_hamiltonian(ed, data, 0)
     lanes = H0._dev.spmv_group
     kname = f"spmm_sell<{'float2' if cls == 'spmm_c64' else 'double2'},{nb},{lanes}>"
@@ -1141,6 +1158,7 @@ def hubbard_bench(args) -> int:
             "sector_dims": [min(dims), max(dims)],
             "nev": args.nev,
             "precision": args.precision,
+            "block": nb,
             "concurrent_sectors": args.workers,
             "schedule": "the K steps x 12 sector solves go to the worker pool as one stream",
         },

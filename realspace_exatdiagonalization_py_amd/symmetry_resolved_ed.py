@@ -845,21 +845,21 @@ def diagonalize_block_matrixfree(H_op: MatrixFreeHamiltonian, n: int, *, nev: in
 
 
 def diagonalize_blocks_batched(H_ops, *, nev: int = 5, precision: int = _edx.PREC_FP64, eig_tol: float = 0.0,
-                               want_vectors: bool = True, tol: float = 1e-10, seed: int = 12345):
+                               want_vectors: bool = True, tol: float = 1e-10, seed: int = 12345, block: int = 0):
     """Lowest ``nev`` eigenpairs of several device operators solved together
     (``edx_eigs_batch``): every solve on its own HIP stream, the block-step
     SpMMs of all solves grouped into shared launches, one host thread.  Each
     result equals ``diagonalize_block_matrixfree`` on the same operator bit for
     bit (same schedules).  Returns a list of ``(vals, vecs)``; each operator's
     ``last_info`` is set.  Sectors of the dense path (n <= 500) and empty ones
-    are solved one by one."""
+    are solved one by one.  ``block``: as in ``diagonalize_block_matrixfree``."""
     ops = list(H_ops)
     out: list = [None] * len(ops)
     batch = [i for i, H in enumerate(ops) if H.n > DENSE_CUTOFF]
     for i, H in enumerate(ops):
         if i not in batch:
             out[i] = diagonalize_block_matrixfree(H, H.n, nev=nev, precision=precision, eig_tol=eig_tol,
-                                                  want_vectors=want_vectors)
+                                                  want_vectors=want_vectors, block=block)
     if not batch:
         return out
     bctx = _edx.current_context()
@@ -869,7 +869,7 @@ def diagonalize_blocks_batched(H_ops, *, nev: int = 5, precision: int = _edx.PRE
     vecs = [np.empty((nev, ops[i].n), dtype=np.complex128) for i in batch] if want_vectors else None
     ham_arr = (_edx.C.c_void_p * n)(*[ops[i]._dev.handle for i in batch])
     vec_arr = (_edx.C.c_void_p * n)(*[v.ctypes.data for v in vecs]) if want_vectors else None
-    opts = _edx.eigs_options(nev=nev, precision=precision, tol=tol, eig_tol=eig_tol, seed=seed)
+    opts = _edx.eigs_options(nev=nev, block=block, precision=precision, tol=tol, eig_tol=eig_tol, seed=seed)
     with _BATCH_LOCK:  # batches share the pool of solve contexts: one batch at a time
         ctxs = _batch_contexts(n)
         ctx_arr = (_edx.C.c_void_p * n)(*[c.handle for c in ctxs])
@@ -883,7 +883,7 @@ def diagonalize_blocks_batched(H_ops, *, nev: int = 5, precision: int = _edx.PRE
 
 def solve_sectors_batched(ed_data: "Symmetry_Resolved_ED_Data", *, nev: int = 5, precision: int = _edx.PREC_MIXED,
                           scanned_sectors=None, eig_tol: float = 0.0, want_vectors: bool = True,
-                          table: str = "auto"):
+                          table: str = "auto", block: int = 0):
     """The matrix-free solves of several sectors (all, or ``scanned_sectors``)
     as one batch (``diagonalize_blocks_batched``); results go to
     ``ed_data.ed_scan_res`` and ``sector_dims`` as ``ed_scan`` stores them."""
@@ -913,7 +913,7 @@ def solve_sectors_batched(ed_data: "Symmetry_Resolved_ED_Data", *, nev: int = 5,
         idx.append(irrep_idx)
         ops.append(H)
     for irrep_idx, r in zip(idx, diagonalize_blocks_batched(ops, nev=nev, precision=precision, eig_tol=eig_tol,
-                                                            want_vectors=want_vectors)):
+                                                            want_vectors=want_vectors, block=block)):
         ed_data.ed_scan_res[irrep_idx] = r
     return [ed_data.ed_scan_res[i] for i in idx]
 

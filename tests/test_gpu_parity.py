@@ -374,6 +374,40 @@ def test_degenerate_multiplets_hubbard():
     assert np.max(np.abs(w - exact)) <= 1e-10, (w, exact)
 
 
+def test_block_width_and_handover_options():
+    """``block`` and ``mixed_tol`` of diagonalize_block_matrixfree on a 2x4
+    Hubbard sector whose 8 lowest levels hold spin multiplets: every block at
+    least as wide as the largest multiplicity gives the exact levels (1e-10),
+    in FP64 and in mixed precision with handovers 1e-8 and 1e-6; a narrower
+    block may return fewer copies of a multiplet, but every value it returns
+    is an exact level (the block's limit, documented, never a wrong value)."""
+    model, ne, G, oc, gc = _both("hub24")
+    irrep = build_irrep_list(G, model.lattice)[0]
+    reps, r, c, v, hd, k = _oracle_block(model, oc, irrep, True)
+    dense = O.csr_hamiltonian(r, c, v, hd, len(reps)).toarray()
+    spectrum = np.linalg.eigvalsh((dense + dense.conj().T) / 2)
+    nev = 8
+    exact = spectrum[:nev]
+    runs = np.split(exact, np.nonzero(np.diff(exact) > 1e-8)[0] + 1)
+    mult = max(len(x) for x in runs)  # copies wanted of the most degenerate level
+    print(f"[block] hub24 irrep=0 dim={len(reps)} multiplicities {[len(x) for x in runs]}", flush=True)
+    basis = ed.build_symmetry_sector_basis(gc, irrep)
+    H = ed.MatrixFreeHamiltonian(basis, model.bilinear_terms, model.density_density_terms, None)
+    for b in sorted({mult, mult + 1, 8}):
+        w, _ = ed.diagonalize_block_matrixfree(H, H.n, nev=nev, block=b)
+        assert np.max(np.abs(w - exact)) <= 1e-10, (b, w, exact)
+        assert int(H.last_info[3]) == b
+        for mtol in (1e-8, 1e-6):
+            w, _ = ed.diagonalize_block_matrixfree(H, H.n, nev=nev, block=b, precision=ed.PREC_MIXED,
+                                                   mixed_tol=mtol, eig_tol=1e-11)
+            assert np.max(np.abs(w - exact)) <= 1e-10, (b, mtol, w, exact)
+    if mult > 2:
+        w, _ = ed.diagonalize_block_matrixfree(H, H.n, nev=nev, block=mult - 1)
+        dist = np.min(np.abs(w[:, None] - spectrum[None, :]), axis=1)
+        print(f"[block] block {mult - 1}: {w.tolist()}", flush=True)
+        assert np.max(dist) <= 1e-9, (w, exact)
+
+
 def test_golden_energy_alignment():
     """Julia golden energies of the reference's test_energy_alignment.py (ATOL 1e-9), on the GPU."""
     import json
