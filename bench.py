@@ -265,6 +265,30 @@ def copy_kernel(args) -> int:
     return 0
 
 
+def read_ceiling(args) -> int:
+    """Practical HBM ceilings: a plain streaming read (edx_time_stream_read)
+    and the runtime's device copy of --read-bytes bytes, best of 5 each."""
+    import ctypes as C
+    from realspace_exatdiagonalization_py_amd import _edx
+
+    ctx = _edx.Context.default()
+    n = int(args.read_bytes) // 16 * 16
+    a = _edx.DeviceArray(ctx, n)
+    b = _edx.DeviceArray(ctx, n)
+    ctx.check(_edx.lib.edx_device_copy(ctx.handle, a.ptr, b.ptr, n))  # touch both
+    ms = C.c_double()
+    ctx.check(_edx.lib.edx_time_stream_read(ctx.handle, a.ptr, n, 5, C.byref(ms)))
+    best_copy = float("inf")
+    for _ in range(6):
+        t0 = time.perf_counter()
+        ctx.check(_edx.lib.edx_device_copy(ctx.handle, b.ptr, a.ptr, n))
+        best_copy = min(best_copy, time.perf_counter() - t0)
+    print(json.dumps({"bytes": n, "stream_read_GBps": round(n / ms.value / 1e6, 1),
+                      "stream_read_frac": round(n / ms.value / 1e6 / HBM_PEAK_GBS, 4),
+                      "copy_GBps_host_timed": round(2 * n / best_copy / 1e9, 1)}))
+    return 0
+
+
 # Headline workload: the 48-site checkerboard flat band (couplings of the
 # reference builder, t'_1 = -t'_2; nearest-neighbour phase 2 pi / 8).
 CHECKERBOARD_PARAMS = {"t": 1.0, "t'_1": 0.3, "t'_2": -0.3, "t''": -0.2, "phi_over_2pi": 1 / 8, "V1": 1.0,
@@ -1018,6 +1042,8 @@ def main() -> int:
     ap.add_argument("--profile-solve", action="store_true",
                     help="one solve of a 48-site checkerboard sector (for counter runs)")
     ap.add_argument("--copy-bytes", type=float, default=0, help="run only a plain device copy of this many bytes")
+    ap.add_argument("--read-bytes", type=float, default=0,
+                    help="run only the practical-ceiling probes (streaming read, copy) over this many bytes")
     ap.add_argument("--flux-sweep", action="store_true",
                     help="twisted-boundary sweep of the Haldane 3x6 boson workload (18 sectors per twist)")
     ap.add_argument("--flux-twists", type=int, default=2, help="timed twists of --flux-sweep")
@@ -1044,6 +1070,8 @@ def main() -> int:
         return profile_solve(args)
     if args.copy_bytes:
         return copy_kernel(args)
+    if args.read_bytes:
+        return read_ceiling(args)
     if args.serve:
         return serve_bench(args)
     if args.flux_sweep:

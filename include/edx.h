@@ -387,6 +387,10 @@ int edx_device_alloc(edx_ctx* ctx, uint64_t bytes, void** out);
 int edx_device_free(edx_ctx* ctx, void* p);
 int edx_memcpy_h2d(edx_ctx* ctx, void* dst, const void* src, uint64_t bytes);
 int edx_memcpy_d2h(edx_ctx* ctx, void* dst, const void* src, uint64_t bytes);
+/* Best time of n_rep launches (after one warm-up) of a plain streaming read of
+ * `bytes` (a positive multiple of 16) at src: the practical read ceiling the
+ * bandwidth-bound kernels are compared with.  No reference counterpart. */
+int edx_time_stream_read(edx_ctx* ctx, const void* src, uint64_t bytes, int32_t n_rep, double* best_ms);
 /* Free and total device memory of the ctx's device (hipMemGetInfo); used by
  * the serving loop to admit work by HBM footprint. */
 int edx_mem_info(edx_ctx* ctx, uint64_t* free_bytes, uint64_t* total_bytes);

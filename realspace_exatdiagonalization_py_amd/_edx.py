@@ -92,6 +92,7 @@ SIGNATURES = {
     "edx_time_matvec": (C.c_int, [P, P, i32, i32, i32, P, P, C.POINTER(C.c_double)]),
     "edx_kernel_stats": (C.c_int, [P, i32, i32, P, P, P]),
     "edx_kernel_flops": (C.c_int, [P, P]),
+    "edx_time_stream_read": (C.c_int, [P, P, u64, i32, P]),
     "edx_eigs_ex": (C.c_int, [P, P, P, P, P, P]),
     "edx_eigs_default_options": (None, [P]),
     "edx_fock_expand": (C.c_int, [P, P, P, P, P, f64, i32, i64, P, P, pi64]),

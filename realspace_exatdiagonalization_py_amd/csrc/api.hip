@@ -202,6 +202,23 @@ void copy_to_host(Ctx& ctx, void* dst, size_t dpitch, const void* src, size_t sp
 
 }  // namespace edx
 
+namespace {
+// Streaming-read reference for the bandwidth figures: a grid-stride sum over
+// n 16-byte words with four loads in flight per lane; the sum is written only
+// if it equals an impossible value, so the loads cannot be dropped.
+__global__ void __launch_bounds__(256) stream_read(const float4* __restrict__ a, size_t n, float* sink) {
+  float s = 0.f;
+  size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  const size_t stride = (size_t)gridDim.x * 256;
+  for (; i + 3 * stride < n; i += 4 * stride) {
+    const float4 x0 = a[i], x1 = a[i + stride], x2 = a[i + 2 * stride], x3 = a[i + 3 * stride];
+    s += x0.x + x1.y + x2.z + x3.w;
+  }
+  for (; i < n; i += stride) s += a[i].x;
+  if (s == -1.2345e-30f) sink[0] = s;
+}
+}  // namespace
+
 extern "C" {
 
 int32_t edx_device_count(void) {
@@ -391,6 +408,34 @@ int edx_device_copy(edx_ctx* ctx, void* dst, const void* src, uint64_t bytes) {
     edx::DeviceGuard dg(ctx->device);
     EDX_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx->stream));
     EDX_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+int edx_time_stream_read(edx_ctx* ctx, const void* src, uint64_t bytes, int32_t n_rep, double* best_ms) {
+  return edx::guarded(ctx, [&] {
+    EDX_REQUIRE(ctx && src && best_ms && n_rep > 0 && bytes >= 16 && bytes % 16 == 0, EDX_ERR_INVALID,
+                "bad argument (bytes: a positive multiple of 16)");
+    edx::DeviceGuard dg(ctx->device);
+    edx::DevBuf<float> sink;
+    sink.alloc(1);
+    const size_t n = bytes / 16;
+    const unsigned grid = (unsigned)std::min<size_t>(65536, (n + 255) / 256);
+    hipEvent_t a, b;
+    EDX_HIP(hipEventCreate(&a));
+    EDX_HIP(hipEventCreate(&b));
+    float best = INFINITY;
+    for (int r = 0; r <= n_rep; ++r) {  // the first launch warms up
+      EDX_HIP(hipEventRecord(a, ctx->stream));
+      hipLaunchKernelGGL(stream_read, dim3(grid), dim3(256), 0, ctx->stream, (const float4*)src, n, sink.p);
+      EDX_HIP(hipEventRecord(b, ctx->stream));
+      EDX_HIP(hipEventSynchronize(b));
+      float ms = 0.f;
+      EDX_HIP(hipEventElapsedTime(&ms, a, b));
+      if (r > 0) best = std::min(best, ms);
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    *best_ms = best;
   });
 }
 

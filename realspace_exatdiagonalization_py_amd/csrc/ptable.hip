@@ -345,13 +345,19 @@ __device__ __forceinline__ void sell_slice(int64_t s, int lane, int64_t dim, con
     if (c >= 0) {
       const V v = vp[t * 64];
       const V* x = X + (int64_t)c * NB;
+      V xv[NB];
+#pragma unroll
+      for (int q = 0; q < NB; ++q) xv[q] = x[q];
+      // every gather of the entry is issued before its products (otherwise
+      // the compiler may hold back the last ones until the first arrive,
+      // which costs a second round trip: complex128 at 6 vectors)
+      __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
       for (int q = 0; q < NB; ++q) {
-        const V xv = x[q];
-        ar[q] = fma(v.x, xv.x, ar[q]);
-        ar[q] = fma(-v.y, xv.y, ar[q]);
-        ai[q] = fma(v.x, xv.y, ai[q]);
-        ai[q] = fma(v.y, xv.x, ai[q]);
+        ar[q] = fma(v.x, xv[q].x, ar[q]);
+        ar[q] = fma(-v.y, xv[q].y, ar[q]);
+        ai[q] = fma(v.x, xv[q].y, ai[q]);
+        ai[q] = fma(v.y, xv[q].x, ai[q]);
       }
     }
   }
