@@ -495,7 +495,7 @@ def hubbard_secondary(ed, _edx, workers=8, passes=2, eig_tol=1e-11):
 
     model = ed.build_spinful_hubbard_second_quantized_model([3, 4], 1.0, 4.0)
     G = build_translation_group(model.lattice)
-    _edx.set_autotune(True, ed.PREC_MIXED, HUBBARD_BLOCK)
+    _edx.set_autotune(True, ed.PREC_MIXED, HUBBARD_BLOCK or 6)  # 0: the library's width, min(6, 8)
     data = ed.build_ed_data(model, filling_fraction=Fraction(12, 24), symmetry_group=G)
     solve_all(ed, data, 6, ed.PREC_MIXED, eig_tol, workers, HUBBARD_BLOCK)
     t0 = time.perf_counter()
