@@ -224,8 +224,8 @@ def kernel_stats(ctx: "Context", enable: int = -1, reset: bool = False) -> dict:
     n = np.zeros(k, dtype=np.int64)
     b = np.zeros(k)
     f = np.zeros(k)
-    ctx.check(lib.edx_kernel_flops(ctx.handle, ptr(f)))
     ctx.check(lib.edx_kernel_stats(ctx.handle, enable, 1 if reset else 0, ptr(ms), ptr(n), ptr(b)))
+    ctx.check(lib.edx_kernel_flops(ctx.handle, ptr(f)))  # after a reset: the cleared totals too
     return {k: (float(ms[i]), int(n[i]), float(b[i]), float(f[i])) for i, k in enumerate(KERNEL_CLASSES)}
 
 
