@@ -49,6 +49,7 @@ sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
+FP32_MATRIX_PEAK_TFLOPS = 157.3
 METRIC = "sector solves/s + p99 latency at |ΔE|≤1e-10 vs FP64; H|ψ⟩ HBM GB/s"
 TRAFFIC_FILE = ROOT / "profiles" / "spmm_hubbard3x4_traffic.json"
 
@@ -336,18 +337,22 @@ def checkerboard_model(size):
 def by_class(ks: dict) -> dict:
     """Every timed class against the HBM peak by algorithmic bytes (SpMM:
     table, gathered and written vectors; Gram: the vectors read; update and
-    combine: the vectors read and written) over the event-timed duration, and
-    the Gram, update and combine also against the FP64 FMA peak by their
-    complex multiply-adds (8 flops each).  The restart combine is bound by
-    FP64 FMAs, the others by HBM."""
+    combinations: the vectors read and written) over the event-timed
+    duration, and the Gram, update and combinations also against an
+    arithmetic peak by their complex multiply-adds (8 flops each): FP64 FMAs,
+    or for the matrix-core combinations the FP32 matrix cores.  The
+    combinations are bound by that arithmetic, the others by HBM."""
     out = {}
     for k, (ms, n, nbytes, flops) in ks.items():
         if n <= 0 or ms <= 0 or nbytes <= 0:
             continue
         e = {"launches": n, "avg_launch_ms": round(ms / n, 5), "achieved": round(nbytes / ms / 1e6, 1),
              "frac": round(nbytes / ms / 1e6 / HBM_PEAK_GBS, 4),
-             "bound": "fp64" if k == "combine" else "hbm"}
-        if flops > 0:
+             "bound": {"combine": "fp64", "combine_mfma": "fp32_matrix"}.get(k, "hbm")}
+        if flops > 0 and k == "combine_mfma":
+            e["fp32_matrix_tflops"] = round(flops / ms / 1e9, 2)
+            e["fp32_matrix_frac"] = round(flops / ms / 1e9 / FP32_MATRIX_PEAK_TFLOPS, 4)
+        elif flops > 0:
             e["fp64_tflops"] = round(flops / ms / 1e9, 2)
             e["fp64_frac"] = round(flops / ms / 1e9 / FP64_PEAK_TFLOPS, 4)
         out[k] = e
@@ -650,7 +655,8 @@ def checkerboard_bench(args) -> int:
     cls = max(ks, key=lambda k: ks[k][0])
     lanes = next((e["lanes"] for e in _edx.autotune_table() if e["block"] == nb), None)
     names = {"spmm_c64": f"spmm_sell<float2,{nb},{lanes}>", "spmm_c128": f"spmm_sell<double2,{max(nb, nev)},{lanes}>",
-             "gram": "gram_tiles + gram_reduce", "gemm": "gemm_tiles", "combine": "gemm_tiles (restart combine)"}
+             "gram": "gram_tiles + gram_reduce", "gemm": "gemm_tiles", "combine": "gemm_tiles (restart combine)",
+             "combine_mfma": "combine_mfma32 (restart combine)"}
     k_ms, k_n, k_bytes = ks[cls][:3]
     avg_ms = k_ms / max(k_n, 1)
     achieved = k_bytes / k_ms / 1e6 if k_ms > 0 else 0.0

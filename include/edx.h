@@ -357,6 +357,26 @@ int edx_hamiltonian_autotune(edx_ctx* ctx, edx_hamiltonian* ham, int32_t precisi
  * No reference counterpart. */
 int edx_schedule_table(int32_t capacity, int32_t* keys, float* ms, int32_t* n);
 int edx_set_solver_schedule(edx_ctx* ctx, int32_t gram, int32_t update);
+/* Combination of device vectors, the restart step of the eigensolver that
+ * replaces diagonalize_block_matrixfree (ref SRE:2248-2299): out[:, k] =
+ * sum_j v[:, j] y[j][k] for j < nv, k < nk (1..240 each), v and out
+ * column-major with leading dimension ld > dim (complex128 for
+ * EDX_PREC_FP64, complex64 for EDX_PREC_MIXED; device addresses; out must
+ * not overlap v), y host row-major complex128 (re, im pairs, nv x nk).
+ * schedule 1 sums in FP64 on vector FMAs (the result is the exact sum
+ * rounded once); 2 runs on the FP32 matrix cores (complex64 only; needs
+ * ld >= dim rounded up to 32, ld even, 16-byte aligned buffers, and
+ * nv <= 112 for nk > 32, nv <= 204 otherwise): coefficients rounded to
+ * FP32, FP32 sums of 8
+ * terms added in FP64, so each output errs by a few FP32 roundings of its
+ * terms; 0 picks 2 for complex64 combinations of more than 16 outputs from
+ * 65536 rows when it applies, else 1.  Synchronous; timed by
+ * edx_kernel_stats as class 4 (vector FMAs) or 5 (matrix cores).  edx_set_combine_schedule fixes the schedule of the
+ * eigensolver's combinations of more than 16 outputs on a context (0 =
+ * automatic).  No reference counterpart as entry points. */
+int edx_combine_vectors(edx_ctx* ctx, int32_t precision, const void* v_dev, int64_t ld, int64_t dim, int32_t nv,
+                        const double* y, int32_t nk, void* out_dev, int32_t schedule);
+int edx_set_combine_schedule(edx_ctx* ctx, int32_t schedule);
 
 /* ---- host-only helper (no device needed) ------------------------------------
  * Eigen-decomposition of a Hermitian n x n matrix (row-major (re, im) pairs)
@@ -366,20 +386,21 @@ int edx_host_herm_eig(int32_t n, const double* a, int32_t upper, double* w, doub
 
 /* ---- live kernel timing -------------------------------------------------------
  * No reference counterpart.  With timing enabled every complex128 SpMM (class
- * 0), complex64 SpMM (1), Gram (2), block update (3) and restart combination
- * (4) launch on the ctx stream is bracketed by HIP events; ms/launches/bytes (arrays of
+ * 0), complex64 SpMM (1), Gram (2), block update (3) and combination of more
+ * than 16 outputs (restarts, Ritz vectors, edx_combine_vectors) on FP64 vector
+ * FMAs (4) or on the FP32 matrix cores (5) launch on the ctx stream is
+ * bracketed by HIP events; ms/launches/bytes (arrays of
  * EDX_KERNEL_CLASSES, NULL skips) receive the totals (bytes = algorithmic
  * bytes of each launch).  enable: 1 on, 0 off, -1 keep; reset != 0 clears the
  * totals first. */
-#define EDX_KERNEL_CLASSES 5
+#define EDX_KERNEL_CLASSES 6
 int edx_kernel_stats(edx_ctx* ctx, int32_t enable, int32_t reset, double* ms, int64_t* launches, double* bytes);
 
 /* Arithmetic of the same timed launches (flops[EDX_KERNEL_CLASSES], totals
  * since the last reset): 8 flops per complex multiply-add of the Gram,
- * update and combine classes (FP64 FMAs, or FP32 matrix-core products for
- * the complex64 matrix-core Gram), 0 for the SpMM classes.  Class 4 is the
- * restart combinations (updates of more than 16 outputs).  No reference
- * counterpart. */
+ * update and combination classes (FP64 FMAs, or FP32 matrix-core products
+ * for the complex64 matrix-core Gram and class 5), 0 for the SpMM classes.
+ * No reference counterpart. */
 int edx_kernel_flops(edx_ctx* ctx, double* flops);
 
 /* ---- device helpers for benchmarks and tests ------------------------------ */

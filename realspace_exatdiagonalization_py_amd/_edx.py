@@ -81,6 +81,8 @@ SIGNATURES = {
     "edx_graph_launch": (C.c_int, [P, P]),
     "edx_graph_destroy": (C.c_int, [P]),
     "edx_set_solver_schedule": (C.c_int, [P, i32, i32]),
+    "edx_combine_vectors": (C.c_int, [P, i32, P, C.c_int64, C.c_int64, i32, P, i32, P, i32]),
+    "edx_set_combine_schedule": (C.c_int, [P, i32]),
     "edx_hamiltonian_autotune": (C.c_int, [P, P, i32, i32, pi32, P]),
     "edx_host_herm_eig": (C.c_int, [i32, P, i32, P, P]),
     "edx_device_alloc": (C.c_int, [P, u64, PP]),
@@ -213,7 +215,7 @@ def current_context(default: "Context | None" = None) -> "Context":
     return default if default is not None else Context.default()
 
 
-KERNEL_CLASSES = ("spmm_c128", "spmm_c64", "gram", "gemm", "combine")
+KERNEL_CLASSES = ("spmm_c128", "spmm_c64", "gram", "gemm", "combine", "combine_mfma")
 
 
 def kernel_stats(ctx: "Context", enable: int = -1, reset: bool = False) -> dict:
@@ -293,6 +295,29 @@ def schedule_table() -> list[dict]:
 def set_solver_schedule(ctx: "Context", gram: int = 0, update: int = 0) -> None:
     """Fixes the eigensolver schedule of solves on ``ctx`` (0: tuned or default)."""
     ctx.check(lib.edx_set_solver_schedule(ctx.handle, int(gram), int(update)))
+
+
+def set_combine_schedule(ctx: "Context", schedule: int = 0) -> None:
+    """Fixes how solves on ``ctx`` form combinations of more than 16 vectors
+    (restarts, Ritz vectors): 1 FP64 vector FMAs, 2 FP32 matrix cores
+    (complex64), 0 automatic."""
+    ctx.check(lib.edx_set_combine_schedule(ctx.handle, int(schedule)))
+
+
+def combine_vectors(ctx: "Context", precision: int, v: "DeviceArray", ld: int, dim: int, y, out: "DeviceArray",
+                    schedule: int = 0) -> None:
+    """out[:, k] = sum_j v[:, j] y[j, k] on device buffers (column-major,
+    leading dimension ``ld``; complex128 for PREC_FP64, complex64 for
+    PREC_MIXED); ``y`` is an (nv, nk) complex array on the host."""
+    y = np.ascontiguousarray(y, dtype=np.complex128)
+    if y.ndim != 2:
+        raise ValueError("y must have shape (nv, nk)")
+    nv, nk = y.shape
+    esz = 16 if precision == PREC_FP64 else 8
+    if nv * ld * esz > v.nbytes or nk * ld * esz > out.nbytes:
+        raise ValueError("buffer smaller than its vectors")
+    ctx.check(lib.edx_combine_vectors(ctx.handle, int(precision), v.ptr, int(ld), int(dim), int(nv),
+                                      y.ctypes.data_as(C.c_void_p), int(nk), out.ptr, int(schedule)))
 
 
 def mem_info(ctx: "Context") -> tuple[int, int]:

@@ -121,9 +121,19 @@ using ObjBuf = DevBuf<T, false>;
 
 // Live per-class kernel timing with HIP events on the launch stream
 // (enabled on request; used by benchmarks to time kernels inside a solve).
-// K_GEMM: block updates (at most 16 outputs); K_COMBINE: restart
-// combinations (more outputs, bound by FP64 FMAs rather than HBM).
-enum KernelClass { K_SPMM = 0, K_SPMM32 = 1, K_GRAM = 2, K_GEMM = 3, K_COMBINE = 4, K_NCLASS = EDX_KERNEL_CLASSES };
+// K_GEMM: block updates (at most 16 outputs); K_COMBINE: combinations of
+// more outputs (restarts, Ritz vectors) on FP64 vector FMAs, which bound
+// them rather than HBM; K_COMBINE_MFMA: the same on the FP32 matrix cores
+// (complex64).
+enum KernelClass {
+  K_SPMM = 0,
+  K_SPMM32 = 1,
+  K_GRAM = 2,
+  K_GEMM = 3,
+  K_COMBINE = 4,
+  K_COMBINE_MFMA = 5,
+  K_NCLASS = EDX_KERNEL_CLASSES
+};
 
 struct KernelTimer {
   bool on = false;
@@ -206,9 +216,12 @@ struct Ctx {
   int64_t graph_captures = 0, graph_replays = 0;
   // orthogonalisation schedule of the eigensolver (0 = tuned or default):
   // Gram 1 = vector FMAs, 2 = FP32 matrix cores (complex64, blocks <= 8);
-  // block update 1..3 = stage/rows variants (kUpdateVariants)
+  // block update 1..3 = stage/rows variants (kUpdateVariants); combination
+  // of more than 16 outputs 1 = vector FMAs, 2 = FP32 matrix cores
+  // (complex64)
   int sched_gram = 0;
   int sched_update = 0;
+  int sched_combine = 0;
   KernelTimer timer;
   // pinned host staging owned by the context (allocated once, reused by
   // every solve on it): projection coefficients device -> host, and the

@@ -538,6 +538,251 @@ gram_mfma32(int64_t D, VecPair<float2> A, const float2* __restrict__ Wb, int64_t
   }
 }
 
+// Combination of complex64 vectors on the FP32 matrix cores (restart
+// combinations): Out_k = sum_j A_j S[j][k] for the NT * 8 outputs k of
+// column group blockIdx.y (S row-major, FP64).  Operand layout as for the
+// Gram above; K runs over the vectors of A, four per step (lane group g takes
+// vector j + g).  In real form Re Out_k = sum Ar Sr - Ai Si and
+// Im Out_k = sum Ar Si + Ai Sr: a 16 x 16 result tile holds 8 outputs
+// (columns n < 8 their real parts, n >= 8 their imaginary parts) and takes
+// two products per step, Ar times B1 = [Sr | Si] and Ai times
+// B2 = [-Si | Sr].  A wave works on strips of 32 rows: lane (g, c) loads rows
+// 2c and 2c + 1 of its vector with one 16-byte load, which feed two row tiles
+// (the even and the odd rows of the strip); loads run two steps ahead.  The
+// FP32 tiles (8 products per entry and step) are added into FP64 sums every
+// kCombineFlush steps, so an output errs by a few FP32 roundings of its
+// terms before its one rounding to complex64 (every step: 48-site restarts,
+// 104 -> 57 vectors, took 10.8 ms against 9.3 ms every second step and 15.6
+// ms on FP64 vector FMAs, and their FP64 refinement needed the matvecs it
+// needs after vector-FMA restarts on 2 of 3 sectors, against 0 of 3 every
+// second step).  The group's coefficients are staged in LDS as complex64 (rows
+// padded so that the four lane groups read distinct banks), zero past nv and
+// nk; vector indices past nv load vector nv - 1, which the zero coefficients
+// cancel.  Strips never read past row ld - 1 (the launch requires ld >= D
+// rounded up to 32); rows past D are not written.  Lane (g, c) ends with the
+// real (c < 8) or imaginary part of output c % 8 of each tile at rows
+// 8g .. 8g + 7 of the strip; it swaps half of them with lane c ^ 8 and writes
+// four rows of that output as two 16-byte stores.
+constexpr int kCombineFlush = 1;
+constexpr int kCombineWorkgroups = 512;  // two per CU
+// Row stride (complex64 entries) of the coefficient staging: 2 * ldc = 16
+// (mod 64) dwords, so that rows j .. j + 3 start 16 LDS banks apart.
+constexpr int combine_ldc(int nt) { return (nt * 8 + 31) / 32 * 32 + 8; }
+
+template <int NT>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+combine_mfma32(int64_t D, VecPair<float2> A, int nk, float2* __restrict__ Ob, int64_t ldo,
+               const double2* __restrict__ S, int ldS, int64_t n_strips, StepPred pred) {
+  constexpr int NC = NT * 8;              // outputs of a column group
+  constexpr int LDC = combine_ldc(NT);    // LDS row stride in complex64 entries
+  extern __shared__ float2 s_coef[];  // [nvp][LDC]
+  if (!pred.run()) return;
+  const int nv = A.n0 + A.n1;
+  const int steps = (nv + 3) / 4;
+  const int k0 = (int)blockIdx.y * NC;
+  for (int e = threadIdx.x; e < steps * 4 * NC; e += 256) {
+    const int i = e / NC, k = e % NC;
+    float2 v = make_float2(0.f, 0.f);
+    if (i < nv && k0 + k < nk) {
+      const double2 s = S[(int64_t)i * ldS + k0 + k];
+      v = make_float2((float)s.x, (float)s.y);
+    }
+    s_coef[i * LDC + k] = v;
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int c = lane & 15, g = lane >> 4;
+  const bool re_lane = c < 8;
+  const float2* crow = s_coef + g * LDC + (c & 7);
+  for (int64_t strip = (int64_t)blockIdx.x * 4 + wave; strip < n_strips; strip += (int64_t)gridDim.x * 4) {
+    const int64_t row0 = strip * 32;
+    const int64_t lr = row0 + 2 * c;
+    f32x4_t acc[2][NT];
+    double sum[2][NT][4];
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        acc[h][t] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int v = 0; v < 4; ++v) sum[h][t][v] = 0.0;
+      }
+    auto load = [&](int s) -> float4 {
+      return *reinterpret_cast<const float4*>(A.vec(min(4 * s + g, nv - 1)) + lr);
+    };
+    auto step = [&](const float4& a, int s) {
+      const float2* cp = crow + 4 * s * LDC;
+      float b1[NT], b2[NT];
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        const float2 sc = cp[t * 8];
+        b1[t] = re_lane ? sc.x : sc.y;
+        b2[t] = re_lane ? -sc.y : sc.x;
+      }
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        acc[0][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.x, b1[t], acc[0][t], 0, 0, 0);
+        acc[1][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.z, b1[t], acc[1][t], 0, 0, 0);
+      }
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        acc[0][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.y, b2[t], acc[0][t], 0, 0, 0);
+        acc[1][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.w, b2[t], acc[1][t], 0, 0, 0);
+      }
+      if ((s + 1) % kCombineFlush == 0 || s + 1 == steps) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+          for (int t = 0; t < NT; ++t) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) sum[h][t][v] += (double)acc[h][t][v];
+            acc[h][t] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+          }
+      }
+    };
+    float4 a0 = load(0), a1 = load(1), a2;
+    for (int s = 0; s < steps; s += 3) {
+      a2 = load(s + 2);
+      EDX_KEEP_PREFETCH();
+      step(a0, s);
+      a0 = load(s + 3);
+      EDX_KEEP_PREFETCH();
+      if (s + 1 < steps) step(a1, s + 1);
+      a1 = load(s + 4);
+      EDX_KEEP_PREFETCH();
+      if (s + 2 < steps) step(a2, s + 2);
+    }
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      float x[4][2], y[2][2];
+#pragma unroll
+      for (int v = 0; v < 4; ++v)
+#pragma unroll
+        for (int h = 0; h < 2; ++h) x[v][h] = (float)sum[h][t][v];
+      // the real lane keeps rows 8g .. 8g + 3 (v = 0, 1), the imaginary lane
+      // rows 8g + 4 .. 8g + 7 (v = 2, 3)
+#pragma unroll
+      for (int w = 0; w < 2; ++w)
+#pragma unroll
+        for (int h = 0; h < 2; ++h) y[w][h] = __shfl_xor(re_lane ? x[2 + w][h] : x[w][h], 8, 64);
+      const int k = k0 + t * 8 + (c & 7);
+      if (k >= nk) continue;
+#pragma unroll
+      for (int w = 0; w < 2; ++w) {
+        const int64_t row = row0 + 8 * g + 2 * (re_lane ? w : 2 + w);
+        const float re0 = re_lane ? x[w][0] : y[w][0], im0 = re_lane ? y[w][0] : x[2 + w][0];
+        const float re1 = re_lane ? x[w][1] : y[w][1], im1 = re_lane ? y[w][1] : x[2 + w][1];
+        float2* op = Ob + (int64_t)k * ldo + row;
+        if (row + 1 < D) *reinterpret_cast<float4*>(op) = make_float4(re0, im0, re1, im1);
+        else if (row < D) *op = make_float2(re0, im0);
+      }
+    }
+  }
+}
+
+// Launch of gemm_tiles: Out[:, 0..nk) = [A0 | A1] S over vectors of D rows
+// (leading dimension ld), output tiles of NK (see the kernel for Xi and
+// aliasing).
+template <typename VT, int NK, int U = kGemmStage, int RPT = 1>
+void launch_gemm_tiles(hipStream_t st, int64_t D, int64_t ld, const void* A0, int n0, const void* A1, int n1,
+                       void* Out, int nk, const double2* S, int ldS, void* Xi, StepPred pred) {
+  VecPair<VT> A{(const VT*)A0, ld, n0, (const VT*)A1, ld, n1};
+  const int nv = n0 + n1;
+  const int n_kt = (nk + NK - 1) / NK;
+  const int64_t n_tiles = (int64_t)n_kt * ((D + 256 * RPT - 1) / (256 * RPT));
+  EDX_REQUIRE(n_tiles < (1ll << 30), EDX_ERR_INTERNAL, "update grid");
+  const int nvp = (nv + 2 * U - 1) / (2 * U) * (2 * U);
+  // LDS: the coefficients, and (with an interleaved copy, room permitting)
+  // the staged outputs
+  size_t lds = (size_t)nvp * NK * sizeof(double2);
+  const size_t stage = (size_t)256 * RPT * nk * sizeof(VT);
+  const int stage_xi = Xi && stage <= kStageLdsBytes ? 1 : 0;
+  if (stage_xi) lds = std::max(lds, stage);
+  hipLaunchKernelGGL((gemm_tiles<VT, NK, U, RPT>), dim3((unsigned)((n_tiles + 7) / 8 * 8)), dim3(256), lds, st, D, A,
+                     nk, (VT*)Out, ld, S, ldS, (VT*)Xi, n_kt, (int)n_tiles, stage_xi, pred);
+}
+
+// Combination schedules (more than 16 outputs): FP64 vector FMAs in tiles
+// of 9 to 12 outputs, or (complex64) the FP32 matrix cores.
+constexpr int kCombineVector = 1, kCombineMatrix = 2;
+// Largest coefficient staging of a matrix-core combination (bytes).
+constexpr size_t kCombineLdsBytes = 64 * 1024;
+// Automatic schedule: complex64 combinations of vectors of at least this
+// many rows run on the matrix cores.
+constexpr int64_t kMfmaCombineRows = int64_t(1) << 16;
+
+int combine_tiles(int nk) { return std::min(8, (nk + 7) / 8); }
+
+// Whether combine_mfma32 takes nv inputs and nk outputs of D rows with
+// leading dimension ld (coefficients within its LDS, whole 32-row strips
+// inside every vector, 16-byte aligned rows).
+bool combine_mfma_fits(int nv, int nk, int64_t D, int64_t ld) {
+  const size_t lds = (size_t)((nv + 3) / 4 * 4) * combine_ldc(combine_tiles(nk)) * sizeof(float2);
+  return lds <= kCombineLdsBytes && ld >= (D + 31) / 32 * 32 && ld % 2 == 0;
+}
+
+template <int NT>
+void launch_combine_nt(hipStream_t st, int64_t D, const VecPair<float2>& A, int nk, float2* Out, int64_t ldo,
+                       const double2* S, int ldS, StepPred pd) {
+  const int nv = A.n0 + A.n1;
+  const size_t lds = (size_t)((nv + 3) / 4 * 4) * combine_ldc(NT) * sizeof(float2);
+  const int64_t n_strips = (D + 31) / 32;
+  const int n_groups = (nk + NT * 8 - 1) / (NT * 8);
+  // each workgroup stages the coefficients once for at least 8 strips
+  const int64_t wgs = std::min<int64_t>(kCombineWorkgroups, std::max<int64_t>(1, n_strips / 8));
+  hipLaunchKernelGGL((combine_mfma32<NT>), dim3((unsigned)wgs, (unsigned)n_groups), dim3(256), lds, st, D, A, nk, Out,
+                     ldo, S, ldS, n_strips, pd);
+}
+
+void launch_combine_mfma(hipStream_t st, int64_t D, const VecPair<float2>& A, int nk, float2* Out, int64_t ldo,
+                         const double2* S, int ldS, StepPred pd) {
+  switch (combine_tiles(nk)) {
+    case 1: launch_combine_nt<1>(st, D, A, nk, Out, ldo, S, ldS, pd); break;
+    case 2: launch_combine_nt<2>(st, D, A, nk, Out, ldo, S, ldS, pd); break;
+    case 3: launch_combine_nt<3>(st, D, A, nk, Out, ldo, S, ldS, pd); break;
+    case 4: launch_combine_nt<4>(st, D, A, nk, Out, ldo, S, ldS, pd); break;
+    case 5: launch_combine_nt<5>(st, D, A, nk, Out, ldo, S, ldS, pd); break;
+    case 6: launch_combine_nt<6>(st, D, A, nk, Out, ldo, S, ldS, pd); break;
+    case 7: launch_combine_nt<7>(st, D, A, nk, Out, ldo, S, ldS, pd); break;
+    default: launch_combine_nt<8>(st, D, A, nk, Out, ldo, S, ldS, pd);
+  }
+}
+
+// Out[:, 0..nk) = [A0 | A1] S with the given schedule (0: automatic).  Out
+// must not overlap the inputs.  A matrix-core schedule that does not apply
+// to the shape is an error when `strict`, else the vector schedule runs.
+// Returns the kernel-timer class of the launch.
+template <typename VT>
+int launch_combine(hipStream_t st, int64_t D, int64_t ld, const void* A0, int n0, const void* A1, int n1, void* Out,
+                    int nk, const double2* S, int ldS, int schedule, bool strict, StepPred pd) {
+  if (std::is_same<VT, float2>::value && schedule != kCombineVector) {
+    auto aligned = [](const void* p) { return ((uintptr_t)p & 15) == 0; };
+    const bool fits = combine_mfma_fits(n0 + n1, nk, D, ld) && aligned(A0) && aligned(A1) && aligned(Out);
+    EDX_REQUIRE(fits || !strict || schedule != kCombineMatrix, EDX_ERR_INVALID,
+                "the matrix-core combination needs ld >= dim rounded up to 32, an even ld, 16-byte aligned "
+                "buffers and a smaller basis");
+    if (fits && (schedule == kCombineMatrix || D >= kMfmaCombineRows)) {
+      VecPair<float2> A{(const float2*)A0, ld, n0, (const float2*)A1, ld, n1};
+      launch_combine_mfma(st, D, A, nk, (float2*)Out, ld, S, ldS, pd);
+      return edx::K_COMBINE_MFMA;
+    }
+  }
+  // as many tiles as with kCombineTile outputs each, but of equal width
+  // (9 to 12 outputs), so the last one carries no padding: the combination
+  // is bound by FP64 FMAs (60 -> 33: 84.8 us in tiles of 11 against 91.4 us
+  // in tiles of 12, complex64, D = 225k)
+  constexpr int R = kUpdateRows;
+  const int n_kt = (nk + kCombineTile - 1) / kCombineTile;
+  switch ((nk + n_kt - 1) / n_kt) {
+    case 9: launch_gemm_tiles<VT, 9, kCombineStage, R>(st, D, ld, A0, n0, A1, n1, Out, nk, S, ldS, nullptr, pd); break;
+    case 10: launch_gemm_tiles<VT, 10, kCombineStage, R>(st, D, ld, A0, n0, A1, n1, Out, nk, S, ldS, nullptr, pd); break;
+    case 11: launch_gemm_tiles<VT, 11, kCombineStage, R>(st, D, ld, A0, n0, A1, n1, Out, nk, S, ldS, nullptr, pd); break;
+    default:
+      launch_gemm_tiles<VT, kCombineTile, kCombineStage, R>(st, D, ld, A0, n0, A1, n1, Out, nk, S, ldS, nullptr, pd);
+  }
+  return edx::K_COMBINE;
+}
+
 // Second Cholesky QR pass of a block, on the device: G (b x b, row-major,
 // Hermitian) = R^H R with R upper triangular, S = R^-1 (row-major, the
 // coefficients of the update dst <- dst S) and R copied to Rout (host-visible)
@@ -1312,20 +1557,7 @@ class BlockLanczos {
   template <typename VT, int NK, int U = kGemmStage, int RPT = 1>
   void launch_gemm_t(const void* A0, int n0, const void* A1, int n1, void* Out, int nk, const double2* S, int ldS,
                      void* Xi, StepPred pred) {
-    VecPair<VT> A{(const VT*)A0, ld_, n0, (const VT*)A1, ld_, n1};
-    const int nv = n0 + n1;
-    const int n_kt = (nk + NK - 1) / NK;
-    const int64_t n_tiles = (int64_t)n_kt * ((D_ + 256 * RPT - 1) / (256 * RPT));
-    EDX_REQUIRE(n_tiles < (1ll << 30), EDX_ERR_INTERNAL, "update grid");
-    const int nvp = (nv + 2 * U - 1) / (2 * U) * (2 * U);
-    // LDS: the coefficients, and (with an interleaved copy, room permitting)
-    // the staged outputs
-    size_t lds = (size_t)nvp * NK * sizeof(double2);
-    const size_t stage = (size_t)256 * RPT * nk * sizeof(VT);
-    const int stage_xi = Xi && stage <= kStageLdsBytes ? 1 : 0;
-    if (stage_xi) lds = std::max(lds, stage);
-    hipLaunchKernelGGL((gemm_tiles<VT, NK, U, RPT>), dim3((unsigned)((n_tiles + 7) / 8 * 8)), dim3(256), lds,
-                       ctx_.stream, D_, A, nk, (VT*)Out, ld_, S, ldS, (VT*)Xi, n_kt, (int)n_tiles, stage_xi, pred);
+    launch_gemm_tiles<VT, NK, U, RPT>(ctx_.stream, D_, ld_, A0, n0, A1, n1, Out, nk, S, ldS, Xi, pred);
   }
 
   // A block update of at most 8 outputs with the schedule of this solve.
@@ -1429,32 +1661,22 @@ class BlockLanczos {
     if (!upd_fixed) upd_var_ = e.update - 1;
   }
 
-  // One tile of the smallest width that holds nk (nk <= 16), otherwise tiles
-  // of up to 12 outputs (restart combinations: 97 us against 129 us with
-  // tiles of 6 at 60 -> 33 vectors, complex64, D = 225k).
+  // One tile of the smallest width that holds nk (nk <= 16), otherwise the
+  // context's combination schedule (vector tiles of up to 12 outputs: 97 us
+  // against 129 us with tiles of 6 at 60 -> 33 vectors, complex64, D = 225k).
+  // Returns the kernel-timer class of the launch.
   template <typename VT>
-  void launch_gemm_v(const void* A0, int n0, const void* A1, int n1, void* Out, int nk, const double2* S, int ldS,
+  int launch_gemm_v(const void* A0, int n0, const void* A1, int n1, void* Out, int nk, const double2* S, int ldS,
                      void* Xi, StepPred pd) {
-    constexpr int R = kUpdateRows;
+    if (nk > 16)
+      return launch_combine<VT>(ctx_.stream, D_, ld_, A0, n0, A1, n1, Out, nk, S, ldS, ctx_.sched_combine, false, pd);
     if (nk <= 2) launch_update<VT, 2>(A0, n0, A1, n1, Out, nk, S, ldS, Xi, pd);
     else if (nk <= 4) launch_update<VT, 4>(A0, n0, A1, n1, Out, nk, S, ldS, Xi, pd);
     else if (nk <= 6) launch_update<VT, 6>(A0, n0, A1, n1, Out, nk, S, ldS, Xi, pd);
     else if (nk <= 8) launch_update<VT, 8>(A0, n0, A1, n1, Out, nk, S, ldS, Xi, pd);
     else if (nk <= 12) launch_gemm_t<VT, 12>(A0, n0, A1, n1, Out, nk, S, ldS, Xi, pd);
-    else if (nk <= 16) launch_gemm_t<VT, 16>(A0, n0, A1, n1, Out, nk, S, ldS, Xi, pd);
-    else {
-      // as many tiles as with kCombineTile outputs each, but of equal width
-      // (9 to 12 outputs), so the last one carries no padding: the
-      // combination is bound by FP64 FMAs (60 -> 33: 84.8 us in tiles of 11
-      // against 91.4 us in tiles of 12, complex64, D = 225k)
-      const int n_kt = (nk + kCombineTile - 1) / kCombineTile;
-      switch ((nk + n_kt - 1) / n_kt) {
-        case 9: launch_gemm_t<VT, 9, kCombineStage, R>(A0, n0, A1, n1, Out, nk, S, ldS, Xi, pd); break;
-        case 10: launch_gemm_t<VT, 10, kCombineStage, R>(A0, n0, A1, n1, Out, nk, S, ldS, Xi, pd); break;
-        case 11: launch_gemm_t<VT, 11, kCombineStage, R>(A0, n0, A1, n1, Out, nk, S, ldS, Xi, pd); break;
-        default: launch_gemm_t<VT, kCombineTile, kCombineStage, R>(A0, n0, A1, n1, Out, nk, S, ldS, Xi, pd);
-      }
-    }
+    else launch_gemm_t<VT, 16>(A0, n0, A1, n1, Out, nk, S, ldS, Xi, pd);
+    return edx::K_GEMM;
   }
 
   // True when the nk output vectors at Out overlap the n vectors at A.
@@ -1475,10 +1697,10 @@ class BlockLanczos {
     EDX_REQUIRE(nk <= 16 || (!Xi && !overlaps(A0, n0, Out, nk) && !overlaps(A1, n1, Out, nk)), EDX_ERR_INTERNAL,
                 "multi-tile update aliases its inputs");
     hipEvent_t t0 = ctx_.timer.start(ctx_.stream);
-    if (prec_ == EDX_PREC_FP64) launch_gemm_v<double2>(A0, n0, A1, n1, Out, nk, S, ldS, Xi, pred);
-    else launch_gemm_v<float2>(A0, n0, A1, n1, Out, nk, S, ldS, Xi, pred);
+    const int cls = prec_ == EDX_PREC_FP64 ? launch_gemm_v<double2>(A0, n0, A1, n1, Out, nk, S, ldS, Xi, pred)
+                                           : launch_gemm_v<float2>(A0, n0, A1, n1, Out, nk, S, ldS, Xi, pred);
     EDX_HIP(hipGetLastError());
-    ctx_.timer.stop(ctx_.stream, t0, nk > 16 ? edx::K_COMBINE : edx::K_GEMM,
+    ctx_.timer.stop(ctx_.stream, t0, cls,
                     (double)(n0 + n1 + (Xi ? 2 : 1) * nk) * D_ * esz_, 8.0 * (n0 + n1) * nk * D_);
   }
 
@@ -1882,6 +2104,47 @@ int edx_eigs_batch(edx_ctx* ctx, int32_t n, edx_ctx* const* ctxs, const edx_hami
       if (!claims.back()->ok) throw edx::Error{EDX_ERR_BUSY, "a solve's context is in use by another thread"};
     }
     eigs_batch_impl(ctx, n, ctxs, hams, *o, vals, vecs, info);
+  });
+}
+
+int edx_combine_vectors(edx_ctx* ctx, int32_t precision, const void* v_dev, int64_t ld, int64_t dim, int32_t nv,
+                        const double* y, int32_t nk, void* out_dev, int32_t schedule) {
+  return edx::guarded(ctx, [&] {
+    EDX_REQUIRE(ctx && v_dev && y && out_dev, EDX_ERR_INVALID, "null argument");
+    EDX_REQUIRE(precision == EDX_PREC_FP64 || precision == EDX_PREC_MIXED, EDX_ERR_INVALID, "bad precision");
+    EDX_REQUIRE(schedule >= 0 && schedule <= 2, EDX_ERR_INVALID, "combination schedule must be 0 (automatic), 1 or 2");
+    // the vector schedule's lanes past dim read (and discard) row dim
+    EDX_REQUIRE(dim >= 1 && ld > dim, EDX_ERR_INVALID, "need dim >= 1 and ld > dim");
+    EDX_REQUIRE(nv >= 1 && nv <= kMaxVec + kMaxW && nk >= 1 && nk <= kMaxVec + kMaxW, EDX_ERR_INVALID,
+                "nv and nk must be 1..240");
+    EDX_REQUIRE(precision == EDX_PREC_MIXED || schedule != kCombineMatrix, EDX_ERR_INVALID,
+                "the matrix-core combination takes complex64 vectors");
+    const size_t esz = precision == EDX_PREC_FP64 ? sizeof(double2) : sizeof(float2);
+    const char* vb = static_cast<const char*>(v_dev);
+    const char* ob = static_cast<const char*>(out_dev);
+    EDX_REQUIRE(ob + (size_t)nk * ld * esz <= vb || vb + (size_t)nv * ld * esz <= ob, EDX_ERR_INVALID,
+                "the output overlaps the input vectors");
+    edx::DeviceGuard dg(ctx->device);
+    DevBuf<double2> S((size_t)nv * nk);
+    EDX_HIP(hipMemcpyAsync(S.p, y, (size_t)nv * nk * sizeof(double2), hipMemcpyHostToDevice, ctx->stream));
+    hipEvent_t t0 = ctx->timer.start(ctx->stream);
+    const int cls =
+        precision == EDX_PREC_FP64
+            ? launch_combine<double2>(ctx->stream, dim, ld, v_dev, nv, nullptr, 0, out_dev, nk, S.p, nk, schedule,
+                                      true, StepPred{})
+            : launch_combine<float2>(ctx->stream, dim, ld, v_dev, nv, nullptr, 0, out_dev, nk, S.p, nk, schedule,
+                                     true, StepPred{});
+    EDX_HIP(hipGetLastError());
+    ctx->timer.stop(ctx->stream, t0, cls, (double)(nv + nk) * dim * esz, 8.0 * nv * nk * dim);
+    EDX_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+int edx_set_combine_schedule(edx_ctx* ctx, int32_t schedule) {
+  return edx::guarded(ctx, [&] {
+    EDX_REQUIRE(ctx, EDX_ERR_INVALID, "null argument");
+    EDX_REQUIRE(schedule >= 0 && schedule <= 2, EDX_ERR_INVALID, "combination schedule must be 0 (automatic), 1 or 2");
+    ctx->sched_combine = schedule;
   });
 }
 

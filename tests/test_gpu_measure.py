@@ -2,7 +2,7 @@
 
 * ``edx_kernel_stats`` / ``edx_kernel_flops``: after a timed solve every
   eigensolver class has launches, time, algorithmic bytes and (Gram, block
-  update, restart combine) FP64 flops, with flops >= 8 D per launch; a
+  update, combinations) flops, with flops >= 8 D per launch; a
   reset clears them; with the timer off nothing is counted.
 * ``edx_time_stream_read``: the streaming-read reference returns a rate
   between 1 TB/s and the 8 TB/s peak and refuses sizes that are not whole
@@ -54,9 +54,10 @@ def test_kernel_timer_classes_and_flops():
             # per row: flops >= 8 D per launch, and per byte moved at most
             # 8 x (vectors per launch) ~ a few hundred
             assert flops >= 8 * H.n * n and flops <= 1000 * nbytes, (cls, ks[cls])
-    ms, n, nbytes, flops = ks["combine"]
-    if n:  # restarts of this solve went through the combine class
-        assert flops > 0 and nbytes > 0
+    for cls in ("combine", "combine_mfma"):
+        ms, n, nbytes, flops = ks[cls]
+        if n:  # restarts of this solve went through this combination class
+            assert flops > 0 and nbytes > 0
     assert _edx.kernel_stats(ctx) == ks  # off: unchanged by reading
     cleared = _edx.kernel_stats(ctx, reset=True)  # a reset clears before reading
     assert all(v == (0.0, 0, 0.0, 0.0) for v in cleared.values()), cleared
