@@ -156,8 +156,9 @@ def test_solve_same_levels_under_both_combine_schedules():
             ms, n, _, _ = ks["combine" if sched == 1 else "combine_mfma"]
             print(f"[combine solve] D={H.n} schedule {sched}: {n} combinations, {ms:.2f} ms, matvecs "
                   f"{H.last_info[0]:.0f}", flush=True)
-            assert n > 0  # the restarts went through the combination of that schedule
-            assert ks["combine_mfma" if sched == 1 else "combine"][1] == 0
+            assert n > 0  # the complex64 restarts went through the combination of that schedule
+            if sched == 1:
+                assert ks["combine_mfma"][1] == 0  # (complex128 ones always run on vector FMAs)
             levels[sched] = np.asarray(w)
     finally:
         _edx.set_combine_schedule(ctx, 0)
