@@ -196,7 +196,8 @@ def time_spmm(_edx, H, precision, nb, n_rep=20):
     X.upload(x if precision == 0 else x.astype(np.float32))
     ms = C.c_double()
     ctx.check(_edx.lib.edx_time_matvec(ctx.handle, dh.handle, precision, nb, n_rep, X.ptr, Y.ptr, C.byref(ms)))
-    nbytes = dh.dim * (s + 2.0 * nb * s) + dh.n_entries * (4.0 + s + nb * s)
+    elem = 0.0 if dh.layout()["packed"] else s  # a packed entry is one 4-byte word
+    nbytes = dh.dim * (s + 2.0 * nb * s) + dh.n_entries * (4.0 + elem + nb * s)
     return ms.value, nbytes
 
 
@@ -247,6 +248,7 @@ def profile_solve(args) -> int:
     w, _ = ed.diagonalize_block_matrixfree(H, H.n, nev=10, precision=ed.PREC_MIXED, eig_tol=1e-11,
                                            want_vectors=False, block=nb, mixed_tol=mtol)
     print(json.dumps({"dim": H.n, "entries": H._dev.n_entries, "lanes": H._dev.spmv_group,
+                      "table_layout": H._dev.layout(),
                       "solve_s": round(time.perf_counter() - t0, 3), "matvecs": int(H.last_info[0]),
                       "levels": [float(x) for x in w]}))
     return 0
@@ -563,7 +565,8 @@ def grouped_products(ed, _edx, data, nb=6, reps=50):
            "single_calls_us": round(wall(singles), 1),
            "grouped_launch_us": round(wall(lambda: _edx.matvec_group(ctx, hams, ed.PREC_MIXED, nb, xs, ys)), 1),
            "graph_replay_us": round(wall(g.launch), 1)}
-    bytes_ = sum(H._dev.dim * (8 + 16 * nb) + H._dev.n_entries * (12 + 8 * nb) for H in ops)
+    bytes_ = sum(H._dev.dim * (8 + 16 * nb) + H._dev.n_entries * (4 + (0 if H._dev.layout()["packed"] else 8) + 8 * nb)
+                 for H in ops)
     out["graph_replay_algorithmic_GBps"] = round(bytes_ / out["graph_replay_us"] / 1e3, 1)
     return out
 
@@ -629,6 +632,8 @@ def checkerboard_bench(args) -> int:
     # path and the accuracy reference), then with eigenvectors to the host
     i_ref, w_mixed = last
     basis, H = sector(i_ref)
+    table_layout = H._dev.layout()
+    log(f"table layout of sector {i_ref}: {table_layout}")
     t0 = time.perf_counter()
     w64, _ = ed.diagonalize_block_matrixfree(H, H.n, nev=nev, precision=ed.PREC_FP64, eig_tol=0.0,
                                              want_vectors=False)
@@ -654,7 +659,9 @@ def checkerboard_bench(args) -> int:
     total_k = sum(v[0] for v in ks.values())
     cls = max(ks, key=lambda k: ks[k][0])
     lanes = next((e["lanes"] for e in _edx.autotune_table() if e["block"] == nb), None)
-    names = {"spmm_c64": f"spmm_sell<float2,{nb},{lanes}>", "spmm_c128": f"spmm_sell<double2,{max(nb, nev)},{lanes}>",
+    pk = ",true" if table_layout["packed"] else ",false"
+    names = {"spmm_c64": f"spmm_sell<float2,{nb},{lanes}{pk}>",
+             "spmm_c128": f"spmm_sell<double2,{max(nb, nev)},{lanes}{pk}>",
              "gram": "gram_tiles + gram_reduce", "gemm": "gemm_tiles", "combine": "gemm_tiles (restart combine)",
              "combine_mfma": "combine_mfma32 (restart combine)"}
     k_ms, k_n, k_bytes = ks[cls][:3]
@@ -715,6 +722,7 @@ def checkerboard_bench(args) -> int:
         "eigvec_true_max_rel_residual": res_true,
         "pcie_inclusive_solve_s": round(t_vec, 3),
         "catalog_ms": round(t_cat * 1e3, 3),
+        "table_layout": table_layout,
         "roofline": roofline,
         "cpu_baseline": cpu,
         "secondary": secondary,
@@ -1042,6 +1050,8 @@ def main() -> int:
                     help="sectors solved concurrently, one HIP stream (context) per worker thread")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="heuristic SpMM lanes instead of timed ones")
+    ap.add_argument("--no-table-packing", action="store_true",
+                    help="unpacked sliced tables (column and element arrays) instead of palette-packed words")
     ap.add_argument("--no-kernel-timer", action="store_true",
                     help="no live per-kernel HIP-event timing (for runs under a profiler, which times kernels itself)")
     ap.add_argument("--profile-spmm", action="store_true", help="run only the SpMM kernels of one sector")
@@ -1070,6 +1080,8 @@ def main() -> int:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}): sectors shard across GPUs "
               "without communication, so the one-GPU figure is the per-GPU rate.", file=sys.stderr)
         return 2
+    if args.no_table_packing:
+        os.environ["EDX_TABLE_PACKING"] = "0"  # read by the library when a context is created
     if args.profile_spmm:
         return profile_spmm(args)
     if args.profile_solve:

@@ -318,6 +318,26 @@ int edx_set_spmv_group(edx_ctx* ctx, int32_t lanes);                 /* default 
 int edx_hamiltonian_set_spmv_group(edx_ctx* ctx, edx_hamiltonian* ham, int32_t lanes);
 int edx_hamiltonian_info(const edx_hamiltonian* ham, int64_t* dim, int64_t* n_entries, int32_t* spmv_group);
 
+/* ---- Palette-packed tables ---------------------------------------------------
+ * A sector table holds few distinct elements (hopping amplitude x sign x
+ * character x stabilizer-norm ratio, SRE:595-693).  With packing on (the
+ * default) a table whose distinct complex128 values number at most
+ * min(1024, 2^(32 - bit length of dim)) stores its sliced copy as one 32-bit
+ * word per slot, column | palette index << bit length of dim, and its values
+ * once; the SpMM then streams 4 bytes per entry instead of 12 (complex64) or
+ * 20 (complex128) and gives bit-identical results.  Other tables (e.g. the
+ * Hermitian parts of twisted blocks) keep the unpacked copy.  No reference
+ * counterpart (storage layout of the device product). */
+/* for tables built on ctx from now on; a new context starts with packing on,
+ * or off when the environment variable EDX_TABLE_PACKING is "0" */
+int edx_set_table_packing(edx_ctx* ctx, int32_t enable);
+/* packed: 1 when the sliced copy is packed; n_palette: its distinct values;
+ * stored_slots: slots of the sliced copy including padding; table_bytes:
+ * device bytes of the sliced copy (offsets, words or columns and elements,
+ * palettes).  Any output may be NULL. */
+int edx_hamiltonian_layout(const edx_hamiltonian* ham, int32_t* packed, int32_t* n_palette, int64_t* stored_slots,
+                           int64_t* table_bytes);
+
 /* ---- SpMM autotuner --------------------------------------------------------
  * With tuning on, every Hamiltonian build (edx_hamiltonian_build,
  * edx_hamiltonian_from_csr) picks the lanes per row of its sliced copy from a

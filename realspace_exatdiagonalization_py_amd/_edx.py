@@ -70,6 +70,8 @@ SIGNATURES = {
     "edx_set_spmv_group": (C.c_int, [P, i32]),
     "edx_hamiltonian_set_spmv_group": (C.c_int, [P, P, i32]),
     "edx_hamiltonian_info": (C.c_int, [P, pi64, pi64, pi32]),
+    "edx_set_table_packing": (C.c_int, [P, i32]),
+    "edx_hamiltonian_layout": (C.c_int, [P, pi32, pi32, pi64, pi64]),
     "edx_set_autotune": (C.c_int, [i32, i32, i32]),
     "edx_autotune_table": (C.c_int, [i32, P, P, pi32]),
     "edx_autotune_clear": (C.c_int, []),
@@ -318,6 +320,21 @@ def combine_vectors(ctx: "Context", precision: int, v: "DeviceArray", ld: int, d
         raise ValueError("buffer smaller than its vectors")
     ctx.check(lib.edx_combine_vectors(ctx.handle, int(precision), v.ptr, int(ld), int(dim), int(nv),
                                       y.ctypes.data_as(C.c_void_p), int(nk), out.ptr, int(schedule)))
+
+
+def set_table_packing(ctx: "Context", enable: bool = True) -> None:
+    """Palette-packed sliced tables for Hamiltonians built on ``ctx`` from now on."""
+    ctx.check(lib.edx_set_table_packing(ctx.handle, 1 if enable else 0))
+
+
+def hamiltonian_layout(handle) -> dict:
+    """Layout of a built Hamiltonian's sliced copy: packed, palette size,
+    stored slots (with padding) and device bytes."""
+    packed, n_pal = C.c_int32(), C.c_int32()
+    slots, nbytes = C.c_int64(), C.c_int64()
+    raise_for(lib.edx_hamiltonian_layout(handle, C.byref(packed), C.byref(n_pal), C.byref(slots), C.byref(nbytes)))
+    return {"packed": bool(packed.value), "n_palette": n_pal.value, "stored_slots": slots.value,
+            "table_bytes": nbytes.value}
 
 
 def mem_info(ctx: "Context") -> tuple[int, int]:

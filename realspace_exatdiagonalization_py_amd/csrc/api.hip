@@ -247,6 +247,8 @@ int edx_ctx_create(int32_t device, edx_ctx** out) {
     (void)hipGetLastError();
     auto ctx = new edx_ctx();
     ctx->device = device;
+    // EDX_TABLE_PACKING=0: new contexts start with table packing off
+    if (const char* pk = std::getenv("EDX_TABLE_PACKING")) ctx->table_pack = pk[0] == '0' ? 0 : 1;
     hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
     // Gram results are written by kernels straight into pinned_in: coherent
     // (fine-grained) host memory, read by the host after a stream sync
@@ -327,6 +329,26 @@ int edx_hamiltonian_info(const edx_hamiltonian* ham, int64_t* dim, int64_t* n_en
   if (dim) *dim = ham->dim;
   if (n_entries) *n_entries = ham->n_entries;
   if (spmv_group) *spmv_group = ham->sell_g;
+  return EDX_OK;
+}
+
+int edx_set_table_packing(edx_ctx* ctx, int32_t enable) {
+  return edx::guarded(ctx, [&] {
+    EDX_REQUIRE(ctx, EDX_ERR_INVALID, "null argument");
+    EDX_REQUIRE(enable == 0 || enable == 1, EDX_ERR_INVALID, "table packing must be 0 or 1");
+    ctx->table_pack = enable;
+  });
+}
+
+int edx_hamiltonian_layout(const edx_hamiltonian* ham, int32_t* packed, int32_t* n_palette, int64_t* stored_slots,
+                           int64_t* table_bytes) {
+  if (!ham) return EDX_ERR_INVALID;
+  if (packed) *packed = ham->sell_packed;
+  if (n_palette) *n_palette = ham->n_pal;
+  if (stored_slots) *stored_slots = ham->sell_entries;
+  if (table_bytes)
+    *table_bytes = (int64_t)(ham->sell_off.bytes() + ham->sell_col.bytes() + ham->sell_val.bytes() +
+                             ham->sell_val32.bytes() + ham->pal.bytes() + ham->pal32.bytes());
   return EDX_OK;
 }
 

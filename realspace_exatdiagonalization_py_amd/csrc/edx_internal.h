@@ -16,6 +16,11 @@
 
 #define EDX_MAX_SITES 63
 #define EDX_MAX_GROUP 64
+
+namespace edx {
+// Largest palette of a packed sliced table (complex128: 16 KiB of LDS per workgroup).
+constexpr int kPalMax = 1024;
+}  // namespace edx
 #define EDX_MAX_TERMS 65535
 
 namespace edx {
@@ -212,6 +217,9 @@ struct Ctx {
   std::atomic<uintptr_t> owner{0};
   int depth = 0;
   int autotune_group = 0;  // lanes per row in the SpMV (0 = pick by heuristics)
+  // 1: tables built on this context get the palette-packed sliced copy when
+  // their values fit (edx_set_table_packing)
+  int table_pack = 1;
   // batched solves driven from this context: round graphs captured, replayed
   int64_t graph_captures = 0, graph_replays = 0;
   // orthogonalisation schedule of the eigensolver (0 = tuned or default):

@@ -71,9 +71,17 @@ struct edx_hamiltonian {
   int64_t sell_slices = 0;
   int64_t sell_entries = 0;           // stored entries including padding
   edx::ObjBuf<int64_t> sell_off;      // [sell_slices + 1]
-  edx::ObjBuf<int32_t> sell_col;      // [sell_entries], -1 = padding
-  edx::ObjBuf<double2> sell_val;      // [sell_entries]
-  edx::ObjBuf<float2> sell_val32;     // [sell_entries] complex64 copy
+  edx::ObjBuf<int32_t> sell_col;      // [sell_entries], -1 = padding (packed: words, below)
+  edx::ObjBuf<double2> sell_val;      // [sell_entries] (empty when packed)
+  edx::ObjBuf<float2> sell_val32;     // [sell_entries] complex64 copy (empty when packed)
+  // 1: palette-packed copy (sellpack.hip): sell_col holds the 32-bit words
+  // column | (palette index << pack_bits), 0xFFFFFFFF = padding, and the
+  // elements are pal[index] (complex128) and pal32[index] (complex64)
+  int sell_packed = 0;
+  int pack_bits = 0;
+  int n_pal = 0;
+  edx::ObjBuf<double2> pal;           // [n_pal]
+  edx::ObjBuf<float2> pal32;          // [n_pal]
   // 1: no stored table; every product recomputes the elements from the terms
   // and the catalog (the arrays below), CSR and sliced arrays are empty
   int on_the_fly = 0;
@@ -131,6 +139,11 @@ void abandon_sell(const edx_hamiltonian& h, SellPlan& p);
 // Slice offsets and storage of a sliced copy with G lanes per row (contents
 // not written); returns the stored entry count.
 int64_t prepare_sell(edx_hamiltonian& h, Ctx& ctx, int G);
+// Bit length of dim: the column field of a packed table word.
+int pack_bits_for(int64_t dim);
+// Replaces h's complete unpacked sliced copy by the palette-packed one when
+// ctx allows packing and the values fit; returns true when packed.
+bool pack_sell(edx_hamiltonian& h, Ctx& ctx);
 // Looks up the device address of edx_matvec_noalloc's static scratch array
 // on `device` (current) once per device (noalloc.hip).
 void noalloc_init(int device);

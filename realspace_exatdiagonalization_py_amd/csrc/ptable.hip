@@ -321,10 +321,13 @@ template <> struct Real<float2> { using T = float; };
 // entry); Y is column-major with leading dimension ldy.
 // ---------------------------------------------------------------------------
 // Slice s of one table (wavefront `lane` lanes); shared by the single-table
-// and the grouped kernels, so both give identical results.
-template <typename V, int NB, int G>
+// and the grouped kernels, so both give identical results.  PACKED: `col`
+// holds the palette-packed words (column | index << bits, 0xFFFFFFFF =
+// padding) and `val` is the palette (in LDS); the elements multiplied carry
+// the bits of the unpacked copy's, so both layouts give identical results.
+template <typename V, int NB, int G, bool PACKED>
 __device__ __forceinline__ void sell_slice(int64_t s, int lane, int64_t dim, const int64_t* __restrict__ off,
-                                           const int32_t* __restrict__ col, const V* __restrict__ val,
+                                           const int32_t* __restrict__ col, const V* __restrict__ val, int bits,
                                            const V* __restrict__ hdiag, const V* __restrict__ X, V* __restrict__ Y,
                                            int64_t ldy) {
   using T = typename Real<V>::T;
@@ -336,14 +339,16 @@ __device__ __forceinline__ void sell_slice(int64_t s, int lane, int64_t dim, con
   for (int q = 0; q < NB; ++q) ar[q] = ai[q] = (T)0;
   const int32_t* cp = col + o0 + lane;
   const V* vp = val + o0 + lane;
+  const uint32_t cmask = (1u << bits) - 1u;
   // the next entry's column is loaded while this entry's element and x
   // values are fetched, so each entry costs one memory round trip
   int cn = L > 0 ? __builtin_nontemporal_load(cp) : -1;
   for (int64_t t = 0; t < L; ++t) {
-    const int c = cn;
+    const int w = cn;
     if (t + 1 < L) cn = __builtin_nontemporal_load(cp + (t + 1) * 64);
-    if (c >= 0) {
-      const V v = vp[t * 64];
+    if (PACKED ? (uint32_t)w != 0xFFFFFFFFu : w >= 0) {
+      const int c = PACKED ? (int)((uint32_t)w & cmask) : w;
+      const V v = PACKED ? val[(uint32_t)w >> bits] : vp[t * 64];
       const V* x = X + (int64_t)c * NB;
       V xv[NB];
 #pragma unroll
@@ -384,26 +389,40 @@ __device__ __forceinline__ void sell_slice(int64_t s, int lane, int64_t dim, con
   }
 }
 
-template <typename V, int NB, int G>
+// Copies a palette of n values into LDS (all threads of the workgroup take
+// part, before any of them returns).
+template <typename V>
+__device__ __forceinline__ const V* stage_palette(const V* __restrict__ pal, int n) {
+  extern __shared__ char smem[];
+  V* p = reinterpret_cast<V*>(smem);
+  for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = pal[i];
+  __syncthreads();
+  return p;
+}
+
+template <typename V, int NB, int G, bool PACKED>
 __global__ void __launch_bounds__(256)
 spmm_sell(int64_t dim, int64_t n_slices, const int64_t* __restrict__ off, const int32_t* __restrict__ col,
-          const V* __restrict__ val, const V* __restrict__ hdiag, const V* __restrict__ X, V* __restrict__ Y,
-          int64_t ldy) {
+          const V* __restrict__ val, int n_pal, int bits, const V* __restrict__ hdiag, const V* __restrict__ X,
+          V* __restrict__ Y, int64_t ldy) {
+  const V* v = PACKED ? stage_palette(val, n_pal) : val;
   const int64_t s = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
   if (s >= n_slices) return;
-  sell_slice<V, NB, G>(s, threadIdx.x & 63, dim, off, col, val, hdiag, X, Y, ldy);
+  sell_slice<V, NB, G, PACKED>(s, threadIdx.x & 63, dim, off, col, v, bits, hdiag, X, Y, ldy);
 }
 
 // Grouped SpMM: the sliced products of up to kGroupMax tables (same element
-// precision, block width and lanes per row) in one launch.  Workgroups
-// [first[i], first[i + 1]) serve table i, four slices each, as in spmm_sell.
+// precision, block width, lanes per row and layout) in one launch.
+// Workgroups [first[i], first[i + 1]) serve table i, four slices each, as in
+// spmm_sell.
 constexpr int kGroupMax = 32;
 template <typename V>
 struct SellTable {
   int64_t dim, n_slices, ldy;
   const int64_t* off;
   const int32_t* col;
-  const V* val;
+  const V* val;  // elements, or the palette of a packed table
+  int n_pal, bits;
   const V* hdiag;
   const V* X;
   V* Y;
@@ -415,35 +434,38 @@ struct SellGroup {
   SellTable<V> t[kGroupMax];
 };
 
-template <typename V, int NB, int G>
+template <typename V, int NB, int G, bool PACKED>
 __global__ void __launch_bounds__(256) spmm_sell_group(SellGroup<V> g) {
   const int64_t blk = blockIdx.x;
   int i = 0;
   while (i + 1 < g.n && blk >= g.first[i + 1]) ++i;
   const SellTable<V>& t = g.t[i];
+  const V* v = PACKED ? stage_palette(t.val, t.n_pal) : t.val;
   const int64_t s = (blk - g.first[i]) * 4 + (threadIdx.x >> 6);
   if (s >= t.n_slices) return;
-  sell_slice<V, NB, G>(s, threadIdx.x & 63, t.dim, t.off, t.col, t.val, t.hdiag, t.X, t.Y, t.ldy);
+  sell_slice<V, NB, G, PACKED>(s, threadIdx.x & 63, t.dim, t.off, t.col, v, t.bits, t.hdiag, t.X, t.Y, t.ldy);
 }
 
-template <typename V, int NB>
+template <typename V, int NB, bool PACKED>
 void launch_sell_nb(const edx_hamiltonian& h, hipStream_t st, const V* val, const V* hd, const V* X, V* Y,
                     int64_t ldy) {
   const dim3 grid((unsigned)((h.sell_slices + 3) / 4));
   const int64_t* off = h.sell_off.p;
   const int32_t* col = h.sell_col.p;
+  const int n_pal = PACKED ? h.n_pal : 0;
+  const size_t lds = PACKED ? (size_t)n_pal * sizeof(V) : 0;
   switch (h.sell_g) {
     case 1:
-      hipLaunchKernelGGL((spmm_sell<V, NB, 1>), grid, dim3(256), 0, st, h.dim, h.sell_slices, off, col, val, hd, X,
-                         Y, ldy);
+      hipLaunchKernelGGL((spmm_sell<V, NB, 1, PACKED>), grid, dim3(256), lds, st, h.dim, h.sell_slices, off, col,
+                         val, n_pal, h.pack_bits, hd, X, Y, ldy);
       break;
     case 2:
-      hipLaunchKernelGGL((spmm_sell<V, NB, 2>), grid, dim3(256), 0, st, h.dim, h.sell_slices, off, col, val, hd, X,
-                         Y, ldy);
+      hipLaunchKernelGGL((spmm_sell<V, NB, 2, PACKED>), grid, dim3(256), lds, st, h.dim, h.sell_slices, off, col,
+                         val, n_pal, h.pack_bits, hd, X, Y, ldy);
       break;
     default:
-      hipLaunchKernelGGL((spmm_sell<V, NB, 4>), grid, dim3(256), 0, st, h.dim, h.sell_slices, off, col, val, hd, X,
-                         Y, ldy);
+      hipLaunchKernelGGL((spmm_sell<V, NB, 4, PACKED>), grid, dim3(256), lds, st, h.dim, h.sell_slices, off, col,
+                         val, n_pal, h.pack_bits, hd, X, Y, ldy);
   }
   EDX_HIP(hipGetLastError());
 }
@@ -454,7 +476,8 @@ void launch_sell(const edx_hamiltonian& h, hipStream_t st, int nb, const V* val,
   switch (nb) {
 #define EDX_SELL_CASE(N) \
   case N:                \
-    launch_sell_nb<V, N>(h, st, val, hd, X, Y, ldy); \
+    if (h.sell_packed) launch_sell_nb<V, N, true>(h, st, val, hd, X, Y, ldy); \
+    else launch_sell_nb<V, N, false>(h, st, val, hd, X, Y, ldy); \
     break;
     EDX_SELL_CASE(1) EDX_SELL_CASE(2) EDX_SELL_CASE(3) EDX_SELL_CASE(4) EDX_SELL_CASE(5) EDX_SELL_CASE(6)
     EDX_SELL_CASE(7) EDX_SELL_CASE(8) EDX_SELL_CASE(9) EDX_SELL_CASE(10) EDX_SELL_CASE(11) EDX_SELL_CASE(12)
@@ -470,7 +493,9 @@ void launch_sell(const edx_hamiltonian& h, hipStream_t st, int nb, const V* val,
 // and one gather of nb x values (padding of the sliced layout not counted).
 double spmm_bytes(const edx_hamiltonian& h, int precision, int nb) {
   const double s = precision == EDX_PREC_FP64 ? 16.0 : 8.0;
-  return (double)h.dim * (s + 2.0 * nb * s) + (double)h.n_entries * (4.0 + s + nb * s);
+  // a packed entry is one 4-byte word (the palette stays in LDS)
+  const double elem = h.sell_packed ? 0.0 : s;
+  return (double)h.dim * (s + 2.0 * nb * s) + (double)h.n_entries * (4.0 + elem + nb * s);
 }
 
 // ---------------------------------------------------------------------------
@@ -662,20 +687,30 @@ void spmm(const edx_hamiltonian& h, Ctx& ctx, int precision, int nb, const void*
     return;
   }
   if (precision == EDX_PREC_FP64)
-    launch_sell<double2>(h, st, nb, h.sell_val.p, h.hdiag.p, (const double2*)X, (double2*)Y, ldy);
+    launch_sell<double2>(h, st, nb, h.sell_packed ? h.pal.p : h.sell_val.p, h.hdiag.p, (const double2*)X,
+                         (double2*)Y, ldy);
   else
-    launch_sell<float2>(h, st, nb, h.sell_val32.p, h.hdiag32.p, (const float2*)X, (float2*)Y, ldy);
+    launch_sell<float2>(h, st, nb, h.sell_packed ? h.pal32.p : h.sell_val32.p, h.hdiag32.p, (const float2*)X,
+                        (float2*)Y, ldy);
   ctx.timer.stop(st, t0, precision == EDX_PREC_FP64 ? K_SPMM : K_SPMM32, spmm_bytes(h, precision, nb));
 }
 
-template <typename V, int NB>
-void launch_group_nb(const SellGroup<V>& g, int lanes, unsigned blocks, hipStream_t st) {
+template <typename V, int NB, bool PACKED>
+void launch_group_packed(const SellGroup<V>& g, int lanes, unsigned blocks, size_t lds, hipStream_t st) {
   switch (lanes) {
-    case 1: hipLaunchKernelGGL((spmm_sell_group<V, NB, 1>), dim3(blocks), dim3(256), 0, st, g); break;
-    case 2: hipLaunchKernelGGL((spmm_sell_group<V, NB, 2>), dim3(blocks), dim3(256), 0, st, g); break;
-    default: hipLaunchKernelGGL((spmm_sell_group<V, NB, 4>), dim3(blocks), dim3(256), 0, st, g);
+    case 1: hipLaunchKernelGGL((spmm_sell_group<V, NB, 1, PACKED>), dim3(blocks), dim3(256), lds, st, g); break;
+    case 2: hipLaunchKernelGGL((spmm_sell_group<V, NB, 2, PACKED>), dim3(blocks), dim3(256), lds, st, g); break;
+    default: hipLaunchKernelGGL((spmm_sell_group<V, NB, 4, PACKED>), dim3(blocks), dim3(256), lds, st, g);
   }
   EDX_HIP(hipGetLastError());
+}
+
+template <typename V, int NB>
+void launch_group_nb(const SellGroup<V>& g, int lanes, bool packed, unsigned blocks, hipStream_t st) {
+  int n_pal = 0;
+  for (int i = 0; i < g.n; ++i) n_pal = std::max(n_pal, g.t[i].n_pal);
+  if (packed) launch_group_packed<V, NB, true>(g, lanes, blocks, (size_t)n_pal * sizeof(V), st);
+  else launch_group_packed<V, NB, false>(g, lanes, blocks, 0, st);
 }
 
 template <typename V>
@@ -693,12 +728,14 @@ void launch_group(const SpmmJob* jobs, const std::vector<int>& idx, int precisio
     t.off = h.sell_off.p;
     t.col = h.sell_col.p;
     if constexpr (std::is_same<V, double2>::value) {
-      t.val = h.sell_val.p;
+      t.val = h.sell_packed ? h.pal.p : h.sell_val.p;
       t.hdiag = h.hdiag.p;
     } else {
-      t.val = h.sell_val32.p;
+      t.val = h.sell_packed ? h.pal32.p : h.sell_val32.p;
       t.hdiag = h.hdiag32.p;
     }
+    t.n_pal = h.sell_packed ? h.n_pal : 0;
+    t.bits = h.pack_bits;
     t.X = (const V*)j.X;
     t.Y = (V*)j.Y;
     g.first[i + 1] = g.first[i] + (h.sell_slices + 3) / 4;
@@ -706,10 +743,11 @@ void launch_group(const SpmmJob* jobs, const std::vector<int>& idx, int precisio
   EDX_REQUIRE(g.first[g.n] < (int64_t(1) << 31), EDX_ERR_INTERNAL, "grouped SpMM grid");
   const unsigned blocks = (unsigned)g.first[g.n];
   const int lanes = jobs[idx[0]].h->sell_g;
+  const bool packed = jobs[idx[0]].h->sell_packed != 0;
   switch (jobs[idx[0]].nb) {
 #define EDX_GROUP_CASE(N) \
   case N:                 \
-    launch_group_nb<V, N>(g, lanes, blocks, st); \
+    launch_group_nb<V, N>(g, lanes, packed, blocks, st); \
     break;
     EDX_GROUP_CASE(1) EDX_GROUP_CASE(2) EDX_GROUP_CASE(3) EDX_GROUP_CASE(4) EDX_GROUP_CASE(5) EDX_GROUP_CASE(6)
     EDX_GROUP_CASE(7) EDX_GROUP_CASE(8) EDX_GROUP_CASE(9) EDX_GROUP_CASE(10) EDX_GROUP_CASE(11) EDX_GROUP_CASE(12)
@@ -752,7 +790,8 @@ void spmm_group(const SpmmJob* jobs, int n, Ctx& ctx, int precision) {
     std::vector<int> idx;
     double bytes = 0.0;
     for (int i = i0; i < n && (int)idx.size() < kGroupMax; ++i)
-      if (left[i] && jobs[i].nb == jobs[i0].nb && jobs[i].h->sell_g == jobs[i0].h->sell_g) {
+      if (left[i] && jobs[i].nb == jobs[i0].nb && jobs[i].h->sell_g == jobs[i0].h->sell_g &&
+          jobs[i].h->sell_packed == jobs[i0].h->sell_packed) {
         idx.push_back(i);
         left[i] = 0;
         bytes += spmm_bytes(*jobs[i].h, precision, jobs[i].nb);
@@ -890,6 +929,11 @@ int64_t prepare_sell(edx_hamiltonian& h, Ctx& ctx, int G) {
   h.sell_g = G;
   h.sell_slices = ns;
   h.sell_entries = 0;
+  h.sell_packed = 0;
+  h.n_pal = 0;
+  h.pack_bits = 0;
+  h.pal.release();
+  h.pal32.release();
   h.sell_off.alloc((size_t)ns + 1);
   if (ns == 0) {
     const int64_t zero = 0;
@@ -925,6 +969,7 @@ void build_sell(edx_hamiltonian& h, Ctx& ctx, int G) {
                      (const int64_t*)h.sell_off.p, h.sell_col.p, h.sell_val.p, h.sell_val32.p);
   EDX_HIP(hipGetLastError());
   EDX_HIP(hipStreamSynchronize(st));
+  pack_sell(h, ctx);
 }
 
 // Column-major [nb][ld] -> interleaved [dim][nb].
@@ -1199,7 +1244,10 @@ int build_hamiltonian(edx_ctx* ctx, const edx_catalog* cat, const edx_sector* se
         EDX_HIP(hipStreamSynchronize(st));
       }
       ham->n_entries = E;
-      if (sell_written) edx::finish_sell(*ham, *ctx, plan);
+      if (sell_written) {
+        edx::pack_sell(*ham, *ctx);
+        edx::finish_sell(*ham, *ctx, plan);
+      }
       else edx::build_sell_for(*ham, *ctx);
     } catch (...) {
       edx::abandon_sell(*ham, plan);

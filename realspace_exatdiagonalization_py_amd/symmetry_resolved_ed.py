@@ -553,6 +553,11 @@ class _DeviceHamiltonian:
             lib.edx_hamiltonian_destroy(self.handle)
             self.handle = None
 
+    def layout(self) -> dict:
+        """Sliced-copy layout: ``packed`` (palette-packed words), ``n_palette``,
+        ``stored_slots`` (with padding) and ``table_bytes`` on the device."""
+        return _edx.hamiltonian_layout(self.handle)
+
     def autotune(self, precision: int = _edx.PREC_MIXED, block: int = 6) -> dict:
         """Times the 1-, 2- and 4-lane SpMM layouts at (precision, block),
         keeps the fastest; returns ``{"lanes": g, "ms": {1: t1, 2: t2, 4: t4}}``.

@@ -1,0 +1,182 @@
+"""GPU: palette-packed sliced tables against the unpacked layout.
+
+A packed table stores one 32-bit word per slot (column | palette index above
+the column's bit length) and its distinct elements once; the SpMM reads the
+element from the palette in LDS.  The elements carry the bits of the
+unpacked copy's and the arithmetic is the same, so the bar here is bit
+equality, not a tolerance:
+
+* per case (spin chain, Hubbard 3x4, checkerboard, Haldane bosons), tables
+  built with packing on and off: the packed one is packed, smaller, holds
+  fewer palette entries than the index bits allow, and ``edx_matvec_device``
+  gives bit-identical outputs for 1, 4, 10 and 16 vectors in complex128 and
+  complex64;
+* the grouped product (one launch over several tables) of packed tables, and
+  of a group mixing packed and unpacked tables, equals each table's own
+  product bit for bit;
+* a mixed-precision solve gives bit-identical levels on both layouts;
+* rebuilding the sliced copy with other lanes per row keeps it packed with
+  the same products;
+* a twisted table (its sliced copy holds the Hermitian part, packed when
+  its values fit) gives bit-identical mixed-precision levels on both;
+* a table handed over as host CSR is packed and applies its operator.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import realspace_exatdiagonalization_py_amd as ed
+from cases import make_case
+from realspace_exatdiagonalization_py_amd import _edx
+from realspace_exatdiagonalization_py_amd._edx import lib
+from realspace_exatdiagonalization_py_amd.groups import build_irrep_list
+
+pytestmark = pytest.mark.gpu
+
+CASES = [("heis16", 1), ("hub34", 2), ("cb34", 3), ("hal34_twist", None), ("hub24", 0)]
+
+
+def _contexts():
+    packed = _edx.Context(0)
+    plain = _edx.Context(0)
+    _edx.set_table_packing(plain, False)
+    return packed, plain
+
+
+def _hamiltonians(name, sector, ctxs):
+    model, ne, G = make_case(name)
+    cat = ed.build_symmetry_orbit_catalog(second_quantized_model=model, n_filled=ne, symmetry_group=G,
+                                          particle_statistics=model.particle_statistics)
+    irreps = build_irrep_list(G, model.lattice)
+    irrep = irreps[sector if sector is not None else len(irreps) // 2]
+    basis = ed.build_symmetry_sector_basis(cat, irrep)
+    out = []
+    for ctx in ctxs:
+        with _edx.use_context(ctx):
+            out.append(ed.MatrixFreeHamiltonian(basis, model.bilinear_terms, model.density_density_terms, None))
+    return out
+
+
+def _product(dh, prec, nb, X):
+    dt = np.complex128 if prec == _edx.PREC_FP64 else np.complex64
+    xs = np.ascontiguousarray(X.astype(dt))
+    dx = _edx.DeviceArray(dh.ctx, xs.nbytes)
+    dy = _edx.DeviceArray(dh.ctx, xs.nbytes)
+    dx.upload(xs)
+    dh.ctx.check(lib.edx_matvec_device(dh.ctx.handle, dh.handle, prec, nb, dx.ptr, dy.ptr))
+    return dy.download(np.empty((nb, dh.dim), dtype=dt))
+
+
+def _block(D, nb, seed):
+    rng = np.random.default_rng(seed)
+    return rng.standard_normal((D, nb)) + 1j * rng.standard_normal((D, nb))
+
+
+@pytest.mark.parametrize("name,sector", CASES)
+def test_packed_product_bit_identical(name, sector):
+    print(f"[packed] {name} sector {sector}", flush=True)
+    ctxs = _contexts()
+    Hp, Hu = _hamiltonians(name, sector, ctxs)
+    lp, lu = Hp._dev.layout(), Hu._dev.layout()
+    twisted = name.endswith("_twist")
+    print(f"[packed] {name}: D={Hp.n} E={Hp._dev.n_entries} layout packed={lp} plain={lu}", flush=True)
+    assert not lu["packed"] and lu["n_palette"] == 0
+    if twisted:
+        # the sliced copy holds the Hermitian part, which edx_matvec_device does
+        # not read: compare the solves, whichever layout the values allowed
+        levels = []
+        for H, ctx in ((Hp, ctxs[0]), (Hu, ctxs[1])):
+            with _edx.use_context(ctx):
+                w, _ = ed.diagonalize_block_matrixfree(H, H.n, nev=3, precision=_edx.PREC_MIXED,
+                                                       want_vectors=False)
+            levels.append(np.asarray(w))
+        assert levels[0].tobytes() == levels[1].tobytes()
+        return
+    bits = max(int(Hp.n).bit_length(), 1)
+    assert lp["packed"] and 0 < lp["n_palette"] <= min(1024, 2 ** (32 - bits))
+    assert lp["stored_slots"] == lu["stored_slots"]
+    assert lp["table_bytes"] < lu["table_bytes"]
+    for nb in (1, 4, 10, 16):
+        X = _block(Hp.n, nb, nb)
+        for prec in (_edx.PREC_FP64, _edx.PREC_MIXED):
+            yp = _product(Hp._dev, prec, nb, X)
+            yu = _product(Hu._dev, prec, nb, X)
+            assert yp.tobytes() == yu.tobytes(), (name, nb, prec, np.abs(yp - yu).max())
+
+
+def test_packed_grouped_products():
+    ctxs = _contexts()
+    packed_ctx = ctxs[0]
+    hams = []
+    for name, sector in (("heis16", 1), ("hub24", 0), ("cb34", 3)):
+        hp, hu = _hamiltonians(name, sector, ctxs)
+        hams += [hp._dev, hu._dev]
+    nb = 4
+    for prec in (_edx.PREC_FP64, _edx.PREC_MIXED):
+        dt = np.complex128 if prec == _edx.PREC_FP64 else np.complex64
+        for group in (hams[0::2], hams):  # packed only; packed and unpacked mixed
+            xs, ys, hx = [], [], []
+            for i, dh in enumerate(group):
+                X = np.ascontiguousarray(_block(dh.dim, nb, 100 + i).astype(dt))
+                dx = _edx.DeviceArray(packed_ctx, X.nbytes)
+                dx.upload(X)
+                xs.append(dx)
+                ys.append(_edx.DeviceArray(packed_ctx, X.nbytes))
+                hx.append(X)
+            _edx.matvec_group(packed_ctx, group, prec, nb, xs, ys)
+            for dh, X, dy in zip(group, hx, ys):
+                got = dy.download(np.empty((nb, dh.dim), dtype=dt))
+                alone = _product(dh, prec, nb, X.astype(np.complex128))
+                assert got.tobytes() == alone.tobytes()
+
+
+def test_packed_solve_bit_identical():
+    ctxs = _contexts()
+    Hp, Hu = _hamiltonians("hub34", 2, ctxs)
+    assert Hp._dev.layout()["packed"]
+    res = []
+    for H, ctx in ((Hp, ctxs[0]), (Hu, ctxs[1])):
+        with _edx.use_context(ctx):
+            w, _ = ed.diagonalize_block_matrixfree(H, H.n, nev=4, precision=_edx.PREC_MIXED, want_vectors=False)
+        res.append(np.asarray(w))
+    print(f"[packed] hub34 levels {res[0]}", flush=True)
+    assert res[0].tobytes() == res[1].tobytes()
+
+
+def test_packed_relayout_keeps_products():
+    ctxs = _contexts()
+    Hp, Hu = _hamiltonians("hub24", 1, ctxs)
+    X = _block(Hp.n, 4, 5)
+    for lanes in (1, 2, 4):
+        Hp._dev.ctx.check(lib.edx_hamiltonian_set_spmv_group(Hp._dev.ctx.handle, Hp._dev.handle, lanes))
+        assert Hp._dev.layout()["packed"]
+        Hu._dev.ctx.check(lib.edx_hamiltonian_set_spmv_group(Hu._dev.ctx.handle, Hu._dev.handle, lanes))
+        ref = _product(Hu._dev, _edx.PREC_MIXED, 4, X)
+        assert _product(Hp._dev, _edx.PREC_MIXED, 4, X).tobytes() == ref.tobytes()
+
+
+def test_packing_from_host_csr():
+    """A table handed over as host CSR (edx_hamiltonian_from_csr) is packed
+    too: two distinct values, dimension 3 (two column bits)."""
+    ctx = _edx.Context(0)
+    D = 3
+    rp = np.array([0, 1, 2, 2], np.int64)
+    col = np.array([1, 0], np.int32)
+    vals = np.array([0.5 + 0.25j, 0.5 - 0.25j], np.complex128)
+    hd = np.array([1.0, -1.0, 0.5], np.complex128)
+    h = _edx.C.c_void_p()
+    ctx.check(lib.edx_hamiltonian_from_csr(ctx.handle, D, 2, _edx.ptr(rp), _edx.ptr(col), _edx.ptr(vals),
+                                           _edx.ptr(hd), _edx.C.byref(h)))
+    try:
+        lay = _edx.hamiltonian_layout(h)
+        assert lay["packed"] and lay["n_palette"] == 2
+        x = np.array([1.0 + 2.0j, -0.5j, 3.0], np.complex128)
+        y = np.empty_like(x)
+        ctx.check(lib.edx_matvec(ctx.handle, h, 1, _edx.ptr(x), _edx.ptr(y)))
+        H = np.diag(hd)
+        H[0, 1], H[1, 0] = vals
+        np.testing.assert_allclose(y, H @ x, rtol=0, atol=1e-15)
+    finally:
+        lib.edx_hamiltonian_destroy(h)
