@@ -217,10 +217,11 @@ def profile_spmm(args) -> int:
     out = {"dim": dh.dim, "entries": dh.n_entries}
     for g in (1, 2, 4):
         dh.ctx.check(_edx.lib.edx_hamiltonian_set_spmv_group(dh.ctx.handle, dh.handle, g))
+        pk = "true" if dh.layout()["packed"] else "false"
         for prec, v in ((1, "float2"), (0, "double2")):
             ms, nbytes = time_spmm(_edx, H, prec, 6)
-            out[f"spmm_sell<{v},6,{g}>"] = {"avg_ms": round(ms, 5), "algorithmic_bytes": int(nbytes),
-                                            "GB/s": round(nbytes / ms / 1e6, 1)}
+            out[f"spmm_sell<{v},6,{g},{pk}>"] = {"avg_ms": round(ms, 5), "algorithmic_bytes": int(nbytes),
+                                                "GB/s": round(nbytes / ms / 1e6, 1)}
     print(json.dumps(out))
     return 0
 
@@ -1152,7 +1153,8 @@ def hubbard_bench(args) -> int:
     spmm_ms, spmm_n, spmm_bytes = ks[cls][:3]
     H0 = _sector_hamiltonian(ed, data, 0)
     lanes = H0._dev.spmv_group
-    kname = f"spmm_sell<{'float2' if cls == 'spmm_c64' else 'double2'},{nb},{lanes}>"
+    pk = "true" if H0._dev.layout()["packed"] else "false"
+    kname = f"spmm_sell<{'float2' if cls == 'spmm_c64' else 'double2'},{nb},{lanes},{pk}>"
     avg_ms = spmm_ms / max(spmm_n, 1)
     achieved = spmm_bytes / spmm_ms / 1e6 if spmm_ms > 0 else 0.0  # GB/s (0: timer off)
     total_k = sum(v[0] for v in ks.values())
