@@ -1060,6 +1060,9 @@ struct Options {
 int auto_basis(int64_t D, int64_t E, int prec, int b, int nev) {
   const int base = std::max({5 * (nev + b), 6 * b, 24});
   if (prec == EDX_PREC_FP64 || nev < 8 || D <= 0) return base;
+  // the count keeps the element bytes of an unpacked table: on the
+  // palette-packed 48-site checkerboard sectors (nev = 10, b = 4) its 104
+  // vectors stay the fastest basis (84, 92, 116 and 128 took 2-12 % longer)
   const double esz = 8.0;
   const double spmm = (double)D * (esz + 2.0 * b * esz) + (double)E * (4.0 + esz + (double)b * esz);
   return std::max(base, (int)(spmm / ((double)D * esz) / 4.5));

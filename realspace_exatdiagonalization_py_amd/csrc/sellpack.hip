@@ -29,6 +29,9 @@ namespace edx {
 namespace {
 
 constexpr int kSlots = 4096;  // hash slots (power of two, >= 4 kPalMax)
+// Longest probe sequence: at most kPalMax keys in kSlots slots (load <= 1/4)
+// keep linear-probing runs far shorter; a longer one ends the packing.
+constexpr int kMaxProbe = 64;
 
 __device__ __forceinline__ uint64_t mix64(uint64_t x) {
   x ^= x >> 30;
@@ -51,19 +54,22 @@ __device__ __forceinline__ bool same_bits(double2 a, double2 b) {
          __double_as_longlong(a.y) == __double_as_longlong(b.y);
 }
 
-// Inserts the value of every stored slot (column >= 0) into the hash table;
-// flag bit 0: the table is full.
+// Inserts the value of every stored slot (column >= 0) into the hash table.
+// flag[0] bit 0: packing is impossible (more than kPalMax distinct values, or
+// a probe run longer than kMaxProbe); every thread then stops.  flag[1]
+// counts the distinct values inserted.
 __global__ void __launch_bounds__(256)
 pal_insert(int64_t n, const int32_t* __restrict__ col, const double2* __restrict__ val, unsigned long long* keys,
            double2* rep, int* flag) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    if (__atomic_load_n(flag, __ATOMIC_RELAXED)) return;
     if (col[i] < 0) continue;
     const double2 v = val[i];
     const unsigned long long k = value_key(v);
     unsigned s = (unsigned)k & (kSlots - 1);
     int probe = 0;
-    for (; probe < kSlots; ++probe) {
+    for (; probe < kMaxProbe; ++probe) {
       // a plain read settles almost every entry; a stale empty slot only costs
       // the compare-and-swap, which returns the key that is there
       unsigned long long cur = __atomic_load_n(&keys[s], __ATOMIC_RELAXED);
@@ -72,13 +78,14 @@ pal_insert(int64_t n, const int32_t* __restrict__ col, const double2* __restrict
         cur = atomicCAS(&keys[s], 0ull, k);
         if (cur == 0ull) {
           rep[s] = v;
+          if (atomicAdd(flag + 1, 1) >= kPalMax) atomicOr(flag, 1);
           break;
         }
         if (cur == k) break;
       }
       s = (s + 1) & (kSlots - 1);
     }
-    if (probe == kSlots) atomicOr(flag, 1);
+    if (probe == kMaxProbe) atomicOr(flag, 1);
   }
 }
 
@@ -111,11 +118,11 @@ pal_encode(int64_t n, const int32_t* __restrict__ col, const double2* __restrict
     const unsigned long long k = value_key(v);
     unsigned s = (unsigned)k & (kSlots - 1);
     int probe = 0;
-    while (probe < kSlots && keys[s] != k) {
+    while (probe < kMaxProbe && keys[s] != k) {
       s = (s + 1) & (kSlots - 1);
       ++probe;
     }
-    if (probe == kSlots || !same_bits(rep[s], v)) {
+    if (probe == kMaxProbe || !same_bits(rep[s], v)) {
       atomicOr(flag, 2);
       continue;
     }
@@ -151,7 +158,7 @@ bool pack_sell(edx_hamiltonian& h, Ctx& ctx) {
   const int64_t n = h.sell_entries;
   DevBuf<unsigned long long> keys(kSlots);
   DevBuf<double2> rep(kSlots);
-  DevBuf<int> flag(1);
+  DevBuf<int> flag(2);
   EDX_HIP(hipMemsetAsync(keys.p, 0, keys.bytes(), st));
   EDX_HIP(hipMemsetAsync(flag.p, 0, flag.bytes(), st));
   hipLaunchKernelGGL(pal_insert, dim3(grid_stride_blocks(n)), dim3(256), 0, st, n, (const int32_t*)h.sell_col.p,

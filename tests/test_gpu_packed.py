@@ -19,7 +19,10 @@ equality, not a tolerance:
   the same products;
 * a twisted table (its sliced copy holds the Hermitian part, packed when
   its values fit) gives bit-identical mixed-precision levels on both;
-* a table handed over as host CSR is packed and applies its operator.
+* a table handed over as host CSR is packed and applies its operator;
+* a table of 50 000 distinct values keeps the unpacked copy, is built
+  promptly (the hash pass stops once packing is impossible) and applies its
+  operator.
 """
 
 from __future__ import annotations
@@ -178,5 +181,46 @@ def test_packing_from_host_csr():
         H = np.diag(hd)
         H[0, 1], H[1, 0] = vals
         np.testing.assert_allclose(y, H @ x, rtol=0, atol=1e-15)
+    finally:
+        lib.edx_hamiltonian_destroy(h)
+
+
+def test_too_many_values_keep_unpacked_copy():
+    """50 000 distinct values (beyond the 1024-value palette and the 4096-slot
+    hash table): the build stops the hash pass early, keeps the unpacked
+    copy, and the product equals the host CSR product."""
+    import time
+
+    import scipy.sparse as sp
+
+    ctx = _edx.Context(0)
+    rng = np.random.default_rng(3)
+    D, per_row = 5000, 10
+    rows = np.repeat(np.arange(D), per_row)
+    cols = rng.integers(0, D, D * per_row)
+    vals = rng.standard_normal(D * per_row) + 1j * rng.standard_normal(D * per_row)
+    A = sp.csr_matrix((vals, (rows, cols)), shape=(D, D))
+    A.setdiag(0)
+    A.eliminate_zeros()
+    A.sort_indices()
+    hd = rng.standard_normal(D) + 0j
+    rp = np.ascontiguousarray(A.indptr, np.int64)
+    ci = np.ascontiguousarray(A.indices, np.int32)
+    dv = np.ascontiguousarray(A.data, np.complex128)
+    h = _edx.C.c_void_p()
+    t0 = time.perf_counter()
+    ctx.check(lib.edx_hamiltonian_from_csr(ctx.handle, D, A.nnz, _edx.ptr(rp), _edx.ptr(ci), _edx.ptr(dv),
+                                           _edx.ptr(hd), _edx.C.byref(h)))
+    t_build = time.perf_counter() - t0
+    try:
+        lay = _edx.hamiltonian_layout(h)
+        print(f"[packed] {A.nnz} distinct values: layout {lay}, build {t_build:.3f} s", flush=True)
+        assert not lay["packed"] and lay["n_palette"] == 0
+        assert t_build < 10.0
+        x = rng.standard_normal(D) + 1j * rng.standard_normal(D)
+        y = np.empty_like(x)
+        ctx.check(lib.edx_matvec(ctx.handle, h, 1, _edx.ptr(x), _edx.ptr(y)))
+        ref = A @ x + hd * x
+        assert np.linalg.norm(y - ref) <= 1e-13 * np.linalg.norm(ref)
     finally:
         lib.edx_hamiltonian_destroy(h)
