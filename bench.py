@@ -38,6 +38,7 @@ import json
 import math
 import os
 import sys
+import threading
 import time
 from fractions import Fraction
 from pathlib import Path
@@ -614,7 +615,7 @@ def checkerboard_bench(args) -> int:
         log(f"warmup step {j + 1}/{args.warmup}: sector {j % n_sec} D={dim} {dt:.2f}s ({info[0]:.0f} matvecs)")
     _edx.kernel_stats_all(enable=0 if args.no_kernel_timer else 1, reset=True)
     ctx.check(_edx.lib.edx_synchronize(ctx.handle))
-    lat, dims, infos, timed = [], [], [], []
+    lat, dims, infos, timed, seq_levels = [], [], [], [], {}
     t0 = time.perf_counter()
     for j in range(args.steps):
         i = (args.warmup + j) % n_sec
@@ -624,10 +625,53 @@ def checkerboard_bench(args) -> int:
         infos.append(info)
         timed.append(i)
         last = (i, w)
+        seq_levels[i] = w
         log(f"step {j + 1}/{args.steps}: sector {i} D={dim} {dt:.2f}s ({info[0]:.0f} matvecs)")
     elapsed = time.perf_counter() - t0
     ks = _edx.kernel_stats_all(enable=0)
     log(f"{args.steps} sector solves in {elapsed:.2f}s")
+
+    # the timed sectors again with K in flight (one context and worker thread
+    # each): the throughput when sectors overlap on the device.  The line's
+    # value, latencies and roofline stay those of one sector at a time, whose
+    # kernel durations are not stretched by the other stream's work.
+    in_flight = None
+    if args.in_flight > 1:
+        ctxs = [_edx.Context(ctx.device) for _ in range(args.in_flight)]
+        # the timed sectors, then the next ones up to a multiple of K solves
+        # (no solve runs alone at the end)
+        n_fl = -(-len(timed) // args.in_flight) * args.in_flight
+        todo = [(args.warmup + j) % n_sec for j in range(n_fl)]
+        got, errs, lock = {}, [], threading.Lock()
+
+        def worker(c):
+            try:
+                with _edx.use_context(c):
+                    while True:
+                        with lock:
+                            if not todo:
+                                return
+                            i = todo.pop(0)
+                        got[i] = solve(i, prec, args.eig_tol)[1]
+            except Exception as e:  # reported after the join
+                errs.append(e)
+
+        t1 = time.perf_counter()
+        threads = [threading.Thread(target=worker, args=(c,)) for c in ctxs]
+        for t in threads:
+            t.start()
+        for t in threads:
+            t.join()
+        el = time.perf_counter() - t1
+        if errs:
+            raise errs[0]
+        in_flight = {"sectors_in_flight": args.in_flight, "solves": n_fl, "sectors": sorted(got),
+                     "solves_per_s": round(n_fl / el, 4),
+                     "max_abs_dE_vs_one_at_a_time": float(max(np.max(np.abs(got[i] - seq_levels[i]))
+                                                              for i in got if i in seq_levels))}
+        del ctxs
+        log(f"{n_fl} sector solves with {args.in_flight} in flight in {el:.2f}s "
+            f"({in_flight['solves_per_s']} solves/s)")
 
     # the last timed sector again: FP64 with ARPACK's residual rule (the plain
     # path and the accuracy reference), then with eigenvectors to the host
@@ -724,6 +768,7 @@ def checkerboard_bench(args) -> int:
         "pcie_inclusive_solve_s": round(t_vec, 3),
         "catalog_ms": round(t_cat * 1e3, 3),
         "table_layout": table_layout,
+        "in_flight": in_flight,
         "roofline": roofline,
         "cpu_baseline": cpu,
         "secondary": secondary,
@@ -1051,6 +1096,8 @@ def main() -> int:
                     help="sectors solved concurrently, one HIP stream (context) per worker thread")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="heuristic SpMM lanes instead of timed ones")
+    ap.add_argument("--in-flight", type=int, default=2,
+                    help="after the timed steps, solve the same sectors again with this many in flight (0/1: skip)")
     ap.add_argument("--no-table-packing", action="store_true",
                     help="unpacked sliced tables (column and element arrays) instead of palette-packed words")
     ap.add_argument("--no-kernel-timer", action="store_true",
