@@ -184,6 +184,19 @@ def _sector_hamiltonian(ed, data, idx=0):
     return ed.MatrixFreeHamiltonian(basis, model.bilinear_terms, model.density_density_terms, cmap)
 
 
+def element_bytes(layout: dict, s: float) -> float:
+    """Bytes per stored entry beyond its 4-byte column: the element (s), none
+    for 32-bit packed words, a 2-byte palette index for the wide layout."""
+    return {1: 0.0, 2: 2.0}.get(layout["packed"], s)
+
+
+def spmm_kernel_name(layout: dict, v: str, nb: int, lanes) -> str:
+    """Kernel name of the sliced SpMM as the summaries write it."""
+    if layout["packed"] == 2:
+        return f"spmm_sell_wide<{v},{nb},{lanes}>"
+    return f"spmm_sell<{v},{nb},{lanes},{'true' if layout['packed'] else 'false'}>"
+
+
 def time_spmm(_edx, H, precision, nb, n_rep=20):
     """Average SpMM launch (ms) on one stream, inputs resident in HBM, and its
     algorithmic bytes."""
@@ -197,7 +210,7 @@ def time_spmm(_edx, H, precision, nb, n_rep=20):
     X.upload(x if precision == 0 else x.astype(np.float32))
     ms = C.c_double()
     ctx.check(_edx.lib.edx_time_matvec(ctx.handle, dh.handle, precision, nb, n_rep, X.ptr, Y.ptr, C.byref(ms)))
-    elem = 0.0 if dh.layout()["packed"] else s  # a packed entry is one 4-byte word
+    elem = element_bytes(dh.layout(), s)
     nbytes = dh.dim * (s + 2.0 * nb * s) + dh.n_entries * (4.0 + elem + nb * s)
     return ms.value, nbytes
 
@@ -218,10 +231,10 @@ def profile_spmm(args) -> int:
     out = {"dim": dh.dim, "entries": dh.n_entries}
     for g in (1, 2, 4):
         dh.ctx.check(_edx.lib.edx_hamiltonian_set_spmv_group(dh.ctx.handle, dh.handle, g))
-        pk = "true" if dh.layout()["packed"] else "false"
+        lay = dh.layout()
         for prec, v in ((1, "float2"), (0, "double2")):
             ms, nbytes = time_spmm(_edx, H, prec, 6)
-            out[f"spmm_sell<{v},6,{g},{pk}>"] = {"avg_ms": round(ms, 5), "algorithmic_bytes": int(nbytes),
+            out[spmm_kernel_name(lay, v, 6, g)] = {"avg_ms": round(ms, 5), "algorithmic_bytes": int(nbytes),
                                                 "GB/s": round(nbytes / ms / 1e6, 1)}
     print(json.dumps(out))
     return 0
@@ -567,7 +580,7 @@ def grouped_products(ed, _edx, data, nb=6, reps=50):
            "single_calls_us": round(wall(singles), 1),
            "grouped_launch_us": round(wall(lambda: _edx.matvec_group(ctx, hams, ed.PREC_MIXED, nb, xs, ys)), 1),
            "graph_replay_us": round(wall(g.launch), 1)}
-    bytes_ = sum(H._dev.dim * (8 + 16 * nb) + H._dev.n_entries * (4 + (0 if H._dev.layout()["packed"] else 8) + 8 * nb)
+    bytes_ = sum(H._dev.dim * (8 + 16 * nb) + H._dev.n_entries * (4 + element_bytes(H._dev.layout(), 8) + 8 * nb)
                  for H in ops)
     out["graph_replay_algorithmic_GBps"] = round(bytes_ / out["graph_replay_us"] / 1e3, 1)
     return out
@@ -704,9 +717,8 @@ def checkerboard_bench(args) -> int:
     total_k = sum(v[0] for v in ks.values())
     cls = max(ks, key=lambda k: ks[k][0])
     lanes = next((e["lanes"] for e in _edx.autotune_table() if e["block"] == nb), None)
-    pk = ",true" if table_layout["packed"] else ",false"
-    names = {"spmm_c64": f"spmm_sell<float2,{nb},{lanes}{pk}>",
-             "spmm_c128": f"spmm_sell<double2,{max(nb, nev)},{lanes}{pk}>",
+    names = {"spmm_c64": spmm_kernel_name(table_layout, "float2", nb, lanes),
+             "spmm_c128": spmm_kernel_name(table_layout, "double2", max(nb, nev), lanes),
              "gram": "gram_tiles + gram_reduce", "gemm": "gemm_tiles", "combine": "gemm_tiles (restart combine)",
              "combine_mfma": "combine_mfma32 (restart combine)"}
     k_ms, k_n, k_bytes = ks[cls][:3]
@@ -1041,6 +1053,8 @@ def flux_sweep(args) -> int:
     # accuracy: sector 0 of the last twist again in FP64
     w32, H = last.pop(0)
     n_mv = int(H.last_info[0])
+    flux_layout = H._dev.layout()
+    log(f"table layout of sector 0 at the last twist: {flux_layout}")
     w64, _ = ed.diagonalize_block_matrixfree(H, H.n, nev=nev, precision=ed.PREC_FP64, eig_tol=args.eig_tol,
                                              want_vectors=False)
     cpu = None
@@ -1069,6 +1083,7 @@ def flux_sweep(args) -> int:
         "precision": "mixed",
         "block": nb,
         "max_abs_dE_vs_fp64_one_sector": float(np.max(np.abs(w32 - w64))),
+        "table_layout": flux_layout,
         "full_sweep_estimate_s": round(576 / (n_solves / elapsed), 1),
         "cpu_baseline": cpu,
     }
@@ -1200,8 +1215,7 @@ def hubbard_bench(args) -> int:
     spmm_ms, spmm_n, spmm_bytes = ks[cls][:3]
     H0 = _sector_hamiltonian(ed, data, 0)
     lanes = H0._dev.spmv_group
-    pk = "true" if H0._dev.layout()["packed"] else "false"
-    kname = f"spmm_sell<{'float2' if cls == 'spmm_c64' else 'double2'},{nb},{lanes},{pk}>"
+    kname = spmm_kernel_name(H0._dev.layout(), "float2" if cls == "spmm_c64" else "double2", nb, lanes)
     avg_ms = spmm_ms / max(spmm_n, 1)
     achieved = spmm_bytes / spmm_ms / 1e6 if spmm_ms > 0 else 0.0  # GB/s (0: timer off)
     total_k = sum(v[0] for v in ks.values())

@@ -323,15 +323,19 @@ int edx_hamiltonian_info(const edx_hamiltonian* ham, int64_t* dim, int64_t* n_en
  * character x stabilizer-norm ratio, SRE:595-693).  With packing on (the
  * default) a table whose distinct complex128 values number at most
  * min(1024, 2^(32 - bit length of dim)) stores its sliced copy as one 32-bit
- * word per slot, column | palette index << bit length of dim, and its values
- * once; the SpMM then streams 4 bytes per entry instead of 12 (complex64) or
- * 20 (complex128) and gives bit-identical results.  Other tables (e.g. the
- * Hermitian parts of twisted blocks) keep the unpacked copy.  No reference
- * counterpart (storage layout of the device product). */
+ * word per slot, column | palette index << bit length of dim (layout 1), and
+ * its values once; the SpMM then streams 4 bytes per entry instead of 12
+ * (complex64) or 20 (complex128).  A table of up to 3072 values that do not
+ * fit those bits (e.g. the Hermitian parts of twisted blocks) keeps its
+ * int32 columns and stores a 16-bit palette index per slot (layout 2, 6 bytes
+ * per entry).  Both give results bit-identical to the unpacked copy, which
+ * tables of more values keep.  No reference counterpart (storage layout of
+ * the device product). */
 /* for tables built on ctx from now on; a new context starts with packing on,
  * or off when the environment variable EDX_TABLE_PACKING is "0" */
 int edx_set_table_packing(edx_ctx* ctx, int32_t enable);
-/* packed: 1 when the sliced copy is packed; n_palette: its distinct values;
+/* packed: 0 unpacked, 1 32-bit words, 2 columns and 16-bit indices;
+ * n_palette: its distinct values;
  * stored_slots: slots of the sliced copy including padding; table_bytes:
  * device bytes of the sliced copy (offsets, words or columns and elements,
  * palettes).  Any output may be NULL. */

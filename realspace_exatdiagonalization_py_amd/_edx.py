@@ -328,12 +328,13 @@ def set_table_packing(ctx: "Context", enable: bool = True) -> None:
 
 
 def hamiltonian_layout(handle) -> dict:
-    """Layout of a built Hamiltonian's sliced copy: packed, palette size,
+    """Layout of a built Hamiltonian's sliced copy: ``packed`` (0 unpacked,
+    1 32-bit words, 2 columns and 16-bit palette indices), palette size,
     stored slots (with padding) and device bytes."""
     packed, n_pal = C.c_int32(), C.c_int32()
     slots, nbytes = C.c_int64(), C.c_int64()
     raise_for(lib.edx_hamiltonian_layout(handle, C.byref(packed), C.byref(n_pal), C.byref(slots), C.byref(nbytes)))
-    return {"packed": bool(packed.value), "n_palette": n_pal.value, "stored_slots": slots.value,
+    return {"packed": packed.value, "n_palette": n_pal.value, "stored_slots": slots.value,
             "table_bytes": nbytes.value}
 
 

@@ -347,8 +347,8 @@ int edx_hamiltonian_layout(const edx_hamiltonian* ham, int32_t* packed, int32_t*
   if (n_palette) *n_palette = ham->n_pal;
   if (stored_slots) *stored_slots = ham->sell_entries;
   if (table_bytes)
-    *table_bytes = (int64_t)(ham->sell_off.bytes() + ham->sell_col.bytes() + ham->sell_val.bytes() +
-                             ham->sell_val32.bytes() + ham->pal.bytes() + ham->pal32.bytes());
+    *table_bytes = (int64_t)(ham->sell_off.bytes() + ham->sell_col.bytes() + ham->sell_idx.bytes() +
+                             ham->sell_val.bytes() + ham->sell_val32.bytes() + ham->pal.bytes() + ham->pal32.bytes());
   return EDX_OK;
 }
 

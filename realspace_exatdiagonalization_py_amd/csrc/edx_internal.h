@@ -18,8 +18,10 @@
 #define EDX_MAX_GROUP 64
 
 namespace edx {
-// Largest palette of a packed sliced table (complex128: 16 KiB of LDS per workgroup).
+// Largest palette of a packed sliced table: 32-bit words (complex128: 16 KiB
+// of LDS per workgroup), and the wide layout (16-bit indices; 48 KiB).
 constexpr int kPalMax = 1024;
+constexpr int kPalMaxWide = 3072;
 }  // namespace edx
 #define EDX_MAX_TERMS 65535
 

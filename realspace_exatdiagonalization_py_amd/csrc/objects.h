@@ -74,14 +74,16 @@ struct edx_hamiltonian {
   edx::ObjBuf<int32_t> sell_col;      // [sell_entries], -1 = padding (packed: words, below)
   edx::ObjBuf<double2> sell_val;      // [sell_entries] (empty when packed)
   edx::ObjBuf<float2> sell_val32;     // [sell_entries] complex64 copy (empty when packed)
-  // 1: palette-packed copy (sellpack.hip): sell_col holds the 32-bit words
-  // column | (palette index << pack_bits), 0xFFFFFFFF = padding, and the
-  // elements are pal[index] (complex128) and pal32[index] (complex64)
+  // palette-packed copy (sellpack.hip); the elements are pal[index]
+  // (complex128) and pal32[index] (complex64).  1: sell_col holds the 32-bit
+  // words column | (palette index << pack_bits), 0xFFFFFFFF = padding;
+  // 2 (wide): sell_col keeps the columns and sell_idx holds the indices
   int sell_packed = 0;
   int pack_bits = 0;
   int n_pal = 0;
   edx::ObjBuf<double2> pal;           // [n_pal]
   edx::ObjBuf<float2> pal32;          // [n_pal]
+  edx::ObjBuf<uint16_t> sell_idx;     // [sell_entries] (wide layout only)
   // 1: no stored table; every product recomputes the elements from the terms
   // and the catalog (the arrays below), CSR and sliced arrays are empty
   int on_the_fly = 0;

@@ -321,15 +321,19 @@ template <> struct Real<float2> { using T = float; };
 // entry); Y is column-major with leading dimension ldy.
 // ---------------------------------------------------------------------------
 // Slice s of one table (wavefront `lane` lanes); shared by the single-table
-// and the grouped kernels, so both give identical results.  PACKED: `col`
+// and the grouped kernels, so both give identical results.  LAYOUT 0: `col`
+// and `val` are the columns (-1 = padding) and elements.  LAYOUT 1: `col`
 // holds the palette-packed words (column | index << bits, 0xFFFFFFFF =
-// padding) and `val` is the palette (in LDS); the elements multiplied carry
-// the bits of the unpacked copy's, so both layouts give identical results.
-template <typename V, int NB, int G, bool PACKED>
+// padding) and `val` is the palette (in LDS).  LAYOUT 2 (wide): `col` holds
+// the columns, `idx` the 16-bit palette indices and `val` the palette (in
+// LDS).  The elements multiplied carry the bits of the unpacked copy's, so all
+// layouts give identical results.
+template <typename V, int NB, int G, int LAYOUT>
 __device__ __forceinline__ void sell_slice(int64_t s, int lane, int64_t dim, const int64_t* __restrict__ off,
-                                           const int32_t* __restrict__ col, const V* __restrict__ val, int bits,
-                                           const V* __restrict__ hdiag, const V* __restrict__ X, V* __restrict__ Y,
-                                           int64_t ldy) {
+                                           const int32_t* __restrict__ col, const uint16_t* __restrict__ idx,
+                                           const V* __restrict__ val, int bits, const V* __restrict__ hdiag,
+                                           const V* __restrict__ X, V* __restrict__ Y, int64_t ldy) {
+  constexpr bool PACKED = LAYOUT == 1;
   using T = typename Real<V>::T;
   const int64_t o0 = off[s];
   const int64_t L = (off[s + 1] - o0) >> 6;
@@ -338,17 +342,24 @@ __device__ __forceinline__ void sell_slice(int64_t s, int lane, int64_t dim, con
 #pragma unroll
   for (int q = 0; q < NB; ++q) ar[q] = ai[q] = (T)0;
   const int32_t* cp = col + o0 + lane;
+  const uint16_t* ip = idx + o0 + lane;
   const V* vp = val + o0 + lane;
   const uint32_t cmask = (1u << bits) - 1u;
-  // the next entry's column is loaded while this entry's element and x
-  // values are fetched, so each entry costs one memory round trip
+  // the next entry's column (and palette index) is loaded while this entry's
+  // element and x values are fetched, so each entry costs one memory round
+  // trip
   int cn = L > 0 ? __builtin_nontemporal_load(cp) : -1;
+  int in = LAYOUT == 2 && L > 0 ? (int)__builtin_nontemporal_load(ip) : 0;
   for (int64_t t = 0; t < L; ++t) {
     const int w = cn;
-    if (t + 1 < L) cn = __builtin_nontemporal_load(cp + (t + 1) * 64);
+    const int wi = in;
+    if (t + 1 < L) {
+      cn = __builtin_nontemporal_load(cp + (t + 1) * 64);
+      if (LAYOUT == 2) in = (int)__builtin_nontemporal_load(ip + (t + 1) * 64);
+    }
     if (PACKED ? (uint32_t)w != 0xFFFFFFFFu : w >= 0) {
       const int c = PACKED ? (int)((uint32_t)w & cmask) : w;
-      const V v = PACKED ? val[(uint32_t)w >> bits] : vp[t * 64];
+      const V v = PACKED ? val[(uint32_t)w >> bits] : LAYOUT == 2 ? val[wi] : vp[t * 64];
       const V* x = X + (int64_t)c * NB;
       V xv[NB];
 #pragma unroll
@@ -408,7 +419,19 @@ spmm_sell(int64_t dim, int64_t n_slices, const int64_t* __restrict__ off, const 
   const V* v = PACKED ? stage_palette(val, n_pal) : val;
   const int64_t s = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
   if (s >= n_slices) return;
-  sell_slice<V, NB, G, PACKED>(s, threadIdx.x & 63, dim, off, col, v, bits, hdiag, X, Y, ldy);
+  sell_slice<V, NB, G, PACKED ? 1 : 0>(s, threadIdx.x & 63, dim, off, col, nullptr, v, bits, hdiag, X, Y, ldy);
+}
+
+// The same on the wide layout (columns and 16-bit palette indices).
+template <typename V, int NB, int G>
+__global__ void __launch_bounds__(256)
+spmm_sell_wide(int64_t dim, int64_t n_slices, const int64_t* __restrict__ off, const int32_t* __restrict__ col,
+               const uint16_t* __restrict__ idx, const V* __restrict__ pal, int n_pal, const V* __restrict__ hdiag,
+               const V* __restrict__ X, V* __restrict__ Y, int64_t ldy) {
+  const V* v = stage_palette(pal, n_pal);
+  const int64_t s = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (s >= n_slices) return;
+  sell_slice<V, NB, G, 2>(s, threadIdx.x & 63, dim, off, col, idx, v, 0, hdiag, X, Y, ldy);
 }
 
 // Grouped SpMM: the sliced products of up to kGroupMax tables (same element
@@ -421,6 +444,7 @@ struct SellTable {
   int64_t dim, n_slices, ldy;
   const int64_t* off;
   const int32_t* col;
+  const uint16_t* idx;  // palette indices (wide layout)
   const V* val;  // elements, or the palette of a packed table
   int n_pal, bits;
   const V* hdiag;
@@ -443,7 +467,20 @@ __global__ void __launch_bounds__(256) spmm_sell_group(SellGroup<V> g) {
   const V* v = PACKED ? stage_palette(t.val, t.n_pal) : t.val;
   const int64_t s = (blk - g.first[i]) * 4 + (threadIdx.x >> 6);
   if (s >= t.n_slices) return;
-  sell_slice<V, NB, G, PACKED>(s, threadIdx.x & 63, t.dim, t.off, t.col, v, t.bits, t.hdiag, t.X, t.Y, t.ldy);
+  sell_slice<V, NB, G, PACKED ? 1 : 0>(s, threadIdx.x & 63, t.dim, t.off, t.col, nullptr, v, t.bits, t.hdiag, t.X,
+                                       t.Y, t.ldy);
+}
+
+template <typename V, int NB, int G>
+__global__ void __launch_bounds__(256) spmm_sell_group_wide(SellGroup<V> g) {
+  const int64_t blk = blockIdx.x;
+  int i = 0;
+  while (i + 1 < g.n && blk >= g.first[i + 1]) ++i;
+  const SellTable<V>& t = g.t[i];
+  const V* v = stage_palette(t.val, t.n_pal);
+  const int64_t s = (blk - g.first[i]) * 4 + (threadIdx.x >> 6);
+  if (s >= t.n_slices) return;
+  sell_slice<V, NB, G, 2>(s, threadIdx.x & 63, t.dim, t.off, t.col, t.idx, v, 0, t.hdiag, t.X, t.Y, t.ldy);
 }
 
 template <typename V, int NB, bool PACKED>
@@ -454,6 +491,24 @@ void launch_sell_nb(const edx_hamiltonian& h, hipStream_t st, const V* val, cons
   const int32_t* col = h.sell_col.p;
   const int n_pal = PACKED ? h.n_pal : 0;
   const size_t lds = PACKED ? (size_t)n_pal * sizeof(V) : 0;
+  if (PACKED && h.sell_packed == 2) {
+    const uint16_t* idx = h.sell_idx.p;
+    switch (h.sell_g) {
+      case 1:
+        hipLaunchKernelGGL((spmm_sell_wide<V, NB, 1>), grid, dim3(256), lds, st, h.dim, h.sell_slices, off, col, idx,
+                           val, n_pal, hd, X, Y, ldy);
+        break;
+      case 2:
+        hipLaunchKernelGGL((spmm_sell_wide<V, NB, 2>), grid, dim3(256), lds, st, h.dim, h.sell_slices, off, col, idx,
+                           val, n_pal, hd, X, Y, ldy);
+        break;
+      default:
+        hipLaunchKernelGGL((spmm_sell_wide<V, NB, 4>), grid, dim3(256), lds, st, h.dim, h.sell_slices, off, col, idx,
+                           val, n_pal, hd, X, Y, ldy);
+    }
+    EDX_HIP(hipGetLastError());
+    return;
+  }
   switch (h.sell_g) {
     case 1:
       hipLaunchKernelGGL((spmm_sell<V, NB, 1, PACKED>), grid, dim3(256), lds, st, h.dim, h.sell_slices, off, col,
@@ -493,8 +548,9 @@ void launch_sell(const edx_hamiltonian& h, hipStream_t st, int nb, const V* val,
 // and one gather of nb x values (padding of the sliced layout not counted).
 double spmm_bytes(const edx_hamiltonian& h, int precision, int nb) {
   const double s = precision == EDX_PREC_FP64 ? 16.0 : 8.0;
-  // a packed entry is one 4-byte word (the palette stays in LDS)
-  const double elem = h.sell_packed ? 0.0 : s;
+  // a packed entry is one 4-byte word, a wide one its column and a 2-byte
+  // index (the palette stays in LDS)
+  const double elem = h.sell_packed == 1 ? 0.0 : h.sell_packed == 2 ? 2.0 : s;
   return (double)h.dim * (s + 2.0 * nb * s) + (double)h.n_entries * (4.0 + elem + nb * s);
 }
 
@@ -706,11 +762,22 @@ void launch_group_packed(const SellGroup<V>& g, int lanes, unsigned blocks, size
 }
 
 template <typename V, int NB>
-void launch_group_nb(const SellGroup<V>& g, int lanes, bool packed, unsigned blocks, hipStream_t st) {
+void launch_group_nb(const SellGroup<V>& g, int lanes, int packed, unsigned blocks, hipStream_t st) {
   int n_pal = 0;
   for (int i = 0; i < g.n; ++i) n_pal = std::max(n_pal, g.t[i].n_pal);
-  if (packed) launch_group_packed<V, NB, true>(g, lanes, blocks, (size_t)n_pal * sizeof(V), st);
-  else launch_group_packed<V, NB, false>(g, lanes, blocks, 0, st);
+  const size_t lds = (size_t)n_pal * sizeof(V);
+  if (packed == 2) {
+    switch (lanes) {
+      case 1: hipLaunchKernelGGL((spmm_sell_group_wide<V, NB, 1>), dim3(blocks), dim3(256), lds, st, g); break;
+      case 2: hipLaunchKernelGGL((spmm_sell_group_wide<V, NB, 2>), dim3(blocks), dim3(256), lds, st, g); break;
+      default: hipLaunchKernelGGL((spmm_sell_group_wide<V, NB, 4>), dim3(blocks), dim3(256), lds, st, g);
+    }
+    EDX_HIP(hipGetLastError());
+  } else if (packed == 1) {
+    launch_group_packed<V, NB, true>(g, lanes, blocks, lds, st);
+  } else {
+    launch_group_packed<V, NB, false>(g, lanes, blocks, 0, st);
+  }
 }
 
 template <typename V>
@@ -736,6 +803,7 @@ void launch_group(const SpmmJob* jobs, const std::vector<int>& idx, int precisio
     }
     t.n_pal = h.sell_packed ? h.n_pal : 0;
     t.bits = h.pack_bits;
+    t.idx = h.sell_idx.p;
     t.X = (const V*)j.X;
     t.Y = (V*)j.Y;
     g.first[i + 1] = g.first[i] + (h.sell_slices + 3) / 4;
@@ -743,7 +811,7 @@ void launch_group(const SpmmJob* jobs, const std::vector<int>& idx, int precisio
   EDX_REQUIRE(g.first[g.n] < (int64_t(1) << 31), EDX_ERR_INTERNAL, "grouped SpMM grid");
   const unsigned blocks = (unsigned)g.first[g.n];
   const int lanes = jobs[idx[0]].h->sell_g;
-  const bool packed = jobs[idx[0]].h->sell_packed != 0;
+  const int packed = jobs[idx[0]].h->sell_packed;
   switch (jobs[idx[0]].nb) {
 #define EDX_GROUP_CASE(N) \
   case N:                 \
@@ -934,6 +1002,7 @@ int64_t prepare_sell(edx_hamiltonian& h, Ctx& ctx, int G) {
   h.pack_bits = 0;
   h.pal.release();
   h.pal32.release();
+  h.sell_idx.release();
   h.sell_off.alloc((size_t)ns + 1);
   if (ns == 0) {
     const int64_t zero = 0;

@@ -15,22 +15,28 @@
 // from the palette held in LDS; the elements it multiplies carry the same bits
 // as the unpacked copy's, so both layouts give identical results.
 //
+// A table with more distinct values than the index bits left above the
+// column allow (the Hermitian parts of twisted blocks: their complex64-rounded
+// alpha give hundreds to a few thousand values) gets the wide layout instead:
+// the int32 columns stay and a 16-bit palette index per slot replaces the
+// elements (6 bytes per entry), for at most kPalMaxWide values.
+//
 // Build: a hash pass collects the distinct values (64-bit fingerprint of the
 // value's bits in an open-addressing table of kSlots slots, one compare-and-
 // swap per new value), the host numbers the occupied slots, and an encode
-// pass writes the words after checking, per entry, that the slot's value has
-// exactly the entry's complex128 and complex64 bits.  A table whose values
-// do not fit (more than kPalMax distinct values, or more than the index bits
-// left above the column) or whose check fails keeps the unpacked copy.
+// pass writes the words or indices after checking, per entry, that the slot's
+// value has exactly the entry's complex128 and complex64 bits.  A table whose
+// values do not fit (more than kPalMaxWide distinct values) or whose check
+// fails keeps the unpacked copy.
 #include "objects.h"
 
 namespace edx {
 
 namespace {
 
-constexpr int kSlots = 4096;  // hash slots (power of two, >= 4 kPalMax)
-// Longest probe sequence: at most kPalMax keys in kSlots slots (load <= 1/4)
-// keep linear-probing runs far shorter; a longer one ends the packing.
+constexpr int kSlots = 16384;  // hash slots (power of two, >= 4 kPalMaxWide)
+// Longest probe sequence: at most kPalMaxWide keys in kSlots slots (load <=
+// 1/4) keep linear-probing runs far shorter; a longer one ends the packing.
 constexpr int kMaxProbe = 64;
 
 __device__ __forceinline__ uint64_t mix64(uint64_t x) {
@@ -55,9 +61,9 @@ __device__ __forceinline__ bool same_bits(double2 a, double2 b) {
 }
 
 // Inserts the value of every stored slot (column >= 0) into the hash table.
-// flag[0] bit 0: packing is impossible (more than kPalMax distinct values, or
-// a probe run longer than kMaxProbe); every thread then stops.  flag[1]
-// counts the distinct values inserted.
+// flag[0] bit 0: packing is impossible (more than kPalMaxWide distinct
+// values, or a probe run longer than kMaxProbe); every thread then stops.
+// flag[1] counts the distinct values inserted.
 __global__ void __launch_bounds__(256)
 pal_insert(int64_t n, const int32_t* __restrict__ col, const double2* __restrict__ val, unsigned long long* keys,
            double2* rep, int* flag) {
@@ -78,7 +84,7 @@ pal_insert(int64_t n, const int32_t* __restrict__ col, const double2* __restrict
         cur = atomicCAS(&keys[s], 0ull, k);
         if (cur == 0ull) {
           rep[s] = v;
-          if (atomicAdd(flag + 1, 1) >= kPalMax) atomicOr(flag, 1);
+          if (atomicAdd(flag + 1, 1) >= kPalMaxWide) atomicOr(flag, 1);
           break;
         }
         if (cur == k) break;
@@ -99,19 +105,21 @@ __global__ void pal_gather(int n_pal, const int* slot_of, const double2* rep, do
   pal32[i] = make_float2((float)v.x, (float)v.y);
 }
 
-// Packed words; flag bit 1: an entry whose slot is missing or holds other
-// bits (a fingerprint collision), bit 2: a complex64 element that is not the
-// rounding of its complex128 one.
+// Packed words (words != null: column | index << bits, padding all ones) or
+// 16-bit indices (idx != null, padding 0); flag bit 1: an entry whose slot is
+// missing or holds other bits (a fingerprint collision), bit 2: a complex64
+// element that is not the rounding of its complex128 one.
 __global__ void __launch_bounds__(256)
 pal_encode(int64_t n, const int32_t* __restrict__ col, const double2* __restrict__ val,
            const float2* __restrict__ val32, const unsigned long long* __restrict__ keys,
            const double2* __restrict__ rep, const int* __restrict__ index_of_slot, int bits,
-           uint32_t* __restrict__ words, int* flag) {
+           uint32_t* __restrict__ words, uint16_t* __restrict__ idx, int* flag) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
     const int32_t c = col[i];
     if (c < 0) {
-      words[i] = 0xFFFFFFFFu;
+      if (words) words[i] = 0xFFFFFFFFu;
+      else idx[i] = 0;
       continue;
     }
     const double2 v = val[i];
@@ -130,7 +138,8 @@ pal_encode(int64_t n, const int32_t* __restrict__ col, const double2* __restrict
     const float2 r = make_float2((float)v.x, (float)v.y);
     if (__float_as_uint(w.x) != __float_as_uint(r.x) || __float_as_uint(w.y) != __float_as_uint(r.y))
       atomicOr(flag, 4);
-    words[i] = (uint32_t)c | ((uint32_t)index_of_slot[s] << bits);
+    if (words) words[i] = (uint32_t)c | ((uint32_t)index_of_slot[s] << bits);
+    else idx[i] = (uint16_t)index_of_slot[s];
   }
 }
 
@@ -147,13 +156,13 @@ int pack_bits_for(int64_t dim) {
   return std::max(b, 1);
 }
 
-// Replaces h's unpacked sliced copy by the packed one when its values allow;
-// returns true when packed.  Called once the copy is complete.
+// Replaces h's unpacked sliced copy by a packed one when its values allow:
+// the 32-bit words when the palette fits the bits above the column, else the
+// wide layout; returns true when packed.  Called once the copy is complete.
 bool pack_sell(edx_hamiltonian& h, Ctx& ctx) {
   if (!ctx.table_pack || h.sell_packed || h.sell_entries == 0 || h.on_the_fly) return false;
   const int bits = pack_bits_for(h.dim);
-  if (bits >= 31) return false;
-  const int cap = std::min(kPalMax, 1 << (32 - bits));
+  const int cap = bits < 31 ? std::min(kPalMax, 1 << (32 - bits)) : 0;
   hipStream_t st = ctx.stream;
   const int64_t n = h.sell_entries;
   DevBuf<unsigned long long> keys(kSlots);
@@ -177,11 +186,13 @@ bool pack_sell(edx_hamiltonian& h, Ctx& ctx) {
       slot_of.push_back(s);
     }
   const int n_pal = (int)slot_of.size();
-  if (n_pal == 0 || n_pal > cap) return false;
+  if (n_pal == 0 || n_pal > kPalMaxWide) return false;
+  const bool wide = n_pal > cap;
   DevBuf<int> d_index(kSlots), d_slot(n_pal);
   ObjBuf<double2> pal(n_pal);
   ObjBuf<float2> pal32(n_pal);
-  ObjBuf<int32_t> words(n);
+  ObjBuf<int32_t> words(wide ? 0 : n);
+  ObjBuf<uint16_t> idx(wide ? n : 0);
   EDX_HIP(hipMemcpyAsync(d_index.p, index_of_slot.data(), d_index.bytes(), hipMemcpyHostToDevice, st));
   EDX_HIP(hipMemcpyAsync(d_slot.p, slot_of.data(), d_slot.bytes(), hipMemcpyHostToDevice, st));
   hipLaunchKernelGGL(pal_gather, dim3((n_pal + 255) / 256), dim3(256), 0, st, n_pal, (const int*)d_slot.p,
@@ -190,19 +201,23 @@ bool pack_sell(edx_hamiltonian& h, Ctx& ctx) {
   hipLaunchKernelGGL(pal_encode, dim3(grid_stride_blocks(n)), dim3(256), 0, st, n, (const int32_t*)h.sell_col.p,
                      (const double2*)h.sell_val.p, (const float2*)h.sell_val32.p,
                      (const unsigned long long*)keys.p, (const double2*)rep.p, (const int*)d_index.p, bits,
-                     (uint32_t*)words.p, flag.p);
+                     wide ? nullptr : (uint32_t*)words.p, wide ? idx.p : nullptr, flag.p);
   EDX_HIP(hipGetLastError());
   EDX_HIP(hipMemcpyAsync(&hflag, flag.p, sizeof(int), hipMemcpyDeviceToHost, st));
   EDX_HIP(hipStreamSynchronize(st));
   if (hflag) return false;
-  h.sell_col = std::move(words);
+  if (wide) {
+    h.sell_idx = std::move(idx);
+  } else {
+    h.sell_col = std::move(words);
+    h.pack_bits = bits;
+  }
   h.sell_val.release();
   h.sell_val32.release();
   h.pal = std::move(pal);
   h.pal32 = std::move(pal32);
   h.n_pal = n_pal;
-  h.pack_bits = bits;
-  h.sell_packed = 1;
+  h.sell_packed = wide ? 2 : 1;
   return true;
 }
 

@@ -20,6 +20,10 @@ equality, not a tolerance:
 * a twisted table (its sliced copy holds the Hermitian part, packed when
   its values fit) gives bit-identical mixed-precision levels on both;
 * a table handed over as host CSR is packed and applies its operator;
+* a table of 2 000 distinct values (beyond the 1024-value palette of the
+  32-bit words) gets the wide layout (columns and 16-bit indices): its
+  single and grouped products in both precisions are bit-identical to the
+  unpacked table's;
 * a table of 50 000 distinct values keeps the unpacked copy, is built
   promptly (the hash pass stops once packing is impossible) and applies its
   operator.
@@ -85,10 +89,13 @@ def test_packed_product_bit_identical(name, sector):
     lp, lu = Hp._dev.layout(), Hu._dev.layout()
     twisted = name.endswith("_twist")
     print(f"[packed] {name}: D={Hp.n} E={Hp._dev.n_entries} layout packed={lp} plain={lu}", flush=True)
-    assert not lu["packed"] and lu["n_palette"] == 0
+    assert lu["packed"] == 0 and lu["n_palette"] == 0
     if twisted:
-        # the sliced copy holds the Hermitian part, which edx_matvec_device does
-        # not read: compare the solves, whichever layout the values allowed
+        # the sliced copy holds the Hermitian part (hundreds of distinct values:
+        # the wide layout), which edx_matvec_device does not read: compare the
+        # solves
+        assert lp["packed"] in (1, 2)
+        assert lp["table_bytes"] < lu["table_bytes"]
         levels = []
         for H, ctx in ((Hp, ctxs[0]), (Hu, ctxs[1])):
             with _edx.use_context(ctx):
@@ -98,7 +105,7 @@ def test_packed_product_bit_identical(name, sector):
         assert levels[0].tobytes() == levels[1].tobytes()
         return
     bits = max(int(Hp.n).bit_length(), 1)
-    assert lp["packed"] and 0 < lp["n_palette"] <= min(1024, 2 ** (32 - bits))
+    assert lp["packed"] == 1 and 0 < lp["n_palette"] <= min(1024, 2 ** (32 - bits))
     assert lp["stored_slots"] == lu["stored_slots"]
     assert lp["table_bytes"] < lu["table_bytes"]
     for nb in (1, 4, 10, 16):
@@ -215,7 +222,7 @@ def test_too_many_values_keep_unpacked_copy():
     try:
         lay = _edx.hamiltonian_layout(h)
         print(f"[packed] {A.nnz} distinct values: layout {lay}, build {t_build:.3f} s", flush=True)
-        assert not lay["packed"] and lay["n_palette"] == 0
+        assert lay["packed"] == 0 and lay["n_palette"] == 0
         assert t_build < 10.0
         x = rng.standard_normal(D) + 1j * rng.standard_normal(D)
         y = np.empty_like(x)
@@ -224,3 +231,69 @@ def test_too_many_values_keep_unpacked_copy():
         assert np.linalg.norm(y - ref) <= 1e-13 * np.linalg.norm(ref)
     finally:
         lib.edx_hamiltonian_destroy(h)
+
+
+def _csr_table(ctx, D, per_row, n_values, seed):
+    """A random Hermitian-free host CSR table whose elements come from
+    n_values distinct complex values, placed on the device through ctx."""
+    import scipy.sparse as sp
+
+    rng = np.random.default_rng(seed)
+    values = rng.standard_normal(n_values) + 1j * rng.standard_normal(n_values)
+    rows = np.repeat(np.arange(D), per_row)
+    cols = (rows + 1 + rng.integers(0, D - 1, D * per_row)) % D  # never the diagonal
+    A = sp.csr_matrix((np.ones(D * per_row), (rows, cols)), shape=(D, D))
+    A.sum_duplicates()
+    A.sort_indices()
+    A.data = values[np.arange(A.nnz) % n_values].astype(np.complex128)
+    hd = rng.standard_normal(D) + 0j
+    h = _edx.C.c_void_p()
+    ctx.check(lib.edx_hamiltonian_from_csr(ctx.handle, D, A.nnz, _edx.ptr(np.ascontiguousarray(A.indptr, np.int64)),
+                                           _edx.ptr(np.ascontiguousarray(A.indices, np.int32)),
+                                           _edx.ptr(np.ascontiguousarray(A.data)), _edx.ptr(hd), _edx.C.byref(h)))
+    return h, A, hd
+
+
+class _Handle:
+    """Minimal stand-in for a device Hamiltonian wrapper (handle, ctx, dim)."""
+
+    def __init__(self, ctx, h, dim):
+        self.ctx, self.handle, self.dim = ctx, h, dim
+
+
+def test_wide_layout_bit_identical():
+    ctxs = _contexts()
+    D = 6000
+    made = [_csr_table(c, D, 12, 2000, 21) for c in ctxs]
+    try:
+        lw, lu = _edx.hamiltonian_layout(made[0][0]), _edx.hamiltonian_layout(made[1][0])
+        print(f"[packed] wide layout {lw}, unpacked {lu}", flush=True)
+        assert lw["packed"] == 2 and lw["n_palette"] == 2000 and lu["packed"] == 0
+        assert lw["table_bytes"] < lu["table_bytes"]
+        hw, hu = _Handle(ctxs[0], made[0][0], D), _Handle(ctxs[1], made[1][0], D)
+        A, hd = made[0][1], made[0][2]
+        for nb in (1, 4, 10):
+            X = _block(D, nb, 40 + nb)
+            for prec in (_edx.PREC_FP64, _edx.PREC_MIXED):
+                yw = _product(hw, prec, nb, X)
+                yu = _product(hu, prec, nb, X)
+                assert yw.tobytes() == yu.tobytes(), (nb, prec)
+            ref = (A @ X + hd[:, None] * X).T
+            y64 = _product(hw, _edx.PREC_FP64, nb, X)
+            assert np.linalg.norm(y64 - ref) <= 1e-13 * np.linalg.norm(ref)
+        # grouped launch over two wide tables (the same table twice)
+        nb = 4
+        X = np.ascontiguousarray(_block(D, nb, 77).astype(np.complex64))
+        xs, ys = [], []
+        for _ in range(2):
+            dx = _edx.DeviceArray(ctxs[0], X.nbytes)
+            dx.upload(X)
+            xs.append(dx)
+            ys.append(_edx.DeviceArray(ctxs[0], X.nbytes))
+        _edx.matvec_group(ctxs[0], [hw, hw], _edx.PREC_MIXED, nb, xs, ys)
+        alone = _product(hw, _edx.PREC_MIXED, nb, X.astype(np.complex128))
+        for dy in ys:
+            assert dy.download(np.empty((nb, D), np.complex64)).tobytes() == alone.tobytes()
+    finally:
+        for h, _, _ in made:
+            lib.edx_hamiltonian_destroy(h)
