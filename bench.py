@@ -192,8 +192,11 @@ def element_bytes(layout: dict, s: float) -> float:
 
 def spmm_kernel_name(layout: dict, v: str, nb: int, lanes) -> str:
     """Kernel name of the sliced SpMM as the summaries write it."""
-    if layout["packed"] == 2:
-        return f"spmm_sell_wide<{v},{nb},{lanes}>"
+    if layout["packed"] == 2:  # palette in LDS per 1024 threads up to 64 KB (complex128: 256 up to 24 KB)
+        pb = layout["n_palette"] * (8 if v == "float2" else 16)
+        shape = ("true,256" if v == "double2" and pb <= 24576 else "true,1024" if pb <= 65536
+                 else "false,256")
+        return f"spmm_sell_wide<{v},{nb},{lanes},{shape}>"
     return f"spmm_sell<{v},{nb},{lanes},{'true' if layout['packed'] else 'false'}>"
 
 

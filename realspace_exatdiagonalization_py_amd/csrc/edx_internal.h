@@ -19,9 +19,15 @@
 
 namespace edx {
 // Largest palette of a packed sliced table: 32-bit words (complex128: 16 KiB
-// of LDS per workgroup), and the wide layout (16-bit indices; 48 KiB).
+// of LDS per workgroup), and the wide layout (16-bit indices), whose palette
+// is staged in LDS per 256-thread workgroup up to kPalLdsBytes (complex128
+// single-table products and grouped products), per 1024-thread workgroup up
+// to kPalLdsBytesLarge (other single-table products), and read through the
+// caches above.
 constexpr int kPalMax = 1024;
-constexpr int kPalMaxWide = 3072;
+constexpr int kPalMaxWide = 16384;
+constexpr int kPalLdsBytes = 24576;
+constexpr int kPalLdsBytesLarge = 65536;
 }  // namespace edx
 #define EDX_MAX_TERMS 65535
 

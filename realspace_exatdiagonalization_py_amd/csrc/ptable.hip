@@ -422,14 +422,17 @@ spmm_sell(int64_t dim, int64_t n_slices, const int64_t* __restrict__ off, const 
   sell_slice<V, NB, G, PACKED ? 1 : 0>(s, threadIdx.x & 63, dim, off, col, nullptr, v, bits, hdiag, X, Y, ldy);
 }
 
-// The same on the wide layout (columns and 16-bit palette indices).
-template <typename V, int NB, int G>
-__global__ void __launch_bounds__(256)
+// The same on the wide layout (columns and 16-bit palette indices); LDS:
+// the palette is staged in LDS, else read through the caches.  BLK threads
+// per workgroup (BLK / 64 slices): large palettes are staged once per 1024
+// threads, so that LDS does not limit the wavefronts per CU.
+template <typename V, int NB, int G, bool LDS, int BLK>
+__global__ void __launch_bounds__(BLK)
 spmm_sell_wide(int64_t dim, int64_t n_slices, const int64_t* __restrict__ off, const int32_t* __restrict__ col,
                const uint16_t* __restrict__ idx, const V* __restrict__ pal, int n_pal, const V* __restrict__ hdiag,
                const V* __restrict__ X, V* __restrict__ Y, int64_t ldy) {
-  const V* v = stage_palette(pal, n_pal);
-  const int64_t s = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const V* v = LDS ? stage_palette(pal, n_pal) : pal;
+  const int64_t s = (int64_t)blockIdx.x * (BLK / 64) + (threadIdx.x >> 6);
   if (s >= n_slices) return;
   sell_slice<V, NB, G, 2>(s, threadIdx.x & 63, dim, off, col, idx, v, 0, hdiag, X, Y, ldy);
 }
@@ -471,16 +474,40 @@ __global__ void __launch_bounds__(256) spmm_sell_group(SellGroup<V> g) {
                                        t.Y, t.ldy);
 }
 
-template <typename V, int NB, int G>
+template <typename V, int NB, int G, bool LDS>
 __global__ void __launch_bounds__(256) spmm_sell_group_wide(SellGroup<V> g) {
   const int64_t blk = blockIdx.x;
   int i = 0;
   while (i + 1 < g.n && blk >= g.first[i + 1]) ++i;
   const SellTable<V>& t = g.t[i];
-  const V* v = stage_palette(t.val, t.n_pal);
+  const V* v = LDS ? stage_palette(t.val, t.n_pal) : t.val;
   const int64_t s = (blk - g.first[i]) * 4 + (threadIdx.x >> 6);
   if (s >= t.n_slices) return;
   sell_slice<V, NB, G, 2>(s, threadIdx.x & 63, t.dim, t.off, t.col, t.idx, v, 0, t.hdiag, t.X, t.Y, t.ldy);
+}
+
+template <typename V, int NB, bool LDS, int BLK>
+void launch_wide(const edx_hamiltonian& h, hipStream_t st, size_t lds, const V* val, int n_pal, const V* hd,
+                 const V* X, V* Y, int64_t ldy) {
+  const int64_t* off = h.sell_off.p;
+  const int32_t* col = h.sell_col.p;
+  const uint16_t* idx = h.sell_idx.p;
+  constexpr int S = BLK / 64;
+  const dim3 grid((unsigned)((h.sell_slices + S - 1) / S));
+  switch (h.sell_g) {
+    case 1:
+      hipLaunchKernelGGL((spmm_sell_wide<V, NB, 1, LDS, BLK>), grid, dim3(BLK), lds, st, h.dim, h.sell_slices, off,
+                         col, idx, val, n_pal, hd, X, Y, ldy);
+      break;
+    case 2:
+      hipLaunchKernelGGL((spmm_sell_wide<V, NB, 2, LDS, BLK>), grid, dim3(BLK), lds, st, h.dim, h.sell_slices, off,
+                         col, idx, val, n_pal, hd, X, Y, ldy);
+      break;
+    default:
+      hipLaunchKernelGGL((spmm_sell_wide<V, NB, 4, LDS, BLK>), grid, dim3(BLK), lds, st, h.dim, h.sell_slices, off,
+                         col, idx, val, n_pal, hd, X, Y, ldy);
+  }
+  EDX_HIP(hipGetLastError());
 }
 
 template <typename V, int NB, bool PACKED>
@@ -492,21 +519,14 @@ void launch_sell_nb(const edx_hamiltonian& h, hipStream_t st, const V* val, cons
   const int n_pal = PACKED ? h.n_pal : 0;
   const size_t lds = PACKED ? (size_t)n_pal * sizeof(V) : 0;
   if (PACKED && h.sell_packed == 2) {
-    const uint16_t* idx = h.sell_idx.p;
-    switch (h.sell_g) {
-      case 1:
-        hipLaunchKernelGGL((spmm_sell_wide<V, NB, 1>), grid, dim3(256), lds, st, h.dim, h.sell_slices, off, col, idx,
-                           val, n_pal, hd, X, Y, ldy);
-        break;
-      case 2:
-        hipLaunchKernelGGL((spmm_sell_wide<V, NB, 2>), grid, dim3(256), lds, st, h.dim, h.sell_slices, off, col, idx,
-                           val, n_pal, hd, X, Y, ldy);
-        break;
-      default:
-        hipLaunchKernelGGL((spmm_sell_wide<V, NB, 4>), grid, dim3(256), lds, st, h.dim, h.sell_slices, off, col, idx,
-                           val, n_pal, hd, X, Y, ldy);
-    }
-    EDX_HIP(hipGetLastError());
+    // measured on the twisted 36-site sectors (palettes of 0.9-1.3k values):
+    // complex64 products run faster with 1024-thread workgroups at any
+    // palette size (1473 against 1523 us), complex128 ones with 256 threads
+    // while their palette fits (4265 against 4573 us)
+    const size_t small = std::is_same<V, float2>::value ? 0 : (size_t)kPalLdsBytes;
+    if (lds <= small) launch_wide<V, NB, true, 256>(h, st, lds, val, n_pal, hd, X, Y, ldy);
+    else if (lds <= (size_t)kPalLdsBytesLarge) launch_wide<V, NB, true, 1024>(h, st, lds, val, n_pal, hd, X, Y, ldy);
+    else launch_wide<V, NB, false, 256>(h, st, 0, val, n_pal, hd, X, Y, ldy);
     return;
   }
   switch (h.sell_g) {
@@ -767,10 +787,18 @@ void launch_group_nb(const SellGroup<V>& g, int lanes, int packed, unsigned bloc
   for (int i = 0; i < g.n; ++i) n_pal = std::max(n_pal, g.t[i].n_pal);
   const size_t lds = (size_t)n_pal * sizeof(V);
   if (packed == 2) {
-    switch (lanes) {
-      case 1: hipLaunchKernelGGL((spmm_sell_group_wide<V, NB, 1>), dim3(blocks), dim3(256), lds, st, g); break;
-      case 2: hipLaunchKernelGGL((spmm_sell_group_wide<V, NB, 2>), dim3(blocks), dim3(256), lds, st, g); break;
-      default: hipLaunchKernelGGL((spmm_sell_group_wide<V, NB, 4>), dim3(blocks), dim3(256), lds, st, g);
+    if (lds <= (size_t)kPalLdsBytes) {
+      switch (lanes) {
+        case 1: hipLaunchKernelGGL((spmm_sell_group_wide<V, NB, 1, true>), dim3(blocks), dim3(256), lds, st, g); break;
+        case 2: hipLaunchKernelGGL((spmm_sell_group_wide<V, NB, 2, true>), dim3(blocks), dim3(256), lds, st, g); break;
+        default: hipLaunchKernelGGL((spmm_sell_group_wide<V, NB, 4, true>), dim3(blocks), dim3(256), lds, st, g);
+      }
+    } else {
+      switch (lanes) {
+        case 1: hipLaunchKernelGGL((spmm_sell_group_wide<V, NB, 1, false>), dim3(blocks), dim3(256), 0, st, g); break;
+        case 2: hipLaunchKernelGGL((spmm_sell_group_wide<V, NB, 2, false>), dim3(blocks), dim3(256), 0, st, g); break;
+        default: hipLaunchKernelGGL((spmm_sell_group_wide<V, NB, 4, false>), dim3(blocks), dim3(256), 0, st, g);
+      }
     }
     EDX_HIP(hipGetLastError());
   } else if (packed == 1) {

@@ -19,7 +19,8 @@
 // column allow (the Hermitian parts of twisted blocks: their complex64-rounded
 // alpha give hundreds to a few thousand values) gets the wide layout instead:
 // the int32 columns stay and a 16-bit palette index per slot replaces the
-// elements (6 bytes per entry), for at most kPalMaxWide values.
+// elements (6 bytes per entry), for at most kPalMaxWide values (palettes
+// beyond kPalLdsBytes are read through the caches instead of LDS).
 //
 // Build: a hash pass collects the distinct values (64-bit fingerprint of the
 // value's bits in an open-addressing table of kSlots slots, one compare-and-
@@ -34,7 +35,7 @@ namespace edx {
 
 namespace {
 
-constexpr int kSlots = 16384;  // hash slots (power of two, >= 4 kPalMaxWide)
+constexpr int kSlots = 65536;  // hash slots (power of two, >= 4 kPalMaxWide)
 // Longest probe sequence: at most kPalMaxWide keys in kSlots slots (load <=
 // 1/4) keep linear-probing runs far shorter; a longer one ends the packing.
 constexpr int kMaxProbe = 64;

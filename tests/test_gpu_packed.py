@@ -20,10 +20,11 @@ equality, not a tolerance:
 * a twisted table (its sliced copy holds the Hermitian part, packed when
   its values fit) gives bit-identical mixed-precision levels on both;
 * a table handed over as host CSR is packed and applies its operator;
-* a table of 2 000 distinct values (beyond the 1024-value palette of the
-  32-bit words) gets the wide layout (columns and 16-bit indices): its
-  single and grouped products in both precisions are bit-identical to the
-  unpacked table's;
+* tables of 1 200, 2 000 and 6 000 distinct values (beyond the 1024-value
+  palette of the 32-bit words) get the wide layout (columns and 16-bit
+  indices; palettes in LDS or read through the caches): their single and
+  grouped products in both precisions are bit-identical to the unpacked
+  table's;
 * a table of 50 000 distinct values keeps the unpacked copy, is built
   promptly (the hash pass stops once packing is impossible) and applies its
   operator.
@@ -193,9 +194,9 @@ def test_packing_from_host_csr():
 
 
 def test_too_many_values_keep_unpacked_copy():
-    """50 000 distinct values (beyond the 1024-value palette and the 4096-slot
-    hash table): the build stops the hash pass early, keeps the unpacked
-    copy, and the product equals the host CSR product."""
+    """50 000 distinct values (beyond the 16384-value wide palette): the build
+    stops the hash pass early, keeps the unpacked copy, and the product
+    equals the host CSR product."""
     import time
 
     import scipy.sparse as sp
@@ -261,14 +262,19 @@ class _Handle:
         self.ctx, self.handle, self.dim = ctx, h, dim
 
 
-def test_wide_layout_bit_identical():
+@pytest.mark.parametrize("n_values", [1200, 2000, 6000])
+def test_wide_layout_bit_identical(n_values):
+    """Palettes staged in LDS per 1024-thread workgroup (complex64 up to 64 KB,
+    complex128 above 24 KB), per 256-thread workgroup (complex128 up to
+    24 KB: 1200 values) and read through the caches (complex128 above 64 KB:
+    6000 values)."""
     ctxs = _contexts()
     D = 6000
-    made = [_csr_table(c, D, 12, 2000, 21) for c in ctxs]
+    made = [_csr_table(c, D, 12, n_values, 21) for c in ctxs]
     try:
         lw, lu = _edx.hamiltonian_layout(made[0][0]), _edx.hamiltonian_layout(made[1][0])
         print(f"[packed] wide layout {lw}, unpacked {lu}", flush=True)
-        assert lw["packed"] == 2 and lw["n_palette"] == 2000 and lu["packed"] == 0
+        assert lw["packed"] == 2 and lw["n_palette"] == n_values and lu["packed"] == 0
         assert lw["table_bytes"] < lu["table_bytes"]
         hw, hu = _Handle(ctxs[0], made[0][0], D), _Handle(ctxs[1], made[1][0], D)
         A, hd = made[0][1], made[0][2]
