@@ -190,14 +190,19 @@ def element_bytes(layout: dict, s: float) -> float:
     return {1: 0.0, 2: 2.0}.get(layout["packed"], s)
 
 
-def spmm_kernel_name(layout: dict, v: str, nb: int, lanes) -> str:
-    """Kernel name of the sliced SpMM as the summaries write it."""
-    if layout["packed"] == 2:  # palette in LDS per 1024 threads up to 64 KB (complex128: 256 up to 24 KB)
+def spmm_kernel_name(layout: dict, v: str, nb: int, lanes, dim: int) -> str:
+    """Kernel name of the sliced SpMM as the summaries write it (the
+    library's workgroup rule: 1024 threads for packed complex64 products of
+    at least 8192 such workgroups; wide palettes in LDS per 256 threads up to
+    24 KB, per 1024 up to 64 KB, else read through the caches)."""
+    slices = -(-dim * int(lanes or 4) // 64)
+    big = layout["packed"] and v == "float2" and -(-slices // 16) >= 8192
+    if layout["packed"] == 2:
         pb = layout["n_palette"] * (8 if v == "float2" else 16)
-        shape = ("true,256" if v == "double2" and pb <= 24576 else "true,1024" if pb <= 65536
-                 else "false,256")
+        shape = "true,256" if pb <= 24576 and not big else "true,1024" if pb <= 65536 else "false,256"
         return f"spmm_sell_wide<{v},{nb},{lanes},{shape}>"
-    return f"spmm_sell<{v},{nb},{lanes},{'true' if layout['packed'] else 'false'}>"
+    pk = "true" if layout["packed"] else "false"
+    return f"spmm_sell<{v},{nb},{lanes},{pk},{1024 if big else 256}>"
 
 
 def time_spmm(_edx, H, precision, nb, n_rep=20):
@@ -237,7 +242,7 @@ def profile_spmm(args) -> int:
         lay = dh.layout()
         for prec, v in ((1, "float2"), (0, "double2")):
             ms, nbytes = time_spmm(_edx, H, prec, 6)
-            out[spmm_kernel_name(lay, v, 6, g)] = {"avg_ms": round(ms, 5), "algorithmic_bytes": int(nbytes),
+            out[spmm_kernel_name(lay, v, 6, g, dh.dim)] = {"avg_ms": round(ms, 5), "algorithmic_bytes": int(nbytes),
                                                 "GB/s": round(nbytes / ms / 1e6, 1)}
     print(json.dumps(out))
     return 0
@@ -694,6 +699,7 @@ def checkerboard_bench(args) -> int:
     i_ref, w_mixed = last
     basis, H = sector(i_ref)
     table_layout = H._dev.layout()
+    dim_ref = H._dev.dim
     log(f"table layout of sector {i_ref}: {table_layout}")
     t0 = time.perf_counter()
     w64, _ = ed.diagonalize_block_matrixfree(H, H.n, nev=nev, precision=ed.PREC_FP64, eig_tol=0.0,
@@ -720,8 +726,8 @@ def checkerboard_bench(args) -> int:
     total_k = sum(v[0] for v in ks.values())
     cls = max(ks, key=lambda k: ks[k][0])
     lanes = next((e["lanes"] for e in _edx.autotune_table() if e["block"] == nb), None)
-    names = {"spmm_c64": spmm_kernel_name(table_layout, "float2", nb, lanes),
-             "spmm_c128": spmm_kernel_name(table_layout, "double2", max(nb, nev), lanes),
+    names = {"spmm_c64": spmm_kernel_name(table_layout, "float2", nb, lanes, dim_ref),
+             "spmm_c128": spmm_kernel_name(table_layout, "double2", max(nb, nev), lanes, dim_ref),
              "gram": "gram_tiles + gram_reduce", "gemm": "gemm_tiles", "combine": "gemm_tiles (restart combine)",
              "combine_mfma": "combine_mfma32 (restart combine)"}
     k_ms, k_n, k_bytes = ks[cls][:3]
@@ -1218,7 +1224,7 @@ def hubbard_bench(args) -> int:
     spmm_ms, spmm_n, spmm_bytes = ks[cls][:3]
     H0 = _sector_hamiltonian(ed, data, 0)
     lanes = H0._dev.spmv_group
-    kname = spmm_kernel_name(H0._dev.layout(), "float2" if cls == "spmm_c64" else "double2", nb, lanes)
+    kname = spmm_kernel_name(H0._dev.layout(), "float2" if cls == "spmm_c64" else "double2", nb, lanes, H0._dev.dim)
     avg_ms = spmm_ms / max(spmm_n, 1)
     achieved = spmm_bytes / spmm_ms / 1e6 if spmm_ms > 0 else 0.0  # GB/s (0: timer off)
     total_k = sum(v[0] for v in ks.values())

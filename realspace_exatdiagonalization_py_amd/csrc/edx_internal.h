@@ -20,10 +20,10 @@
 namespace edx {
 // Largest palette of a packed sliced table: 32-bit words (complex128: 16 KiB
 // of LDS per workgroup), and the wide layout (16-bit indices), whose palette
-// is staged in LDS per 256-thread workgroup up to kPalLdsBytes (complex128
-// single-table products and grouped products), per 1024-thread workgroup up
-// to kPalLdsBytesLarge (other single-table products), and read through the
-// caches above.
+// is staged in LDS per 256-thread workgroup up to kPalLdsBytes (small or
+// complex128 single-table products, grouped products), per 1024-thread
+// workgroup up to kPalLdsBytesLarge (large complex64 products and larger
+// palettes), and read through the caches above.
 constexpr int kPalMax = 1024;
 constexpr int kPalMaxWide = 16384;
 constexpr int kPalLdsBytes = 24576;

@@ -411,13 +411,13 @@ __device__ __forceinline__ const V* stage_palette(const V* __restrict__ pal, int
   return p;
 }
 
-template <typename V, int NB, int G, bool PACKED>
-__global__ void __launch_bounds__(256)
+template <typename V, int NB, int G, bool PACKED, int BLK = 256>
+__global__ void __launch_bounds__(BLK)
 spmm_sell(int64_t dim, int64_t n_slices, const int64_t* __restrict__ off, const int32_t* __restrict__ col,
           const V* __restrict__ val, int n_pal, int bits, const V* __restrict__ hdiag, const V* __restrict__ X,
           V* __restrict__ Y, int64_t ldy) {
   const V* v = PACKED ? stage_palette(val, n_pal) : val;
-  const int64_t s = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int64_t s = (int64_t)blockIdx.x * (BLK / 64) + (threadIdx.x >> 6);
   if (s >= n_slices) return;
   sell_slice<V, NB, G, PACKED ? 1 : 0>(s, threadIdx.x & 63, dim, off, col, nullptr, v, bits, hdiag, X, Y, ldy);
 }
@@ -518,15 +518,35 @@ void launch_sell_nb(const edx_hamiltonian& h, hipStream_t st, const V* val, cons
   const int32_t* col = h.sell_col.p;
   const int n_pal = PACKED ? h.n_pal : 0;
   const size_t lds = PACKED ? (size_t)n_pal * sizeof(V) : 0;
+  // Packed complex64 products of large tables run on 1024-thread workgroups
+  // (16 slices each, the palette staged once per 16 wavefronts): 48-site
+  // sector 6.52 -> 6.31 ms, twisted 36-site sector 1523 -> 1473 us.
+  // Complex128 products (48-site, 10 vectors: 27.1 -> 30.7 ms) and tables of
+  // fewer than 8192 such workgroups (Hubbard 3x4, 6 vectors: 47.6 -> 52.6 us)
+  // keep 256 threads.
+  const bool big = PACKED && std::is_same<V, float2>::value && (h.sell_slices + 15) / 16 >= 8192;
   if (PACKED && h.sell_packed == 2) {
-    // measured on the twisted 36-site sectors (palettes of 0.9-1.3k values):
-    // complex64 products run faster with 1024-thread workgroups at any
-    // palette size (1473 against 1523 us), complex128 ones with 256 threads
-    // while their palette fits (4265 against 4573 us)
-    const size_t small = std::is_same<V, float2>::value ? 0 : (size_t)kPalLdsBytes;
-    if (lds <= small) launch_wide<V, NB, true, 256>(h, st, lds, val, n_pal, hd, X, Y, ldy);
+    if (lds <= (size_t)kPalLdsBytes && !big) launch_wide<V, NB, true, 256>(h, st, lds, val, n_pal, hd, X, Y, ldy);
     else if (lds <= (size_t)kPalLdsBytesLarge) launch_wide<V, NB, true, 1024>(h, st, lds, val, n_pal, hd, X, Y, ldy);
     else launch_wide<V, NB, false, 256>(h, st, 0, val, n_pal, hd, X, Y, ldy);
+    return;
+  }
+  if (PACKED && big) {
+    const dim3 g2((unsigned)((h.sell_slices + 15) / 16));
+    switch (h.sell_g) {
+      case 1:
+        hipLaunchKernelGGL((spmm_sell<V, NB, 1, PACKED, 1024>), g2, dim3(1024), lds, st, h.dim, h.sell_slices, off,
+                           col, val, n_pal, h.pack_bits, hd, X, Y, ldy);
+        break;
+      case 2:
+        hipLaunchKernelGGL((spmm_sell<V, NB, 2, PACKED, 1024>), g2, dim3(1024), lds, st, h.dim, h.sell_slices, off,
+                           col, val, n_pal, h.pack_bits, hd, X, Y, ldy);
+        break;
+      default:
+        hipLaunchKernelGGL((spmm_sell<V, NB, 4, PACKED, 1024>), g2, dim3(1024), lds, st, h.dim, h.sell_slices, off,
+                           col, val, n_pal, h.pack_bits, hd, X, Y, ldy);
+    }
+    EDX_HIP(hipGetLastError());
     return;
   }
   switch (h.sell_g) {

@@ -25,6 +25,9 @@ equality, not a tolerance:
   indices; palettes in LDS or read through the caches): their single and
   grouped products in both precisions are bit-identical to the unpacked
   table's;
+* tables of 2.2M rows, whose complex64 products run on 1024-thread
+  workgroups, give products bit-identical to the unpacked copy in both
+  packed layouts;
 * a table of 50 000 distinct values keeps the unpacked copy, is built
   promptly (the hash pass stops once packing is impossible) and applies its
   operator.
@@ -300,6 +303,28 @@ def test_wide_layout_bit_identical(n_values):
         alone = _product(hw, _edx.PREC_MIXED, nb, X.astype(np.complex128))
         for dy in ys:
             assert dy.download(np.empty((nb, D), np.complex64)).tobytes() == alone.tobytes()
+    finally:
+        for h, _, _ in made:
+            lib.edx_hamiltonian_destroy(h)
+
+
+@pytest.mark.parametrize("n_values", [40, 1500])
+def test_large_table_workgroup_shape_bit_identical(n_values):
+    """A table of 2.2M rows (over 8192 workgroups of 16 slices): its complex64
+    products run on 1024-thread workgroups (32-bit words with 40 values, the
+    wide layout with 1500), complex128 on 256 threads; both bit-identical to
+    the unpacked copy."""
+    ctxs = _contexts()
+    D = 2_200_000
+    made = [_csr_table(c, D, 17, n_values, 5) for c in ctxs]
+    try:
+        lw, lu = _edx.hamiltonian_layout(made[0][0]), _edx.hamiltonian_layout(made[1][0])
+        print(f"[packed] D={D} {n_values} values: layout {lw}", flush=True)
+        assert lw["packed"] == (1 if n_values <= 1024 else 2) and lu["packed"] == 0
+        hp, hu = _Handle(ctxs[0], made[0][0], D), _Handle(ctxs[1], made[1][0], D)
+        for prec in (_edx.PREC_MIXED, _edx.PREC_FP64):
+            X = _block(D, 4, 9)
+            assert _product(hp, prec, 4, X).tobytes() == _product(hu, prec, 4, X).tobytes(), prec
     finally:
         for h, _, _ in made:
             lib.edx_hamiltonian_destroy(h)
