@@ -188,9 +188,40 @@ __global__ void __launch_bounds__(NT) ptab_sort(int64_t dim, const int64_t* row_
   for (int i = 0; i < n; ++i) keys[b + i] = s[i * NT];
 }
 
-// 32 KB of LDS per block up to rows of 64 entries, 64 KB above.
+// Rows of at most kRankCap entries: one wavefront per row (four per
+// workgroup).  The row is staged in LDS and every lane places its keys at
+// their rank, the number of smaller keys plus the equal ones before it, read
+// as LDS broadcasts: the stable order of the insertion sort above, without
+// its one-thread-per-row serial shifts and its LDS-bound occupancy.
+constexpr int kRankCap = 128;
+__global__ void __launch_bounds__(256) ptab_sort_rank(int64_t dim, const int64_t* row_ptr, uint64_t* keys) {
+  __shared__ uint64_t buf[4][kRankCap];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t r = (int64_t)blockIdx.x * 4 + w;
+  int64_t b = 0, n = 0;
+  if (r < dim) {
+    b = row_ptr[r];
+    n = row_ptr[r + 1] - b;
+  }
+  for (int i = lane; i < n; i += 64) buf[w][i] = keys[b + i];
+  __syncthreads();
+  for (int i = lane; i < n; i += 64) {
+    const uint64_t v = buf[w][i];
+    int pos = 0;
+    for (int j = 0; j < n; ++j) {
+      const uint64_t u = buf[w][j];
+      pos += (u < v) || (u == v && j < i);
+    }
+    keys[b + pos] = v;
+  }
+}
+
+// Rank sort up to rows of kRankCap entries; above, 32 KB of LDS per block up
+// to rows of 64 entries, 64 KB above (only the longest rows can reach it).
 void launch_sort(hipStream_t st, int64_t dim, uint32_t max_row, const int64_t* row_ptr, uint64_t* keys) {
-  if (max_row <= 32)
+  if (max_row <= (uint32_t)kRankCap)
+    hipLaunchKernelGGL(ptab_sort_rank, dim3(edx::grid_for(dim, 4)), dim3(256), 0, st, dim, row_ptr, keys);
+  else if (max_row <= 32)
     hipLaunchKernelGGL((ptab_sort<32, 128>), dim3(edx::grid_for(dim, 128)), dim3(128), 0, st, dim, row_ptr, keys);
   else if (max_row <= 64)
     hipLaunchKernelGGL((ptab_sort<64, 64>), dim3(edx::grid_for(dim, 64)), dim3(64), 0, st, dim, row_ptr, keys);
