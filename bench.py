@@ -714,17 +714,17 @@ def checkerboard_bench(args) -> int:
     ks = _edx.kernel_stats_all(enable=0)
     log(f"{args.steps} sector solves in {elapsed:.2f}s")
 
-    # the timed sectors again with K in flight (one context and worker thread
-    # each): the throughput when sectors overlap on the device.  The line's
+    # the timed sectors again, each K times, with K in flight (one context and
+    # worker thread each): the throughput when sectors overlap on the device.  The line's
     # value, latencies and roofline stay those of one sector at a time, whose
     # kernel durations are not stretched by the other stream's work.
     in_flight = None
     if args.in_flight > 1:
         ctxs = [_edx.Context(ctx.device) for _ in range(args.in_flight)]
-        # the timed sectors, then the next ones up to a multiple of K solves
-        # (no solve runs alone at the end)
-        n_fl = -(-len(timed) // args.in_flight) * args.in_flight
-        todo = [(args.warmup + j) % n_sec for j in range(n_fl)]
+        # the timed sectors, each K times: the same sector mix as the timed
+        # steps, in a multiple of K solves (no solve runs alone at the end)
+        todo = list(timed) * args.in_flight
+        n_fl = len(todo)
         got, errs, lock = {}, [], threading.Lock()
 
         def worker(c):
