@@ -435,7 +435,7 @@ __device__ __forceinline__ void sell_slice(int64_t s, int lane, int64_t dim, con
 // part, before any of them returns).
 template <typename V>
 __device__ __forceinline__ const V* stage_palette(const V* __restrict__ pal, int n) {
-  extern __shared__ char smem[];
+  extern __shared__ __attribute__((aligned(16))) char smem[];  // holds double2 palettes
   V* p = reinterpret_cast<V*>(smem);
   for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = pal[i];
   __syncthreads();
