@@ -1308,6 +1308,10 @@ def hubbard_bench(args) -> int:
         "concurrent_streams": args.workers,
         "isolated": {"avg_launch_ms": round(iso_ms, 5), "achieved": round(iso_bytes / iso_ms / 1e6, 1),
                      "frac": round(iso_bytes / iso_ms / 1e6 / HBM_PEAK_GBS, 4),
+                     # counter bytes over the isolated launch (the timed launches share the device with
+                     # the other streams, so their event times include the others' work)
+                     "hbm_frac": (round(_traffic(kname) / iso_ms / 1e6 / HBM_PEAK_GBS, 4)
+                                  if _traffic(kname) and iso_ms > 0 else None),
                      "note": "same kernel, sector k=(0,0), one stream, 20 launches after the timed region"},
         "kernel_time_share": {k: round(v[0] / total_k, 3) for k, v in ks.items() if total_k > 0},
         # (the Gram's duration includes its partial-sum reduction)
